@@ -1,0 +1,626 @@
+"""Pins the CPU oracle (oracle/galax_oracle.cpp) before it is trusted as the
+checker of the HIP path: the reference's own known-answer tests
+(galax-core/src/lib.rs:1757-2908, galax-integrate/src/lib.rs:1-144), closed
+forms (sympy fixtures), exact Taylor identities and brute force on tiny inputs.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from galax_amd import ic
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rng_bodies(n, seed, lo=0.0, hi=100.0, mlo=1.0, mhi=1.0):
+    r = ic.SplitMix64(seed)
+    b = ic._bodies(n)
+    b["x"][:] = lo + (hi - lo) * r.uniform(n)
+    b["y"][:] = lo + (hi - lo) * r.uniform(n)
+    b["m"][:] = mlo + (mhi - mlo) * r.uniform(n)
+    return b
+
+
+# ---------------------------------------------------------------- Morton ----
+
+def test_morton_round_trip():  # lib.rs:1758
+    bnd = (-50.0, 50.0, -50.0, 50.0)
+    max_err = 100.0 / (1 << 21)
+    for x, y in [(0.0, 0.0), (25.0, -30.0), (-40.0, 10.0), (49.99, -49.99), (-0.001, 0.001)]:
+        k = O.morton_encode(x, y, *bnd)
+        rx, ry = O.morton_decode(k, *bnd)
+        assert math.hypot(x - rx, y - ry) < max_err
+
+
+def test_morton_injective_and_separation():  # lib.rs:1782, 2833
+    keys = {O.morton_encode(x, y, 0, 1, 0, 1) for x in [0.0, 0.1, 0.5, 0.9, 0.999] for y in [0.0, 0.2, 0.5, 0.8, 0.999]}
+    assert len(keys) == 25
+    assert O.morton_encode(0.1, 0.2, 0, 1, 0, 1) != O.morton_encode(0.9, 0.8, 0, 1, 0, 1)
+
+
+def test_morton_bit_interleave_closed_form():
+    # key = interleave(xi, yi), x in even bits; brute force on the grid
+    r = ic.SplitMix64(7)
+    xs = (r.next_u64(200) >> np.uint64(43)).astype(np.int64)  # 21-bit ints
+    ys = (r.next_u64(200) >> np.uint64(43)).astype(np.int64)
+    for xi, yi in zip(xs, ys):
+        # grid-cell centre maps back to its own cell
+        x = (int(xi) + 0.5) / (1 << 21)
+        y = (int(yi) + 0.5) / (1 << 21)
+        want = 0
+        for b in range(21):
+            want |= ((int(xi) >> b) & 1) << (2 * b)
+            want |= ((int(yi) >> b) & 1) << (2 * b + 1)
+        assert O.morton_encode(x, y, 0.0, 1.0, 0.0, 1.0) == want
+
+
+def test_morton_clamp_semantics():
+    # out-of-bounds positions clamp to the edge cells (t clamped to [0, 1-1e-15])
+    top = (1 << 21) - 1
+    k_hi = O.morton_encode(1e9, 1e9, 0, 1, 0, 1)
+    assert k_hi == (1 << 42) - 1
+    assert O.morton_encode(-5.0, -5.0, 0, 1, 0, 1) == 0
+    assert O.morton_encode(1.0, 0.0, 0, 1, 0, 1) == sum(1 << (2 * b) for b in range(21))
+    assert top == 2097151
+
+
+def test_f64_as_i64_semantics():
+    assert O.lib().orc_f64_as_i64(-0.7) == 0
+    assert O.lib().orc_f64_as_i64(-1.7) == -1
+    assert O.lib().orc_f64_as_i64(2.999) == 2
+    assert O.lib().orc_f64_as_i64(float("nan")) == 0
+    assert O.lib().orc_f64_as_i64(1e300) == 2 ** 63 - 1
+
+
+# ------------------------------------------------------------------ Tree ----
+
+def test_tree_sort_is_stable_and_sorted():
+    b = rng_bodies(3000, 11)
+    b["x"][:1500] = np.round(b["x"][:1500], 0)  # force duplicate keys
+    b["y"][:1500] = np.round(b["y"][:1500], 0)
+    x0 = b["x"].copy()
+    y0 = b["y"].copy()
+    keys0 = np.array([O.morton_encode(x, y, 0, 100, 0, 100) for x, y in zip(x0, y0)], np.uint64)
+    t, perm = O.tree_build(b, 32, (0, 100, 0, 100), return_perm=True)
+    assert np.array_equal(b["x"], x0[perm.astype(np.int64)])
+    assert np.array_equal(perm.astype(np.int64), np.argsort(keys0, kind="stable"))
+    assert len(np.unique(keys0)) < len(keys0)
+
+
+def test_tree_leaves_cover_and_size():  # lib.rs:1796, 1823
+    b = rng_bodies(1000, 12)
+    t = O.tree_build(b, 64, (0, 100, 0, 100))
+    d = t.export()
+    leaf = d["is_leaf"] == 1
+    ranges = sorted(zip(d["body_start"][leaf], d["body_end"][leaf]))
+    cov = np.concatenate([np.arange(s, e) for s, e in ranges])
+    assert np.array_equal(cov, np.arange(1000))
+    assert ((d["body_end"] - d["body_start"])[leaf] <= 64).all()
+
+
+def test_tree_parent_covers_children():  # lib.rs:1850
+    b = rng_bodies(500, 13)
+    t = O.tree_build(b, 32, (0, 100, 0, 100))
+    d = t.export()
+    for i in np.nonzero(d["is_leaf"] == 0)[0]:
+        for c in d["children"][i]:
+            if c < 0:
+                continue
+            assert d["body_start"][i] <= d["body_start"][c] and d["body_end"][i] >= d["body_end"][c]
+            assert abs(d["center_x"][c] - d["center_x"][i]) + d["half_width"][c] <= d["half_width"][i] + 1e-12
+            assert abs(d["center_y"][c] - d["center_y"][i]) + d["half_width"][c] <= d["half_width"][i] + 1e-12
+
+
+def test_tree_single_body():  # lib.rs:1892
+    b = ic._bodies(1)
+    b["x"][0] = b["y"][0] = 0.5
+    b["m"][0] = 1.0
+    t = O.tree_build(b, 32, (0, 1, 0, 1))
+    d = t.export()
+    assert len(t) == 1 and d["is_leaf"][0] == 1 and d["body_end"][0] - d["body_start"][0] == 1
+
+
+def test_parent_half_width_is_double_child():  # lib.rs:2466
+    b = rng_bodies(16, 14, 0, 10)
+    t = O.tree_build(b, 2, (0, 10, 0, 10))
+    d = t.export()
+    for i in np.nonzero(d["is_leaf"] == 0)[0]:
+        ch = [c for c in d["children"][i] if c >= 0]
+        assert abs(d["half_width"][i] - 2 * max(d["half_width"][c] for c in ch)) < 1e-12
+
+
+def test_tree_geometry_closed_form():
+    # leaf mass/COM/bbox against numpy; parent mass = sum of children; level
+    # structure: every leaf has the same depth and children are consecutive
+    b = rng_bodies(2000, 15, mlo=0.1, mhi=10.1)
+    t = O.tree_build(b, 32, (0, 100, 0, 100))
+    d = t.export()
+    lv = t.levels()
+    for i in np.nonzero(d["is_leaf"] == 1)[0]:
+        s, e = int(d["body_start"][i]), int(d["body_end"][i])
+        m = b["m"][s:e]
+        assert d["mass"][i] == pytest.approx(m.sum(), rel=1e-13)
+        assert d["com_x"][i] == pytest.approx(np.average(b["x"][s:e], weights=m), rel=1e-12)
+        hx = (b["x"][s:e].max() - b["x"][s:e].min()) / 2
+        hy = (b["y"][s:e].max() - b["y"][s:e].min()) / 2
+        assert d["half_width"][i] == pytest.approx(max(hx, hy), abs=1e-12)
+        assert lv[i] == 0
+    for i in np.nonzero(d["is_leaf"] == 0)[0]:
+        ch = [c for c in d["children"][i] if c >= 0]
+        assert ch == list(range(ch[0], ch[0] + len(ch)))
+        assert d["mass"][i] == pytest.approx(sum(d["mass"][c] for c in ch), rel=1e-13)
+        assert all(lv[c] + 1 == lv[i] for c in ch)
+    # ids are level-ordered
+    assert (np.diff(lv.astype(np.int64)) >= 0).all()
+
+
+# ----------------------------------------------------------------- Lists ----
+
+def _touch(d, a, b):
+    h = d["half_width"][a] + d["half_width"][b]
+    return (abs(d["center_x"][a] - d["center_x"][b]) <= h + 1e-15 and
+            abs(d["center_y"][a] - d["center_y"][b]) <= h + 1e-15)
+
+
+def test_interaction_lists_well_separated_and_capped():  # lib.rs:2083, 2122
+    b = rng_bodies(2000, 16)
+    t = O.tree_build(b, 32, (0, 100, 0, 100))
+    d = t.export()
+    lists = O.m2l_lists(t).as_lists()
+    assert max(len(l) for l in lists) <= 64
+    for node, lst in enumerate(lists):
+        for pid in lst:
+            h = d["half_width"][node] + d["half_width"][pid]
+            adj = (abs(d["center_x"][node] - d["center_x"][pid]) <= h + 1e-14 and
+                   abs(d["center_y"][node] - d["center_y"][pid]) <= h + 1e-14)
+            assert not adj
+            pa, pb = d["parent"][node], d["parent"][pid]
+            assert _touch(d, pa, pb)
+
+
+def _brute_m2l_lists(d, lv, cap):
+    """Independent statement of the list rule from its specification: for node
+    c with parent P, every child q of every same-level node Q != P touching P,
+    in ascending (Q, child) order, with q not touching c, truncated to cap."""
+    nn = len(lv)
+    out = [[] for _ in range(nn)]
+    ids_by_lv = {}
+    for i in range(nn):
+        ids_by_lv.setdefault(int(lv[i]), []).append(i)
+    for c in range(nn):
+        P = d["parent"][c]
+        if P < 0:
+            continue
+        cand = [Q for Q in ids_by_lv[int(lv[P])] if Q != P and _touch(d, P, Q)]
+        for Q in cand:
+            for q in d["children"][Q]:
+                if q >= 0 and not _touch(d, q, c):
+                    out[c].append(int(q))
+        out[c] = out[c][:cap]
+    return out
+
+
+def test_m2l_lists_brute_force_small():
+    for seed, n, nmax in [(17, 300, 4), (18, 600, 8), (19, 120, 1)]:
+        b = rng_bodies(n, seed)
+        t = O.tree_build(b, nmax, (0, 100, 0, 100))
+        d = t.export()
+        lv = t.levels()
+        assert O.m2l_lists(t).as_lists() == _brute_m2l_lists(d, lv, 64)
+
+
+def test_p2p_lists_brute_force_small():
+    b = rng_bodies(400, 20)
+    t = O.tree_build(b, 4, (0, 100, 0, 100))
+    d = t.export()
+    leaves = [i for i in range(len(t)) if d["is_leaf"][i]]
+    want = [[] for _ in range(len(t))]
+    for a in leaves:
+        want[a] = ([a] + [c for c in leaves if c != a and _touch(d, a, c)])[:64]
+    assert O.p2p_lists(t).as_lists() == want
+
+
+def test_p2p_lists_cap_truncates():
+    # a dense clump of tiny leaves inside a uniform field overflows the cap
+    b = rng_bodies(4000, 21)
+    b["x"][:3000] = 50 + 0.01 * (b["x"][:3000] / 100)
+    b["y"][:3000] = 50 + 0.01 * (b["y"][:3000] / 100)
+    t = O.tree_build(b, 2, (0, 100, 0, 100))
+    assert max(len(l) for l in O.p2p_lists(t).as_lists()) == 64
+
+
+# ----------------------------------------------------- kernel derivatives ----
+
+def test_kernel_derivatives_closed_form():  # lib.rs:2562-2596 incl. printed ones
+    p = 2
+    dx, dy, eps = 3.0, 4.0, 0.1
+    G = O.kernel_derivs(dx, dy, eps, p)
+    r2 = dx * dx + dy * dy + eps * eps
+    r = math.sqrt(r2)
+    r5 = r2 * r2 * r
+    k = lambda a, b_: O.kernel_deriv_idx(a, b_, p)
+    assert abs(G[k(0, 0)] + 1 / r) < 1e-14
+    assert abs(G[k(1, 0)] - dx / (r2 * r)) < 1e-14
+    assert abs(G[k(0, 1)] - dy / (r2 * r)) < 1e-14
+    assert G[k(2, 0)] == pytest.approx((-2 * dx * dx + dy * dy + eps * eps) / r5, rel=1e-13)
+    assert G[k(1, 1)] == pytest.approx(-3 * dx * dy / r5, rel=1e-13)
+    assert G[k(0, 2)] == pytest.approx((dx * dx - 2 * dy * dy + eps * eps) / r5, rel=1e-13)
+
+
+def test_kernel_derivatives_sympy_fixtures():
+    with open(os.path.join(GOLD, "kernel_derivs.json")) as f:
+        pts = json.load(f)
+    for pt in pts:
+        p = pt["p"]
+        G = O.kernel_derivs(pt["dx"], pt["dy"], pt["eps"], p)
+        scale = max(abs(v) for v in pt["G"].values())
+        for kl, want in pt["G"].items():
+            k, l = map(int, kl.split(","))
+            got = G[O.kernel_deriv_idx(k, l, p)]
+            # the polynomial evaluation cancels at high order: relative to the
+            # largest derivative of the same order
+            n = k + l
+            sc = max(abs(v) for kk, v in pt["G"].items() if sum(map(int, kk.split(","))) == n)
+            assert abs(got - want) <= 1e-9 * sc + 1e-300, (pt["dx"], pt["dy"], k, l, got, want)
+
+
+# --------------------------------------------------- expansion operators ----
+
+def test_p2m_single_body_moments():  # lib.rs:1906
+    b = ic._bodies(1)
+    b["x"][0], b["y"][0], b["m"][0] = 0.75, 0.25, 2.0
+    t = O.tree_build(b, 32, (0, 1, 0, 1))
+    mp = O.p2m(b["x"], b["y"], b["m"], t, 4)
+    d = t.export()
+    dx = b["x"][0] - d["com_x"][0]
+    assert mp[0] == 2.0
+    assert abs(mp[O.moment_index(1, 0)] - (-2 * dx)) < 1e-14
+
+
+def test_m2m_equals_direct_p2m_about_parent():
+    # M2M is exact: the shifted child moments equal P2M taken about the parent COM
+    b = rng_bodies(300, 22, mlo=0.1, mhi=5.0)
+    t = O.tree_build(b, 8, (0, 100, 0, 100))
+    d = t.export()
+    p = 6
+    S = O.num_moments(p)
+    mp = O.m2m(O.p2m(b["x"], b["y"], b["m"], t, p), t, p)
+    fact = [math.factorial(i) for i in range(p + 1)]
+    for node in np.nonzero(d["is_leaf"] == 0)[0][:40]:
+        s, e = int(d["body_start"][node]), int(d["body_end"][node])
+        dx = b["x"][s:e] - d["com_x"][node]
+        dy = b["y"][s:e] - d["com_y"][node]
+        for i in range(p + 1):
+            for j in range(p + 1 - i):
+                want = np.sum(b["m"][s:e] * dx ** i * dy ** j) / (fact[i] * fact[j])
+                got = mp[node * S + O.moment_index(i, j)]
+                assert got == pytest.approx(want, rel=1e-9, abs=1e-9 * np.sum(b["m"][s:e]) * 50 ** (i + j) / (fact[i] * fact[j]))
+
+
+def test_m2m_root_monopole_dipole():  # lib.rs:1943
+    b = ic._bodies(2)
+    b["x"][:] = [0.25, 0.75]
+    b["y"][:] = [0.25, 0.75]
+    b["m"][:] = [1.0, 2.0]
+    t = O.tree_build(b, 1, (0, 1, 0, 1))
+    p = 4
+    S = O.num_moments(p)
+    mp = O.m2m(O.p2m(b["x"], b["y"], b["m"], t, p), t, p)
+    root = (len(t) - 1) * S
+    assert abs(mp[root] - 3.0) < 1e-14
+    assert abs(mp[root + 1]) < 1e-14 and abs(mp[root + 2]) < 1e-14
+
+
+def _eval_local(L, base, dx, dy, p):
+    v = 0.0
+    for i in range(p + 1):
+        for j in range(p + 1 - i):
+            v += L[base + O.moment_index(i, j)] * dx ** i * dy ** j
+    return v
+
+
+def test_l2l_exact_reexpansion():  # lib.rs:2182, strengthened to random locals at every node pair
+    b = rng_bodies(200, 23)
+    t = O.tree_build(b, 4, (0, 100, 0, 100))
+    d = t.export()
+    p = 5
+    S = O.num_moments(p)
+    r = ic.SplitMix64(5)
+    L = np.zeros(len(t) * S)
+    # random locals on non-leaf nodes only; ascending-id L2L then moves each
+    # parent's own coefficients into its children
+    nonleaf = np.nonzero(d["is_leaf"] == 0)[0]
+    for i in nonleaf:
+        L[i * S:(i + 1) * S] = r.uniform(S) - 0.5
+    L0 = L.copy()
+    O.l2l(L, t, p)
+    for c in range(len(t)):
+        P = d["parent"][c]
+        if P < 0:
+            continue
+        for tx, ty in [(d["com_x"][c] + 0.3, d["com_y"][c] - 0.2), (d["com_x"][c] - 0.1, d["com_y"][c] + 0.05)]:
+            want = _eval_local(L0, P * S, tx - d["com_x"][P], ty - d["com_y"][P], p) + \
+                _eval_local(L0, c * S, tx - d["com_x"][c], ty - d["com_y"][c], p)
+            got = _eval_local(L, c * S, tx - d["com_x"][c], ty - d["com_y"][c], p)
+            assert got == pytest.approx(want, rel=1e-9, abs=1e-9)
+
+
+def test_l2p_is_minus_gradient_and_value():
+    b = rng_bodies(64, 24)
+    t = O.tree_build(b, 64, (0, 100, 0, 100))
+    d = t.export()
+    p = 4
+    S = O.num_moments(p)
+    L = ic.SplitMix64(9).uniform(S) - 0.5
+    phi = np.zeros(64)
+    ax, ay = O.l2p(b["x"], b["y"], L, t, p, phi=phi)
+    h = 1e-6
+    for i in range(0, 64, 7):
+        ux, uy = b["x"][i] - d["com_x"][0], b["y"][i] - d["com_y"][0]
+        gx = (_eval_local(L, 0, ux + h, uy, p) - _eval_local(L, 0, ux - h, uy, p)) / (2 * h)
+        gy = (_eval_local(L, 0, ux, uy + h, p) - _eval_local(L, 0, ux, uy - h, p)) / (2 * h)
+        assert ax[i] == pytest.approx(-gx, rel=1e-6, abs=1e-6)
+        assert ay[i] == pytest.approx(-gy, rel=1e-6, abs=1e-6)
+        assert phi[i] == pytest.approx(_eval_local(L, 0, ux, uy, p), rel=1e-12)
+
+
+def test_m2l_far_field_converges_to_direct():
+    """Two well-separated clusters; the target cluster's far field from the
+    source cluster via P2M -> M2L(exact) -> L2P converges to the direct sum
+    as p grows. A wrong sign, index or factorial breaks the convergence."""
+    n = 64
+    r = ic.SplitMix64(31)
+    b = ic._bodies(2 * n)
+    b["x"][:n] = 10 + r.uniform(n)
+    b["y"][:n] = 10 + r.uniform(n)
+    b["x"][n:] = 20 + r.uniform(n)
+    b["y"][n:] = 14 + r.uniform(n)
+    b["m"][:] = 0.5 + r.uniform(2 * n)
+    t = O.tree_build(b, n, (0, 100, 0, 100))
+    d = t.export()
+    assert int(d["is_leaf"].sum()) == 2 and d["is_leaf"][0] == 1 and d["is_leaf"][1] == 1
+    eps = 0.01
+    errs = []
+    for p in (2, 4, 6, 8):
+        S = O.num_moments(p)
+        mp = O.p2m(b["x"], b["y"], b["m"], t, p)
+        L = np.zeros(len(t) * S)
+        # leaf 0 (first sorted cluster) <- leaf 1 by hand through the oracle m2l:
+        G = O.kernel_derivs(d["com_x"][0] - d["com_x"][1], d["com_y"][0] - d["com_y"][1], eps, p)
+        fact = [math.factorial(i) for i in range(p + 1)]
+        for i in range(p + 1):
+            for j in range(p + 1 - i):
+                for I in range(p + 1):
+                    for J in range(p + 1 - I):
+                        L[O.moment_index(I, J)] += mp[S + O.moment_index(i, j)] * (-1) ** (i + j) / (fact[I] * fact[J]) * G[O.kernel_deriv_idx(i + I, j + J, p)]
+        ax, ay = O.l2p(b["x"], b["y"], L, t, p)
+        s0, e0 = int(d["body_start"][0]), int(d["body_end"][0])
+        s1, e1 = int(d["body_start"][1]), int(d["body_end"][1])
+        exx = np.zeros(e0 - s0)
+        for k, i in enumerate(range(s0, e0)):
+            dx = b["x"][s1:e1] - b["x"][i]
+            dy = b["y"][s1:e1] - b["y"][i]
+            inv = 1 / np.sqrt(dx * dx + dy * dy + eps * eps)
+            exx[k] = np.sum(b["m"][s1:e1] * inv ** 3 * dx)
+        errs.append(np.max(np.abs(ax[s0:e0] - exx)) / np.max(np.abs(exx)))
+    assert errs[-1] < 1e-6 and all(errs[i + 1] < errs[i] for i in range(3)), errs
+
+
+def test_m2l_oracle_matches_hand_contraction_and_cache():
+    """orc_m2l with the exact (uncached) derivatives equals the plan contraction
+    written out term by term; with the reference cache, pairs sharing a
+    truncated key reuse the first pair's derivatives."""
+    b = rng_bodies(300, 25, 0, 10, 0.5, 1.5)
+    t = O.tree_build(b, 8, (0, 10, 0, 10))
+    d = t.export()
+    p, eps = 3, 0.05
+    S = O.num_moments(p)
+    mp = O.m2m(O.p2m(b["x"], b["y"], b["m"], t, p), t, p)
+    lists = O.m2l_lists(t)
+    ll = lists.as_lists()
+    fact = [math.factorial(i) for i in range(p + 1)]
+
+    def contract(G, src):
+        out = np.zeros(S)
+        for i in range(p + 1):
+            for j in range(p + 1 - i):
+                mv = mp[src * S + O.moment_index(i, j)]
+                for I in range(p + 1):
+                    for J in range(p + 1 - I):
+                        out[O.moment_index(I, J)] += mv * (-1) ** (i + j) / (fact[I] * fact[J]) * G[O.kernel_deriv_idx(i + I, j + J, p)]
+        return out
+
+    Lx = O.m2l(np.zeros(len(t) * S), mp, lists, t, p, eps, cache_mode=1)
+    Lc = O.m2l(np.zeros(len(t) * S), mp, lists, t, p, eps, cache_mode=0)
+    cache = {}
+    for tg in range(len(t)):
+        want_x = np.zeros(S)
+        want_c = np.zeros(S)
+        for s in ll[tg]:
+            dx = d["com_x"][tg] - d["com_x"][s]
+            dy = d["com_y"][tg] - d["com_y"][s]
+            G = O.kernel_derivs(dx, dy, eps, p)
+            key = (int(dx), int(dy))  # Python int() truncates toward zero like `as i64`
+            cache.setdefault(key, G)
+            want_x += contract(G, s)
+            want_c += contract(cache[key], s)
+        np.testing.assert_allclose(Lx[tg * S:(tg + 1) * S], want_x, rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(Lc[tg * S:(tg + 1) * S], want_c, rtol=1e-10, atol=1e-12)
+    assert len(cache) < sum(len(l) for l in ll)  # the cache really is shared
+
+
+# ------------------------------------------------------------ P2P / N^2 ----
+
+def test_direct_sum_kats():  # lib.rs:2841, 2860, 2876, 2892
+    ax, ay, _ = O.n2_force([0.0, 1.0], [0.0, 0.0], [1.0, 2.0], 0.1)
+    assert abs(ax[0] * 1.0 + ax[1] * 2.0) < 1e-14
+    ax, ay, _ = O.n2_force([0.0, 0.0], [0.0, 0.0], [1.0, 1e6], 0.1)
+    assert ax[0] == 0.0 and ax[1] == 0.0
+    ax, ay, _ = O.n2_force([0.0, 1e-10], [0.0, 0.0], [1.0, 1.0], 0.1)
+    assert abs(ax[0] - 1e-10 / 0.1 ** 3) / (1e-10 / 0.1 ** 3) < 1e-6
+    ax, ay, _ = O.n2_force([0.0, 1.0], [0.0, 0.0], [1.0, 1.0], 0.1)
+    e = 1 / (1 + 0.01) ** 1.5
+    assert abs(ax[0] - e) < 1e-14 and abs(ax[1] + e) < 1e-14 and ay[0] == 0.0
+
+
+def test_p2p_single_leaf_equals_direct_sum():
+    # with one leaf the P2P self pass is the whole direct sum
+    b = rng_bodies(50, 26, 0, 10, 0.5, 2.0)
+    t = O.tree_build(b, 64, (0, 10, 0, 10))
+    ax, ay = O.p2p(b["x"], b["y"], b["m"], t, O.p2p_lists(t), 0.01)
+    ex, ey, _ = O.n2_force(b["x"], b["y"], b["m"], 0.01)
+    np.testing.assert_allclose(ax, ex, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(ay, ey, rtol=1e-12, atol=1e-12)
+
+
+def test_fmm_p0_equals_p2p():  # lib.rs:2320-2357
+    b = ic.reference_test_bodies(200)
+    t = O.tree_build(b, 16, (0, 100, 0, 100))
+    m2l, p2p = O.m2l_lists(t), O.p2p_lists(t)
+    ax, ay, _ = O.compute_fmm_force(b["x"], b["y"], b["m"], t, m2l, p2p, 0, 0.1)
+    px, py = O.p2p(b["x"], b["y"], b["m"], t, p2p, 0.1)
+    assert np.max(np.abs(ax - px)) < 1e-14 and np.max(np.abs(ay - py)) < 1e-14
+
+
+def test_fmm_finite_regressions():  # lib.rs:2769, 2805
+    b = rng_bodies(1000, 27, -50, 50, 0.1, 10.1)
+    t = O.tree_build(b, 32, (-50, 50, -50, 50))
+    eps = O.default_softening(b["x"], b["y"])
+    ax, ay, _ = O.compute_fmm_force(b["x"], b["y"], b["m"], t, O.m2l_lists(t), O.p2p_lists(t), 4, eps)
+    assert np.isfinite(ax).all() and np.isfinite(ay).all()
+    pb = ic.plummer(1000, seed=3)
+    t = O.tree_build(pb, 32, (-50, 50, -50, 50))
+    ax, ay, _ = O.compute_fmm_force(pb["x"], pb["y"], pb["m"], t, O.m2l_lists(t), O.p2p_lists(t), 4, 0.1)
+    assert np.isfinite(ax).all() and np.isfinite(ay).all()
+
+
+def test_sanitize_semantics():  # lib.rs:1475-1482
+    b = rng_bodies(100, 28, 0, 10)
+    t = O.tree_build(b, 8, (0, 10, 0, 10))
+    m2l, p2p = O.m2l_lists(t), O.p2p_lists(t)
+    x, y, m = b["x"].copy(), b["y"].copy(), b["m"].copy()
+    x[3], y[5], m[7], m[9] = np.nan, 1e101, -1.0, np.inf
+    ax, ay, _ = O.compute_fmm_force(x, y, m, t, m2l, p2p, 2, 0.1)
+    x2, y2, m2 = x.copy(), y.copy(), m.copy()
+    x2[3] = y2[3] = 0.0
+    x2[5] = y2[5] = 0.0
+    m2[7] = m2[9] = 1.0
+    bx, by, _ = O.compute_fmm_force(x2, y2, m2, t, m2l, p2p, 2, 0.1)
+    assert np.array_equal(ax, bx) and np.array_equal(ay, by)
+
+
+def test_corrected_variant_is_accurate():
+    """ALGO_CORRECTED (dual-tree lists with an acceptance test, exact
+    derivatives, top-down L2L) converges to the direct sum as p grows;
+    ALGO_REFERENCE at the same p does not come close."""
+    b = ic.plummer(4000, seed=5)
+    t = O.tree_build(b, 32, (-50, 50, -50, 50))
+    ex, ey, _ = O.n2_force(b["x"], b["y"], b["m"], 0.05)
+    mag = np.hypot(ex, ey)
+    rms = lambda ax, ay: np.sqrt(np.mean(((ax - ex) ** 2 + (ay - ey) ** 2) / mag ** 2))
+    ax, ay, _ = O.compute_fmm_force(b["x"], b["y"], b["m"], t, O.m2l_lists(t), O.p2p_lists(t), 8, 0.05)
+    e_ref = rms(ax, ay)
+    m2l, p2p = O.corrected_lists(t, b["x"], b["y"], 0.5)
+    e = []
+    for p in (2, 4, 6, 8):
+        ax, ay, _ = O.compute_fmm_force(b["x"], b["y"], b["m"], t, m2l, p2p, p, 0.05, algo=1)
+        e.append(rms(ax, ay))
+    assert all(e[i + 1] < e[i] for i in range(3)), e
+    assert e[-1] < 1e-4, e
+    assert e_ref > 100 * e[-1], (e_ref, e)
+
+
+def test_threaded_oracle_matches_sequential_uncached():
+    b = ic.plummer(3000, seed=6)
+    t = O.tree_build(b, 32, (-50, 50, -50, 50))
+    m2l, p2p = O.corrected_lists(t, b["x"], b["y"], 0.5)
+    a1 = O.compute_fmm_force(b["x"], b["y"], b["m"], t, m2l, p2p, 4, 0.05, algo=1, threads=1)
+    a4 = O.compute_fmm_force(b["x"], b["y"], b["m"], t, m2l, p2p, 4, 0.05, algo=1, threads=4)
+    assert np.array_equal(a1[0], a4[0]) and np.array_equal(a1[1], a4[1])
+
+
+# ---------------------------------------------------------- constrained ----
+
+def test_neighbors_geometric_no_self_and_adjacent():  # lib.rs:2689
+    b = rng_bodies(200, 29, 0, 10)
+    t = O.tree_build(b, 16, (0, 10, 0, 10))
+    d = t.export()
+    for i in range(len(t)):
+        nb = O.neighbors_geometric(t, i)
+        if i != len(t) - 1:
+            assert (nb >= 0).any()
+        for k in nb:
+            if k >= 0:
+                assert k != i
+                ratio = max(d["half_width"][i], d["half_width"][k]) / min(d["half_width"][i], d["half_width"][k])
+                assert ratio <= 8.0 and _touch(d, i, k)
+
+
+def test_build_constrained_ratio():  # lib.rs:2716
+    def max_ratio(t):
+        d = t.export()
+        mr = 0.0
+        for lid in np.nonzero(d["is_leaf"] == 1)[0]:
+            for k in O.neighbors_geometric(t, lid):
+                if k >= 0:
+                    a, c = d["half_width"][lid], d["half_width"][k]
+                    mr = max(mr, max(a, c) / min(a, c))
+        return mr
+    b1 = ic.reference_test_bodies(500)
+    b2 = ic.reference_test_bodies(500)
+    t1 = O.tree_build(b1, 16, (0, 100, 0, 100))
+    t2 = O.build_constrained(b2, 16, (0, 100, 0, 100))
+    assert max_ratio(t2) <= max_ratio(t1) + 0.5
+    d = t2.export()
+    leaf = d["is_leaf"] == 1
+    cov = np.concatenate([np.arange(s, e) for s, e in sorted(zip(d["body_start"][leaf], d["body_end"][leaf]))])
+    assert np.array_equal(cov, np.arange(500))
+
+
+# ------------------------------------------------------------ integrator ----
+
+def test_single_body_drift():  # galax-integrate/src/lib.rs:9
+    b = ic._bodies(1)
+    b["vx"][0], b["vy"][0], b["m"][0] = 10.0, 5.0, 1.0
+    t = O.tree_build(b, 32, (-100, 100, -100, 100))
+    O.simulate(b, t, O.m2l_lists(t), O.p2p_lists(t), 2, 0.1, 0.1, 1)
+    assert b["vx"][0] == 10.0 and b["vy"][0] == 5.0
+    assert abs(b["x"][0] - 1.0) < 1e-14 and abs(b["y"][0] - 0.5) < 1e-14
+
+
+def test_energy_diagnostics_known_values():  # galax-integrate/src/lib.rs:33
+    dg = O.diagnostics([0, 3.0], [0, 0.0], [2.0, -1.0], [1.0, 2.0], [5.0, 3.0], [0.5, -0.3], [0.0, 0.1])
+    assert abs(dg["kinetic"] - 0.5 * (5 * 5 + 3 * 5)) < 1e-14
+    assert abs(dg["momentum_x"] - 7.0) < 1e-14 and abs(dg["momentum_y"] - 11.0) < 1e-14
+    assert abs(dg["angular_momentum"] - 18.0) < 1e-14
+
+
+def test_kepler_orbit_through_fmm_one_leaf():  # galax-integrate/src/lib.rs:57 (one leaf => exact)
+    a, m = 10.0, 1.0
+    b = ic._bodies(2)
+    b["x"][:] = [-a / 2, a / 2]
+    b["m"][:] = [m, m]
+    v = math.sqrt(2 * m / a) / 2
+    b["vy"][:] = [v, -v]
+    t = O.tree_build(b, 32, (-50, 50, -50, 50))
+    m2l, p2p = O.m2l_lists(t), O.p2p_lists(t)
+    ax, ay, _ = O.compute_fmm_force(b["x"], b["y"], b["m"], t, m2l, p2p, 4, 1e-8)
+    b["ax"][:], b["ay"][:] = ax, ay
+    e0 = 0.5 * m * 2 * v * v - m * m / a
+    T = 2 * math.pi * math.sqrt(a ** 3 / (2 * m))
+    O.simulate(b, t, m2l, p2p, 4, 1e-8, T / 500, 500)
+    r = math.hypot(b["x"][1] - b["x"][0], b["y"][1] - b["y"][0])
+    e1 = 0.5 * m * (b["vx"] ** 2 + b["vy"] ** 2).sum() - m * m / r
+    assert abs(e1 - e0) / abs(e0) < 0.01
+
+
+def test_default_softening_and_dt():  # galax-cli/src/main.rs:182-190, 232-244
+    x = np.array([0.0, 3.0, 3.0])
+    y = np.array([0.0, 4.0, 0.0])
+    assert O.default_softening(x, y) == pytest.approx(0.1 * (5 + 4 + 3) / 3)
+    assert O.default_dt(np.array([3.0]), np.array([4.0])) == pytest.approx(0.5 / math.sqrt(5))
+    assert O.default_dt(np.array([0.0]), np.array([0.0])) == 0.01
