@@ -1,0 +1,645 @@
+// api.hip — the extern "C" entry points of include/galax_fmm.h and the
+// host-side orchestration around the device passes.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.h"
+
+using namespace gx;
+
+namespace gx {
+
+static inline unsigned nb256(size_t n) { return (unsigned)((n + 255) / 256); }
+
+// --------------------------------------------- corrected-variant lists ----
+
+// R_n = max |body - com_n| over the node's bodies; one wave per node.
+__global__ void k_radii(const double* __restrict__ x, const double* __restrict__ y, const uint32_t* __restrict__ bs,
+                        const uint32_t* __restrict__ be, const double* __restrict__ comx,
+                        const double* __restrict__ comy, uint32_t nn, double* __restrict__ R) {
+#pragma clang fp contract(off)
+  uint32_t nd = blockIdx.x * 4 + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
+  if (nd >= nn) return;
+  double cx = comx[nd], cy = comy[nd];
+  double r2 = 0.0;
+  for (uint32_t b = bs[nd] + lane; b < be[nd]; b += 64) {
+    double dx = x[b] - cx, dy = y[b] - cy;
+    r2 = fmax(r2, dx * dx + dy * dy);
+  }
+  for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o, 64));
+  if (lane == 0) R[nd] = sqrt(r2);
+}
+
+namespace {
+struct DttCtx {
+  const double *comx, *comy, *R;
+  const uint32_t* cf;
+  const uint8_t* cc;
+  double theta;
+  std::vector<std::vector<uint32_t>> m2l, p2p;
+};
+
+void dtt(DttCtx& c, uint32_t a, uint32_t b) {
+  const bool la = c.cc[a] == 0, lb = c.cc[b] == 0;
+  if (a == b) {
+    if (la) {
+      c.p2p[a].insert(c.p2p[a].begin(), a);
+      return;
+    }
+    for (uint32_t i = 0; i < c.cc[a]; ++i)
+      for (uint32_t j = i; j < c.cc[a]; ++j) dtt(c, c.cf[a] + i, c.cf[a] + j);
+    return;
+  }
+  double dx = c.comx[a] - c.comx[b], dy = c.comy[a] - c.comy[b];
+  double d = std::sqrt(dx * dx + dy * dy);
+  if (c.R[a] + c.R[b] < c.theta * d) {
+    c.m2l[a].push_back(b);
+    c.m2l[b].push_back(a);
+    return;
+  }
+  if (la && lb) {
+    c.p2p[a].push_back(b);
+    c.p2p[b].push_back(a);
+    return;
+  }
+  bool split_a = lb || (!la && c.R[a] >= c.R[b]);
+  if (split_a) {
+    for (uint32_t i = 0; i < c.cc[a]; ++i) dtt(c, c.cf[a] + i, b);
+  } else {
+    for (uint32_t i = 0; i < c.cc[b]; ++i) dtt(c, a, c.cf[b] + i);
+  }
+}
+}  // namespace
+
+// Dual tree traversal over the installed tree with the acceptance test
+// R_A + R_B < theta |com_A - com_B| (radii from the given positions).
+int build_lists_corrected(galax_fmm* h, const double* d_x, const double* d_y) {
+  DevTree& t = h->tree;
+  hipStream_t s = h->stream;
+  size_t nn = t.n_nodes;
+  h->corrected_ready = false;
+  if (nn == 0) {
+    h->corrected_ready = true;
+    return 0;
+  }
+  GX_CHECK(dbuf_reserve(h->ws.a, nn * 8, s));
+  k_radii<<<(unsigned)((nn + 3) / 4), 256, 0, s>>>(d_x, d_y, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(),
+                                                   t.com_x.as<double>(), t.com_y.as<double>(), (uint32_t)nn,
+                                                   h->ws.a.as<double>());
+  GX_HIP(hipGetLastError());
+  std::vector<double> comx(nn), comy(nn), R(nn);
+  std::vector<uint32_t> cf(nn);
+  std::vector<uint8_t> cc(nn);
+  GX_HIP(hipMemcpyAsync(comx.data(), t.com_x.p, nn * 8, hipMemcpyDeviceToHost, s));
+  GX_HIP(hipMemcpyAsync(comy.data(), t.com_y.p, nn * 8, hipMemcpyDeviceToHost, s));
+  GX_HIP(hipMemcpyAsync(R.data(), h->ws.a.p, nn * 8, hipMemcpyDeviceToHost, s));
+  GX_HIP(hipMemcpyAsync(cf.data(), t.child_first.p, nn * 4, hipMemcpyDeviceToHost, s));
+  GX_HIP(hipMemcpyAsync(cc.data(), t.child_count.p, nn, hipMemcpyDeviceToHost, s));
+  GX_HIP(hipStreamSynchronize(s));
+  DttCtx c{comx.data(), comy.data(), R.data(), cf.data(), cc.data(), h->cfg.theta, {}, {}};
+  c.m2l.assign(nn, {});
+  c.p2p.assign(nn, {});
+  uint32_t root = (uint32_t)nn - 1;
+  dtt(c, root, root);
+  std::vector<uint32_t> moff(nn + 1, 0), goff(t.n_leaves + 1, 0), mdat, gdat;
+  for (size_t i = 0; i < nn; ++i) {
+    std::sort(c.m2l[i].begin(), c.m2l[i].end());
+    moff[i + 1] = moff[i] + (uint32_t)c.m2l[i].size();
+  }
+  mdat.reserve(moff[nn]);
+  for (size_t i = 0; i < nn; ++i) mdat.insert(mdat.end(), c.m2l[i].begin(), c.m2l[i].end());
+  for (size_t i = 0; i < t.n_leaves; ++i) {
+    auto& v = c.p2p[i];
+    if (!v.empty()) std::sort(v.begin() + 1, v.end());
+    goff[i + 1] = goff[i] + (uint32_t)v.size();
+  }
+  gdat.reserve(goff[t.n_leaves]);
+  for (size_t i = 0; i < t.n_leaves; ++i) gdat.insert(gdat.end(), c.p2p[i].begin(), c.p2p[i].end());
+  h->c_m2l_total = mdat.size();
+  h->c_gat_total = gdat.size();
+  GX_CHECK(dbuf_reserve(h->c_m2l_off, moff.size() * 4, s));
+  GX_CHECK(dbuf_reserve(h->c_m2l_dat, mdat.size() * 4 + 4, s));
+  GX_CHECK(dbuf_reserve(h->c_gat_off, goff.size() * 4, s));
+  GX_CHECK(dbuf_reserve(h->c_gat_dat, gdat.size() * 4 + 4, s));
+  GX_HIP(hipMemcpyAsync(h->c_m2l_off.p, moff.data(), moff.size() * 4, hipMemcpyHostToDevice, s));
+  if (!mdat.empty()) GX_HIP(hipMemcpyAsync(h->c_m2l_dat.p, mdat.data(), mdat.size() * 4, hipMemcpyHostToDevice, s));
+  GX_HIP(hipMemcpyAsync(h->c_gat_off.p, goff.data(), goff.size() * 4, hipMemcpyHostToDevice, s));
+  if (!gdat.empty()) GX_HIP(hipMemcpyAsync(h->c_gat_dat.p, gdat.data(), gdat.size() * 4, hipMemcpyHostToDevice, s));
+  GX_HIP(hipStreamSynchronize(s));
+  GX_CHECK(count_interactions(h, h->c_gat_off.as<uint32_t>(), h->c_gat_dat.as<uint32_t>(), &h->c_p2p_interactions));
+  h->corrected_ready = true;
+  return 0;
+}
+
+// ---------------------------------------------------------------- KDK ----
+
+__global__ void k_kick(double* __restrict__ vx, double* __restrict__ vy, const double* __restrict__ ax,
+                       const double* __restrict__ ay, size_t n, double half) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  vx[i] += ax[i] * half;
+  vy[i] += ay[i] * half;
+}
+
+__global__ void k_drift(double* __restrict__ x, double* __restrict__ y, const double* __restrict__ vx,
+                        const double* __restrict__ vy, size_t n, double dt) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i] += vx[i] * dt;
+  y[i] += vy[i] * dt;
+}
+
+// per-block partial sums of {KE, 1/2 m phi, heuristic PE, px, py, Lz}
+__global__ void k_diag(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ vx,
+                       const double* __restrict__ vy, const double* __restrict__ m, const double* __restrict__ ax,
+                       const double* __restrict__ ay, const double* __restrict__ phi, size_t n,
+                       double* __restrict__ part) {
+  __shared__ double sh[6][256];
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  if (i < n) {
+    double mi = m[i];
+    v[0] = 0.5 * mi * (vx[i] * vx[i] + vy[i] * vy[i]);
+    v[1] = 0.5 * mi * (phi ? phi[i] : 0.0);
+    double a = sqrt(ax[i] * ax[i] + ay[i] * ay[i]);
+    double r = fmax(sqrt(x[i] * x[i] + y[i] * y[i]), 1.0);
+    v[2] = mi * a * r;
+    v[3] = mi * vx[i];
+    v[4] = mi * vy[i];
+    v[5] = mi * (x[i] * vy[i] - y[i] * vx[i]);
+  }
+  for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 6; ++q) part[(size_t)blockIdx.x * 6 + q] = sh[q][0];
+}
+
+static int diagnostics(galax_fmm* h, uint64_t step, galax_diag_t* out) {
+  size_t n = h->tree.n_bodies;
+  hipStream_t s = h->stream;
+  unsigned nb = nb256(n);
+  GX_CHECK(dbuf_reserve(h->ws.b, (size_t)nb * 6 * 8, s));
+  k_diag<<<nb, 256, 0, s>>>(h->x.as<double>(), h->y.as<double>(), h->vx.as<double>(), h->vy.as<double>(),
+                            h->m.as<double>(), h->ax.as<double>(), h->ay.as<double>(), h->phi.as<double>(), n,
+                            h->ws.b.as<double>());
+  GX_HIP(hipGetLastError());
+  std::vector<double> part((size_t)nb * 6);
+  GX_HIP(hipMemcpyAsync(part.data(), h->ws.b.p, part.size() * 8, hipMemcpyDeviceToHost, s));
+  GX_HIP(hipStreamSynchronize(s));
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (unsigned b = 0; b < nb; ++b)
+    for (int q = 0; q < 6; ++q) acc[q] += part[(size_t)b * 6 + q];
+  out->step = step;
+  out->kinetic = acc[0];
+  out->potential = acc[1];
+  out->potential_heuristic = -0.5 * acc[2];
+  out->momentum_x = acc[3];
+  out->momentum_y = acc[4];
+  out->angular_momentum = acc[5];
+  return 0;
+}
+
+int kdk_device(galax_fmm* h, double dt, uint64_t n_steps, uint64_t every, galax_diag_t* diag) {
+  size_t n = h->tree.n_bodies;
+  hipStream_t s = h->stream;
+  double half = 0.5 * dt;
+  uint64_t nd = 0;
+  for (uint64_t st = 0; st < n_steps; ++st) {
+    k_kick<<<nb256(n), 256, 0, s>>>(h->vx.as<double>(), h->vy.as<double>(), h->ax.as<double>(), h->ay.as<double>(),
+                                    n, half);
+    k_drift<<<nb256(n), 256, 0, s>>>(h->x.as<double>(), h->y.as<double>(), h->vx.as<double>(), h->vy.as<double>(),
+                                     n, dt);
+    GX_HIP(hipGetLastError());
+    GX_CHECK(evaluate(h, h->x.as<double>(), h->y.as<double>(), h->m.as<double>(), h->ax.as<double>(),
+                      h->ay.as<double>(), h->phi.as<double>()));
+    k_kick<<<nb256(n), 256, 0, s>>>(h->vx.as<double>(), h->vy.as<double>(), h->ax.as<double>(), h->ay.as<double>(),
+                                    n, half);
+    GX_HIP(hipGetLastError());
+    if (diag && every > 0 && (st + 1) % every == 0) GX_CHECK(diagnostics(h, st + 1, &diag[nd++]));
+  }
+  GX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+}  // namespace gx
+
+// ====================================================================== ABI
+
+namespace {
+
+int check_cfg(const galax_fmm_config_t* c) {
+  if (!c) return set_error(GALAX_ERR_INVALID, "config is NULL");
+  if (c->p > (uint32_t)MAX_P) return set_error(GALAX_ERR_INVALID, "p = %u outside 0..%d", c->p, MAX_P);
+  if (c->n_max == 0) return set_error(GALAX_ERR_INVALID, "n_max must be > 0");
+  if (!(c->softening >= 0.0) || !std::isfinite(c->softening))
+    return set_error(GALAX_ERR_INVALID, "softening must be finite and >= 0");
+  if (c->tree_mode > GALAX_TREE_CONSTRAINED) return set_error(GALAX_ERR_INVALID, "unknown tree_mode");
+  if (c->algo > GALAX_ALGO_CORRECTED) return set_error(GALAX_ERR_INVALID, "unknown algo");
+  if (c->precision != GALAX_PREC_FP64)
+    return set_error(GALAX_ERR_UNSUPPORTED, "precision %u is not provided by this build", c->precision);
+  if (c->algo == GALAX_ALGO_CORRECTED && !(c->theta > 0.0 && c->theta < 1.0))
+    return set_error(GALAX_ERR_INVALID, "theta must be in (0, 1) for the corrected variant");
+  return 0;
+}
+
+int reserve_bodies(galax_fmm* h, size_t n) {
+  hipStream_t s = h->stream;
+  DBuf* b[8] = {&h->x, &h->y, &h->m, &h->vx, &h->vy, &h->ax, &h->ay, &h->phi};
+  for (DBuf* d : b) GX_CHECK(dbuf_reserve(*d, std::max<size_t>(n, 1) * 8, s));
+  return 0;
+}
+
+int after_tree_installed(galax_fmm* h) {
+  h->has_tree = true;
+  h->keys_valid = false;
+  h->corrected_ready = false;
+  return 0;
+}
+
+int up(hipStream_t s, DBuf& d, const double* src, size_t n) {
+  if (!src || n == 0) return 0;
+  GX_HIP(hipMemcpyAsync(d.p, src, n * 8, hipMemcpyHostToDevice, s));
+  return 0;
+}
+int down(hipStream_t s, double* dst, const DBuf& d, size_t n) {
+  if (!dst || n == 0) return 0;
+  GX_HIP(hipMemcpyAsync(dst, d.p, n * 8, hipMemcpyDeviceToHost, s));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* galax_fmm_last_error(void) { return gx::last_error(); }
+
+const char* galax_fmm_version(void) { return "galax_fmm 0.1 (gfx950)"; }
+
+int galax_fmm_create(const galax_fmm_config_t* cfg, size_t max_n, galax_fmm_t** out) {
+  if (!out) return set_error(GALAX_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  GX_CHECK(check_cfg(cfg));
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(GALAX_ERR_INIT, "no HIP device");
+  galax_fmm* h = new (std::nothrow) galax_fmm();
+  if (!h) return set_error(GALAX_ERR_ALLOC, "out of host memory");
+  h->cfg = *cfg;
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    delete h;
+    return set_error(GALAX_ERR_INIT, "hipGetDevice failed");
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, h->device) == hipSuccess && std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    delete h;
+    return set_error(GALAX_ERR_COMPILE, "device is %s; this library carries gfx950 code only", prop.gcnArchName);
+  }
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return set_error(GALAX_ERR_INIT, "hipStreamCreate failed");
+  }
+  h->max_n = max_n;
+  int rc = reserve_bodies(h, max_n);
+  if (rc) {
+    galax_fmm_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return 0;
+}
+
+void galax_fmm_destroy(galax_fmm_t* h) {
+  if (!h) return;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  DBuf* bufs[] = {&h->x, &h->y, &h->m, &h->vx, &h->vy, &h->ax, &h->ay, &h->phi, &h->mp, &h->loc, &h->mp32,
+                  &h->loc32, &h->gtab, &h->key_of_pair, &h->perm, &h->hash, &h->tmp_u32, &h->flag,
+                  &h->c_m2l_off, &h->c_m2l_dat, &h->c_gat_off, &h->c_gat_dat, &h->c_src_pairs};
+  for (DBuf* b : bufs) dbuf_free(*b);
+  DevTree& t = h->tree;
+  DBuf* tb[] = {&t.center_x, &t.center_y, &t.half_width, &t.mass, &t.com_x, &t.com_y, &t.body_start, &t.body_end,
+                &t.child_first, &t.child_count, &t.parent, &t.level, &t.keys, &t.m2l_off, &t.m2l_dat, &t.p2p_off,
+                &t.p2p_dat, &t.gat_off, &t.gat_dat};
+  for (DBuf* b : tb) dbuf_free(*b);
+  Workspace& w = h->ws;
+  DBuf* wb[] = {&w.a, &w.b, &w.c, &w.d, &w.e, &w.f, &w.g, &w.h};
+  for (DBuf* b : wb) dbuf_free(*b);
+  for (DBuf& b : w.scan) dbuf_free(b);
+  for (hipEvent_t& e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+static int build_common(galax_fmm_t* h, size_t n) {
+  int rc = h->cfg.tree_mode == GALAX_TREE_CONSTRAINED ? build_tree_constrained(h, n) : build_tree_plain(h, n);
+  if (rc) return rc;
+  GX_CHECK(build_lists_reference(h));
+  return after_tree_installed(h);
+}
+
+int galax_fmm_build_tree(galax_fmm_t* h, galax_bodies_t* b, uint64_t* perm_out) {
+  if (!h || !b) return set_error(GALAX_ERR_INVALID, "NULL argument");
+  if (b->n > h->max_n) return set_error(GALAX_ERR_INVALID, "n = %zu exceeds max_n = %zu", b->n, h->max_n);
+  if (b->n > 0 && (!b->x || !b->y || !b->m)) return set_error(GALAX_ERR_INVALID, "x, y and m are required");
+  if (b->n >= 0xFFFFFFF0ull) return set_error(GALAX_ERR_INVALID, "n too large");
+  hipStream_t s = h->stream;
+  size_t n = b->n;
+  GX_HIP(hipSetDevice(h->device));
+  GX_CHECK(up(s, h->x, b->x, n));
+  GX_CHECK(up(s, h->y, b->y, n));
+  GX_CHECK(up(s, h->m, b->m, n));
+  // arrays the caller did not pass are not permuted (they stay zero on device)
+  DBuf* opt[4] = {&h->vx, &h->vy, &h->ax, &h->ay};
+  double* src[4] = {b->vx, b->vy, b->ax, b->ay};
+  for (int k = 0; k < 4; ++k) {
+    if (src[k])
+      GX_CHECK(up(s, *opt[k], src[k], n));
+    else if (n)
+      GX_HIP(hipMemsetAsync(opt[k]->p, 0, n * 8, s));
+  }
+  GX_CHECK(build_common(h, n));
+  GX_CHECK(down(s, b->x, h->x, n));
+  GX_CHECK(down(s, b->y, h->y, n));
+  GX_CHECK(down(s, b->m, h->m, n));
+  for (int k = 0; k < 4; ++k) GX_CHECK(down(s, src[k], *opt[k], n));
+  if (perm_out && n) {
+    std::vector<uint32_t> pv(n);
+    if (h->cfg.tree_mode == GALAX_TREE_CONSTRAINED) {
+      for (size_t i = 0; i < n; ++i) pv[i] = (uint32_t)i;
+    } else {
+      GX_HIP(hipMemcpyAsync(pv.data(), h->perm.p, n * 4, hipMemcpyDeviceToHost, s));
+    }
+    GX_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i < n; ++i) perm_out[i] = pv[i];
+  }
+  GX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int galax_fmm_build_tree_device(galax_fmm_t* h, double* d_x, double* d_y, double* d_vx, double* d_vy, double* d_m,
+                                double* d_ax, double* d_ay, size_t n) {
+  if (!h) return set_error(GALAX_ERR_INVALID, "NULL handle");
+  if (n > h->max_n) return set_error(GALAX_ERR_INVALID, "n = %zu exceeds max_n = %zu", n, h->max_n);
+  if (n > 0 && (!d_x || !d_y || !d_m)) return set_error(GALAX_ERR_INVALID, "x, y and m are required");
+  hipStream_t s = h->stream;
+  double* arr[7] = {d_x, d_y, d_vx, d_vy, d_m, d_ax, d_ay};
+  DBuf* own[7] = {&h->x, &h->y, &h->vx, &h->vy, &h->m, &h->ax, &h->ay};
+  for (int k = 0; k < 7; ++k) {
+    if (arr[k] && n)
+      GX_HIP(hipMemcpyAsync(own[k]->p, arr[k], n * 8, hipMemcpyDeviceToDevice, s));
+    else if (n)
+      GX_HIP(hipMemsetAsync(own[k]->p, 0, n * 8, s));
+  }
+  GX_CHECK(build_common(h, n));
+  for (int k = 0; k < 7; ++k)
+    if (arr[k] && n) GX_HIP(hipMemcpyAsync(arr[k], own[k]->p, n * 8, hipMemcpyDeviceToDevice, s));
+  GX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int galax_fmm_upload_tree(galax_fmm_t* h, const galax_tree_view_t* tree, const galax_lists_view_t* m2l,
+                          const galax_lists_view_t* p2p) {
+  if (!h || !tree || !m2l || !p2p) return set_error(GALAX_ERR_INVALID, "NULL argument");
+  GX_CHECK(install_tree_host(h, tree, m2l, p2p));
+  if (h->tree.n_bodies > h->max_n) return set_error(GALAX_ERR_INVALID, "tree holds more bodies than max_n");
+  return after_tree_installed(h);
+}
+
+int galax_fmm_set_softening(galax_fmm_t* h, double softening) {
+  if (!h) return set_error(GALAX_ERR_INVALID, "NULL handle");
+  if (!(softening >= 0.0) || !std::isfinite(softening))
+    return set_error(GALAX_ERR_INVALID, "softening must be finite and >= 0");
+  h->cfg.softening = softening;
+  h->keys_valid = false;  // derivative tables depend on eps
+  return 0;
+}
+
+int galax_fmm_tree_info(galax_fmm_t* h, size_t* n_nodes, size_t* n_leaves, uint32_t* n_levels, size_t* m2l_total,
+                        size_t* p2p_total) {
+  if (!h) return set_error(GALAX_ERR_INVALID, "NULL handle");
+  if (n_nodes) *n_nodes = h->tree.n_nodes;
+  if (n_leaves) *n_leaves = h->tree.n_leaves;
+  if (n_levels) *n_levels = h->tree.n_levels;
+  if (m2l_total) *m2l_total = h->tree.m2l_total;
+  if (p2p_total) *p2p_total = h->tree.p2p_total;
+  return 0;
+}
+
+int galax_fmm_export_tree(galax_fmm_t* h, galax_tree_view_t* tv, galax_lists_view_t* m2l, galax_lists_view_t* p2p) {
+  if (!h) return set_error(GALAX_ERR_INVALID, "NULL handle");
+  DevTree& t = h->tree;
+  hipStream_t s = h->stream;
+  size_t nn = t.n_nodes;
+  if (tv) {
+    if (tv->n_nodes != nn) return set_error(GALAX_ERR_INVALID, "tree view holds %zu nodes, tree has %zu", tv->n_nodes, nn);
+    std::vector<uint32_t> bs(nn), be(nn), cf(nn), par(nn);
+    std::vector<uint8_t> cc(nn);
+    GX_HIP(hipMemcpyAsync(tv->center_x, t.center_x.p, nn * 8, hipMemcpyDeviceToHost, s));
+    GX_HIP(hipMemcpyAsync(tv->center_y, t.center_y.p, nn * 8, hipMemcpyDeviceToHost, s));
+    GX_HIP(hipMemcpyAsync(tv->half_width, t.half_width.p, nn * 8, hipMemcpyDeviceToHost, s));
+    GX_HIP(hipMemcpyAsync(tv->mass, t.mass.p, nn * 8, hipMemcpyDeviceToHost, s));
+    GX_HIP(hipMemcpyAsync(tv->com_x, t.com_x.p, nn * 8, hipMemcpyDeviceToHost, s));
+    GX_HIP(hipMemcpyAsync(tv->com_y, t.com_y.p, nn * 8, hipMemcpyDeviceToHost, s));
+    GX_HIP(hipMemcpyAsync(bs.data(), t.body_start.p, nn * 4, hipMemcpyDeviceToHost, s));
+    GX_HIP(hipMemcpyAsync(be.data(), t.body_end.p, nn * 4, hipMemcpyDeviceToHost, s));
+    GX_HIP(hipMemcpyAsync(cf.data(), t.child_first.p, nn * 4, hipMemcpyDeviceToHost, s));
+    GX_HIP(hipMemcpyAsync(par.data(), t.parent.p, nn * 4, hipMemcpyDeviceToHost, s));
+    GX_HIP(hipMemcpyAsync(cc.data(), t.child_count.p, nn, hipMemcpyDeviceToHost, s));
+    GX_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i < nn; ++i) {
+      tv->body_start[i] = bs[i];
+      tv->body_end[i] = be[i];
+      for (int k = 0; k < 4; ++k) tv->children[i * 4 + k] = (k < cc[i]) ? (int32_t)(cf[i] + k) : -1;
+      tv->parent[i] = par[i] == NONE ? -1 : (int32_t)par[i];
+      tv->is_leaf[i] = cc[i] == 0 ? 1 : 0;
+    }
+  }
+  auto ex = [&](galax_lists_view_t* lv, DBuf& off, DBuf& dat, size_t total) -> int {
+    if (!lv) return 0;
+    if (lv->n_nodes != nn) return set_error(GALAX_ERR_INVALID, "list view size mismatch");
+    std::vector<uint32_t> o(nn + 1);
+    GX_HIP(hipMemcpyAsync(o.data(), off.p, (nn + 1) * 4, hipMemcpyDeviceToHost, s));
+    if (total) GX_HIP(hipMemcpyAsync(lv->data, dat.p, total * 4, hipMemcpyDeviceToHost, s));
+    GX_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i <= nn; ++i) lv->offsets[i] = o[i];
+    return 0;
+  };
+  if (nn) {
+    GX_CHECK(ex(m2l, t.m2l_off, t.m2l_dat, t.m2l_total));
+    GX_CHECK(ex(p2p, t.p2p_off, t.p2p_dat, t.p2p_total));
+  }
+  return 0;
+}
+
+static int prepare_eval(galax_fmm_t* h, const double* d_x, const double* d_y) {
+  if (!h->has_tree) return set_error(GALAX_ERR_INVALID, "no tree: call galax_fmm_build_tree or galax_fmm_upload_tree");
+  if (h->cfg.algo == GALAX_ALGO_CORRECTED && !h->corrected_ready) GX_CHECK(build_lists_corrected(h, d_x, d_y));
+  return 0;
+}
+
+int galax_fmm_evaluate(galax_fmm_t* h, galax_bodies_t* b) {
+  if (!h || !b) return set_error(GALAX_ERR_INVALID, "NULL argument");
+  size_t n = h->tree.n_bodies;
+  if (b->n != n) return set_error(GALAX_ERR_INVALID, "bodies.n = %zu but the tree holds %zu bodies", b->n, n);
+  if (n && (!b->x || !b->y || !b->m || !b->ax || !b->ay))
+    return set_error(GALAX_ERR_INVALID, "x, y, m, ax, ay are required");
+  hipStream_t s = h->stream;
+  GX_CHECK(up(s, h->x, b->x, n));
+  GX_CHECK(up(s, h->y, b->y, n));
+  GX_CHECK(up(s, h->m, b->m, n));
+  GX_CHECK(prepare_eval(h, h->x.as<double>(), h->y.as<double>()));
+  GX_CHECK(evaluate(h, h->x.as<double>(), h->y.as<double>(), h->m.as<double>(), h->ax.as<double>(),
+                    h->ay.as<double>(), b->phi ? h->phi.as<double>() : nullptr));
+  uint32_t changed = 0;
+  GX_HIP(hipMemcpyAsync(&changed, h->flag.as<uint32_t>() + 4, 4, hipMemcpyDeviceToHost, s));
+  GX_CHECK(down(s, b->ax, h->ax, n));
+  GX_CHECK(down(s, b->ay, h->ay, n));
+  if (b->phi) GX_CHECK(down(s, b->phi, h->phi, n));
+  GX_HIP(hipStreamSynchronize(s));
+  if (changed) {  // sanitize mutates the caller's bodies, as the reference does
+    GX_CHECK(down(s, b->x, h->x, n));
+    GX_CHECK(down(s, b->y, h->y, n));
+    GX_CHECK(down(s, b->m, h->m, n));
+    GX_HIP(hipStreamSynchronize(s));
+  }
+  return 0;
+}
+
+int galax_fmm_evaluate_device(galax_fmm_t* h, double* d_x, double* d_y, double* d_m, double* d_ax, double* d_ay,
+                              double* d_phi) {
+  if (!h) return set_error(GALAX_ERR_INVALID, "NULL handle");
+  if (h->tree.n_bodies && (!d_x || !d_y || !d_m || !d_ax || !d_ay))
+    return set_error(GALAX_ERR_INVALID, "x, y, m, ax, ay are required");
+  GX_CHECK(prepare_eval(h, d_x, d_y));
+  return evaluate(h, d_x, d_y, d_m, d_ax, d_ay, d_phi);
+}
+
+int galax_fmm_last_timings(galax_fmm_t* h, galax_timings_t* out) {
+  if (!h || !out) return set_error(GALAX_ERR_INVALID, "NULL argument");
+  *out = h->timings;
+  return 0;
+}
+
+int galax_fmm_kdk(galax_fmm_t* h, galax_bodies_t* b, double dt, uint64_t n_steps, uint64_t every,
+                  galax_diag_t* diag) {
+  if (!h || !b) return set_error(GALAX_ERR_INVALID, "NULL argument");
+  size_t n = h->tree.n_bodies;
+  if (b->n != n) return set_error(GALAX_ERR_INVALID, "bodies.n = %zu but the tree holds %zu bodies", b->n, n);
+  if (n && (!b->x || !b->y || !b->vx || !b->vy || !b->m || !b->ax || !b->ay))
+    return set_error(GALAX_ERR_INVALID, "x, y, vx, vy, m, ax, ay are required");
+  hipStream_t s = h->stream;
+  GX_CHECK(up(s, h->x, b->x, n));
+  GX_CHECK(up(s, h->y, b->y, n));
+  GX_CHECK(up(s, h->vx, b->vx, n));
+  GX_CHECK(up(s, h->vy, b->vy, n));
+  GX_CHECK(up(s, h->m, b->m, n));
+  GX_CHECK(up(s, h->ax, b->ax, n));
+  GX_CHECK(up(s, h->ay, b->ay, n));
+  GX_CHECK(prepare_eval(h, h->x.as<double>(), h->y.as<double>()));
+  GX_CHECK(kdk_device(h, dt, n_steps, every, diag));
+  double* outs[7] = {b->x, b->y, b->vx, b->vy, b->m, b->ax, b->ay};
+  DBuf* own[7] = {&h->x, &h->y, &h->vx, &h->vy, &h->m, &h->ax, &h->ay};
+  for (int k = 0; k < 7; ++k) GX_CHECK(down(s, outs[k], *own[k], n));
+  if (b->phi) GX_CHECK(down(s, b->phi, h->phi, n));
+  GX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int galax_fmm_vjp(galax_fmm_t* h, const galax_bodies_t* b, const double* axb, const double* ayb, double* xb,
+                  double* yb, double* mb) {
+  (void)b;
+  (void)axb;
+  (void)ayb;
+  (void)xb;
+  (void)yb;
+  (void)mb;
+  if (!h) return set_error(GALAX_ERR_INVALID, "NULL handle");
+  return set_error(GALAX_ERR_UNSUPPORTED, "vjp is not provided by this build yet");
+}
+
+int galax_direct_sum(const double* x, const double* y, const double* m, size_t n, double softening,
+                     const uint64_t* targets, size_t n_targets, double* ax, double* ay, double* phi) {
+  if (n && (!x || !y || !m)) return set_error(GALAX_ERR_INVALID, "x, y, m are required");
+  size_t nt = targets ? n_targets : n;
+  if (nt && (!ax || !ay)) return set_error(GALAX_ERR_INVALID, "ax, ay are required");
+  if (targets)
+    for (size_t k = 0; k < nt; ++k)
+      if (targets[k] >= n) return set_error(GALAX_ERR_INVALID, "target index %llu >= n", (unsigned long long)targets[k]);
+  hipStream_t s;
+  GX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  double *dx = nullptr, *dy = nullptr, *dm = nullptr, *dax = nullptr, *day = nullptr, *dphi = nullptr;
+  uint64_t* dt = nullptr;
+  int rc = 0;
+  auto fail = [&](hipError_t e) {
+    if (e != hipSuccess && rc == 0) rc = set_error(GALAX_ERR_DEVICE, "direct sum: %s", hipGetErrorString(e));
+  };
+  size_t nb = std::max<size_t>(n, 1) * 8, nbt = std::max<size_t>(nt, 1) * 8;
+  fail(hipMalloc(&dx, nb));
+  fail(hipMalloc(&dy, nb));
+  fail(hipMalloc(&dm, nb));
+  fail(hipMalloc(&dax, nbt));
+  fail(hipMalloc(&day, nbt));
+  fail(hipMalloc(&dphi, nbt));
+  if (targets) fail(hipMalloc(&dt, nbt));
+  if (rc == 0 && n) {
+    fail(hipMemcpyAsync(dx, x, n * 8, hipMemcpyHostToDevice, s));
+    fail(hipMemcpyAsync(dy, y, n * 8, hipMemcpyHostToDevice, s));
+    fail(hipMemcpyAsync(dm, m, n * 8, hipMemcpyHostToDevice, s));
+    if (targets) fail(hipMemcpyAsync(dt, targets, nt * 8, hipMemcpyHostToDevice, s));
+  }
+  if (rc == 0) rc = direct_sum_device(s, dx, dy, dm, n, softening, dt, nt, dax, day, dphi);
+  if (rc == 0 && nt) {
+    fail(hipMemcpyAsync(ax, dax, nt * 8, hipMemcpyDeviceToHost, s));
+    fail(hipMemcpyAsync(ay, day, nt * 8, hipMemcpyDeviceToHost, s));
+    if (phi) fail(hipMemcpyAsync(phi, dphi, nt * 8, hipMemcpyDeviceToHost, s));
+  }
+  fail(hipStreamSynchronize(s));
+  for (void* p : {(void*)dx, (void*)dy, (void*)dm, (void*)dax, (void*)day, (void*)dphi, (void*)dt})
+    if (p) (void)hipFree(p);
+  (void)hipStreamDestroy(s);
+  return rc;
+}
+
+int galax_fmm_validate(galax_fmm_t* h, galax_bodies_t* b, const uint64_t* samples, size_t ns, double* max_rel,
+                       double* mean_rel, double* rms_vec) {
+  if (!h || !b || (!samples && ns)) return set_error(GALAX_ERR_INVALID, "NULL argument");
+  size_t n = b->n;
+  std::vector<double> ax(n), ay(n), ex(ns), ey(ns);
+  galax_bodies_t c = *b;
+  std::vector<double> x(b->x, b->x + n), y(b->y, b->y + n), m(b->m, b->m + n);
+  c.x = x.data();
+  c.y = y.data();
+  c.m = m.data();
+  c.ax = ax.data();
+  c.ay = ay.data();
+  c.phi = nullptr;
+  GX_CHECK(galax_fmm_evaluate(h, &c));
+  GX_CHECK(galax_direct_sum(x.data(), y.data(), m.data(), n, h->cfg.softening, samples, ns, ex.data(), ey.data(),
+                            nullptr));
+  double mx = 0.0, sum = 0.0, sq = 0.0;
+  for (size_t k = 0; k < ns; ++k) {
+    size_t i = samples[k];
+    double fa = std::sqrt(ax[i] * ax[i] + ay[i] * ay[i]);
+    double ea = std::sqrt(ex[k] * ex[k] + ey[k] * ey[k]);
+    double den = std::max(std::fabs(ea), 1e-20);
+    double r = std::fabs(fa - ea) / den;  // galax-core/src/lib.rs:1554-1557
+    mx = std::max(mx, r);
+    sum += r;
+    double vx = ax[i] - ex[k], vy = ay[i] - ey[k];
+    sq += (vx * vx + vy * vy) / (den * den);
+  }
+  if (max_rel) *max_rel = mx;
+  if (mean_rel) *mean_rel = ns ? sum / (double)ns : 0.0;
+  if (rms_vec) *rms_vec = ns ? std::sqrt(sq / (double)ns) : 0.0;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,140 @@
+// common.h — internal definitions shared by the HIP translation units of
+// libgalax_fmm.so (context, error plumbing, device helpers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/galax_fmm.h"
+
+namespace gx {
+
+constexpr int MAX_P = 16;          // plans exist for p <= 16 (galax-core/src/lib.rs:1163-1164)
+constexpr uint32_t LIST_CAP = 64;  // InteractionLists data: [u32; 64] (galax-core/src/lib.rs:942)
+constexpr uint32_t MORTON_BITS = 21;
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+
+inline __host__ __device__ constexpr int num_moments(int p) { return (p + 1) * (p + 2) / 2; }
+inline __host__ __device__ constexpr int moment_index(int i, int j) { return (i + j) * (i + j + 1) / 2 + j; }
+// Derivative table index of G^(k,l) in a (2p+1)^2 buffer (galax-core/src/lib.rs:1033-1036).
+inline __host__ __device__ constexpr int deriv_index(int k, int l, int p) { return (k + l) * (2 * p + 1) + l; }
+
+// Thread-local error text; set_error returns the code for `return set_error(...)`.
+int set_error(int code, const char* fmt, ...);
+const char* last_error();
+
+#define GX_HIP(call)                                                                              \
+  do {                                                                                            \
+    hipError_t e_ = (call);                                                                       \
+    if (e_ != hipSuccess)                                                                         \
+      return ::gx::set_error(GALAX_ERR_DEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                             __FILE__, __LINE__);                                                 \
+  } while (0)
+
+#define GX_CHECK(expr)        \
+  do {                        \
+    int rc_ = (expr);         \
+    if (rc_ != 0) return rc_; \
+  } while (0)
+
+// Grow-only device buffer owned by a handle.
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  template <class T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+};
+int dbuf_reserve(DBuf& b, size_t bytes, hipStream_t s);
+void dbuf_free(DBuf& b);
+
+// Device-side tree in structure-of-arrays form. Node ids are level-ordered
+// (leaves first, root last) and each node's children are consecutive ids
+// [child_first, child_first + child_count), as Tree::build creates them.
+struct DevTree {
+  size_t n_nodes = 0, n_leaves = 0, n_bodies = 0;
+  uint32_t n_levels = 0;                // levels 0 (leaves) .. n_levels-1 (root)
+  std::vector<uint32_t> level_offset;   // host copy, n_levels + 1 entries
+  DBuf center_x, center_y, half_width, mass, com_x, com_y;  // double
+  DBuf body_start, body_end;            // uint32
+  DBuf child_first;                     // uint32 (NONE for leaves)
+  DBuf child_count;                     // uint8
+  DBuf parent;                          // uint32 (NONE for the root)
+  DBuf level;                           // uint8
+  DBuf keys;                            // uint64, Morton key per sorted body
+  // Lists as CSR (uint32 offsets): the reference's M2L and P2P lists for every
+  // node, and per leaf the symmetric P2P gather list (self first).
+  DBuf m2l_off, m2l_dat, p2p_off, p2p_dat, gat_off, gat_dat;
+  size_t m2l_total = 0, p2p_total = 0, gat_total = 0;
+  uint64_t p2p_interactions = 0;  // directed body-body pairs of the gather lists
+  size_t capacity = 0;  // node capacity of the node arrays
+};
+
+struct Workspace {
+  DBuf a, b, c, d, e, f, g, h;  // scratch reused across build stages
+  DBuf scan[6];                 // partial sums of the multi-level scans
+};
+
+}  // namespace gx
+
+struct galax_fmm {
+  galax_fmm_config_t cfg{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t max_n = 0;
+  // bodies in tree order, owned (used by host-array entry points)
+  gx::DBuf x, y, m, vx, vy, ax, ay, phi;
+  gx::DevTree tree;
+  bool has_tree = false;
+  // expansions and M2L key tables
+  gx::DBuf mp, loc, mp32, loc32, gtab, key_of_pair, perm, hash, tmp_u32, flag;
+  size_t n_keys = 0;
+  size_t n_live_targets = 0, n_live_pairs = 0;
+  bool keys_valid = false;
+  gx::Workspace ws;
+  // corrected-variant lists (built on the host from the device tree)
+  gx::DBuf c_m2l_off, c_m2l_dat, c_gat_off, c_gat_dat, c_src_pairs;
+  size_t c_m2l_total = 0, c_gat_total = 0;
+  uint64_t c_p2p_interactions = 0;
+  bool corrected_ready = false;
+  galax_timings_t timings{};
+  hipEvent_t ev[12]{};
+};
+
+namespace gx {
+
+// ---------------------------------------------------------------- scans ----
+// Exclusive (or inclusive) prefix sum / max over n uint32 values in d_in,
+// written to d_out (may alias). Uses ws scratch. Returns the total in *total
+// when non-null (sum only; synchronises the stream).
+int scan_u32(galax_fmm* h, const uint32_t* d_in, uint32_t* d_out, size_t n, bool inclusive, bool max_op,
+             uint32_t* total);
+
+// ---------------------------------------------------------- tree build ----
+int build_tree_plain(galax_fmm* h, size_t n);             // bodies already in h->x/y/... unsorted
+int build_tree_constrained(galax_fmm* h, size_t n);       // bodies in h->x/y/m, current order
+int build_lists_reference(galax_fmm* h);                  // M2L + P2P lists + gather CSR
+int install_tree_host(galax_fmm* h, const galax_tree_view_t* t, const galax_lists_view_t* m2l,
+                      const galax_lists_view_t* p2p);
+int build_lists_corrected(galax_fmm* h, const double* d_x, const double* d_y);
+int ensure_node_capacity(galax_fmm* h, size_t nodes);
+int count_interactions(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, uint64_t* out);
+
+// ---------------------------------------------------------- evaluation ----
+int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, double* d_ay, double* d_phi);
+int vjp(galax_fmm* h, const double* d_x, const double* d_y, const double* d_m, const double* d_axb,
+        const double* d_ayb, double* d_xb, double* d_yb, double* d_mb);
+int direct_sum_device(hipStream_t s, const double* x, const double* y, const double* m, size_t n, double eps,
+                      const uint64_t* targets, size_t nt, double* ax, double* ay, double* phi);
+int kdk_device(galax_fmm* h, double dt, uint64_t n_steps, uint64_t every, galax_diag_t* diag);
+
+// Host restatement of the kernel-derivative numerator polynomials
+// N_{k,l}[a][b] (galax-core/src/lib.rs:1059-1130); they depend on eps and p only.
+std::vector<double> kernel_polys(double eps, int p);
+
+}  // namespace gx

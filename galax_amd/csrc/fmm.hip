@@ -1,0 +1,866 @@
+// fmm.hip — the FMM passes of compute_fmm_force (galax-core/src/lib.rs:1466-1499)
+// on the device, FP64:
+//   sanitize  :1475-1482     P2M  :848-887       M2M :890-938
+//   M2L       :1199-1259 (derivatives :1040-1151, plan :1153-1196)
+//   L2L       :1262-1302     L2P  :1308-1355     P2P :1395-1462
+// Functions marked "exact" run without FMA contraction and in the reference's
+// summation order, so their results match galax-core bit for bit given the
+// same inputs; P2P sums in a gather order and uses FMA.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "common.h"
+
+namespace gx {
+
+// ----------------------------------------------------------- tables ----
+
+struct Tabs {
+  double fact[MAX_P + 1];
+  double binom[MAX_P + 1][MAX_P + 1];
+};
+
+__constant__ Tabs c_tabs;
+
+static Tabs host_tabs() {
+  Tabs t;
+  std::memset(&t, 0, sizeof(t));
+  // galax-core/src/lib.rs:15-39
+  for (int n = 0; n <= MAX_P; ++n) {
+    t.binom[n][0] = 1.0;
+    t.binom[n][n] = 1.0;
+    for (int k = 1; k < n; ++k) t.binom[n][k] = t.binom[n - 1][k - 1] + t.binom[n - 1][k];
+  }
+  t.fact[0] = 1.0;
+  for (int i = 1; i <= MAX_P; ++i) t.fact[i] = t.fact[i - 1] * (double)i;
+  return t;
+}
+
+static int upload_tabs(hipStream_t s) {
+  static thread_local int done_dev = -1;
+  int dev = 0;
+  GX_HIP(hipGetDevice(&dev));
+  if (done_dev == dev) return 0;
+  Tabs t = host_tabs();
+  GX_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_tabs), &t, sizeof(t), 0, hipMemcpyHostToDevice, s));
+  GX_HIP(hipStreamSynchronize(s));
+  done_dev = dev;
+  return 0;
+}
+
+template <int P>
+__device__ __forceinline__ void coef_ij(int k, int& i, int& j) {
+  int n = 0;
+#pragma unroll
+  for (int q = 0; q <= P; ++q)
+    if (k >= (q + 1) * (q + 2) / 2) n = q + 1;
+  j = k - n * (n + 1) / 2;
+  i = n - j;
+}
+
+static inline unsigned nblk(size_t n, unsigned bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+// ---------------------------------------------------------- sanitize ----
+
+// galax-core/src/lib.rs:1475-1482 (mutates its input)
+__global__ void k_sanitize(double* __restrict__ x, double* __restrict__ y, double* __restrict__ m, size_t n,
+                           uint32_t* __restrict__ changed) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double xi = x[i], yi = y[i], mi = m[i];
+  bool c = false;
+  if (fabs(xi) > 1e100 || fabs(yi) > 1e100 || !isfinite(xi) || !isfinite(yi)) {
+    x[i] = 0.0;
+    y[i] = 0.0;
+    c = true;
+  }
+  if (!isfinite(mi) || mi <= 0.0) {
+    m[i] = 1.0;
+    c = true;
+  }
+  if (c) atomicAdd(changed, 1u);
+}
+
+// --------------------------------------------------------------- P2M ----
+
+// exact: one wave per leaf, one lane per coefficient, bodies in order
+// (galax-core/src/lib.rs:854-885; coeff = mass*dx^i*dy^j/(i!*j!), non-finite -> 0).
+template <int P>
+__global__ void __launch_bounds__(256) k_p2m(const double* __restrict__ x, const double* __restrict__ y,
+                                             const double* __restrict__ m, const uint32_t* __restrict__ bs,
+                                             const uint32_t* __restrict__ be, const double* __restrict__ comx,
+                                             const double* __restrict__ comy, uint32_t n_leaves,
+                                             double* __restrict__ mp) {
+#pragma clang fp contract(off)
+  constexpr int S = num_moments(P);
+  const uint32_t leaf = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (leaf >= n_leaves) return;
+  const uint32_t s = bs[leaf], e = be[leaf];
+  const double cx = comx[leaf], cy = comy[leaf];
+  for (int k = lane; k < S; k += 64) {
+    int i, j;
+    coef_ij<P>(k, i, j);
+    const double div = c_tabs.fact[i] * c_tabs.fact[j];
+    double acc = 0.0;
+    for (uint32_t b = s; b < e; ++b) {
+      double dx = x[b] - cx, dy = y[b] - cy, mb = m[b];
+      double px = 1.0, py = 1.0;
+      for (int q = 0; q < i; ++q) px = px * dx;
+      for (int q = 0; q < j; ++q) py = py * dy;
+      acc += mb * px * py / div;
+    }
+    mp[(size_t)leaf * S + k] = isfinite(acc) ? acc : 0.0;
+  }
+}
+
+// --------------------------------------------------------------- M2M ----
+
+// exact: parents in [lo, hi), one thread per (parent, coefficient); children in
+// order, each child's shifted sum added to the parent (galax-core/src/lib.rs:894-932).
+template <int P>
+__global__ void k_m2m(const uint32_t* __restrict__ cf, const uint8_t* __restrict__ cc, const double* __restrict__ comx,
+                      const double* __restrict__ comy, uint32_t lo, uint32_t hi, double* __restrict__ mp) {
+#pragma clang fp contract(off)
+  constexpr int S = num_moments(P);
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)(hi - lo) * S) return;
+  uint32_t p = lo + (uint32_t)(t / S);
+  int k = (int)(t % S);
+  int oi, oj;
+  coef_ij<P>(k, oi, oj);
+  double acc = 0.0;
+  uint32_t c0 = cf[p];
+  for (uint32_t c = c0; c < c0 + cc[p]; ++c) {
+    double dx = comx[c] - comx[p], dy = comy[c] - comy[p];
+    double dxp[P + 1], dyp[P + 1];
+    dxp[0] = 1.0;
+    dyp[0] = 1.0;
+#pragma unroll
+    for (int q = 1; q <= P; ++q) {
+      dxp[q] = dxp[q - 1] * dx;
+      dyp[q] = dyp[q - 1] * dy;
+    }
+    const double* mc = mp + (size_t)c * S;
+    double sum = 0.0;
+    for (int i = 0; i <= oi; ++i) {
+      for (int j = 0; j <= oj; ++j) {
+        if (i + j > P) continue;
+        sum += mc[moment_index(i, j)] * dxp[oi - i] * dyp[oj - j] / (c_tabs.fact[oi - i] * c_tabs.fact[oj - j]);
+      }
+    }
+    acc += sum;
+  }
+  mp[(size_t)p * S + k] = acc;
+}
+
+// ----------------------------------------------- kernel derivatives ----
+
+// exact: G^(k,l) for one (dx, dy) by evaluating the numerator polynomials
+// N_{k,l} (computed on the host, kernel_polys) over R^{1+2(k+l)}, each term
+// c * x^a * y^b with powi by binary exponentiation as LLVM lowers Rust's
+// f64::powi (galax-core/src/lib.rs:1132-1148). One lane per (k, l).
+__device__ __forceinline__ double powi_dev(double a, int b) {
+#pragma clang fp contract(off)
+  double r = 1.0;
+  for (;;) {
+    if (b & 1) r *= a;
+    b /= 2;
+    if (b == 0) break;
+    a *= a;
+  }
+  return r;
+}
+
+// polys layout: [(k*stride + l) * stride*stride + a*stride + b], stride = 2p+1
+__global__ void k_derivs_poly(const double* __restrict__ polys, int p, const double* __restrict__ dxy, uint32_t nk,
+                              double eps, double* __restrict__ gtab) {
+#pragma clang fp contract(off)
+  const int stride = 2 * p + 1, GS = stride * stride;
+  uint32_t key = blockIdx.x;
+  if (key >= nk) return;
+  const double x = dxy[2 * key], y = dxy[2 * key + 1];
+  const double r2 = x * x + y * y + eps * eps;
+  const double r = sqrt(r2);
+  for (int t = threadIdx.x; t < GS; t += blockDim.x) {
+    int k = t / stride, l = t % stride;
+    double out = 0.0;
+    if (k + l <= 2 * p) {
+      const double* c = polys + (size_t)(k * stride + l) * GS;
+      double val = 0.0;
+      for (int a = 0; a < stride; ++a)
+        for (int b = 0; b < stride - a; ++b) {
+          double cc = c[a * stride + b];
+          if (cc != 0.0) val += cc * powi_dev(x, a) * powi_dev(y, b);
+        }
+      // inv_r[1] = 1/r; inv_r[n] = inv_r[n-2] / r2 (galax-core/src/lib.rs:1053-1057)
+      double ir = 1.0 / r;
+      for (int q = 0; q < k + l; ++q) ir = ir / r2;
+      out = val * ir;
+      gtab[(size_t)key * GS + deriv_index(k, l, p)] = out;
+    }
+  }
+}
+
+// ----------------------------------------------- M2L key cache (setup) ----
+// The reference caches derivative tables in a map keyed by
+// (dx as i64, dy as i64) of the COM offset target - source; with one thread the
+// first pair in (target id, list position) order fills each entry and every
+// later pair with that key reuses it (galax-core/src/lib.rs:1209-1247). The
+// tree is static, so the keys and tables are computed once per tree.
+
+__device__ __forceinline__ int64_t rust_f64_as_i64(double v) {
+  if (isnan(v)) return 0;
+  if (v >= 9223372036854775808.0) return INT64_MAX;
+  if (v <= -9223372036854775808.0) return INT64_MIN;
+  return (int64_t)v;
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+constexpr uint64_t EMPTY_KEY = ~0ull;
+
+__device__ __forceinline__ bool pack_key(double dx, double dy, uint64_t* out) {
+  int64_t kx = rust_f64_as_i64(dx), ky = rust_f64_as_i64(dy);
+  const int64_t lim = 2147483647ll;
+  if (kx <= -lim || kx >= lim || ky <= -lim || ky >= lim) return false;
+  *out = ((uint64_t)(uint32_t)(kx + 2147483648ll) << 32) | (uint64_t)(uint32_t)(ky + 2147483648ll);
+  return true;
+}
+
+// one thread per target: insert each pair's key, keeping the smallest pair index
+__global__ void k_keys_insert(const uint32_t* __restrict__ off, const uint32_t* __restrict__ dat, uint32_t n_targets,
+                              const double* __restrict__ comx, const double* __restrict__ comy,
+                              unsigned long long* __restrict__ hkeys, uint32_t* __restrict__ hrep, uint64_t mask,
+                              uint32_t* __restrict__ tgt_of_pair, uint32_t* __restrict__ err) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_targets) return;
+  for (uint32_t e = off[t]; e < off[t + 1]; ++e) {
+    uint32_t s = dat[e];
+    tgt_of_pair[e] = t;
+    uint64_t key;
+    if (!pack_key(comx[t] - comx[s], comy[t] - comy[s], &key)) {
+      atomicOr(err, 1u);
+      continue;
+    }
+    uint64_t slot = mix64(key) & mask;
+    for (;;) {
+      unsigned long long prev = atomicCAS(&hkeys[slot], (unsigned long long)EMPTY_KEY, (unsigned long long)key);
+      if (prev == EMPTY_KEY || prev == key) {
+        atomicMin(&hrep[slot], e);
+        break;
+      }
+      slot = (slot + 1) & mask;
+    }
+  }
+}
+
+__global__ void k_keys_flag(const unsigned long long* __restrict__ hkeys, uint64_t nslots, uint32_t* __restrict__ f) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nslots) f[i] = hkeys[i] != EMPTY_KEY ? 1u : 0u;
+}
+
+// compact occupied slots; representative offsets (dx, dy) of each key
+__global__ void k_keys_compact(const unsigned long long* __restrict__ hkeys, const uint32_t* __restrict__ hrep,
+                               uint64_t nslots, const uint32_t* __restrict__ kid, uint32_t* __restrict__ slot_kid,
+                               const uint32_t* __restrict__ dat, const uint32_t* __restrict__ tgt_of_pair,
+                               const double* __restrict__ comx, const double* __restrict__ comy,
+                               double* __restrict__ dxy) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots) return;
+  if (hkeys[i] == EMPTY_KEY) return;
+  uint32_t id = kid[i];
+  slot_kid[i] = id;
+  uint32_t e = hrep[i];
+  uint32_t t = tgt_of_pair[e], s = dat[e];
+  dxy[2 * id] = comx[t] - comx[s];
+  dxy[2 * id + 1] = comy[t] - comy[s];
+}
+
+__global__ void k_keys_lookup(const uint32_t* __restrict__ dat, const uint32_t* __restrict__ tgt_of_pair,
+                              uint32_t n_pairs, const double* __restrict__ comx, const double* __restrict__ comy,
+                              const unsigned long long* __restrict__ hkeys, const uint32_t* __restrict__ slot_kid,
+                              uint64_t mask, uint32_t* __restrict__ key_of_pair) {
+  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_pairs) return;
+  uint32_t t = tgt_of_pair[e], s = dat[e];
+  uint64_t key;
+  if (!pack_key(comx[t] - comx[s], comy[t] - comy[s], &key)) return;
+  uint64_t slot = mix64(key) & mask;
+  while (hkeys[slot] != key) slot = (slot + 1) & mask;
+  key_of_pair[e] = slot_kid[slot];
+}
+
+// --------------------------------------------------------------- M2L ----
+
+// exact: one wave per target, one lane per local coefficient (I, J); pairs in
+// list order, source terms in plan order (i outer, j inner), zero multipole
+// terms skipped, non-finite multipoles read as 0 (the post-M2M scrub,
+// galax-core/src/lib.rs:935-937).
+template <int P>
+__global__ void __launch_bounds__(256) k_m2l_ref(const uint32_t* __restrict__ off, const uint32_t* __restrict__ dat,
+                                                 const uint32_t* __restrict__ key_of_pair,
+                                                 const double* __restrict__ gtab, uint32_t n_targets,
+                                                 const double* __restrict__ mp, double* __restrict__ loc) {
+#pragma clang fp contract(off)
+  constexpr int S = num_moments(P);
+  constexpr int ST = 2 * P + 1, GS = ST * ST;
+  const uint32_t t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (t >= n_targets) return;
+  for (int k = lane; k < S; k += 64) {
+    int I, J;
+    coef_ij<P>(k, I, J);
+    const double lf = 1.0 / (c_tabs.fact[I] * c_tabs.fact[J]);
+    double acc = 0.0;
+    for (uint32_t e = off[t]; e < off[t + 1]; ++e) {
+      const uint32_t s = dat[e];
+      const double* ms = mp + (size_t)s * S;
+      const double* G = gtab + (size_t)key_of_pair[e] * GS;
+#pragma unroll
+      for (int i = 0; i <= P; ++i) {
+#pragma unroll
+        for (int j = 0; j <= P - i; ++j) {
+          double mv = ms[moment_index(i, j)];
+          if (!isfinite(mv)) mv = 0.0;
+          if (mv == 0.0) continue;
+          const double w = ((i + j) % 2 == 0) ? lf : -lf;
+          acc += mv * w * G[(i + I + j + J) * ST + (j + J)];
+        }
+      }
+    }
+    loc[(size_t)t * S + k] = acc;
+  }
+}
+
+// Exact per-pair derivatives of G = -1/sqrt(x^2+y^2+eps^2) by the recurrence
+// obtained from R^2 dphi/dx = -x phi (phi = 1/R) with Leibniz' rule:
+//   R^2 D(k,l) = -(2k-1) x D(k-1,l) - (k-1)^2 D(k-2,l) - 2l y D(k,l-1) - l(l-1) D(k,l-2)   (k >= 1)
+//   R^2 D(0,l) = -(2l-1) y D(0,l-1) - (l-1)^2 D(0,l-2)
+// with D = d^k/dx^k d^l/dy^l phi and G^(k,l) = -D(k,l). Order n needs orders
+// n-1 and n-2 only, so a wave computes one order per step.
+template <int P>
+__device__ __forceinline__ void derivs_recurrence(double x, double y, double eps2, double* G /* LDS, ST*ST */,
+                                                  int lane) {
+  constexpr int ST = 2 * P + 1;
+  const double r2 = x * x + y * y + eps2;
+  const double ir2 = 1.0 / r2;
+  if (lane == 0) G[0] = 1.0 / sqrt(r2);  // D(0,0)
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  for (int n = 1; n <= 2 * P; ++n) {
+    for (int k = lane; k <= n; k += 64) {
+      int l = n - k;
+      double v;
+      if (k >= 1) {
+        v = -(2.0 * k - 1.0) * x * G[(n - 1) * ST + l];
+        if (k >= 2) v -= (double)(k - 1) * (k - 1) * G[(n - 2) * ST + l];
+        if (l >= 1) v -= 2.0 * l * y * G[(n - 1) * ST + (l - 1)];
+        if (l >= 2) v -= (double)l * (l - 1) * G[(n - 2) * ST + (l - 2)];
+      } else {
+        v = -(2.0 * l - 1.0) * y * G[(n - 1) * ST + (l - 1)];
+        if (l >= 2) v -= (double)(l - 1) * (l - 1) * G[(n - 2) * ST + (l - 2)];
+      }
+      G[n * ST + l] = v * ir2;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+}
+
+// CORRECTED variant: exact derivatives per pair (D stored; G = -D).
+template <int P>
+__global__ void __launch_bounds__(256) k_m2l_exact(const uint32_t* __restrict__ off, const uint32_t* __restrict__ dat,
+                                                   uint32_t n_targets, const double* __restrict__ comx,
+                                                   const double* __restrict__ comy, double eps2,
+                                                   const double* __restrict__ mp, double* __restrict__ loc) {
+  constexpr int S = num_moments(P);
+  constexpr int ST = 2 * P + 1;
+  __shared__ double sG[4][ST * ST];
+  const int w = threadIdx.x >> 6;
+  const uint32_t t = blockIdx.x * 4 + w;
+  const int lane = threadIdx.x & 63;
+  if (t >= n_targets) return;
+  double* G = sG[w];
+  double acc[(S + 63) / 64];
+#pragma unroll
+  for (int q = 0; q < (S + 63) / 64; ++q) acc[q] = 0.0;
+  const double tx = comx[t], ty = comy[t];
+  for (uint32_t e = off[t]; e < off[t + 1]; ++e) {
+    const uint32_t s = dat[e];
+    derivs_recurrence<P>(tx - comx[s], ty - comy[s], eps2, G, lane);
+    const double* ms = mp + (size_t)s * S;
+#pragma unroll
+    for (int q = 0; q < (S + 63) / 64; ++q) {
+      int k = lane + 64 * q;
+      if (k < S) {
+        int I, J;
+        coef_ij<P>(k, I, J);
+        const double lf = 1.0 / (c_tabs.fact[I] * c_tabs.fact[J]);
+        double a = 0.0;
+#pragma unroll
+        for (int i = 0; i <= P; ++i)
+#pragma unroll
+          for (int j = 0; j <= P - i; ++j) {
+            double mv = ms[moment_index(i, j)];
+            if (!isfinite(mv)) mv = 0.0;
+            // w * G = sign/(I!J!) * (-D)
+            const double sg = ((i + j) % 2 == 0) ? -1.0 : 1.0;
+            a += mv * G[(i + I + j + J) * ST + (j + J)] * sg;
+          }
+        acc[q] += a * lf;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+#pragma unroll
+  for (int q = 0; q < (S + 63) / 64; ++q) {
+    int k = lane + 64 * q;
+    if (k < S) loc[(size_t)t * S + k] = acc[q];
+  }
+}
+
+// --------------------------------------------------------------- L2L ----
+
+// exact: children in [lo, hi), one thread per (child, coefficient);
+// L_child[oi,oj] += sum_{i>=oi, j>=oj, i+j<=p} L_par[i,j] C(i,oi) C(j,oj) dx^(i-oi) dy^(j-oj)
+// (galax-core/src/lib.rs:1274-1299)
+template <int P>
+__global__ void k_l2l(const uint32_t* __restrict__ par, const double* __restrict__ comx,
+                      const double* __restrict__ comy, uint32_t lo, uint32_t hi, double* __restrict__ loc) {
+#pragma clang fp contract(off)
+  constexpr int S = num_moments(P);
+  size_t tt = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tt >= (size_t)(hi - lo) * S) return;
+  uint32_t c = lo + (uint32_t)(tt / S);
+  int k = (int)(tt % S);
+  uint32_t p = par[c];
+  if (p == NONE) return;
+  int oi, oj;
+  coef_ij<P>(k, oi, oj);
+  double dx = comx[c] - comx[p], dy = comy[c] - comy[p];
+  double dxp[P + 1], dyp[P + 1];
+  dxp[0] = 1.0;
+  dyp[0] = 1.0;
+#pragma unroll
+  for (int q = 1; q <= P; ++q) {
+    dxp[q] = dxp[q - 1] * dx;
+    dyp[q] = dyp[q - 1] * dy;
+  }
+  const double* lp = loc + (size_t)p * S;
+  double sum = 0.0;
+#pragma unroll
+  for (int i = 0; i <= P; ++i) {
+    if (i < oi) continue;
+#pragma unroll
+    for (int j = 0; j <= P - i; ++j) {
+      if (j < oj) continue;
+      sum += lp[moment_index(i, j)] * c_tabs.binom[i][oi] * c_tabs.binom[j][oj] * dxp[i - oi] * dyp[j - oj];
+    }
+  }
+  loc[(size_t)c * S + k] += sum;
+}
+
+// ------------------------------------------------------ L2P + P2P ----
+
+// exact far field of one body (galax-core/src/lib.rs:1321-1345)
+template <int P>
+__device__ __forceinline__ void l2p_body(const double* __restrict__ L, double dux, double duy, double* fx,
+                                         double* fy, double* fphi) {
+#pragma clang fp contract(off)
+  double xp[P + 1], yp[P + 1];
+  xp[0] = 1.0;
+  yp[0] = 1.0;
+#pragma unroll
+  for (int q = 1; q <= P; ++q) {
+    xp[q] = xp[q - 1] * dux;
+    yp[q] = yp[q - 1] * duy;
+  }
+  double ax = 0.0, ay = 0.0, ph = 0.0;
+#pragma unroll
+  for (int i = 0; i <= P; ++i)
+#pragma unroll
+    for (int j = 0; j <= P - i; ++j) {
+      double c = L[moment_index(i, j)];
+      if (i >= 1) ax += c * (double)i * xp[i - 1] * yp[j];
+      if (j >= 1) ay += c * (double)j * xp[i] * yp[j - 1];
+      ph += c * xp[i] * yp[j];
+    }
+  *fx = ax;
+  *fy = ay;
+  *fphi = ph;
+}
+
+// One wave per target leaf (up to 64 bodies per pass); lane = target body.
+// Far field from the leaf's local expansion, then the near field gathered from
+// the leaf's P2P gather list with source bodies staged through LDS.
+template <int P>
+__global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, const double* __restrict__ y,
+                                                const double* __restrict__ m, const uint32_t* __restrict__ bs,
+                                                const uint32_t* __restrict__ be, const double* __restrict__ comx,
+                                                const double* __restrict__ comy, const double* __restrict__ loc,
+                                                const uint32_t* __restrict__ goff, const uint32_t* __restrict__ gdat,
+                                                double eps2, int use_far, double* __restrict__ ax,
+                                                double* __restrict__ ay, double* __restrict__ phi) {
+  constexpr int S = num_moments(P);
+  __shared__ double sx[64], sy[64], sm[64];
+  const uint32_t leaf = blockIdx.x;
+  const int lane = threadIdx.x;
+  const uint32_t ts = bs[leaf], te = be[leaf];
+  for (uint32_t tb = ts; tb < te; tb += 64) {
+    const uint32_t i = tb + lane;
+    const bool valid = i < te;
+    double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0;
+    double fx = 0.0, fy = 0.0, fp = 0.0;
+    if (use_far && valid) l2p_body<P>(loc + (size_t)leaf * S, xi - comx[leaf], yi - comy[leaf], &fx, &fy, &fp);
+    double accx = 0.0 - fx, accy = 0.0 - fy, accp = fp;
+    for (uint32_t g = goff[leaf]; g < goff[leaf + 1]; ++g) {
+      const uint32_t sl = gdat[g];
+      const uint32_t ss = bs[sl], se = be[sl];
+      for (uint32_t sb = ss; sb < se; sb += 64) {
+        const uint32_t cnt = min(64u, se - sb);
+        __syncthreads();
+        if ((uint32_t)lane < cnt) {
+          sx[lane] = x[sb + lane];
+          sy[lane] = y[sb + lane];
+          sm[lane] = m[sb + lane];
+        }
+        __syncthreads();
+        if (valid) {
+          for (uint32_t jj = 0; jj < cnt; ++jj) {
+            if (sb + jj == i) continue;
+            const double dx = sx[jj] - xi, dy = sy[jj] - yi;
+            const double r2 = fma(dx, dx, fma(dy, dy, eps2));
+            const double inv = 1.0 / sqrt(r2);
+            const double mj = sm[jj];
+            const double f = mj * (inv * inv * inv);
+            accx = fma(f, dx, accx);
+            accy = fma(f, dy, accy);
+            accp = fma(-mj, inv, accp);
+          }
+        }
+      }
+    }
+    if (valid) {
+      ax[i] = accx;
+      ay[i] = accy;
+      if (phi) phi[i] = accp;
+    }
+  }
+}
+
+// ------------------------------------------------------ direct sum ----
+
+__global__ void k_direct(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ m,
+                         size_t n, double eps2, const uint64_t* __restrict__ targets, size_t nt,
+                         double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ phi) {
+  __shared__ double sx[256], sy[256], sm[256];
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool valid = k < nt;
+  size_t i = valid ? (targets ? (size_t)targets[k] : k) : 0;
+  double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0;
+  double accx = 0.0, accy = 0.0, accp = 0.0;
+  for (size_t b = 0; b < n; b += 256) {
+    __syncthreads();
+    size_t j = b + threadIdx.x;
+    if (j < n) {
+      sx[threadIdx.x] = x[j];
+      sy[threadIdx.x] = y[j];
+      sm[threadIdx.x] = m[j];
+    }
+    __syncthreads();
+    size_t cnt = n - b < 256 ? n - b : 256;
+    if (valid) {
+      for (size_t jj = 0; jj < cnt; ++jj) {
+        if (b + jj == i) continue;
+        double dx = sx[jj] - xi, dy = sy[jj] - yi;
+        double r2 = dx * dx + dy * dy + eps2;
+        double inv = 1.0 / sqrt(r2);
+        double f = sm[jj] * (inv * inv * inv);
+        accx += f * dx;
+        accy += f * dy;
+        accp -= sm[jj] * inv;
+      }
+    }
+  }
+  if (valid) {
+    ax[k] = accx;
+    ay[k] = accy;
+    if (phi) phi[k] = accp;
+  }
+}
+
+int direct_sum_device(hipStream_t s, const double* x, const double* y, const double* m, size_t n, double eps,
+                      const uint64_t* targets, size_t nt, double* ax, double* ay, double* phi) {
+  if (nt == 0) return 0;
+  k_direct<<<nblk(nt), 256, 0, s>>>(x, y, m, n, eps * eps, targets, nt, ax, ay, phi);
+  GX_HIP(hipGetLastError());
+  return 0;
+}
+
+// ------------------------------------------------------ orchestration ----
+
+#define GX_P_CASES(F) \
+  F(0) F(1) F(2) F(3) F(4) F(5) F(6) F(7) F(8) F(9) F(10) F(11) F(12) F(13) F(14) F(15) F(16)
+
+// Host restatement of compute_kernel_derivs' polynomial construction
+// (galax-core/src/lib.rs:1059-1130); same operations in the same order.
+std::vector<double> kernel_polys(double eps, int p) {
+  const int order = 2 * p, stride = order + 1, GS = stride * stride;
+  std::vector<double> poly((size_t)GS * GS, 0.0);
+  auto P_ = [&](int k, int l) { return poly.data() + (size_t)(k * stride + l) * GS; };
+  const double eps2 = eps * eps;
+  P_(0, 0)[0] = -1.0;
+  std::vector<double> coeff(GS);
+  for (int n = 1; n <= order; ++n) {
+    for (int k = 0; k <= n; ++k) {
+      int l = n - k;
+      std::fill(coeff.begin(), coeff.end(), 0.0);
+      if (k > 0) {
+        const double* prev = P_(k - 1, l);
+        int mm = 1 + 2 * (k - 1) + 2 * l;
+        for (int a = 0; a < stride; ++a)
+          for (int b = 0; b < stride - a; ++b) {
+            double c = prev[a * stride + b];
+            if (c == 0.0) continue;
+            if (a > 0) {
+              double dc = c * (double)a;
+              if (a + 1 < stride) coeff[(a + 1) * stride + b] += dc;
+              if (b + 2 < stride) coeff[(a - 1) * stride + b + 2] += dc;
+              coeff[(a - 1) * stride + b] += dc * eps2;
+            }
+          }
+        for (int a = 0; a < stride; ++a)
+          for (int b = 0; b < stride - a; ++b) {
+            double c = prev[a * stride + b];
+            if (c == 0.0) continue;
+            if (a + 1 < stride) coeff[(a + 1) * stride + b] -= (double)mm * c;
+          }
+      } else if (l > 0) {
+        const double* prev = P_(k, l - 1);
+        int mm = 1 + 2 * k + 2 * (l - 1);
+        for (int a = 0; a < stride; ++a)
+          for (int b = 0; b < stride - a; ++b) {
+            double c = prev[a * stride + b];
+            if (c == 0.0) continue;
+            if (b > 0) {
+              double dc = c * (double)b;
+              if (a + 2 < stride) coeff[(a + 2) * stride + b - 1] += dc;
+              if (b + 1 < stride) coeff[a * stride + b + 1] += dc;
+              coeff[a * stride + b - 1] += dc * eps2;
+            }
+          }
+        for (int a = 0; a < stride; ++a)
+          for (int b = 0; b < stride - a; ++b) {
+            double c = prev[a * stride + b];
+            if (c == 0.0) continue;
+            if (b + 1 < stride) coeff[a * stride + b + 1] -= (double)mm * c;
+          }
+      }
+      std::copy(coeff.begin(), coeff.end(), P_(k, l));
+    }
+  }
+  return poly;
+}
+
+// Builds the key table of the live M2L pairs [0, m2l_off[n_targets]).
+static int m2l_keys_setup(galax_fmm* h, uint32_t n_targets) {
+  DevTree& t = h->tree;
+  hipStream_t s = h->stream;
+  const int p = (int)h->cfg.p;
+  std::vector<uint32_t> offs(2);
+  GX_HIP(hipMemcpyAsync(offs.data() + 1, t.m2l_off.as<uint32_t>() + n_targets, 4, hipMemcpyDeviceToHost, s));
+  GX_HIP(hipStreamSynchronize(s));
+  uint32_t n_pairs = offs[1];
+  h->n_live_pairs = n_pairs;
+  h->n_live_targets = n_targets;
+  h->n_keys = 0;
+  const int GS = (2 * p + 1) * (2 * p + 1);
+  if (n_pairs == 0) {
+    h->keys_valid = true;
+    return 0;
+  }
+  uint64_t nslots = 64;
+  while (nslots < 2ull * n_pairs) nslots <<= 1;
+  uint64_t mask = nslots - 1;
+  GX_CHECK(dbuf_reserve(h->hash, nslots * (8 + 4 + 4 + 4), s));
+  unsigned long long* hkeys = h->hash.as<unsigned long long>();
+  uint32_t* hrep = reinterpret_cast<uint32_t*>(hkeys + nslots);
+  uint32_t* kid = hrep + nslots;
+  uint32_t* slot_kid = kid + nslots;
+  GX_HIP(hipMemsetAsync(hkeys, 0xFF, nslots * 8, s));
+  GX_HIP(hipMemsetAsync(hrep, 0xFF, nslots * 4, s));
+  GX_CHECK(dbuf_reserve(h->tmp_u32, (size_t)n_pairs * 4, s));
+  GX_CHECK(dbuf_reserve(h->key_of_pair, (size_t)n_pairs * 4, s));
+  GX_CHECK(dbuf_reserve(h->flag, 64, s));
+  uint32_t* err = h->flag.as<uint32_t>();
+  GX_HIP(hipMemsetAsync(err, 0, 4, s));
+  const double* comx = t.com_x.as<double>();
+  const double* comy = t.com_y.as<double>();
+  k_keys_insert<<<nblk(n_targets), 256, 0, s>>>(t.m2l_off.as<uint32_t>(), t.m2l_dat.as<uint32_t>(), n_targets, comx,
+                                                comy, hkeys, hrep, mask, h->tmp_u32.as<uint32_t>(), err);
+  GX_HIP(hipGetLastError());
+  uint32_t herr = 0;
+  GX_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s));
+  GX_HIP(hipStreamSynchronize(s));
+  if (herr)
+    return set_error(GALAX_ERR_UNSUPPORTED, "M2L cache key (dx as i64, dy as i64) outside +-2^31 is not supported");
+  k_keys_flag<<<nblk(nslots), 256, 0, s>>>(hkeys, nslots, kid);
+  GX_HIP(hipGetLastError());
+  uint32_t nk = 0;
+  GX_CHECK(scan_u32(h, kid, kid, nslots, false, false, &nk));
+  h->n_keys = nk;
+  GX_CHECK(dbuf_reserve(h->ws.g, (size_t)nk * 16 + 16, s));
+  double* dxy = h->ws.g.as<double>();
+  k_keys_compact<<<nblk(nslots), 256, 0, s>>>(hkeys, hrep, nslots, kid, slot_kid, t.m2l_dat.as<uint32_t>(),
+                                              h->tmp_u32.as<uint32_t>(), comx, comy, dxy);
+  GX_HIP(hipGetLastError());
+  k_keys_lookup<<<nblk(n_pairs), 256, 0, s>>>(t.m2l_dat.as<uint32_t>(), h->tmp_u32.as<uint32_t>(), n_pairs, comx,
+                                              comy, hkeys, slot_kid, mask, h->key_of_pair.as<uint32_t>());
+  GX_HIP(hipGetLastError());
+  // derivative tables of the representatives, with the reference's polynomials
+  std::vector<double> polys = kernel_polys(h->cfg.softening, p);
+  GX_CHECK(dbuf_reserve(h->ws.f, polys.size() * 8, s));
+  GX_HIP(hipMemcpyAsync(h->ws.f.p, polys.data(), polys.size() * 8, hipMemcpyHostToDevice, s));
+  GX_CHECK(dbuf_reserve(h->gtab, (size_t)nk * GS * 8, s));
+  GX_HIP(hipMemsetAsync(h->gtab.p, 0, (size_t)nk * GS * 8, s));
+  k_derivs_poly<<<nk, 64, 0, s>>>(h->ws.f.as<double>(), p, dxy, nk, h->cfg.softening, h->gtab.as<double>());
+  GX_HIP(hipGetLastError());
+  GX_HIP(hipStreamSynchronize(s));
+  h->keys_valid = true;
+  return 0;
+}
+
+template <int P>
+static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, double* d_ay,
+                      double* d_phi) {
+  DevTree& t = h->tree;
+  hipStream_t s = h->stream;
+  constexpr int S = num_moments(P);
+  const bool corrected = h->cfg.algo == GALAX_ALGO_CORRECTED;
+  const size_t nn = t.n_nodes, nl = t.n_leaves;
+  const double eps = h->cfg.softening;
+  const double* comx = t.com_x.as<double>();
+  const double* comy = t.com_y.as<double>();
+  GX_CHECK(dbuf_reserve(h->mp, nn * S * 8, s));
+  GX_CHECK(dbuf_reserve(h->loc, nn * S * 8, s));
+  double* mp = h->mp.as<double>();
+  double* loc = h->loc.as<double>();
+  // Levels whose expansions reach a body. REFERENCE: the ascending-id L2L
+  // (galax-core/src/lib.rs:1266) pushes each node's local into its children
+  // before its own parent has pushed into it, so a leaf receives its own M2L
+  // and its parent's M2L only; multipoles above level 1 feed only M2L at
+  // levels >= 2, whose locals never reach a body. Those passes are skipped.
+  const uint32_t top = corrected ? t.n_levels : std::min<uint32_t>(t.n_levels, 2);
+  const uint32_t live_hi = t.level_offset[top];
+  hipEvent_t* ev = h->ev;
+  GX_HIP(hipEventRecord(ev[1], s));
+  {
+    k_p2m<P><<<nblk(nl, 4), 256, 0, s>>>(d_x, d_y, d_m, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), comx,
+                                         comy, (uint32_t)nl, mp);
+    GX_HIP(hipGetLastError());
+  }
+  GX_HIP(hipEventRecord(ev[2], s));
+  {
+    for (uint32_t l = 1; l < top; ++l) {
+      uint32_t lo = t.level_offset[l], hi = t.level_offset[l + 1];
+      k_m2m<P><<<nblk((size_t)(hi - lo) * S), 256, 0, s>>>(t.child_first.as<uint32_t>(), t.child_count.as<uint8_t>(),
+                                                          comx, comy, lo, hi, mp);
+      GX_HIP(hipGetLastError());
+    }
+  }
+  GX_HIP(hipEventRecord(ev[3], s));
+  GX_HIP(hipEventRecord(ev[4], s));
+  if (corrected) {
+    k_m2l_exact<P><<<nblk(nn, 4), 256, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(),
+                                               (uint32_t)nn, comx, comy, eps * eps, mp, loc);
+    GX_HIP(hipGetLastError());
+  } else if (live_hi > 0) {
+    k_m2l_ref<P><<<nblk(live_hi, 4), 256, 0, s>>>(t.m2l_off.as<uint32_t>(), t.m2l_dat.as<uint32_t>(),
+                                                  h->key_of_pair.as<uint32_t>(), h->gtab.as<double>(), live_hi, mp,
+                                                  loc);
+    GX_HIP(hipGetLastError());
+  }
+  GX_HIP(hipEventRecord(ev[5], s));
+  if (corrected) {
+    for (uint32_t l = t.n_levels - 1; l >= 1; --l) {
+      uint32_t lo = t.level_offset[l - 1], hi = t.level_offset[l];
+      k_l2l<P><<<nblk((size_t)(hi - lo) * S), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, lo, hi, loc);
+      GX_HIP(hipGetLastError());
+    }
+  } else if (t.n_levels >= 2) {
+    k_l2l<P><<<nblk(nl * S), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, 0, (uint32_t)nl, loc);
+    GX_HIP(hipGetLastError());
+  }
+  GX_HIP(hipEventRecord(ev[6], s));
+  const uint32_t* goff = corrected ? h->c_gat_off.as<uint32_t>() : t.gat_off.as<uint32_t>();
+  const uint32_t* gdat = corrected ? h->c_gat_dat.as<uint32_t>() : t.gat_dat.as<uint32_t>();
+  k_l2p_p2p<P><<<(unsigned)nl, 64, 0, s>>>(d_x, d_y, d_m, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), comx,
+                                           comy, loc, goff, gdat, eps * eps, 1, d_ax,
+                                           d_ay, d_phi);
+  GX_HIP(hipGetLastError());
+  GX_HIP(hipEventRecord(ev[7], s));
+  return 0;
+}
+
+int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, double* d_ay, double* d_phi) {
+  DevTree& t = h->tree;
+  hipStream_t s = h->stream;
+  const size_t n = t.n_bodies;
+  GX_CHECK(upload_tabs(s));
+  for (int i = 0; i < 12; ++i)
+    if (!h->ev[i]) GX_HIP(hipEventCreate(&h->ev[i]));
+  if (n == 0) return 0;
+  if (t.n_nodes == 0) return set_error(GALAX_ERR_INVALID, "no tree installed");
+  const bool corrected = h->cfg.algo == GALAX_ALGO_CORRECTED;
+  if (corrected && !h->corrected_ready) return set_error(GALAX_ERR_INVALID, "corrected lists not built");
+  if (!corrected && !h->keys_valid) {
+    uint32_t top = std::min<uint32_t>(t.n_levels, 2);
+    GX_CHECK(m2l_keys_setup(h, t.level_offset[top]));
+  }
+  GX_CHECK(dbuf_reserve(h->flag, 64, s));
+  uint32_t* changed = h->flag.as<uint32_t>() + 4;
+  GX_HIP(hipEventRecord(h->ev[0], s));
+  GX_HIP(hipMemsetAsync(changed, 0, 4, s));
+  k_sanitize<<<nblk(n), 256, 0, s>>>(d_x, d_y, d_m, n, changed);
+  GX_HIP(hipGetLastError());
+  int rc = 0;
+  switch (h->cfg.p) {
+#define GX_CASE(PP)                                                  \
+  case PP:                                                           \
+    rc = run_passes<PP>(h, d_x, d_y, d_m, d_ax, d_ay, d_phi);        \
+    break;
+    GX_P_CASES(GX_CASE)
+#undef GX_CASE
+    default:
+      return set_error(GALAX_ERR_INVALID, "p = %u outside 0..16", h->cfg.p);
+  }
+  GX_CHECK(rc);
+  GX_HIP(hipStreamSynchronize(s));
+  float ms[8];
+  for (int i = 0; i < 7; ++i) GX_HIP(hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1]));
+  galax_timings_t& tm = h->timings;
+  tm.sanitize = ms[0];
+  tm.p2m = ms[1];
+  tm.m2m = ms[2];
+  tm.m2l_setup = ms[3];
+  tm.m2l = ms[4];
+  tm.l2l = ms[5];
+  tm.l2p_p2p = ms[6];
+  float tot = 0.f;
+  GX_HIP(hipEventElapsedTime(&tot, h->ev[0], h->ev[7]));
+  tm.total = tot;
+  tm.m2l_pairs_executed = corrected ? h->c_m2l_total : h->n_live_pairs;
+  tm.m2l_pairs_reference = corrected ? h->c_m2l_total : t.m2l_total;
+  tm.m2l_keys = corrected ? 0 : h->n_keys;
+  tm.p2p_interactions = corrected ? h->c_p2p_interactions : t.p2p_interactions;
+  return 0;
+}
+
+}  // namespace gx

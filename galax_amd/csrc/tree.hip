@@ -1,0 +1,970 @@
+// tree.hip — the reference's Morton-chunk tree and its interaction lists, built
+// on the device and bit-exact with galax-core:
+//   Morton keys            galax-core/src/lib.rs:73-109
+//   stable sort + permute  galax-core/src/lib.rs:146-217 (sort_by_key is stable)
+//   leaves                 galax-core/src/lib.rs:219-267
+//   parent rounds          galax-core/src/lib.rs:269-354
+//   M2L lists              galax-core/src/lib.rs:947-1025
+//   P2P lists              galax-core/src/lib.rs:1358-1392
+// No FMA contraction anywhere in this file: every product is rounded before it
+// is added, as in Rust, so the f64 geometry matches bit for bit.
+#pragma clang fp contract(off)
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "common.h"
+
+namespace gx {
+
+// ------------------------------------------------------ Rust semantics ----
+
+__device__ __forceinline__ uint64_t rust_f64_as_u64(double v) {
+  if (!(v > 0.0)) return 0;  // NaN, zero and negatives
+  if (v >= 18446744073709551616.0) return ~0ull;
+  return (uint64_t)v;
+}
+
+__device__ __forceinline__ uint64_t normalize(double x, double lo, double hi) {
+  double t = (x - lo) / (hi - lo);
+  const double top = 1.0 - 1e-15;  // f64::clamp(0.0, 1.0 - 1e-15), NaN passes
+  if (t < 0.0) t = 0.0;
+  if (t > top) t = top;
+  return rust_f64_as_u64(t * 2097152.0);  // (1 << MORTON_BITS) as f64
+}
+
+__device__ __forceinline__ uint64_t spread_bits(uint64_t v) {
+  uint64_t x = v & ((1ull << MORTON_BITS) - 1);
+  x = (x | (x << 16)) & 0x0000ffff0000ffffull;
+  x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
+  x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0full;
+  x = (x | (x << 2)) & 0x3333333333333333ull;
+  x = (x | (x << 1)) & 0x5555555555555555ull;
+  return x;
+}
+
+__global__ void k_morton(const double* __restrict__ x, const double* __restrict__ y, size_t n, double x0,
+                         double x1, double y0, double y1, uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = spread_bits(normalize(x[i], x0, x1)) | (spread_bits(normalize(y[i], y0, y1)) << 1);
+  if (idx) idx[i] = (uint32_t)i;
+}
+
+// ------------------------------------------------- stable radix sort ----
+// LSD radix sort of (42-bit key, u32 payload), 7-bit digits, 6 passes. Each
+// 256-thread block owns a tile of 4096 keys read in 16 rounds of 256
+// consecutive keys; within a round, lanes holding the same digit are found
+// with 7 wave ballots and ranked by lane order, and per-wave digit counts are
+// combined through LDS. Ranks therefore follow input order: the sort is stable.
+
+constexpr int RS_BITS = 7, RS_RADIX = 1 << RS_BITS, RS_THREADS = 256, RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int RS_WAVES = RS_THREADS / 64;
+
+__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint64_t* __restrict__ keys, size_t n, int shift,
+                                                        uint32_t* __restrict__ hist, uint32_t nb) {
+  __shared__ uint32_t hcount[RS_RADIX];
+  for (int t = threadIdx.x; t < RS_RADIX; t += RS_THREADS) hcount[t] = 0;
+  __syncthreads();
+  size_t base = (size_t)blockIdx.x * RS_TILE;
+#pragma unroll 4
+  for (int k = 0; k < RS_ITEMS; ++k) {
+    size_t i = base + (size_t)k * RS_THREADS + threadIdx.x;
+    if (i < n) atomicAdd(&hcount[(keys[i] >> shift) & (RS_RADIX - 1)], 1u);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < RS_RADIX; t += RS_THREADS) hist[(size_t)t * nb + blockIdx.x] = hcount[t];
+}
+
+__global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t* __restrict__ kin,
+                                                           const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
+                                                           uint32_t* __restrict__ vout, size_t n, int shift,
+                                                           const uint32_t* __restrict__ offs, uint32_t nb) {
+  __shared__ uint32_t run[RS_RADIX];
+  __shared__ uint32_t boff[RS_RADIX];
+  __shared__ uint32_t cnt[RS_WAVES][RS_RADIX];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int t = tid; t < RS_RADIX; t += RS_THREADS) {
+    run[t] = 0;
+    boff[t] = offs[(size_t)t * nb + blockIdx.x];
+  }
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  size_t base = (size_t)blockIdx.x * RS_TILE;
+  for (int k = 0; k < RS_ITEMS; ++k) {
+    for (int t = tid; t < RS_WAVES * RS_RADIX; t += RS_THREADS) (&cnt[0][0])[t] = 0;
+    __syncthreads();
+    size_t i = base + (size_t)k * RS_THREADS + tid;
+    bool valid = i < n;
+    uint64_t key = valid ? kin[i] : 0ull;
+    uint32_t d = (uint32_t)(key >> shift) & (RS_RADIX - 1);
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < RS_BITS; ++b) {
+      bool bit = (d >> b) & 1u;
+      uint64_t bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+    bool leader = valid && ((peers & lt_mask) == 0);
+    if (leader) cnt[w][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    for (int t = tid; t < RS_RADIX; t += RS_THREADS) {
+      uint32_t s = run[t];
+#pragma unroll
+      for (int ww = 0; ww < RS_WAVES; ++ww) {
+        uint32_t c = cnt[ww][t];
+        cnt[ww][t] = s;
+        s += c;
+      }
+      run[t] = s;
+    }
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = boff[d] + cnt[w][d] + rank;
+      kout[pos] = key;
+      vout[pos] = vin[i];
+    }
+    __syncthreads();
+  }
+}
+
+// Sorts keys (in a) with payload vals (in va); result in place. kb/vb scratch.
+static int radix_sort(galax_fmm* h, uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb, size_t n) {
+  if (n <= 1) return 0;
+  hipStream_t s = h->stream;
+  uint32_t nb = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
+  GX_CHECK(dbuf_reserve(h->ws.d, (size_t)RS_RADIX * nb * sizeof(uint32_t), s));
+  uint32_t* hist = h->ws.d.as<uint32_t>();
+  uint64_t *ki = ka, *ko = kb;
+  uint32_t *vi = va, *vo = vb;
+  for (int pass = 0; pass < 6; ++pass) {
+    int shift = pass * RS_BITS;
+    k_rs_hist<<<nb, RS_THREADS, 0, s>>>(ki, n, shift, hist, nb);
+    GX_HIP(hipGetLastError());
+    GX_CHECK(scan_u32(h, hist, hist, (size_t)RS_RADIX * nb, false, false, nullptr));
+    k_rs_scatter<<<nb, RS_THREADS, 0, s>>>(ki, vi, ko, vo, n, shift, hist, nb);
+    GX_HIP(hipGetLastError());
+    std::swap(ki, ko);
+    std::swap(vi, vo);
+  }
+  // 6 passes: the result is back in (ka, va)
+  return 0;
+}
+
+// ---------------------------------------------------------- permute ----
+
+__global__ void k_gather_f64(const double* __restrict__ src, const uint32_t* __restrict__ idx, size_t n,
+                             double* __restrict__ dst) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+// ------------------------------------------------------------ leaves ----
+
+__device__ __forceinline__ void leaf_geometry(const double* __restrict__ x, const double* __restrict__ y,
+                                              const double* __restrict__ m, uint32_t s, uint32_t e, double* cx,
+                                              double* cy, double* hw, double* ms, double* comx, double* comy) {
+  double mass_sum = 0.0, sx = 0.0, sy = 0.0;
+  double bx0 = 1.7976931348623157e308, bx1 = -1.7976931348623157e308;
+  double by0 = 1.7976931348623157e308, by1 = -1.7976931348623157e308;
+  for (uint32_t i = s; i < e; ++i) {
+    double mi = m[i], xi = x[i], yi = y[i];
+    mass_sum += mi;
+    sx += xi * mi;
+    sy += yi * mi;
+    bx0 = fmin(bx0, xi);  // f64::min / f64::max
+    bx1 = fmax(bx1, xi);
+    by0 = fmin(by0, yi);
+    by1 = fmax(by1, yi);
+  }
+  double c_x = (bx0 + bx1) * 0.5, c_y = (by0 + by1) * 0.5;
+  *cx = c_x;
+  *cy = c_y;
+  *hw = fmax(bx1 - bx0, by1 - by0) * 0.5 + 1e-15;
+  *ms = mass_sum;
+  *comx = mass_sum > 0.0 ? sx / mass_sum : c_x;
+  *comy = mass_sum > 0.0 ? sy / mass_sum : c_y;
+}
+
+struct NodeArrays {
+  double *cx, *cy, *hw, *mass, *comx, *comy;
+  uint32_t *bs, *be, *cf, *par;
+  uint8_t *cc, *lv;
+};
+
+static NodeArrays node_arrays(galax_fmm* h) {
+  DevTree& t = h->tree;
+  return NodeArrays{t.center_x.as<double>(),   t.center_y.as<double>(),   t.half_width.as<double>(),
+                    t.mass.as<double>(),       t.com_x.as<double>(),      t.com_y.as<double>(),
+                    t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), t.child_first.as<uint32_t>(),
+                    t.parent.as<uint32_t>(),   t.child_count.as<uint8_t>(), t.level.as<uint8_t>()};
+}
+
+// leaf ranges: [lstart[i], lstart[i+1]) with lstart[n_leaves] = n
+__global__ void k_leaves(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ m,
+                         const uint32_t* __restrict__ lstart, uint32_t n_leaves, NodeArrays a) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_leaves) return;
+  uint32_t s = lstart[i], e = lstart[i + 1];
+  leaf_geometry(x, y, m, s, e, &a.cx[i], &a.cy[i], &a.hw[i], &a.mass[i], &a.comx[i], &a.comy[i]);
+  a.bs[i] = s;
+  a.be[i] = e;
+  a.cf[i] = NONE;
+  a.cc[i] = 0;
+  a.par[i] = NONE;
+  a.lv[i] = 0;
+}
+
+__global__ void k_uniform_leaf_starts(uint32_t* lstart, uint32_t n_leaves, uint32_t n_max, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n_leaves) return;
+  uint64_t v = (uint64_t)i * n_max;
+  lstart[i] = (uint32_t)(v < n ? v : n);
+}
+
+// ----------------------------------------------------- parent rounds ----
+// One round groups the nodes [cs, ce) into runs of equal keys[body_start] >> shift,
+// split into chunks of at most 4 from the start of each run; chunk j becomes
+// parent ce + j (galax-core/src/lib.rs:274-354).
+
+__global__ void k_round_heads(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ bs, uint32_t cs,
+                              uint32_t c, uint32_t shift, uint32_t* __restrict__ head) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= c) return;
+  uint32_t sh = shift & 63;  // release-mode Rust masks the shift amount
+  uint64_t g = keys[bs[cs + j]] >> sh;
+  bool start = (j == 0) || (g != (keys[bs[cs + j - 1]] >> sh));
+  head[j] = start ? j : 0u;
+}
+
+__global__ void k_round_chunks(const uint32_t* __restrict__ run_start, uint32_t c, uint32_t* __restrict__ flag) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= c) return;
+  flag[j] = ((j - run_start[j]) % 4u == 0u) ? 1u : 0u;
+}
+
+__global__ void k_round_mark(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pidx, uint32_t cs,
+                             uint32_t ce, uint32_t c, uint32_t* __restrict__ cf) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= c) return;
+  if (flag[j]) cf[ce + pidx[j]] = cs + j;
+}
+
+__global__ void k_round_parents(NodeArrays a, uint32_t ce, uint32_t np, uint8_t level, int constrained_style) {
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  uint32_t p = ce + q;
+  uint32_t first = a.cf[p];
+  uint32_t last = (q + 1 < np) ? a.cf[p + 1] : ce;
+  double mass_sum = 0.0, sx = 0.0, sy = 0.0;
+  double bx0 = 1.7976931348623157e308, bx1 = -1.7976931348623157e308;
+  double by0 = 1.7976931348623157e308, by1 = -1.7976931348623157e308;
+  double max_hw = 0.0;
+  for (uint32_t c = first; c < last; ++c) {
+    double cm = a.mass[c], chw = a.hw[c], ccx = a.cx[c], ccy = a.cy[c];
+    mass_sum += cm;
+    sx += a.comx[c] * cm;
+    sy += a.comy[c] * cm;
+    if (constrained_style) {  // build_with_constraints, galax-core/src/lib.rs:660-668
+      max_hw = fmax(max_hw, chw);
+      bx0 = fmin(bx0, ccx - chw);
+      bx1 = fmax(bx1, ccx + chw);
+      by0 = fmin(by0, ccy - chw);
+      by1 = fmax(by1, ccy + chw);
+    } else {  // Tree::build, galax-core/src/lib.rs:302-318
+      if (chw > max_hw) max_hw = chw;
+      double cl = ccx - chw, cr = ccx + chw;
+      if (cl < bx0) bx0 = cl;
+      if (cr > bx1) bx1 = cr;
+      double cb = ccy - chw, ct = ccy + chw;
+      if (cb < by0) by0 = cb;
+      if (ct > by1) by1 = ct;
+    }
+    a.par[c] = p;
+  }
+  double c_x = (bx0 + bx1) * 0.5, c_y = (by0 + by1) * 0.5;
+  a.cx[p] = c_x;
+  a.cy[p] = c_y;
+  a.hw[p] = max_hw * 2.0;
+  a.mass[p] = mass_sum;
+  a.comx[p] = mass_sum > 0.0 ? sx / mass_sum : c_x;
+  a.comy[p] = mass_sum > 0.0 ? sy / mass_sum : c_y;
+  a.bs[p] = a.bs[first];
+  a.be[p] = a.be[last - 1];
+  a.cc[p] = (uint8_t)(last - first);
+  a.par[p] = NONE;
+  a.lv[p] = level;
+}
+
+template <class T>
+static int grow_keep(DBuf& b, size_t old_n, size_t new_n, hipStream_t s) {
+  if (new_n * sizeof(T) <= b.bytes) return 0;
+  DBuf nb;
+  GX_CHECK(dbuf_reserve(nb, new_n * sizeof(T), s));
+  if (b.p && old_n) GX_HIP(hipMemcpyAsync(nb.p, b.p, old_n * sizeof(T), hipMemcpyDeviceToDevice, s));
+  GX_HIP(hipStreamSynchronize(s));
+  dbuf_free(b);
+  b = nb;
+  return 0;
+}
+
+int ensure_node_capacity(galax_fmm* h, size_t nodes) {
+  DevTree& t = h->tree;
+  if (nodes <= t.capacity) return 0;
+  size_t nc = std::max(nodes, t.capacity * 3 / 2 + 64);
+  size_t old = t.capacity;
+  hipStream_t s = h->stream;
+  GX_CHECK(grow_keep<double>(t.center_x, old, nc, s));
+  GX_CHECK(grow_keep<double>(t.center_y, old, nc, s));
+  GX_CHECK(grow_keep<double>(t.half_width, old, nc, s));
+  GX_CHECK(grow_keep<double>(t.mass, old, nc, s));
+  GX_CHECK(grow_keep<double>(t.com_x, old, nc, s));
+  GX_CHECK(grow_keep<double>(t.com_y, old, nc, s));
+  GX_CHECK(grow_keep<uint32_t>(t.body_start, old, nc, s));
+  GX_CHECK(grow_keep<uint32_t>(t.body_end, old, nc, s));
+  GX_CHECK(grow_keep<uint32_t>(t.child_first, old, nc, s));
+  GX_CHECK(grow_keep<uint32_t>(t.parent, old, nc, s));
+  GX_CHECK(grow_keep<uint8_t>(t.child_count, old, nc, s));
+  GX_CHECK(grow_keep<uint8_t>(t.level, old, nc, s));
+  t.capacity = nc;
+  return 0;
+}
+
+static inline unsigned nblk(size_t n, unsigned bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+// Builds leaves from the leaf starts in ws.c (n_leaves + 1 entries) and runs the
+// parent rounds. Bodies are in h->x, h->y, h->m (tree order); keys in tree.keys.
+static int build_from_leaf_starts(galax_fmm* h, size_t n, size_t n_leaves, bool constrained_style) {
+  DevTree& t = h->tree;
+  hipStream_t s = h->stream;
+  GX_CHECK(ensure_node_capacity(h, n_leaves * 4 + 64));
+  NodeArrays a = node_arrays(h);
+  k_leaves<<<nblk(n_leaves), 256, 0, s>>>(h->x.as<double>(), h->y.as<double>(), h->m.as<double>(),
+                                          h->ws.c.as<uint32_t>(), (uint32_t)n_leaves, a);
+  GX_HIP(hipGetLastError());
+  t.level_offset.assign(1, 0);
+  t.level_offset.push_back((uint32_t)n_leaves);
+  size_t cs = 0, ce = n_leaves;
+  uint32_t shift = 2;
+  uint32_t lvl = 0;
+  GX_CHECK(dbuf_reserve(h->ws.e, (n_leaves + 1) * sizeof(uint32_t), s));
+  GX_CHECK(dbuf_reserve(h->ws.f, (n_leaves + 1) * sizeof(uint32_t), s));
+  GX_CHECK(dbuf_reserve(h->ws.g, (n_leaves + 1) * sizeof(uint32_t), s));
+  while (ce - cs > 1) {
+    uint32_t c = (uint32_t)(ce - cs);
+    ++lvl;
+    if (lvl > 250) return set_error(GALAX_ERR_INVALID, "tree deeper than 250 levels");
+    uint32_t* head = h->ws.e.as<uint32_t>();
+    uint32_t* flag = h->ws.f.as<uint32_t>();
+    uint32_t* pidx = h->ws.g.as<uint32_t>();
+    k_round_heads<<<nblk(c), 256, 0, s>>>(t.keys.as<uint64_t>(), a.bs, (uint32_t)cs, c, shift, head);
+    GX_HIP(hipGetLastError());
+    GX_CHECK(scan_u32(h, head, head, c, true, true, nullptr));
+    k_round_chunks<<<nblk(c), 256, 0, s>>>(head, c, flag);
+    GX_HIP(hipGetLastError());
+    uint32_t np = 0;
+    GX_CHECK(scan_u32(h, flag, pidx, c, false, false, &np));
+    GX_CHECK(ensure_node_capacity(h, ce + np));
+    a = node_arrays(h);
+    k_round_mark<<<nblk(c), 256, 0, s>>>(flag, pidx, (uint32_t)cs, (uint32_t)ce, c, a.cf);
+    GX_HIP(hipGetLastError());
+    k_round_parents<<<nblk(np), 256, 0, s>>>(a, (uint32_t)ce, np, (uint8_t)lvl, constrained_style ? 1 : 0);
+    GX_HIP(hipGetLastError());
+    cs = ce;
+    ce += np;
+    t.level_offset.push_back((uint32_t)ce);
+    shift += 2;
+  }
+  t.n_nodes = ce;
+  t.n_leaves = n_leaves;
+  t.n_bodies = n;
+  t.n_levels = (uint32_t)t.level_offset.size() - 1;
+  GX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// Tree::build: keys, stable sort, permute the 7 arrays, re-key, leaves, rounds.
+int build_tree_plain(galax_fmm* h, size_t n) {
+  DevTree& t = h->tree;
+  hipStream_t s = h->stream;
+  const galax_fmm_config_t& c = h->cfg;
+  t.n_nodes = t.n_leaves = 0;
+  t.n_levels = 0;
+  t.n_bodies = n;
+  t.level_offset.assign(1, 0);
+  if (n == 0) return 0;
+  GX_CHECK(dbuf_reserve(t.keys, n * sizeof(uint64_t), s));
+  GX_CHECK(dbuf_reserve(h->ws.a, n * sizeof(uint64_t), s));
+  GX_CHECK(dbuf_reserve(h->ws.b, n * sizeof(uint32_t) * 2, s));
+  uint64_t* ka = t.keys.as<uint64_t>();
+  uint64_t* kb = h->ws.a.as<uint64_t>();
+  uint32_t* va = h->ws.b.as<uint32_t>();
+  uint32_t* vb = va + n;
+  k_morton<<<nblk(n), 256, 0, s>>>(h->x.as<double>(), h->y.as<double>(), n, c.x_min, c.x_max, c.y_min, c.y_max, ka,
+                                   va);
+  GX_HIP(hipGetLastError());
+  GX_CHECK(radix_sort(h, ka, va, kb, vb, n));
+  // permutation: new[i] = old[va[i]]; kept for galax_fmm_build_tree's perm_out
+  GX_CHECK(dbuf_reserve(h->perm, n * sizeof(uint32_t), s));
+  GX_HIP(hipMemcpyAsync(h->perm.p, va, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+  GX_CHECK(dbuf_reserve(h->ws.h, n * sizeof(double), s));
+  double* tmp = h->ws.h.as<double>();
+  DBuf* arrs[7] = {&h->x, &h->y, &h->vx, &h->vy, &h->m, &h->ax, &h->ay};
+  for (DBuf* b : arrs) {
+    if (!b->p) continue;
+    k_gather_f64<<<nblk(n), 256, 0, s>>>(b->as<double>(), va, n, tmp);
+    GX_HIP(hipGetLastError());
+    GX_HIP(hipMemcpyAsync(b->p, tmp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+  // Re-encoding the permuted bodies reproduces the sorted keys exactly (the
+  // encoding is a pure function of the body), so ka already holds them.
+  size_t n_leaves = (n + c.n_max - 1) / c.n_max;
+  GX_CHECK(dbuf_reserve(h->ws.c, (n_leaves + 1) * sizeof(uint32_t), s));
+  k_uniform_leaf_starts<<<nblk(n_leaves + 1), 256, 0, s>>>(h->ws.c.as<uint32_t>(), (uint32_t)n_leaves, c.n_max,
+                                                           (uint32_t)n);
+  GX_HIP(hipGetLastError());
+  return build_from_leaf_starts(h, n, n_leaves, false);
+}
+
+// ------------------------------------------- constrained tree (A6/A7) ----
+
+// neighbors_geometric (galax-core/src/lib.rs:382-423) for one leaf and all 8
+// test points, as a pruned traversal: nodes whose box contains the point and
+// that are cells_adjacent to the leaf; the nearest centre wins, the lowest id
+// among equal distances (the reference scans ids in ascending order with `<`).
+struct UBox {
+  const double *x0, *x1, *y0, *y1;
+};
+
+__global__ void k_ubox(NodeArrays a, uint32_t lo, uint32_t hi, double* ux0, double* ux1, double* uy0, double* uy1) {
+  uint32_t i = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= hi) return;
+  double x0 = a.cx[i] - a.hw[i], x1 = a.cx[i] + a.hw[i], y0 = a.cy[i] - a.hw[i], y1 = a.cy[i] + a.hw[i];
+  uint32_t cf = a.cf[i];
+  if (cf != NONE) {
+    for (uint32_t c = cf; c < cf + a.cc[i]; ++c) {
+      x0 = fmin(x0, ux0[c]);
+      x1 = fmax(x1, ux1[c]);
+      y0 = fmin(y0, uy0[c]);
+      y1 = fmax(y1, uy1[c]);
+    }
+  }
+  ux0[i] = x0;
+  ux1[i] = x1;
+  uy0[i] = y0;
+  uy1[i] = y1;
+}
+
+static int compute_uboxes(galax_fmm* h) {
+  DevTree& t = h->tree;
+  hipStream_t s = h->stream;
+  size_t nn = t.n_nodes;
+  GX_CHECK(dbuf_reserve(h->ws.h, nn * 4 * sizeof(double), s));
+  double* u = h->ws.h.as<double>();
+  NodeArrays a = node_arrays(h);
+  for (uint32_t l = 0; l < t.n_levels; ++l) {
+    uint32_t lo = t.level_offset[l], hi = t.level_offset[l + 1];
+    k_ubox<<<nblk(hi - lo), 256, 0, s>>>(a, lo, hi, u, u + nn, u + 2 * nn, u + 3 * nn);
+    GX_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+constexpr int DFS_STACK = 256;
+
+__device__ __forceinline__ bool cells_adjacent(const NodeArrays& a, uint32_t i, uint32_t j) {
+  double hi = a.hw[i], hj = a.hw[j];
+  double ratio = fmax(hi, hj) / fmin(hi, hj);
+  if (ratio > 8.0) return false;
+  double dx = fabs(a.cx[i] - a.cx[j]), dy = fabs(a.cy[i] - a.cy[j]);
+  return dx <= hi + hj + 1e-15 && dy <= hi + hj + 1e-15;
+}
+
+// For each leaf: does any geometric neighbour have hw*2.1 < this leaf's hw?
+// (the 2:1 check of build_constrained, galax-core/src/lib.rs:558-578)
+__global__ void k_constrained_check(NodeArrays a, uint32_t n_leaves, uint32_t root, const double* __restrict__ ux0,
+                                    const double* __restrict__ ux1, const double* __restrict__ uy0,
+                                    const double* __restrict__ uy1, uint32_t* __restrict__ violate,
+                                    uint32_t* __restrict__ err) {
+  uint32_t lid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lid >= n_leaves) return;
+  double cx = a.cx[lid], cy = a.cy[lid], hw = a.hw[lid];
+  const double px[8] = {cx + hw * 2.0, cx + hw * 2.0, cx, cx - hw * 2.0, cx - hw * 2.0, cx - hw * 2.0, cx, cx + hw * 2.0};
+  const double py[8] = {cy, cy + hw * 2.0, cy + hw * 2.0, cy + hw * 2.0, cy, cy - hw * 2.0, cy - hw * 2.0, cy - hw * 2.0};
+  uint32_t result = 0;
+  for (int k = 0; k < 8; ++k) {
+    uint32_t best = NONE;
+    double best_d = 1.7976931348623157e308;
+    uint32_t stack[DFS_STACK];
+    int sp = 0;
+    stack[sp++] = root;
+    while (sp > 0) {
+      uint32_t nd = stack[--sp];
+      // prune with the subtree box (conservative slack on top of the 1e-15)
+      double tol = 1e-15 + 1e-12 * (fabs(px[k]) + fabs(py[k]) + 1.0);
+      if (ux0[nd] > px[k] + tol || ux1[nd] < px[k] - tol || uy0[nd] > py[k] + tol || uy1[nd] < py[k] - tol) continue;
+      if (nd != lid) {
+        double dx = fabs(px[k] - a.cx[nd]), dy = fabs(py[k] - a.cy[nd]);
+        if (dx <= a.hw[nd] + 1e-15 && dy <= a.hw[nd] + 1e-15 && cells_adjacent(a, lid, nd)) {
+          double ex = px[k] - a.cx[nd], ey = py[k] - a.cy[nd];
+          double dist = ex * ex + ey * ey;
+          if (dist < best_d || (dist == best_d && nd < best)) {
+            best_d = dist;
+            best = nd;
+          }
+        }
+      }
+      uint32_t cf = a.cf[nd];
+      if (cf != NONE) {
+        for (int c = (int)a.cc[nd] - 1; c >= 0; --c) {
+          if (sp < DFS_STACK)
+            stack[sp++] = cf + (uint32_t)c;
+          else
+            atomicOr(err, 1u);
+        }
+      }
+    }
+    if (best != NONE && hw > a.hw[best] * 2.1) {
+      result = 1;
+      break;  // one split per leaf per iteration
+    }
+  }
+  violate[lid] = result;
+}
+
+// build_constrained (galax-core/src/lib.rs:541-588): up to 10 rebuilds on the
+// current body order; a leaf of more than 2 bodies with a geometric neighbour
+// of less than 1/2.1 its half width is split at (start+end)/2.
+int build_tree_constrained(galax_fmm* h, size_t n) {
+  DevTree& t = h->tree;
+  hipStream_t s = h->stream;
+  const galax_fmm_config_t& c = h->cfg;
+  t.n_nodes = t.n_leaves = 0;
+  t.n_levels = 0;
+  t.n_bodies = n;
+  t.level_offset.assign(1, 0);
+  if (n == 0) return 0;
+  GX_CHECK(dbuf_reserve(t.keys, n * sizeof(uint64_t), s));
+  k_morton<<<nblk(n), 256, 0, s>>>(h->x.as<double>(), h->y.as<double>(), n, c.x_min, c.x_max, c.y_min, c.y_max,
+                                   t.keys.as<uint64_t>(), nullptr);
+  GX_HIP(hipGetLastError());
+  std::vector<uint64_t> splits;
+  std::vector<uint32_t> starts;
+  auto make_starts = [&]() {
+    starts.clear();
+    uint64_t st = 0;
+    while (st < n) {
+      uint64_t bound = std::min<uint64_t>(st + c.n_max, n);
+      uint64_t e = bound;
+      auto it = std::upper_bound(splits.begin(), splits.end(), st);
+      if (it != splits.end() && *it < bound) e = *it;
+      starts.push_back((uint32_t)st);
+      st = e;
+    }
+    starts.push_back((uint32_t)n);
+  };
+  for (int iter = 0; iter <= 10; ++iter) {
+    make_starts();
+    size_t nl = starts.size() - 1;
+    GX_CHECK(dbuf_reserve(h->ws.c, starts.size() * sizeof(uint32_t), s));
+    GX_HIP(hipMemcpyAsync(h->ws.c.p, starts.data(), starts.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    GX_CHECK(build_from_leaf_starts(h, n, nl, true));
+    if (iter == 10) break;
+    GX_CHECK(compute_uboxes(h));
+    GX_CHECK(dbuf_reserve(h->ws.d, (nl + 16) * sizeof(uint32_t), s));
+    uint32_t* viol = h->ws.d.as<uint32_t>();
+    uint32_t* err = viol + nl;
+    GX_HIP(hipMemsetAsync(err, 0, sizeof(uint32_t), s));
+    double* u = h->ws.h.as<double>();
+    size_t nn = t.n_nodes;
+    NodeArrays a = node_arrays(h);
+    k_constrained_check<<<nblk(nl), 128, 0, s>>>(a, (uint32_t)nl, (uint32_t)(nn - 1), u, u + nn, u + 2 * nn,
+                                                  u + 3 * nn, viol, err);
+    GX_HIP(hipGetLastError());
+    std::vector<uint32_t> hv(nl + 1);
+    GX_HIP(hipMemcpyAsync(hv.data(), viol, (nl + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    GX_HIP(hipStreamSynchronize(s));
+    if (hv[nl]) return set_error(GALAX_ERR_INVALID, "constrained build: traversal stack overflow");
+    std::vector<uint64_t> new_splits;
+    for (size_t l = 0; l < nl; ++l) {
+      if (!hv[l]) continue;
+      uint64_t st = starts[l], e = starts[l + 1];
+      if (e - st > 2) {
+        uint64_t mid = (st + e) / 2;
+        if (!std::binary_search(splits.begin(), splits.end(), mid) &&
+            std::find(new_splits.begin(), new_splits.end(), mid) == new_splits.end())
+          new_splits.push_back(mid);
+      }
+    }
+    if (new_splits.empty()) break;
+    splits.insert(splits.end(), new_splits.begin(), new_splits.end());
+    std::sort(splits.begin(), splits.end());
+    splits.erase(std::unique(splits.begin(), splits.end()), splits.end());
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------ lists ----
+
+// Same-level neighbours: for each node P, the ids Q != P on P's level whose
+// boxes touch P's box, in ascending id order. The traversal descends from the
+// root through subtree boxes; a left-to-right preorder reaches the nodes of
+// one level in ascending id order because parents are created in the order
+// of their consecutive children.
+template <bool FILL>
+__global__ void k_same_level_nbrs(NodeArrays a, uint32_t lo, uint32_t hi, uint32_t root, const double* __restrict__ ux0,
+                                  const double* __restrict__ ux1, const double* __restrict__ uy0,
+                                  const double* __restrict__ uy1, uint32_t* __restrict__ cnt_or_off,
+                                  uint32_t* __restrict__ out, uint32_t* __restrict__ err) {
+  uint32_t P = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (P >= hi) return;
+  const uint8_t lvP = a.lv[P];
+  const double cx = a.cx[P], cy = a.cy[P], hw = a.hw[P];
+  const double tol = 1e-15 + 1e-12 * (fabs(cx) + fabs(cy) + hw + 1.0);
+  const double qx0 = cx - hw - tol, qx1 = cx + hw + tol, qy0 = cy - hw - tol, qy1 = cy + hw + tol;
+  uint32_t base = FILL ? cnt_or_off[P] : 0u;
+  uint32_t cnt = 0;
+  uint32_t stack[DFS_STACK];
+  int sp = 0;
+  stack[sp++] = root;
+  while (sp > 0) {
+    uint32_t nd = stack[--sp];
+    if (a.lv[nd] == lvP) {
+      if (nd != P) {
+        double dx = fabs(a.cx[nd] - cx), dy = fabs(a.cy[nd] - cy);
+        double hs = hw + a.hw[nd];
+        bool t;
+        if (lvP == 0)  // build_p2p_lists form, galax-core/src/lib.rs:1380-1383
+          t = dx <= hs + 1e-15 && dy <= hs + 1e-15;
+        else  // InteractionLists::build form, galax-core/src/lib.rs:990-995
+          t = !(dx > hs + 1e-15 || dy > hs + 1e-15);
+        if (t) {
+          if (FILL) out[base + cnt] = nd;
+          ++cnt;
+        }
+      }
+      continue;
+    }
+    if (ux0[nd] > qx1 || ux1[nd] < qx0 || uy0[nd] > qy1 || uy1[nd] < qy0) continue;
+    uint32_t cf = a.cf[nd];
+    for (int c = (int)a.cc[nd] - 1; c >= 0; --c) {
+      if (sp < DFS_STACK)
+        stack[sp++] = cf + (uint32_t)c;
+      else
+        atomicOr(err, 1u);
+    }
+  }
+  if (!FILL) cnt_or_off[P] = cnt;
+}
+
+__device__ __forceinline__ bool touch_child(const NodeArrays& a, uint32_t child, uint32_t node) {
+  // galax-core/src/lib.rs:1000-1004
+  double cdx = fabs(a.cx[child] - a.cx[node]), cdy = fabs(a.cy[child] - a.cy[node]);
+  double hs = a.hw[node] + a.hw[child];
+  return cdx <= hs + 1e-15 && cdy <= hs + 1e-15;
+}
+
+// M2L list of node c: children of the parent's same-level neighbours (ascending
+// neighbour id, then child order) that do not touch c, capped at 64.
+template <bool FILL>
+__global__ void k_m2l_lists(NodeArrays a, uint32_t nn, const uint32_t* __restrict__ nb_off,
+                            const uint32_t* __restrict__ nb_dat, uint32_t* __restrict__ cnt_or_off,
+                            uint32_t* __restrict__ out) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nn) return;
+  uint32_t P = a.par[c];
+  uint32_t cnt = 0, base = FILL ? cnt_or_off[c] : 0u;
+  if (P != NONE) {
+    for (uint32_t k = nb_off[P]; k < nb_off[P + 1] && cnt < LIST_CAP; ++k) {
+      uint32_t Q = nb_dat[k];
+      uint32_t cf = a.cf[Q];
+      for (uint32_t q = cf; q < cf + a.cc[Q]; ++q) {
+        if (touch_child(a, q, c)) continue;
+        if (cnt < LIST_CAP) {
+          if (FILL) out[base + cnt] = q;
+          ++cnt;
+        }
+      }
+    }
+  }
+  if (!FILL) cnt_or_off[c] = cnt;
+}
+
+// P2P list of leaf a: a itself, then its touching leaves in ascending id, capped at 64.
+template <bool FILL>
+__global__ void k_p2p_lists(uint32_t n_leaves, uint32_t nn, const uint32_t* __restrict__ nb_off,
+                            const uint32_t* __restrict__ nb_dat, uint32_t* __restrict__ cnt_or_off,
+                            uint32_t* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  if (i >= n_leaves) {
+    if (!FILL) cnt_or_off[i] = 0;
+    return;
+  }
+  uint32_t nnb = nb_off[i + 1] - nb_off[i];
+  uint32_t cnt = 1 + (nnb < LIST_CAP - 1 ? nnb : LIST_CAP - 1);
+  if (!FILL) {
+    cnt_or_off[i] = cnt;
+    return;
+  }
+  uint32_t base = cnt_or_off[i];
+  out[base] = i;
+  for (uint32_t k = 1; k < cnt; ++k) out[base + k] = nb_dat[nb_off[i] + k - 1];
+}
+
+// Symmetric P2P gather lists from the reference lists: leaf t gathers from
+// itself, from b in list(t) with b > t, and from every a < t with t in list(a)
+// (galax-core/src/lib.rs:1415-1437 computes pair (a,b) iff b in list(a), a < b).
+__global__ void k_gather_count(uint32_t n_leaves, const uint32_t* __restrict__ off, const uint32_t* __restrict__ dat,
+                               uint32_t* __restrict__ gcnt) {
+  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n_leaves) return;
+  uint32_t own = 0;
+  for (uint32_t k = off[a]; k < off[a + 1]; ++k) {
+    uint32_t b = dat[k];
+    if (b > a) {
+      ++own;
+      atomicAdd(&gcnt[b], 1u);
+    }
+  }
+  atomicAdd(&gcnt[a], own + 1u);
+}
+
+__global__ void k_gather_fill(uint32_t n_leaves, const uint32_t* __restrict__ off, const uint32_t* __restrict__ dat,
+                              const uint32_t* __restrict__ goff, uint32_t* __restrict__ cursor,
+                              uint32_t* __restrict__ gdat) {
+  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n_leaves) return;
+  gdat[goff[a]] = a;
+  for (uint32_t k = off[a]; k < off[a + 1]; ++k) {
+    uint32_t b = dat[k];
+    if (b > a) {
+      uint32_t pa = atomicAdd(&cursor[a], 1u);
+      gdat[goff[a] + 1 + pa] = b;
+      uint32_t pb = atomicAdd(&cursor[b], 1u);
+      gdat[goff[b] + 1 + pb] = a;
+    }
+  }
+}
+
+__global__ void k_sort_segments(uint32_t n_seg, const uint32_t* __restrict__ off, uint32_t* __restrict__ dat,
+                                uint32_t skip_front) {
+  uint32_t sgi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sgi >= n_seg) return;
+  uint32_t s = off[sgi] + skip_front, e = off[sgi + 1];
+  for (uint32_t i = s + 1; i < e; ++i) {
+    uint32_t v = dat[i];
+    uint32_t j = i;
+    while (j > s && dat[j - 1] > v) {
+      dat[j] = dat[j - 1];
+      --j;
+    }
+    dat[j] = v;
+  }
+}
+
+// Directed body-body interactions of the gather lists (self pairs excluded).
+__global__ void k_count_interactions(uint32_t n_leaves, const uint32_t* __restrict__ goff,
+                                     const uint32_t* __restrict__ gdat, const uint32_t* __restrict__ bs,
+                                     const uint32_t* __restrict__ be, unsigned long long* __restrict__ total) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_leaves) return;
+  unsigned long long src = 0;
+  for (uint32_t g = goff[t]; g < goff[t + 1]; ++g) src += be[gdat[g]] - bs[gdat[g]];
+  unsigned long long nt = be[t] - bs[t];
+  atomicAdd(total, nt * src - nt);
+}
+
+int count_interactions(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, uint64_t* out) {
+  DevTree& t = h->tree;
+  hipStream_t s = h->stream;
+  GX_CHECK(dbuf_reserve(h->flag, 64, s));
+  unsigned long long* d = reinterpret_cast<unsigned long long*>(h->flag.as<uint32_t>() + 12);
+  GX_HIP(hipMemsetAsync(d, 0, 8, s));
+  k_count_interactions<<<nblk(t.n_leaves), 256, 0, s>>>((uint32_t)t.n_leaves, goff, gdat,
+                                                        t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), d);
+  GX_HIP(hipGetLastError());
+  unsigned long long v = 0;
+  GX_HIP(hipMemcpyAsync(&v, d, 8, hipMemcpyDeviceToHost, s));
+  GX_HIP(hipStreamSynchronize(s));
+  *out = v;
+  return 0;
+}
+
+static int build_gather(galax_fmm* h) {
+  DevTree& t = h->tree;
+  hipStream_t s = h->stream;
+  uint32_t nl = (uint32_t)t.n_leaves;
+  GX_CHECK(dbuf_reserve(t.gat_off, (nl + 1) * sizeof(uint32_t), s));
+  GX_CHECK(dbuf_reserve(h->ws.e, (nl + 1) * sizeof(uint32_t), s));
+  uint32_t* goff = t.gat_off.as<uint32_t>();
+  GX_HIP(hipMemsetAsync(goff, 0, (nl + 1) * sizeof(uint32_t), s));
+  k_gather_count<<<nblk(nl), 256, 0, s>>>(nl, t.p2p_off.as<uint32_t>(), t.p2p_dat.as<uint32_t>(), goff);
+  GX_HIP(hipGetLastError());
+  uint32_t total = 0;
+  GX_CHECK(scan_u32(h, goff, goff, nl + 1, false, false, &total));
+  t.gat_total = total;
+  GX_CHECK(dbuf_reserve(t.gat_dat, (size_t)total * sizeof(uint32_t) + 4, s));
+  uint32_t* cur = h->ws.e.as<uint32_t>();
+  GX_HIP(hipMemsetAsync(cur, 0, (nl + 1) * sizeof(uint32_t), s));
+  k_gather_fill<<<nblk(nl), 256, 0, s>>>(nl, t.p2p_off.as<uint32_t>(), t.p2p_dat.as<uint32_t>(), goff, cur,
+                                         t.gat_dat.as<uint32_t>());
+  GX_HIP(hipGetLastError());
+  k_sort_segments<<<nblk(nl), 256, 0, s>>>(nl, goff, t.gat_dat.as<uint32_t>(), 1);
+  GX_HIP(hipGetLastError());
+  GX_CHECK(count_interactions(h, goff, t.gat_dat.as<uint32_t>(), &t.p2p_interactions));
+  return 0;
+}
+
+int build_lists_reference(galax_fmm* h) {
+  DevTree& t = h->tree;
+  hipStream_t s = h->stream;
+  size_t nn = t.n_nodes;
+  if (nn == 0) {
+    t.m2l_total = t.p2p_total = t.gat_total = 0;
+    return 0;
+  }
+  GX_CHECK(compute_uboxes(h));
+  double* u = h->ws.h.as<double>();
+  NodeArrays a = node_arrays(h);
+  // same-level neighbours of every node
+  GX_CHECK(dbuf_reserve(h->ws.c, (nn + 1) * sizeof(uint32_t), s));
+  GX_CHECK(dbuf_reserve(h->flag, 64, s));
+  uint32_t* err = h->flag.as<uint32_t>();
+  GX_HIP(hipMemsetAsync(err, 0, sizeof(uint32_t), s));
+  uint32_t* nb_off = h->ws.c.as<uint32_t>();
+  k_same_level_nbrs<false><<<nblk(nn, 128), 128, 0, s>>>(a, 0, (uint32_t)nn, (uint32_t)(nn - 1), u, u + nn,
+                                                         u + 2 * nn, u + 3 * nn, nb_off, nullptr, err);
+  GX_HIP(hipGetLastError());
+  uint32_t nb_total = 0;
+  GX_CHECK(scan_u32(h, nb_off, nb_off, nn + 1, false, false, &nb_total));
+  GX_CHECK(dbuf_reserve(h->ws.d, (size_t)nb_total * sizeof(uint32_t) + 4, s));
+  uint32_t* nb_dat = h->ws.d.as<uint32_t>();
+  k_same_level_nbrs<true><<<nblk(nn, 128), 128, 0, s>>>(a, 0, (uint32_t)nn, (uint32_t)(nn - 1), u, u + nn,
+                                                        u + 2 * nn, u + 3 * nn, nb_off, nb_dat, err);
+  GX_HIP(hipGetLastError());
+  uint32_t herr = 0;
+  GX_HIP(hipMemcpyAsync(&herr, err, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  GX_HIP(hipStreamSynchronize(s));
+  if (herr) return set_error(GALAX_ERR_INVALID, "neighbour traversal stack overflow (tree too deep)");
+  // M2L lists
+  GX_CHECK(dbuf_reserve(t.m2l_off, (nn + 1) * sizeof(uint32_t), s));
+  uint32_t* moff = t.m2l_off.as<uint32_t>();
+  k_m2l_lists<false><<<nblk(nn), 256, 0, s>>>(a, (uint32_t)nn, nb_off, nb_dat, moff, nullptr);
+  GX_HIP(hipGetLastError());
+  uint32_t mt = 0;
+  GX_CHECK(scan_u32(h, moff, moff, nn + 1, false, false, &mt));
+  t.m2l_total = mt;
+  GX_CHECK(dbuf_reserve(t.m2l_dat, (size_t)mt * sizeof(uint32_t) + 4, s));
+  k_m2l_lists<true><<<nblk(nn), 256, 0, s>>>(a, (uint32_t)nn, nb_off, nb_dat, moff, t.m2l_dat.as<uint32_t>());
+  GX_HIP(hipGetLastError());
+  // P2P lists
+  GX_CHECK(dbuf_reserve(t.p2p_off, (nn + 1) * sizeof(uint32_t), s));
+  uint32_t* poff = t.p2p_off.as<uint32_t>();
+  k_p2p_lists<false><<<nblk(nn), 256, 0, s>>>((uint32_t)t.n_leaves, (uint32_t)nn, nb_off, nb_dat, poff, nullptr);
+  GX_HIP(hipGetLastError());
+  uint32_t pt = 0;
+  GX_CHECK(scan_u32(h, poff, poff, nn + 1, false, false, &pt));
+  t.p2p_total = pt;
+  GX_CHECK(dbuf_reserve(t.p2p_dat, (size_t)pt * sizeof(uint32_t) + 4, s));
+  k_p2p_lists<true><<<nblk(nn), 256, 0, s>>>((uint32_t)t.n_leaves, (uint32_t)nn, nb_off, nb_dat, poff,
+                                             t.p2p_dat.as<uint32_t>());
+  GX_HIP(hipGetLastError());
+  GX_CHECK(build_gather(h));
+  h->keys_valid = false;
+  return 0;
+}
+
+// --------------------------------------------------- host tree install ----
+
+int install_tree_host(galax_fmm* h, const galax_tree_view_t* tv, const galax_lists_view_t* m2l,
+                      const galax_lists_view_t* p2p) {
+  DevTree& t = h->tree;
+  hipStream_t s = h->stream;
+  size_t nn = tv->n_nodes;
+  if (m2l->n_nodes != nn || p2p->n_nodes != nn) return set_error(GALAX_ERR_INVALID, "list sizes != tree size");
+  std::vector<uint32_t> level(nn, 0), cf(nn, NONE), par(nn, NONE), bs(nn), be(nn);
+  std::vector<uint8_t> cc(nn, 0), lv8(nn);
+  size_t nl = 0, nb = 0;
+  for (size_t i = 0; i < nn; ++i) {
+    uint32_t mx = 0;
+    int k = 0;
+    for (; k < 4; ++k) {
+      int32_t c = tv->children[i * 4 + k];
+      if (c < 0) break;
+      if ((size_t)c >= i) return set_error(GALAX_ERR_INVALID, "child id %d >= parent id %zu", c, i);
+      if (k > 0 && c != tv->children[i * 4 + k - 1] + 1)
+        return set_error(GALAX_ERR_INVALID, "children of node %zu are not consecutive", i);
+      mx = std::max(mx, level[c] + 1);
+    }
+    for (int k2 = k; k2 < 4; ++k2)
+      if (tv->children[i * 4 + k2] >= 0) return set_error(GALAX_ERR_INVALID, "gap in children of node %zu", i);
+    if (k > 0) {
+      cf[i] = (uint32_t)tv->children[i * 4];
+      cc[i] = (uint8_t)k;
+    }
+    if ((k == 0) != (tv->is_leaf[i] != 0)) return set_error(GALAX_ERR_INVALID, "is_leaf inconsistent at %zu", i);
+    level[i] = mx;
+    if (mx > 255) return set_error(GALAX_ERR_INVALID, "tree deeper than 255 levels");
+    lv8[i] = (uint8_t)mx;
+    par[i] = tv->parent[i] < 0 ? NONE : (uint32_t)tv->parent[i];
+    bs[i] = (uint32_t)tv->body_start[i];
+    be[i] = (uint32_t)tv->body_end[i];
+    if (k == 0) {
+      ++nl;
+      nb = std::max<size_t>(nb, be[i]);
+    }
+  }
+  for (size_t i = 1; i < nn; ++i)
+    if (level[i] < level[i - 1]) return set_error(GALAX_ERR_INVALID, "node ids are not level-ordered");
+  for (size_t i = 0; i < nl; ++i)
+    if (level[i] != 0) return set_error(GALAX_ERR_INVALID, "leaves must come first");
+  GX_CHECK(ensure_node_capacity(h, nn));
+  auto up = [&](DBuf& d, const void* src, size_t bytes) -> int {
+    GX_HIP(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, s));
+    return 0;
+  };
+  GX_CHECK(up(t.center_x, tv->center_x, nn * 8));
+  GX_CHECK(up(t.center_y, tv->center_y, nn * 8));
+  GX_CHECK(up(t.half_width, tv->half_width, nn * 8));
+  GX_CHECK(up(t.mass, tv->mass, nn * 8));
+  GX_CHECK(up(t.com_x, tv->com_x, nn * 8));
+  GX_CHECK(up(t.com_y, tv->com_y, nn * 8));
+  GX_CHECK(up(t.body_start, bs.data(), nn * 4));
+  GX_CHECK(up(t.body_end, be.data(), nn * 4));
+  GX_CHECK(up(t.child_first, cf.data(), nn * 4));
+  GX_CHECK(up(t.parent, par.data(), nn * 4));
+  GX_CHECK(up(t.child_count, cc.data(), nn));
+  GX_CHECK(up(t.level, lv8.data(), nn));
+  t.n_nodes = nn;
+  t.n_leaves = nl;
+  t.n_bodies = nb;
+  t.n_levels = nn ? level[nn - 1] + 1 : 0;
+  t.level_offset.assign(t.n_levels + 1, (uint32_t)nn);
+  t.level_offset[0] = 0;
+  for (uint32_t l = 1; l <= t.n_levels; ++l) {
+    size_t i = t.level_offset[l - 1];
+    while (i < nn && level[i] < l) ++i;
+    t.level_offset[l] = (uint32_t)i;
+  }
+  auto up_lists = [&](const galax_lists_view_t* lv, DBuf& off, DBuf& dat, size_t& total) -> int {
+    std::vector<uint32_t> o(nn + 1);
+    for (size_t i = 0; i <= nn; ++i) o[i] = (uint32_t)lv->offsets[i];
+    total = lv->offsets[nn];
+    GX_CHECK(dbuf_reserve(off, (nn + 1) * 4, s));
+    GX_CHECK(dbuf_reserve(dat, total * 4 + 4, s));
+    GX_HIP(hipMemcpyAsync(off.p, o.data(), (nn + 1) * 4, hipMemcpyHostToDevice, s));
+    if (total) GX_HIP(hipMemcpyAsync(dat.p, lv->data, total * 4, hipMemcpyHostToDevice, s));
+    GX_HIP(hipStreamSynchronize(s));
+    return 0;
+  };
+  GX_CHECK(up_lists(m2l, t.m2l_off, t.m2l_dat, t.m2l_total));
+  GX_CHECK(up_lists(p2p, t.p2p_off, t.p2p_dat, t.p2p_total));
+  GX_CHECK(build_gather(h));
+  h->keys_valid = false;
+  return 0;
+}
+
+}  // namespace gx

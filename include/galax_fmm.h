@@ -1,0 +1,220 @@
+/* galax_fmm.h — C ABI of the MI355X (gfx950) FMM force backend.
+ *
+ * Drop-in for galax's force backends (knekvasil/galax):
+ *   galax-core  compute_fmm_force(&mut BodiesSoA, &Tree, &InteractionLists m2l,
+ *               &InteractionLists p2p, p, softening)   galax-core/src/lib.rs:1466-1499
+ *   galax-gpu   GpuContext::{new, upload_tree_data, step, read_accelerations,
+ *               crosscheck, n2_debug}                  galax-gpu/src/lib.rs:230-718
+ * Bodies in (structure of arrays, caller-owned), accelerations out, with the
+ * same p, n_max, softening and tree bounds. Potentials are an extra output.
+ *
+ * Conventions: every entry point returns 0 on success or a negative
+ * galax_status_t; nothing aborts or throws across the ABI. The text of the
+ * last error on the calling thread is galax_fmm_last_error(). A handle owns
+ * one HIP stream and its device buffers; calls on one handle are not
+ * reentrant. Host-array entry points are synchronous on return.
+ * Only C types appear here, so a Rust `extern "C"` block (repr(C) structs)
+ * binds it directly.
+ */
+#ifndef GALAX_FMM_H
+#define GALAX_FMM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(__GNUC__) || defined(__clang__)
+#define GALAX_API __attribute__((visibility("default")))
+#else
+#define GALAX_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes; the reference's GpuError variants (galax-gpu/src/lib.rs:42-59)
+ * and its panics (map failure :556, crosscheck threshold :663-665) map here. */
+typedef enum {
+  GALAX_OK = 0,
+  GALAX_ERR_INIT = -1,        /* GpuError::InitFailed: no HIP device / stream */
+  GALAX_ERR_COMPILE = -2,     /* GpuError::ShaderCompilation: no gfx950 code object */
+  GALAX_ERR_ALLOC = -3,       /* GpuError::BufferAllocation */
+  GALAX_ERR_INVALID = -4,     /* bad argument, size above capacity, no tree yet */
+  GALAX_ERR_DEVICE = -5,      /* a HIP call or kernel failed */
+  GALAX_ERR_UNSUPPORTED = -6, /* a combination this build does not provide */
+  GALAX_ERR_THRESHOLD = -7    /* crosscheck error above its threshold */
+} galax_status_t;
+
+/* Algorithm variants (same tree, same kernels). */
+enum {
+  /* galax-core's semantics: 64-capped id-ordered lists, P2P pair rule
+   * `partner > node`, M2L derivative cache keyed by (dx as i64, dy as i64)
+   * with sequential first-touch representatives, ascending-id L2L. */
+  GALAX_ALGO_REFERENCE = 0,
+  /* dual-tree-traversal lists with acceptance test R_A+R_B < theta*d, exact
+   * per-pair derivatives, top-down L2L (no counterpart in the reference). */
+  GALAX_ALGO_CORRECTED = 1
+};
+
+enum {
+  GALAX_TREE_PLAIN = 0,      /* Tree::build: stable Morton sort + n_max chunks (lib.rs:163-361) */
+  GALAX_TREE_CONSTRAINED = 1 /* build_constrained on the current order (lib.rs:541-588) */
+};
+
+enum {
+  GALAX_PREC_FP64 = 0,       /* everything in FP64 */
+  GALAX_PREC_MIXED_FP32 = 1, /* FP32 expansions, FP32 MFMA M2L, FP32 P2P on leaf-relative coordinates */
+  GALAX_PREC_MIXED_BF16X3 = 2 /* as MIXED_FP32 with M2L as a 3-term split BF16 MFMA product */
+};
+
+/* Mirrors GpuConfig (galax-gpu/src/lib.rs:22-40) plus the tree bounds that
+ * galax-cli hard-codes (galax-cli/src/main.rs:159) and the variant selectors. */
+typedef struct {
+  uint32_t p;          /* expansion order, 0..16 (MAX_P = 17, lib.rs:6) */
+  uint32_t n_max;      /* bodies per leaf */
+  double softening;    /* eps in 1/sqrt(r^2+eps^2) */
+  double x_min, x_max, y_min, y_max; /* Morton bounds */
+  uint32_t tree_mode;  /* GALAX_TREE_* */
+  uint32_t algo;       /* GALAX_ALGO_* */
+  uint32_t precision;  /* GALAX_PREC_* */
+  double theta;        /* acceptance parameter of GALAX_ALGO_CORRECTED (galax-cli --theta) */
+} galax_fmm_config_t;
+
+/* Caller-owned structure of arrays; mirrors BodiesSoA (lib.rs:41-49) plus phi.
+ * Pointers not needed by an entry point may be NULL (phi == NULL: not written). */
+typedef struct {
+  double *x, *y, *vx, *vy, *m, *ax, *ay, *phi;
+  size_t n;
+} galax_bodies_t;
+
+/* Caller-owned view of a tree in galax-core's node order (Node, lib.rs:117-129).
+ * Arrays have n_nodes entries (children: 4 per node, -1 = None; parent -1 = None). */
+typedef struct {
+  size_t n_nodes;
+  double *center_x, *center_y, *half_width, *mass, *com_x, *com_y;
+  uint64_t *body_start, *body_end;
+  int32_t *children;
+  int32_t *parent;
+  uint8_t *is_leaf;
+} galax_tree_view_t;
+
+/* Interaction lists as CSR: partners of node i are data[offsets[i] .. offsets[i+1]),
+ * in the reference's order (InteractionLists{data:[u32;64], lengths:u8}, lib.rs:941-944). */
+typedef struct {
+  size_t n_nodes;
+  uint64_t *offsets; /* n_nodes + 1 */
+  uint32_t *data;    /* offsets[n_nodes] */
+} galax_lists_view_t;
+
+/* Energy / momentum diagnostics (galax-integrate/src/lib.rs:192-199). potential
+ * is the true -1/2 sum m_i phi_i; potential_heuristic is galax-core's
+ * compute_fmm_potential_energy (lib.rs:1671-1683). */
+typedef struct {
+  uint64_t step;
+  double kinetic, potential, potential_heuristic, momentum_x, momentum_y, angular_momentum;
+} galax_diag_t;
+
+typedef struct galax_fmm galax_fmm_t;
+
+/* Per-pass device times of the last evaluate, milliseconds (HIP events on the
+ * handle's stream). */
+typedef struct {
+  double sanitize, p2m, m2m, m2l_setup, m2l, l2l, l2p_p2p, total;
+  uint64_t m2l_pairs_executed, m2l_pairs_reference, m2l_keys, p2p_interactions;
+} galax_timings_t;
+
+/* ~GpuContext::new (galax-gpu/src/lib.rs:268-390): validates the config,
+ * selects the device current on the calling thread, creates the stream and
+ * preallocates body buffers for max_n bodies. */
+GALAX_API int galax_fmm_create(const galax_fmm_config_t* cfg, size_t max_n, galax_fmm_t** out);
+
+/* ~Tree::build + InteractionLists::build + build_p2p_lists
+ * (galax-core/src/lib.rs:163-361, 947-1025, 1358-1392; galax-cli/src/main.rs:159-193):
+ * builds the tree and both lists on the device. Permutes all non-NULL arrays of
+ * `bodies` in place exactly as Tree::build does (stable Morton order; no sort in
+ * GALAX_TREE_CONSTRAINED). perm_out (optional, n entries): new[i] = old[perm_out[i]].
+ * The tree is then static, as in the reference (built once, galax-cli/src/main.rs:159). */
+GALAX_API int galax_fmm_build_tree(galax_fmm_t* h, galax_bodies_t* bodies, uint64_t* perm_out);
+
+/* As galax_fmm_build_tree with device-resident, already-allocated arrays of
+ * n bodies (x, y, m required; vx, vy, ax, ay optional) permuted in place. */
+GALAX_API int galax_fmm_build_tree_device(galax_fmm_t* h, double* d_x, double* d_y, double* d_vx, double* d_vy, double* d_m,
+                                double* d_ax, double* d_ay, size_t n);
+
+/* ~GpuContext::upload_tree_data (galax-gpu/src/lib.rs:392-462): installs a tree
+ * and its two lists built elsewhere (e.g. by galax-core). Node ids must be
+ * level-ordered with consecutive children, as Tree::build produces. */
+GALAX_API int galax_fmm_upload_tree(galax_fmm_t* h, const galax_tree_view_t* tree, const galax_lists_view_t* m2l,
+                          const galax_lists_view_t* p2p);
+
+/* Changes the softening of an existing context (galax-cli derives its default
+ * from the Morton-sorted bodies after Tree::build, galax-cli/src/main.rs:182-190). */
+GALAX_API int galax_fmm_set_softening(galax_fmm_t* h, double softening);
+
+/* Sizes of the installed tree: nodes, leaves, levels, M2L and P2P list lengths. */
+GALAX_API int galax_fmm_tree_info(galax_fmm_t* h, size_t* n_nodes, size_t* n_leaves, uint32_t* n_levels, size_t* m2l_total,
+                        size_t* p2p_total);
+
+/* Copies the installed tree and lists out (for bit-exactness checks).
+ * Views must be sized from galax_fmm_tree_info; any view may be NULL. */
+GALAX_API int galax_fmm_export_tree(galax_fmm_t* h, galax_tree_view_t* tree, galax_lists_view_t* m2l,
+                          galax_lists_view_t* p2p);
+
+/* ~compute_fmm_force (galax-core/src/lib.rs:1466-1499) and
+ * GpuContext::step + read_accelerations (galax-gpu/src/lib.rs:464-566):
+ * reads x, y, m in the tree's body order, sanitizes them like :1475-1482
+ * (writing the sanitized values back to the caller's arrays when any changed),
+ * writes ax, ay and, when non-NULL, phi. Synchronous. */
+GALAX_API int galax_fmm_evaluate(galax_fmm_t* h, galax_bodies_t* bodies);
+
+/* As galax_fmm_evaluate on device-resident arrays of the installed tree's n
+ * bodies, enqueued on the handle's stream and synchronised before return.
+ * d_phi may be NULL. Inputs are sanitized in place. */
+GALAX_API int galax_fmm_evaluate_device(galax_fmm_t* h, double* d_x, double* d_y, double* d_m, double* d_ax, double* d_ay,
+                              double* d_phi);
+
+/* Device times and work counts of the last evaluate. */
+GALAX_API int galax_fmm_last_timings(galax_fmm_t* h, galax_timings_t* out);
+
+/* ~galax-integrate simulate / leapfrog_kdk_gpu (galax-integrate/src/lib.rs:158-237,
+ * 292-368): n_steps kick-drift-kick steps on the device (positions and
+ * velocities in FP64, tree static). ax, ay must hold the current
+ * accelerations on entry and hold the final ones on return. When diag != NULL
+ * and every > 0, diagnostics after every `every` steps are written to
+ * diag[0 .. n_steps/every). */
+GALAX_API int galax_fmm_kdk(galax_fmm_t* h, galax_bodies_t* bodies, double dt, uint64_t n_steps, uint64_t every,
+                  galax_diag_t* diag);
+
+/* Vector-Jacobian product of the accelerations: given cotangents ax_bar, ay_bar
+ * (tree body order), returns x_bar, y_bar, m_bar = J^T (ax_bar, ay_bar) for
+ * the map (x, y, m) -> (ax, ay) with the tree topology and lists frozen
+ * (no reference counterpart; for checking through torch.autograd). */
+GALAX_API int galax_fmm_vjp(galax_fmm_t* h, const galax_bodies_t* bodies, const double* ax_bar, const double* ay_bar,
+                  double* x_bar, double* y_bar, double* m_bar);
+
+/* ~compute_n2_force (galax-core/src/lib.rs:1685-1721) / GpuContext::n2_debug
+ * (galax-gpu/src/lib.rs:670-714) in FP64 on the device, for the targets listed
+ * in targets (all bodies when NULL). */
+GALAX_API int galax_direct_sum(const double* x, const double* y, const double* m, size_t n, double softening,
+                     const uint64_t* targets, size_t n_targets, double* ax, double* ay, double* phi);
+
+/* ~validate_fmm (galax-core/src/lib.rs:1503-1564) with caller-chosen sample
+ * indices: evaluates the FMM, computes the direct sum on the device for the
+ * samples and returns (max, mean) of | |a_fmm| - |a_exact| | / max(|a_exact|, 1e-20)
+ * plus the RMS of the vector error |a_fmm - a_exact| / |a_exact|. */
+GALAX_API int galax_fmm_validate(galax_fmm_t* h, galax_bodies_t* bodies, const uint64_t* samples, size_t n_samples,
+                       double* max_rel, double* mean_rel, double* rms_vec);
+
+GALAX_API void galax_fmm_destroy(galax_fmm_t* h);
+
+/* Text of the last error raised on the calling thread ("" if none). */
+GALAX_API const char* galax_fmm_last_error(void);
+
+/* Library version string. */
+GALAX_API const char* galax_fmm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GALAX_FMM_H */

@@ -1,0 +1,263 @@
+"""Parity of the HIP path (libgalax_fmm.so through its C ABI) with the CPU
+oracle, on the GPU: bit-exact tree, permutation and lists; forces within the
+north-star tolerance (median relative difference <= 1e-6 in FP64); the
+reference's known-answer tests; edge cases."""
+import numpy as np
+import pytest
+
+from galax_amd import fmm, ic
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TREE_FIELDS = ("center_x", "center_y", "half_width", "mass", "com_x", "com_y", "body_start", "body_end",
+               "children", "parent", "is_leaf")
+
+
+def _bodies(d):
+    return fmm.BodiesSoA.from_dict(d)
+
+
+def _ctx(n, p=4, n_max=32, eps=0.1, bounds=(0.0, 100.0, 0.0, 100.0), **kw):
+    return fmm.GpuContext(fmm.GpuConfig(max_n=max(n, 1), p=p, eps=eps, n_max=n_max, bounds=bounds, **kw))
+
+
+def _gpu_build(d, **kw):
+    n = len(d["x"])
+    ctx = _ctx(n, **kw)
+    b = _bodies(d)
+    perm = ctx.build_tree(b)
+    return ctx, b, perm
+
+
+def _oracle_build(d, n_max, bounds, constrained=False):
+    od = {k: np.array(d[k], np.float64) for k in ("x", "y", "vx", "vy", "m", "ax", "ay")}
+    if constrained:
+        t = O.build_constrained(od, n_max, bounds)
+        perm = np.arange(len(od["x"]), dtype=np.uint64)
+    else:
+        t, perm = O.tree_build(od, n_max, bounds, return_perm=True)
+    return t, od, perm
+
+
+def _assert_tree_equal(gt, ot):
+    for k in TREE_FIELDS:
+        a, b = np.asarray(gt[k]), np.asarray(ot[k])
+        assert a.shape == b.shape, k
+        if a.dtype == np.float64:
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), f"{k} differs at {np.nonzero(a != b)[0][:5]}"
+        else:
+            assert np.array_equal(a.astype(np.int64), b.astype(np.int64)), f"{k} differs"
+
+
+def _assert_lists_equal(g_csr, o_lists):
+    go, gd = g_csr
+    oo, od = o_lists.csr()
+    assert np.array_equal(go.astype(np.int64), oo.astype(np.int64))
+    assert np.array_equal(gd.astype(np.int64), od.astype(np.int64))
+
+
+CASES = [
+    ("uniform", lambda: ic.uniform(6000, seed=101), 32, (0.0, 100.0, 0.0, 100.0)),
+    ("plummer", lambda: ic.plummer(8000, seed=102), 32, (-50.0, 50.0, -50.0, 50.0)),
+    ("clamped_dupes", lambda: ic.uniform(5000, seed=103), 32, (-50.0, 50.0, -50.0, 50.0)),  # 75% clamp (CLI bounds)
+    ("galaxy", lambda: ic.galaxy(4096, seed=104), 32, (-50.0, 50.0, -50.0, 50.0)),
+    ("tiny_leaves", lambda: ic.uniform(700, seed=105), 1, (0.0, 100.0, 0.0, 100.0)),
+    ("n_max64", lambda: ic.clustered(8192, seed=106, n_clusters=8), 64, "data"),
+]
+
+
+def _bounds(d, bounds):
+    if bounds == "data":
+        return (float(d["x"].min()), float(d["x"].max()), float(d["y"].min()), float(d["y"].max()))
+    return bounds
+
+
+@pytest.mark.parametrize("name,gen,n_max,bounds", CASES, ids=[c[0] for c in CASES])
+def test_tree_and_lists_bit_exact(name, gen, n_max, bounds):
+    d = gen()
+    bounds = _bounds(d, bounds)
+    ctx, b, perm = _gpu_build(d, n_max=n_max, bounds=bounds)
+    ot, od, operm = _oracle_build(d, n_max, bounds)
+    assert np.array_equal(perm, operm)
+    assert np.array_equal(b.x.view(np.uint64), od["x"].view(np.uint64))
+    assert np.array_equal(b.mass, od["m"])
+    gt, gm2l, gp2p = ctx.export_tree()
+    _assert_tree_equal(gt, ot.export())
+    _assert_lists_equal(gm2l, O.m2l_lists(ot))
+    _assert_lists_equal(gp2p, O.p2p_lists(ot))
+
+
+def test_tree_bit_exact_full_size_plummer1m():
+    """Tree and permutation at the 2^20-body Plummer workload (lists are
+    checked at the smaller sizes above: the oracle's list builder is O(n^2))."""
+    d = ic.plummer(1 << 20)
+    bounds = (-50.0, 50.0, -50.0, 50.0)
+    ctx, b, perm = _gpu_build(d, n_max=32, bounds=bounds)
+    ot, od, operm = _oracle_build(d, 32, bounds)
+    assert np.array_equal(perm, operm)
+    gt, _, _ = ctx.export_tree()
+    _assert_tree_equal(gt, ot.export())
+
+
+def test_constrained_tree_bit_exact():
+    d = ic.reference_test_bodies(500)
+    bounds = (0.0, 100.0, 0.0, 100.0)
+    ctx, b, perm = _gpu_build(d, n_max=16, bounds=bounds, tree_mode=fmm.TREE_CONSTRAINED)
+    ot, od, _ = _oracle_build(d, 16, bounds, constrained=True)
+    gt, gm2l, gp2p = ctx.export_tree()
+    _assert_tree_equal(gt, ot.export())
+    _assert_lists_equal(gm2l, O.m2l_lists(ot))
+    _assert_lists_equal(gp2p, O.p2p_lists(ot))
+
+
+def _rel_diff(a, b):
+    den = np.maximum(np.hypot(b[0], b[1]), 1e-300)
+    return np.hypot(a[0] - b[0], a[1] - b[1]) / den
+
+
+@pytest.mark.parametrize("p", [0, 2, 4, 6, 8])
+def test_forces_match_oracle_reference(p):
+    d = ic.plummer(12000, seed=201)
+    bounds = (-50.0, 50.0, -50.0, 50.0)
+    eps = 0.02
+    ctx, b, _ = _gpu_build(d, p=p, eps=eps, bounds=bounds)
+    ctx.evaluate(b)
+    ot, od, _ = _oracle_build(d, 32, bounds)
+    ax, ay, phi = O.compute_fmm_force(od["x"], od["y"], od["m"], ot, O.m2l_lists(ot), O.p2p_lists(ot), p, eps,
+                                      want_phi=True)
+    rd = _rel_diff((b.ax, b.ay), (ax, ay))
+    assert np.median(rd) <= 1e-6, np.median(rd)
+    assert np.max(rd) <= 1e-9, np.max(rd)
+    np.testing.assert_allclose(b.phi, phi, rtol=1e-9, atol=1e-12 * np.abs(phi).max())
+
+
+def test_uniform10k_forces_and_validate():
+    """The 10,000-body uniform workload at p=4: forces vs the oracle, and the
+    --validate metric vs the oracle's on the same samples within 5%."""
+    d = ic.uniform(10_000)
+    bounds = (0.0, 100.0, 0.0, 100.0)
+    ot, od, _ = _oracle_build(d, 32, bounds)
+    eps = O.default_softening(od["x"], od["y"])
+    ctx, b, _ = _gpu_build(d, p=4, eps=eps, bounds=bounds)
+    ctx.evaluate(b)
+    m2l, p2p = O.m2l_lists(ot), O.p2p_lists(ot)
+    ax, ay, _ = O.compute_fmm_force(od["x"], od["y"], od["m"], ot, m2l, p2p, 4, eps)
+    rd = _rel_diff((b.ax, b.ay), (ax, ay))
+    assert np.median(rd) <= 1e-6
+    samples = np.arange(0, 10_000, 10, dtype=np.uint64)
+    mx, mean, _ = ctx.validate(b, samples)
+    ex, ey, _ = O.n2_force(od["x"], od["y"], od["m"], eps, targets=samples)
+    fa = np.hypot(ax[samples.astype(int)], ay[samples.astype(int)])
+    ea = np.hypot(ex, ey)
+    rel = np.abs(fa - ea) / np.maximum(ea, 1e-20)
+    assert abs(mx - rel.max()) <= 0.05 * rel.max()
+    assert abs(mean - rel.mean()) <= 0.05 * rel.mean()
+
+
+def test_p0_equals_p2p_only():  # galax-core/src/lib.rs:2320-2357
+    d = ic.reference_test_bodies(200)
+    bounds = (0.0, 100.0, 0.0, 100.0)
+    ctx, b, _ = _gpu_build(d, p=0, n_max=16, eps=0.1, bounds=bounds)
+    ctx.evaluate(b)
+    ot, od, _ = _oracle_build(d, 16, bounds)
+    px, py = O.p2p(od["x"], od["y"], od["m"], ot, O.p2p_lists(ot), 0.1)
+    assert np.max(np.abs(b.ax - px)) < 1e-14 and np.max(np.abs(b.ay - py)) < 1e-14
+
+
+def test_upload_tree_matches_device_build():
+    d = ic.plummer(5000, seed=202)
+    bounds = (-50.0, 50.0, -50.0, 50.0)
+    ot, od, _ = _oracle_build(d, 32, bounds)
+    ctx = _ctx(5000, p=6, eps=0.05, bounds=bounds)
+    ctx.upload_tree_data(ot.export(), O.m2l_lists(ot).csr(), O.p2p_lists(ot).csr())
+    b = _bodies(od)
+    ctx.evaluate(b)
+    ax, ay, _ = O.compute_fmm_force(od["x"], od["y"], od["m"], ot, O.m2l_lists(ot), O.p2p_lists(ot), 6, 0.05)
+    assert np.median(_rel_diff((b.ax, b.ay), (ax, ay))) <= 1e-6
+
+
+def test_corrected_variant_matches_oracle_and_direct_sum():
+    d = ic.plummer(20000, seed=203)
+    bounds = (-50.0, 50.0, -50.0, 50.0)
+    eps, theta, p = 0.05, 0.3, 6
+    ctx, b, _ = _gpu_build(d, p=p, eps=eps, bounds=bounds, algo=fmm.ALGO_CORRECTED, theta=theta)
+    ctx.evaluate(b)
+    ot, od, _ = _oracle_build(d, 32, bounds)
+    m2l, p2p = O.corrected_lists(ot, od["x"], od["y"], theta)
+    ax, ay, _ = O.compute_fmm_force(od["x"], od["y"], od["m"], ot, m2l, p2p, p, eps, algo=1)
+    assert np.median(_rel_diff((b.ax, b.ay), (ax, ay))) <= 1e-6
+    ex, ey, _ = fmm.compute_n2_force(b.x, b.y, b.mass, eps)
+    rms = np.sqrt(np.mean(_rel_diff((b.ax, b.ay), (ex, ey)) ** 2))
+    assert rms < 2e-5, rms
+
+
+def test_direct_sum_matches_oracle():
+    d = ic.uniform(3000, seed=204)
+    ax, ay, phi = fmm.compute_n2_force(d["x"], d["y"], d["m"], 0.1, want_phi=True)
+    ox, oy, oph = O.n2_force(d["x"], d["y"], d["m"], 0.1, want_phi=True)
+    np.testing.assert_allclose(ax, ox, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(ay, oy, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(phi, oph, rtol=1e-12)
+
+
+def test_direct_sum_kats():  # galax-core/src/lib.rs:2841-2908
+    ax, ay, _ = fmm.compute_n2_force([0.0, 1.0], [0.0, 0.0], [1.0, 1.0], 0.1)
+    e = 1.0 / (1.0 + 0.01) ** 1.5
+    assert abs(ax[0] - e) < 1e-14 and abs(ax[1] + e) < 1e-14 and ay[0] == 0.0
+    ax, _, _ = fmm.compute_n2_force([0.0, 0.0], [0.0, 0.0], [1.0, 1e6], 0.1)
+    assert ax[0] == 0.0 and ax[1] == 0.0
+
+
+def test_sanitize_mutates_like_reference():
+    d = ic.uniform(400, seed=205, hi=10.0)
+    bounds = (0.0, 10.0, 0.0, 10.0)
+    ctx, b, _ = _gpu_build(d, p=2, n_max=8, bounds=bounds)
+    b.x[3], b.y[5], b.mass[7], b.mass[9] = np.nan, 1e101, -1.0, np.inf
+    ctx.evaluate(b)
+    assert b.x[3] == 0.0 and b.y[3] == 0.0 and b.x[5] == 0.0 and b.mass[7] == 1.0 and b.mass[9] == 1.0
+    assert np.isfinite(b.ax).all() and np.isfinite(b.ay).all()
+
+
+@pytest.mark.parametrize("n", [1, 2, 31, 33])
+def test_small_n(n):
+    d = ic.uniform(n, seed=206)
+    ctx, b, _ = _gpu_build(d, p=4, bounds=(0.0, 100.0, 0.0, 100.0))
+    ctx.evaluate(b)
+    ex, ey, _ = fmm.compute_n2_force(b.x, b.y, b.mass, 0.1)
+    if n <= 32:  # one leaf: P2P is the whole direct sum
+        np.testing.assert_allclose(b.ax, ex, rtol=1e-12, atol=1e-15)
+    assert np.isfinite(b.ax).all()
+
+
+def test_kdk_matches_oracle_simulate():
+    d = ic.plummer(4000, seed=207)
+    bounds = (-50.0, 50.0, -50.0, 50.0)
+    eps, p = 0.05, 4
+    ctx, b, _ = _gpu_build(d, p=p, eps=eps, bounds=bounds)
+    ctx.evaluate(b)
+    ot, od, _ = _oracle_build(d, 32, bounds)
+    m2l, p2p = O.m2l_lists(ot), O.p2p_lists(ot)
+    ax, ay, _ = O.compute_fmm_force(od["x"], od["y"], od["m"], ot, m2l, p2p, p, eps)
+    od["ax"][:], od["ay"][:] = ax, ay
+    dt = 0.5 / np.sqrt(np.max(np.hypot(b.ax, b.ay)))
+    diags = ctx.simulate(b, dt, 10, energy_every=5)
+    O.simulate(od, ot, m2l, p2p, p, eps, dt, 10)
+    rd = np.hypot(b.x - od["x"], b.y - od["y"]) / np.maximum(np.hypot(od["x"], od["y"]), 1e-12)
+    assert np.median(rd) < 1e-9
+    assert len(diags) == 2
+    dg = O.diagnostics(od["x"], od["y"], od["vx"], od["vy"], od["m"], od["ax"], od["ay"])
+    assert diags[-1]["kinetic"] == pytest.approx(dg["kinetic"], rel=1e-6)
+    assert diags[-1]["potential_heuristic"] == pytest.approx(dg["potential"], rel=1e-6)
+
+
+def test_errors_are_codes_not_crashes():
+    with pytest.raises(fmm.GalaxError):
+        fmm.GpuContext(fmm.GpuConfig(max_n=10, p=17))
+    ctx = _ctx(10)
+    b = _bodies(ic.uniform(10))
+    with pytest.raises(fmm.GalaxError):
+        ctx.evaluate(b)  # no tree yet
+    big = _bodies(ic.uniform(11))
+    with pytest.raises(fmm.GalaxError):
+        ctx.build_tree(big)  # above max_n
