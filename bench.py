@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 METRIC = "N=1M FMM forces/s: FP32-MFMA vs FP64 GPU at ≤1e-5 RMS force err; VJP/fwd time"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
+FP32_PEAK_TFS = 157.3
 
 
 def _num_moments(p):
@@ -43,6 +44,8 @@ def main():
     ap.add_argument("--err-samples", type=int, default=10_000)
     ap.add_argument("--cpu-sample-n", type=int, default=1 << 18)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--form", default="mixed_fp32", choices=["fp64", "mixed_fp32", "mixed_bf16x3"],
+                    help="the form whose throughput is `value`")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         sys.exit("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)")
@@ -56,75 +59,93 @@ def main():
     d, spec = ic.make(args.workload)
     n = len(d["x"])
     algo = fmm.ALGO_CORRECTED if args.algo == "corrected" else fmm.ALGO_REFERENCE
-    cfg = fmm.GpuConfig(max_n=n, p=args.p, eps=0.1, n_max=spec["n_max"], bounds=spec["bounds"], algo=algo,
-                        theta=args.theta)
-    ctx = fmm.GpuContext(cfg)
-    b = fmm.BodiesSoA.from_dict(d)
-    log(f"[bench] workload={args.workload} N={n} p={args.p} algo={args.algo}: building tree")
-    t0 = time.perf_counter()
-    ctx.build_tree(b)
-    t_build = time.perf_counter() - t0
-    info = ctx.tree_info()
-    eps = fmm.default_softening(b.x, b.y)  # galax-cli/src/main.rs:182-190, on the sorted bodies
-    ctx.set_softening(eps)
-    log(f"[bench] tree {info} in {t_build:.3f}s, softening={eps:.6g}")
+    forms = [("fp64", fmm.PREC_FP64), ("mixed_fp32", fmm.PREC_MIXED_FP32),
+             ("mixed_bf16x3", fmm.PREC_MIXED_BF16X3)]
+    headline = args.form
+    res = {}
+    eps = None
+    for name, prec in forms:
+        cfg = fmm.GpuConfig(max_n=n, p=args.p, eps=0.1, n_max=spec["n_max"], bounds=spec["bounds"], algo=algo,
+                            theta=args.theta, precision=prec)
+        ctx = fmm.GpuContext(cfg)
+        b = fmm.BodiesSoA.from_dict(d)
+        log(f"[bench] {name}: workload={args.workload} N={n} p={args.p} algo={args.algo}: building tree")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.build_tree(b)
+        t_build = time.perf_counter() - t0
+        info = ctx.tree_info()
+        if eps is None:
+            eps = fmm.default_softening(b.x, b.y)  # galax-cli/src/main.rs:182-190, on the sorted bodies
+        ctx.set_softening(eps)
+        x = torch.from_numpy(b.x).to(dev)
+        y = torch.from_numpy(b.y).to(dev)
+        m = torch.from_numpy(b.mass).to(dev)
+        ax, ay, phi = torch.empty_like(x), torch.empty_like(x), torch.empty_like(x)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.evaluate_device(x, y, m, ax, ay, phi)  # the first call also builds the static M2L key tables
+        torch.cuda.synchronize()
+        t_first = time.perf_counter() - t0
+        for _ in range(args.warmup):
+            ctx.evaluate_device(x, y, m, ax, ay, phi)
+        torch.cuda.synchronize()
+        passes = {}
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.evaluate_device(x, y, m, ax, ay, phi)
+            tm = ctx.timings()
+            for k, v in tm.passes_ms.items():
+                passes[k] = passes.get(k, 0.0) + v
+        torch.cuda.synchronize()
+        t_total = time.perf_counter() - t0
+        ms = 1e3 * t_total / args.steps
+        passes = {k: v / args.steps for k, v in passes.items()}
+        log(f"[bench] {name}: {ms:.3f} ms/step (build {t_build:.3f}s, first eval {t_first:.3f}s); passes {passes}")
+        res[name] = dict(ms=ms, passes=passes, t_build=t_build, t_first=t_first, tm=tm, info=info,
+                         ax=ax.cpu().numpy(), ay=ay.cpu().numpy(), bodies=b)
+        del ctx, x, y, m, ax, ay, phi
+        torch.cuda.empty_cache()
 
-    x = torch.from_numpy(b.x).to(dev)
-    y = torch.from_numpy(b.y).to(dev)
-    m = torch.from_numpy(b.mass).to(dev)
-    ax = torch.empty_like(x)
-    ay = torch.empty_like(x)
-    phi = torch.empty_like(x)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ctx.evaluate_device(x, y, m, ax, ay, phi)  # first call also builds the static M2L key tables
-    torch.cuda.synchronize()
-    t_first = time.perf_counter() - t0
-    for _ in range(args.warmup):
-        ctx.evaluate_device(x, y, m, ax, ay, phi)
-    torch.cuda.synchronize()
-    passes = {}
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.evaluate_device(x, y, m, ax, ay, phi)
-        tm = ctx.timings()
-        for k, v in tm.passes_ms.items():
-            passes[k] = passes.get(k, 0.0) + v
-    torch.cuda.synchronize()
-    t_total = time.perf_counter() - t0
-    ms_per_step = 1e3 * t_total / args.steps
-    passes = {k: v / args.steps for k, v in passes.items()}
-    log(f"[bench] {ms_per_step:.3f} ms/step; passes (ms) {passes}")
-
-    # error vs the FP64 direct sum on sampled targets (device direct sum)
-    axh, ayh = ax.cpu().numpy(), ay.cpu().numpy()
+    b = res["fp64"]["bodies"]
     rs = ic.SplitMix64(4242)
     samples = np.unique((rs.next_u64(args.err_samples) % np.uint64(n)).astype(np.uint64))
-    ex, ey, _ = fmm.compute_n2_force(b.x, b.y, b.mass, eps, targets=samples)
     si = samples.astype(np.int64)
-    mag = np.maximum(np.hypot(ex, ey), 1e-300)
-    vec = np.hypot(axh[si] - ex, ayh[si] - ey) / mag
-    magerr = np.abs(np.hypot(axh[si], ayh[si]) - np.hypot(ex, ey)) / np.maximum(np.hypot(ex, ey), 1e-20)
+    ex, ey, _ = fmm.compute_n2_force(b.x, b.y, b.mass, eps, targets=samples)
+    emag = np.hypot(ex, ey)
+    ref_ax, ref_ay = res["fp64"]["ax"], res["fp64"]["ay"]
+    ref_mag = np.maximum(np.hypot(ref_ax, ref_ay), 1e-300)
+    errs = {}
+    for name, r in res.items():
+        vec = np.hypot(r["ax"][si] - ex, r["ay"][si] - ey) / np.maximum(emag, 1e-300)
+        magerr = np.abs(np.hypot(r["ax"][si], r["ay"][si]) - emag) / np.maximum(emag, 1e-20)
+        e = {"rms_vec_vs_direct": float(np.sqrt(np.mean(vec ** 2))), "max_vec_vs_direct": float(vec.max()),
+             "validate_max_mag": float(magerr.max()), "validate_mean_mag": float(magerr.mean())}
+        if name != "fp64":
+            d64 = np.hypot(r["ax"] - ref_ax, r["ay"] - ref_ay) / ref_mag
+            e["rms_vs_fp64"] = float(np.sqrt(np.mean(d64 ** 2)))
+            e["max_vs_fp64"] = float(d64.max())
+            e["speedup_vs_fp64"] = res["fp64"]["ms"] / r["ms"]
+        errs[name] = e
+    log(f"[bench] errors {errs}")
 
-    # roofline of the dominant kernel (algorithmic flops / bytes per launch)
+    # roofline of the headline form's dominant kernel (algorithmic flops or bytes per launch)
+    H = res[headline]
+    passes, tm, info = H["passes"], H["tm"], H["info"]
     S = _num_moments(args.p)
     I = tm.p2p_interactions
     n_leaves = info["n_leaves"]
     dominant = max(("l2p_p2p", "m2l", "p2m", "m2m", "l2l"), key=lambda k: passes.get(k, 0.0))
-    flops = {
-        "l2p_p2p": 17.0 * I + 4.0 * S * n,  # near field incl. potential + local evaluation
-        "m2l": 2.0 * S * S * tm.m2l_pairs_executed,
-    }
-    bytes_ = {
-        "p2m": 24.0 * n + n_leaves * S * 8.0,
-        "m2m": 2.0 * info["n_nodes"] * S * 8.0,
-        "l2l": 2.0 * n_leaves * S * 8.0,
-    }
+    fp32 = headline != "fp64"
+    peak_c = FP32_PEAK_TFS if fp32 else FP64_PEAK_TFS
+    flops = {"l2p_p2p": 17.0 * I + 4.0 * S * n, "m2l": 2.0 * S * S * tm.m2l_pairs_executed}
+    bytes_ = {"p2m": 24.0 * n + n_leaves * S * 8.0, "m2m": 2.0 * info["n_nodes"] * S * 8.0,
+              "l2l": 2.0 * n_leaves * S * 8.0}
     t_dom = passes[dominant] * 1e-3
     if dominant in flops:
         achieved = flops[dominant] / t_dom / 1e12
         roof = {"kernel": dominant, "bound": "valu" if dominant == "l2p_p2p" else "mfma", "achieved": achieved,
-                "peak": FP64_PEAK_TFS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFS, "traffic": None}
+                "peak": peak_c, "unit": "TFLOP/s", "frac": achieved / peak_c, "traffic": None}
     else:
         achieved = bytes_[dominant] / t_dom / 1e9
         roof = {"kernel": dominant, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -136,34 +157,32 @@ def main():
 
     out = {
         "metric": METRIC,
-        "value": n / (ms_per_step * 1e-3),
+        "value": n / (H["ms"] * 1e-3),
         "unit": "bodies/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": H["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded SplitMix64 Plummer sphere, 2D)",
+        "dtype": "f32" if fp32 else "f64",
+        "data": "synthetic (seeded SplitMix64 2D Plummer sphere)",
         "config": {"workload": args.workload, "N": n, "p": args.p, "n_max": spec["n_max"], "algo": args.algo,
-                   "precision": "fp64", "softening": eps, "bounds": list(spec["bounds"]),
+                   "form": headline, "softening": eps, "bounds": list(spec["bounds"]),
                    "theta": args.theta if args.algo == "corrected" else None},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "passes_ms": passes,
-        "t_build_s": t_build,
-        "t_first_eval_s": t_first,
+        "forms": {k: {"ms_per_step": r["ms"], "bodies_per_s": n / (r["ms"] * 1e-3), "passes_ms": r["passes"],
+                      "t_build_s": r["t_build"], "t_first_eval_s": r["t_first"],
+                      "p2p_interactions_per_s": I / (r["passes"]["l2p_p2p"] * 1e-3), **errs[k]}
+                  for k, r in res.items()},
         "tree": info,
         "p2p_interactions": I,
-        "p2p_interactions_per_s": I / (passes["l2p_p2p"] * 1e-3),
         "m2l_pairs_executed": tm.m2l_pairs_executed,
         "m2l_pairs_reference": tm.m2l_pairs_reference,
         "m2l_keys": tm.m2l_keys,
-        "err_vs_direct": {"samples": int(len(samples)), "rms_vec": float(np.sqrt(np.mean(vec ** 2))),
-                          "max_vec": float(vec.max()), "max_mag": float(magerr.max()),
-                          "mean_mag": float(magerr.mean())},
+        "vjp_over_fwd": None,
     }
     print(json.dumps(out), flush=True)
 

@@ -244,8 +244,7 @@ int check_cfg(const galax_fmm_config_t* c) {
     return set_error(GALAX_ERR_INVALID, "softening must be finite and >= 0");
   if (c->tree_mode > GALAX_TREE_CONSTRAINED) return set_error(GALAX_ERR_INVALID, "unknown tree_mode");
   if (c->algo > GALAX_ALGO_CORRECTED) return set_error(GALAX_ERR_INVALID, "unknown algo");
-  if (c->precision != GALAX_PREC_FP64)
-    return set_error(GALAX_ERR_UNSUPPORTED, "precision %u is not provided by this build", c->precision);
+  if (c->precision > GALAX_PREC_MIXED_BF16X3) return set_error(GALAX_ERR_INVALID, "unknown precision");
   if (c->algo == GALAX_ALGO_CORRECTED && !(c->theta > 0.0 && c->theta < 1.0))
     return set_error(GALAX_ERR_INVALID, "theta must be in (0, 1) for the corrected variant");
   return 0;
@@ -261,6 +260,7 @@ int reserve_bodies(galax_fmm* h, size_t n) {
 int after_tree_installed(galax_fmm* h) {
   h->has_tree = true;
   h->keys_valid = false;
+  h->mx_valid = false;
   h->corrected_ready = false;
   return 0;
 }
@@ -321,7 +321,8 @@ void galax_fmm_destroy(galax_fmm_t* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   DBuf* bufs[] = {&h->x, &h->y, &h->m, &h->vx, &h->vy, &h->ax, &h->ay, &h->phi, &h->mp, &h->loc, &h->mp32,
                   &h->loc32, &h->gtab, &h->key_of_pair, &h->perm, &h->hash, &h->tmp_u32, &h->flag,
-                  &h->c_m2l_off, &h->c_m2l_dat, &h->c_gat_off, &h->c_gat_dat, &h->c_src_pairs};
+                  &h->c_m2l_off, &h->c_m2l_dat, &h->c_gat_off, &h->c_gat_dat, &h->c_src_pairs,
+                  &h->key_dxy, &h->mx_pos, &h->mx_src, &h->mx_tiles, &h->mx_that, &h->mx_sig, &h->mx_R};
   for (DBuf* b : bufs) dbuf_free(*b);
   DevTree& t = h->tree;
   DBuf* tb[] = {&t.center_x, &t.center_y, &t.half_width, &t.mass, &t.com_x, &t.com_y, &t.body_start, &t.body_end,
@@ -419,6 +420,7 @@ int galax_fmm_set_softening(galax_fmm_t* h, double softening) {
     return set_error(GALAX_ERR_INVALID, "softening must be finite and >= 0");
   h->cfg.softening = softening;
   h->keys_valid = false;  // derivative tables depend on eps
+  h->mx_valid = false;
   return 0;
 }
 

@@ -93,6 +93,11 @@ struct galax_fmm {
   bool has_tree = false;
   // expansions and M2L key tables
   gx::DBuf mp, loc, mp32, loc32, gtab, key_of_pair, perm, hash, tmp_u32, flag;
+  gx::DBuf key_dxy;  // representative offset (dx, dy) per M2L key
+  // mixed-precision M2L: pairs grouped by key into tiles of 16
+  gx::DBuf mx_pos, mx_src, mx_tiles, mx_that, mx_sig, mx_R;
+  size_t mx_npad = 0, mx_ntiles = 0;
+  bool mx_valid = false;
   size_t n_keys = 0;
   size_t n_live_targets = 0, n_live_pairs = 0;
   bool keys_valid = false;
@@ -129,6 +134,8 @@ int count_interactions(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat,
 int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, double* d_ay, double* d_phi);
 int vjp(galax_fmm* h, const double* d_x, const double* d_y, const double* d_m, const double* d_axb,
         const double* d_ayb, double* d_xb, double* d_yb, double* d_mb);
+int m2l_mfma_setup(galax_fmm* h);
+int m2l_mfma_run(galax_fmm* h, const double* mp, double* loc);
 int direct_sum_device(hipStream_t s, const double* x, const double* y, const double* m, size_t n, double eps,
                       const uint64_t* targets, size_t nt, double* ax, double* ay, double* phi);
 int kdk_device(galax_fmm* h, double dt, uint64_t n_steps, uint64_t every, galax_diag_t* diag);

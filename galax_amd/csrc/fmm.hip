@@ -497,60 +497,92 @@ __device__ __forceinline__ void l2p_body(const double* __restrict__ L, double du
   *fphi = ph;
 }
 
-// One wave per target leaf (up to 64 bodies per pass); lane = target body.
-// Far field from the leaf's local expansion, then the near field gathered from
-// the leaf's P2P gather list with source bodies staged through LDS.
-template <int P>
+// Near-field arithmetic type: FP64 (rsqrt: v_rsq_f64 + one refinement) or
+// FP32 (v_rsq_f32) on coordinates relative to the target leaf's COM.
+template <typename T>
+struct Near;
+template <>
+struct Near<double> {
+  typedef double2 V2;
+  static __device__ __forceinline__ double rsq(double v) { return rsqrt(v); }
+};
+template <>
+struct Near<float> {
+  typedef float2 V2;
+  static __device__ __forceinline__ float rsq(float v) { return __builtin_amdgcn_rsqf(v); }
+};
+
+// One 64-lane wave per target leaf. Lanes 0-31 and 32-63 hold the same 32
+// target bodies and split the sources of every staged chunk (even / odd), so
+// no lane idles at n_max = 32; the halves are combined by one xor-shuffle.
+// Far field: the leaf's local expansion at each body (FP64, exact). Near field:
+// the leaf's gather list, sources staged through LDS as (x, y) relative to the
+// leaf COM plus m, in T precision.
+template <int P, typename T, bool PHI>
 __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, const double* __restrict__ y,
                                                 const double* __restrict__ m, const uint32_t* __restrict__ bs,
                                                 const uint32_t* __restrict__ be, const double* __restrict__ comx,
                                                 const double* __restrict__ comy, const double* __restrict__ loc,
                                                 const uint32_t* __restrict__ goff, const uint32_t* __restrict__ gdat,
-                                                double eps2, int use_far, double* __restrict__ ax,
-                                                double* __restrict__ ay, double* __restrict__ phi) {
+                                                double eps2d, double* __restrict__ ax, double* __restrict__ ay,
+                                                double* __restrict__ phi) {
   constexpr int S = num_moments(P);
-  __shared__ double sx[64], sy[64], sm[64];
+  typedef typename Near<T>::V2 V2;
+  __shared__ V2 sxy[64];
+  __shared__ T sm[64];
   const uint32_t leaf = blockIdx.x;
   const int lane = threadIdx.x;
+  const int tl = lane & 31, half = lane >> 5;
   const uint32_t ts = bs[leaf], te = be[leaf];
-  for (uint32_t tb = ts; tb < te; tb += 64) {
-    const uint32_t i = tb + lane;
+  const double ox = comx[leaf], oy = comy[leaf];
+  const T eps2 = (T)eps2d;
+  const uint32_t g0 = goff[leaf], g1 = goff[leaf + 1];
+  for (uint32_t tb = ts; tb < te; tb += 32) {
+    const uint32_t i = tb + tl;
     const bool valid = i < te;
-    double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0;
+    const double xi = valid ? x[i] : ox, yi = valid ? y[i] : oy;
+    const T xr = (T)(xi - ox), yr = (T)(yi - oy);
     double fx = 0.0, fy = 0.0, fp = 0.0;
-    if (use_far && valid) l2p_body<P>(loc + (size_t)leaf * S, xi - comx[leaf], yi - comy[leaf], &fx, &fy, &fp);
-    double accx = 0.0 - fx, accy = 0.0 - fy, accp = fp;
-    for (uint32_t g = goff[leaf]; g < goff[leaf + 1]; ++g) {
+    if (valid && half == 0) l2p_body<P>(loc + (size_t)leaf * S, xi - ox, yi - oy, &fx, &fy, &fp);
+    T accx = 0, accy = 0, accp = 0;
+    for (uint32_t g = g0; g < g1; ++g) {
       const uint32_t sl = gdat[g];
       const uint32_t ss = bs[sl], se = be[sl];
       for (uint32_t sb = ss; sb < se; sb += 64) {
         const uint32_t cnt = min(64u, se - sb);
         __syncthreads();
         if ((uint32_t)lane < cnt) {
-          sx[lane] = x[sb + lane];
-          sy[lane] = y[sb + lane];
-          sm[lane] = m[sb + lane];
+          V2 v;
+          v.x = (T)(x[sb + lane] - ox);
+          v.y = (T)(y[sb + lane] - oy);
+          sxy[lane] = v;
+          sm[lane] = (T)m[sb + lane];
         }
         __syncthreads();
         if (valid) {
-          for (uint32_t jj = 0; jj < cnt; ++jj) {
-            if (sb + jj == i) continue;
-            const double dx = sx[jj] - xi, dy = sy[jj] - yi;
-            const double r2 = fma(dx, dx, fma(dy, dy, eps2));
-            const double inv = 1.0 / sqrt(r2);
-            const double mj = sm[jj];
-            const double f = mj * (inv * inv * inv);
+          const int self = (int)i - (int)sb;  // skip j == i
+          for (int jj = half; jj < (int)cnt; jj += 2) {
+            if (jj == self) continue;
+            const V2 sj = sxy[jj];
+            const T dx = sj.x - xr, dy = sj.y - yr;
+            const T r2 = fma(dx, dx, fma(dy, dy, eps2));
+            const T inv = Near<T>::rsq(r2);
+            const T mj = sm[jj];
+            const T f = mj * (inv * inv * inv);
             accx = fma(f, dx, accx);
             accy = fma(f, dy, accy);
-            accp = fma(-mj, inv, accp);
+            if (PHI) accp = fma(-mj, inv, accp);
           }
         }
       }
     }
-    if (valid) {
-      ax[i] = accx;
-      ay[i] = accy;
-      if (phi) phi[i] = accp;
+    accx += __shfl_xor(accx, 32, 64);
+    accy += __shfl_xor(accy, 32, 64);
+    if (PHI) accp += __shfl_xor(accp, 32, 64);
+    if (valid && half == 0) {
+      ax[i] = (0.0 - fx) + (double)accx;
+      ay[i] = (0.0 - fy) + (double)accy;
+      if (PHI) phi[i] = fp + (double)accp;
     }
   }
 }
@@ -716,8 +748,8 @@ static int m2l_keys_setup(galax_fmm* h, uint32_t n_targets) {
   uint32_t nk = 0;
   GX_CHECK(scan_u32(h, kid, kid, nslots, false, false, &nk));
   h->n_keys = nk;
-  GX_CHECK(dbuf_reserve(h->ws.g, (size_t)nk * 16 + 16, s));
-  double* dxy = h->ws.g.as<double>();
+  GX_CHECK(dbuf_reserve(h->key_dxy, (size_t)nk * 16 + 16, s));
+  double* dxy = h->key_dxy.as<double>();
   k_keys_compact<<<nblk(nslots), 256, 0, s>>>(hkeys, hrep, nslots, kid, slot_kid, t.m2l_dat.as<uint32_t>(),
                                               h->tmp_u32.as<uint32_t>(), comx, comy, dxy);
   GX_HIP(hipGetLastError());
@@ -781,6 +813,8 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
     k_m2l_exact<P><<<nblk(nn, 4), 256, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(),
                                                (uint32_t)nn, comx, comy, eps * eps, mp, loc);
     GX_HIP(hipGetLastError());
+  } else if (live_hi > 0 && h->cfg.precision != GALAX_PREC_FP64) {
+    GX_CHECK(m2l_mfma_run(h, mp, loc));
   } else if (live_hi > 0) {
     k_m2l_ref<P><<<nblk(live_hi, 4), 256, 0, s>>>(t.m2l_off.as<uint32_t>(), t.m2l_dat.as<uint32_t>(),
                                                   h->key_of_pair.as<uint32_t>(), h->gtab.as<double>(), live_hi, mp,
@@ -801,9 +835,23 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   GX_HIP(hipEventRecord(ev[6], s));
   const uint32_t* goff = corrected ? h->c_gat_off.as<uint32_t>() : t.gat_off.as<uint32_t>();
   const uint32_t* gdat = corrected ? h->c_gat_dat.as<uint32_t>() : t.gat_dat.as<uint32_t>();
-  k_l2p_p2p<P><<<(unsigned)nl, 64, 0, s>>>(d_x, d_y, d_m, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), comx,
-                                           comy, loc, goff, gdat, eps * eps, 1, d_ax,
-                                           d_ay, d_phi);
+  const bool fp32_near = h->cfg.precision != GALAX_PREC_FP64;
+#define GX_LAUNCH_P2P(T, PH)                                                                                       \
+  k_l2p_p2p<P, T, PH><<<(unsigned)nl, 64, 0, s>>>(d_x, d_y, d_m, t.body_start.as<uint32_t>(),                    \
+                                                   t.body_end.as<uint32_t>(), comx, comy, loc, goff, gdat, eps * eps, \
+                                                   d_ax, d_ay, d_phi)
+  if (fp32_near) {
+    if (d_phi)
+      GX_LAUNCH_P2P(float, true);
+    else
+      GX_LAUNCH_P2P(float, false);
+  } else {
+    if (d_phi)
+      GX_LAUNCH_P2P(double, true);
+    else
+      GX_LAUNCH_P2P(double, false);
+  }
+#undef GX_LAUNCH_P2P
   GX_HIP(hipGetLastError());
   GX_HIP(hipEventRecord(ev[7], s));
   return 0;
@@ -823,7 +871,9 @@ int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, 
   if (!corrected && !h->keys_valid) {
     uint32_t top = std::min<uint32_t>(t.n_levels, 2);
     GX_CHECK(m2l_keys_setup(h, t.level_offset[top]));
+    h->mx_valid = false;
   }
+  if (!corrected && h->cfg.precision != GALAX_PREC_FP64 && !h->mx_valid) GX_CHECK(m2l_mfma_setup(h));
   GX_CHECK(dbuf_reserve(h->flag, 64, s));
   uint32_t* changed = h->flag.as<uint32_t>() + 4;
   GX_HIP(hipEventRecord(h->ev[0], s));

@@ -835,6 +835,8 @@ int build_lists_reference(galax_fmm* h) {
   uint32_t* err = h->flag.as<uint32_t>();
   GX_HIP(hipMemsetAsync(err, 0, sizeof(uint32_t), s));
   uint32_t* nb_off = h->ws.c.as<uint32_t>();
+  // the count kernels write n entries; entry n must be 0 before the scan
+  GX_HIP(hipMemsetAsync(nb_off, 0, (nn + 1) * sizeof(uint32_t), s));
   k_same_level_nbrs<false><<<nblk(nn, 128), 128, 0, s>>>(a, 0, (uint32_t)nn, (uint32_t)(nn - 1), u, u + nn,
                                                          u + 2 * nn, u + 3 * nn, nb_off, nullptr, err);
   GX_HIP(hipGetLastError());
@@ -852,6 +854,7 @@ int build_lists_reference(galax_fmm* h) {
   // M2L lists
   GX_CHECK(dbuf_reserve(t.m2l_off, (nn + 1) * sizeof(uint32_t), s));
   uint32_t* moff = t.m2l_off.as<uint32_t>();
+  GX_HIP(hipMemsetAsync(moff, 0, (nn + 1) * sizeof(uint32_t), s));
   k_m2l_lists<false><<<nblk(nn), 256, 0, s>>>(a, (uint32_t)nn, nb_off, nb_dat, moff, nullptr);
   GX_HIP(hipGetLastError());
   uint32_t mt = 0;
@@ -863,6 +866,7 @@ int build_lists_reference(galax_fmm* h) {
   // P2P lists
   GX_CHECK(dbuf_reserve(t.p2p_off, (nn + 1) * sizeof(uint32_t), s));
   uint32_t* poff = t.p2p_off.as<uint32_t>();
+  GX_HIP(hipMemsetAsync(poff, 0, (nn + 1) * sizeof(uint32_t), s));
   k_p2p_lists<false><<<nblk(nn), 256, 0, s>>>((uint32_t)t.n_leaves, (uint32_t)nn, nb_off, nb_dat, poff, nullptr);
   GX_HIP(hipGetLastError());
   uint32_t pt = 0;

@@ -53,8 +53,15 @@ def _assert_tree_equal(gt, ot):
 def _assert_lists_equal(g_csr, o_lists):
     go, gd = g_csr
     oo, od = o_lists.csr()
-    assert np.array_equal(go.astype(np.int64), oo.astype(np.int64))
-    assert np.array_equal(gd.astype(np.int64), od.astype(np.int64))
+    go, oo = go.astype(np.int64), oo.astype(np.int64)
+    if not np.array_equal(go, oo):
+        bad = np.nonzero(go != oo)[0]
+        raise AssertionError(f"list offsets differ first at node {bad[0]}: {go[bad[0]-1:bad[0]+2]} vs "
+                             f"{oo[bad[0]-1:bad[0]+2]} (totals {go[-1]} vs {oo[-1]})")
+    gd, od = gd.astype(np.int64), od.astype(np.int64)
+    if not np.array_equal(gd, od):
+        bad = np.nonzero(gd != od)[0]
+        raise AssertionError(f"list entries differ at {len(bad)} places, first {bad[0]}: {gd[bad[0]]} vs {od[bad[0]]}")
 
 
 CASES = [
@@ -261,3 +268,47 @@ def test_errors_are_codes_not_crashes():
     big = _bodies(ic.uniform(11))
     with pytest.raises(fmm.GalaxError):
         ctx.build_tree(big)  # above max_n
+
+
+# ------------------------------------------------------ mixed precision ----
+
+def _forms_run(d, p, eps, bounds, algo=fmm.ALGO_REFERENCE, n_max=32):
+    out = {}
+    for prec in (fmm.PREC_FP64, fmm.PREC_MIXED_FP32, fmm.PREC_MIXED_BF16X3):
+        ctx, b, _ = _gpu_build(d, p=p, eps=eps, bounds=bounds, precision=prec, algo=algo, n_max=n_max, theta=0.3)
+        ctx.evaluate(b)
+        out[prec] = (b.ax.copy(), b.ay.copy(), b.phi.copy())
+    return out
+
+
+@pytest.mark.parametrize("p", [4, 6, 8])
+def test_mixed_forms_match_fp64_reference_algo(p):
+    """FP32-MFMA M2L + FP32 P2P, and the BF16x3 M2L, against the FP64 path of the
+    same algorithm: RMS relative force difference <= 1e-5 (BASELINE metric)."""
+    d = ic.plummer(50_000, seed=301)
+    res = _forms_run(d, p, 0.02, (-50.0, 50.0, -50.0, 50.0))
+    ref = res[fmm.PREC_FP64]
+    for prec in (fmm.PREC_MIXED_FP32, fmm.PREC_MIXED_BF16X3):
+        rd = _rel_diff(res[prec][:2], ref[:2])
+        assert np.sqrt(np.mean(rd ** 2)) <= 1e-5, (prec, np.sqrt(np.mean(rd ** 2)), rd.max())
+        np.testing.assert_allclose(res[prec][2], ref[2], rtol=1e-5)
+
+
+def test_mixed_forms_match_fp64_plummer1m_p6():
+    """The full 2^20-body Plummer workload at p = 6 (size-independent check:
+    mixed vs FP64 of the same algorithm)."""
+    d = ic.plummer(1 << 20)
+    eps = 0.05
+    res = _forms_run(d, 6, eps, (-50.0, 50.0, -50.0, 50.0))
+    ref = res[fmm.PREC_FP64]
+    assert np.isfinite(ref[0]).all()
+    for prec in (fmm.PREC_MIXED_FP32, fmm.PREC_MIXED_BF16X3):
+        rd = _rel_diff(res[prec][:2], ref[:2])
+        assert np.sqrt(np.mean(rd ** 2)) <= 1e-5, (prec, np.sqrt(np.mean(rd ** 2)))
+
+
+def test_mixed_forms_corrected_algo():
+    d = ic.plummer(30_000, seed=302)
+    res = _forms_run(d, 6, 0.05, (-50.0, 50.0, -50.0, 50.0), algo=fmm.ALGO_CORRECTED)
+    rd = _rel_diff(res[fmm.PREC_MIXED_FP32][:2], res[fmm.PREC_FP64][:2])
+    assert np.sqrt(np.mean(rd ** 2)) <= 1e-5
