@@ -259,6 +259,7 @@ int reserve_bodies(galax_fmm* h, size_t n) {
 
 int after_tree_installed(galax_fmm* h) {
   h->has_tree = true;
+  h->vj_rev_valid = false;
   h->keys_valid = false;
   h->mx_valid = false;
   h->corrected_ready = false;
@@ -322,7 +323,8 @@ void galax_fmm_destroy(galax_fmm_t* h) {
   DBuf* bufs[] = {&h->x, &h->y, &h->m, &h->vx, &h->vy, &h->ax, &h->ay, &h->phi, &h->mp, &h->loc, &h->mp32,
                   &h->loc32, &h->gtab, &h->key_of_pair, &h->perm, &h->hash, &h->tmp_u32, &h->flag,
                   &h->c_m2l_off, &h->c_m2l_dat, &h->c_gat_off, &h->c_gat_dat, &h->c_src_pairs,
-                  &h->key_dxy, &h->mx_pos, &h->mx_src, &h->mx_tiles, &h->mx_that, &h->mx_sig, &h->mx_R};
+                  &h->key_dxy, &h->mx_pos, &h->mx_src, &h->mx_tiles, &h->mx_that, &h->mx_sig, &h->mx_R,
+                  &h->vj_lbar, &h->vj_mbar, &h->vj_leaf_of, &h->vj_roff, &h->vj_rpair, &h->vj_fwd};
   for (DBuf* b : bufs) dbuf_free(*b);
   DevTree& t = h->tree;
   DBuf* tb[] = {&t.center_x, &t.center_y, &t.half_width, &t.mass, &t.com_x, &t.com_y, &t.body_start, &t.body_end,
@@ -557,14 +559,38 @@ int galax_fmm_kdk(galax_fmm_t* h, galax_bodies_t* b, double dt, uint64_t n_steps
 
 int galax_fmm_vjp(galax_fmm_t* h, const galax_bodies_t* b, const double* axb, const double* ayb, double* xb,
                   double* yb, double* mb) {
-  (void)b;
-  (void)axb;
-  (void)ayb;
-  (void)xb;
-  (void)yb;
-  (void)mb;
+  if (!h || !b) return set_error(GALAX_ERR_INVALID, "NULL argument");
+  size_t n = h->tree.n_bodies;
+  if (b->n != n) return set_error(GALAX_ERR_INVALID, "bodies.n = %zu but the tree holds %zu bodies", b->n, n);
+  if (n && (!b->x || !b->y || !b->m || !axb || !ayb || !xb || !yb || !mb))
+    return set_error(GALAX_ERR_INVALID, "x, y, m, cotangents and outputs are required");
+  hipStream_t s = h->stream;
+  GX_CHECK(up(s, h->x, b->x, n));
+  GX_CHECK(up(s, h->y, b->y, n));
+  GX_CHECK(up(s, h->m, b->m, n));
+  GX_CHECK(up(s, h->ax, axb, n));  // the body buffers hold the cotangents and adjoints
+  GX_CHECK(up(s, h->ay, ayb, n));
+  GX_CHECK(dbuf_reserve(h->ws.b, std::max<size_t>(n, 1) * 24, s));
+  double* out = h->ws.b.as<double>();
+  GX_CHECK(prepare_eval(h, h->x.as<double>(), h->y.as<double>()));
+  GX_CHECK(gx::vjp(h, h->x.as<double>(), h->y.as<double>(), h->m.as<double>(), h->ax.as<double>(),
+                   h->ay.as<double>(), out, out + n, out + 2 * n));
+  if (n) {
+    GX_HIP(hipMemcpyAsync(xb, out, n * 8, hipMemcpyDeviceToHost, s));
+    GX_HIP(hipMemcpyAsync(yb, out + n, n * 8, hipMemcpyDeviceToHost, s));
+    GX_HIP(hipMemcpyAsync(mb, out + 2 * n, n * 8, hipMemcpyDeviceToHost, s));
+  }
+  GX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int galax_fmm_vjp_device(galax_fmm_t* h, double* d_x, double* d_y, double* d_m, const double* d_axb,
+                         const double* d_ayb, double* d_xb, double* d_yb, double* d_mb) {
   if (!h) return set_error(GALAX_ERR_INVALID, "NULL handle");
-  return set_error(GALAX_ERR_UNSUPPORTED, "vjp is not provided by this build yet");
+  if (h->tree.n_bodies && (!d_x || !d_y || !d_m || !d_axb || !d_ayb || !d_xb || !d_yb || !d_mb))
+    return set_error(GALAX_ERR_INVALID, "NULL device array");
+  GX_CHECK(prepare_eval(h, d_x, d_y));
+  return gx::vjp(h, d_x, d_y, d_m, d_axb, d_ayb, d_xb, d_yb, d_mb);
 }
 
 int galax_direct_sum(const double* x, const double* y, const double* m, size_t n, double softening,

@@ -98,6 +98,9 @@ struct galax_fmm {
   gx::DBuf mx_pos, mx_src, mx_tiles, mx_that, mx_sig, mx_R;
   size_t mx_npad = 0, mx_ntiles = 0;
   bool mx_valid = false;
+  // adjoint (VJP) buffers
+  gx::DBuf vj_lbar, vj_mbar, vj_leaf_of, vj_roff, vj_rpair, vj_fwd;
+  bool vj_rev_valid = false;
   size_t n_keys = 0;
   size_t n_live_targets = 0, n_live_pairs = 0;
   bool keys_valid = false;

@@ -1,0 +1,473 @@
+// vjp.hip — vector-Jacobian product of the FMM accelerations (no reference
+// counterpart; for differentiable use and checking through torch.autograd).
+//
+// For cotangents (ax_bar, ay_bar) the adjoint passes run in reverse order of
+// the forward ones, on the same static tree, lists and node centres (the
+// reference freezes node geometry at build time, galax-cli/src/main.rs:159,
+// galax-integrate/src/lib.rs:217), so the map (x, y, m) -> (ax, ay) is the
+// composition of polynomial and linear stages:
+//   near  a_i += sum_j m_j F(x_j - x_i),  F(d) = d / (|d|^2 + eps^2)^{3/2}
+//         x_bar_j += sum_i K(x_i - x_j) (m_j a_bar_i - m_i a_bar_j),   K = dF/dd
+//         m_bar_j -= sum_i a_bar_i . F(x_i - x_j)
+//   L2P   a_b = -grad Phi_leaf(u_b):  x_bar_b -= H(u_b) a_bar_b (Hessian),
+//         Lbar[IJ] -= sum_b a_bar_b . grad(u^I v^J)(u_b)
+//   L2L^T, M2L^T (transposed operators), M2M^T, then P2M^T:
+//         x_bar_b += m_b sum Mbar[ij] d/dx (u^i v^j / i!j!), m_bar_b += sum Mbar[ij] u^i v^j / i!j!
+// Every stage is a gather (one writer per output), so results are deterministic.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+
+namespace gx {
+
+static inline unsigned nblk(size_t n, unsigned bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+template <int P>
+__device__ __forceinline__ void cij(int k, int& i, int& j) {
+  int n = 0;
+#pragma unroll
+  for (int q = 0; q <= P; ++q)
+    if (k >= (q + 1) * (q + 2) / 2) n = q + 1;
+  j = k - n * (n + 1) / 2;
+  i = n - j;
+}
+
+__device__ __forceinline__ double dfact(int n) {
+  double f = 1.0;
+  for (int q = 2; q <= n; ++q) f *= q;
+  return f;
+}
+
+__device__ __forceinline__ double dbinom(int n, int k) {
+  double r = 1.0;
+  for (int q = 1; q <= k; ++q) r = r * (n - k + q) / q;
+  return r;
+}
+
+// ------------------------------------------------------------- near ----
+
+// one wave per target leaf j-block; lanes 0-31 / 32-63 split the sources
+__global__ void __launch_bounds__(64) k_vjp_near(const double* __restrict__ x, const double* __restrict__ y,
+                                                 const double* __restrict__ m, const double* __restrict__ axb,
+                                                 const double* __restrict__ ayb, const uint32_t* __restrict__ bs,
+                                                 const uint32_t* __restrict__ be, const uint32_t* __restrict__ goff,
+                                                 const uint32_t* __restrict__ gdat, double eps2,
+                                                 double* __restrict__ xb, double* __restrict__ yb,
+                                                 double* __restrict__ mb) {
+  __shared__ double sx[64], sy[64], sm[64], sax[64], say[64];
+  const uint32_t leaf = blockIdx.x;
+  const int lane = threadIdx.x, tl = lane & 31, half = lane >> 5;
+  const uint32_t ts = bs[leaf], te = be[leaf];
+  for (uint32_t tb = ts; tb < te; tb += 32) {
+    const uint32_t j = tb + tl;
+    const bool valid = j < te;
+    const double xj = valid ? x[j] : 0.0, yj = valid ? y[j] : 0.0, mj = valid ? m[j] : 0.0;
+    const double abx = valid ? axb[j] : 0.0, aby = valid ? ayb[j] : 0.0;
+    double gx_ = 0.0, gy_ = 0.0, gm_ = 0.0;
+    for (uint32_t g = goff[leaf]; g < goff[leaf + 1]; ++g) {
+      const uint32_t sl = gdat[g];
+      const uint32_t ss = bs[sl], se = be[sl];
+      for (uint32_t sb = ss; sb < se; sb += 64) {
+        const uint32_t cnt = min(64u, se - sb);
+        __syncthreads();
+        if ((uint32_t)lane < cnt) {
+          sx[lane] = x[sb + lane];
+          sy[lane] = y[sb + lane];
+          sm[lane] = m[sb + lane];
+          sax[lane] = axb[sb + lane];
+          say[lane] = ayb[sb + lane];
+        }
+        __syncthreads();
+        if (valid) {
+          const int self = (int)j - (int)sb;
+          for (int ii = half; ii < (int)cnt; ii += 2) {
+            if (ii == self) continue;
+            const double dx = sx[ii] - xj, dy = sy[ii] - yj;  // d = x_i - x_j
+            const double r2 = dx * dx + dy * dy + eps2;
+            const double inv = 1.0 / sqrt(r2);
+            const double inv3 = inv * inv * inv, inv5 = inv3 * inv * inv;
+            const double mi = sm[ii];
+            const double vx = mj * sax[ii] - mi * abx, vy = mj * say[ii] - mi * aby;
+            const double dv = dx * vx + dy * vy;
+            gx_ += vx * inv3 - 3.0 * dx * dv * inv5;
+            gy_ += vy * inv3 - 3.0 * dy * dv * inv5;
+            gm_ -= (sax[ii] * dx + say[ii] * dy) * inv3;
+          }
+        }
+      }
+    }
+    gx_ += __shfl_xor(gx_, 32, 64);
+    gy_ += __shfl_xor(gy_, 32, 64);
+    gm_ += __shfl_xor(gm_, 32, 64);
+    if (valid && half == 0) {
+      xb[j] = gx_;
+      yb[j] = gy_;
+      mb[j] = gm_;
+    }
+  }
+}
+
+// ------------------------------------------------------ L2P adjoint ----
+
+// Lbar of each leaf: lane per coefficient, bodies in order
+template <int P>
+__global__ void __launch_bounds__(256) k_vjp_l2p_coef(const double* __restrict__ x, const double* __restrict__ y,
+                                                      const double* __restrict__ axb, const double* __restrict__ ayb,
+                                                      const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be,
+                                                      const double* __restrict__ comx, const double* __restrict__ comy,
+                                                      uint32_t n_leaves, double* __restrict__ lbar) {
+  constexpr int S = num_moments(P);
+  const uint32_t leaf = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (leaf >= n_leaves) return;
+  for (int k = lane; k < S; k += 64) {
+    int I, J;
+    cij<P>(k, I, J);
+    double acc = 0.0;
+    for (uint32_t b = bs[leaf]; b < be[leaf]; ++b) {
+      const double u = x[b] - comx[leaf], v = y[b] - comy[leaf];
+      double gu = 0.0, gv = 0.0;
+      if (I >= 1) gu = (double)I * pow(u, I - 1) * pow(v, J);
+      if (J >= 1) gv = (double)J * pow(u, I) * pow(v, J - 1);
+      acc -= axb[b] * gu + ayb[b] * gv;
+    }
+    lbar[(size_t)leaf * S + k] = acc;
+  }
+}
+
+// per body: far-field Hessian term and the P2M adjoint (leaf COM frozen)
+template <int P>
+__global__ void k_vjp_body(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ m,
+                           const double* __restrict__ axb, const double* __restrict__ ayb,
+                           const uint32_t* __restrict__ leaf_of, const double* __restrict__ comx,
+                           const double* __restrict__ comy, const double* __restrict__ loc,
+                           const double* __restrict__ mbar, size_t n, double* __restrict__ xb, double* __restrict__ yb,
+                           double* __restrict__ mb) {
+  constexpr int S = num_moments(P);
+  size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  const uint32_t lf = leaf_of[b];
+  const double u = x[b] - comx[lf], v = y[b] - comy[lf];
+  double up[P + 1], vp[P + 1];
+  up[0] = 1.0;
+  vp[0] = 1.0;
+#pragma unroll
+  for (int q = 1; q <= P; ++q) {
+    up[q] = up[q - 1] * u;
+    vp[q] = vp[q - 1] * v;
+  }
+  const double* L = loc + (size_t)lf * S;
+  const double* Mb = mbar + (size_t)lf * S;
+  double hxx = 0.0, hxy = 0.0, hyy = 0.0, px = 0.0, py = 0.0, pm = 0.0;
+  double fi = 1.0;
+#pragma unroll
+  for (int i = 0; i <= P; ++i) {
+    if (i > 0) fi *= i;
+    double fj = 1.0;
+#pragma unroll
+    for (int j = 0; j <= P - i; ++j) {
+      if (j > 0) fj *= j;
+      const double c = L[moment_index(i, j)];
+      if (i >= 2) hxx += c * i * (i - 1) * up[i - 2] * vp[j];
+      if (i >= 1 && j >= 1) hxy += c * i * j * up[i - 1] * vp[j - 1];
+      if (j >= 2) hyy += c * j * (j - 1) * up[i] * vp[j - 2];
+      const double mm = Mb[moment_index(i, j)] / (fi * fj);
+      if (i >= 1) px += mm * i * up[i - 1] * vp[j];
+      if (j >= 1) py += mm * j * up[i] * vp[j - 1];
+      pm += mm * up[i] * vp[j];
+    }
+  }
+  const double ax_ = axb[b], ay_ = ayb[b], mass = m[b];
+  xb[b] += -(ax_ * hxx + ay_ * hxy) + mass * px;
+  yb[b] += -(ax_ * hxy + ay_ * hyy) + mass * py;
+  mb[b] += pm;
+}
+
+__global__ void k_leaf_of(const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be, uint32_t n_leaves,
+                          uint32_t* __restrict__ leaf_of) {
+  uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n_leaves) return;
+  for (uint32_t b = bs[l]; b < be[l]; ++b) leaf_of[b] = l;
+}
+
+// ------------------------------------------------------ L2L adjoint ----
+// parents in [lo, hi): Lbar_P[i,j] += sum_c sum_{oi<=i, oj<=j} Lbar_c[oi,oj] C(i,oi) C(j,oj) dx^(i-oi) dy^(j-oj)
+template <int P>
+__global__ void k_vjp_l2l(const uint32_t* __restrict__ cf, const uint8_t* __restrict__ cc,
+                          const double* __restrict__ comx, const double* __restrict__ comy, uint32_t lo, uint32_t hi,
+                          double* __restrict__ lbar) {
+  constexpr int S = num_moments(P);
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)(hi - lo) * S) return;
+  uint32_t p = lo + (uint32_t)(t / S);
+  int k = (int)(t % S);
+  int i, j;
+  cij<P>(k, i, j);
+  double acc = 0.0;
+  for (uint32_t c = cf[p]; c < cf[p] + cc[p]; ++c) {
+    const double dx = comx[c] - comx[p], dy = comy[c] - comy[p];
+    const double* lc = lbar + (size_t)c * S;
+    for (int oi = 0; oi <= i; ++oi)
+      for (int oj = 0; oj <= j; ++oj)
+        acc += lc[moment_index(oi, oj)] * dbinom(i, oi) * dbinom(j, oj) * pow(dx, i - oi) * pow(dy, j - oj);
+  }
+  lbar[(size_t)p * S + k] += acc;
+}
+
+// ------------------------------------------------------ M2M adjoint ----
+// children in [lo, hi): Mbar_c[i,j] += sum_{I>=i, J>=j, I+J<=p} Mbar_P[I,J] dx^(I-i) dy^(J-j) / ((I-i)!(J-j)!)
+template <int P>
+__global__ void k_vjp_m2m(const uint32_t* __restrict__ par, const double* __restrict__ comx,
+                          const double* __restrict__ comy, uint32_t lo, uint32_t hi, double* __restrict__ mbar) {
+  constexpr int S = num_moments(P);
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)(hi - lo) * S) return;
+  uint32_t c = lo + (uint32_t)(t / S);
+  int k = (int)(t % S);
+  uint32_t p = par[c];
+  if (p == NONE) return;
+  int i, j;
+  cij<P>(k, i, j);
+  const double dx = comx[c] - comx[p], dy = comy[c] - comy[p];
+  const double* mp_ = mbar + (size_t)p * S;
+  double acc = 0.0;
+  for (int I = i; I <= P; ++I)
+    for (int J = j; J <= P - I; ++J)
+      acc += mp_[moment_index(I, J)] * pow(dx, I - i) * pow(dy, J - j) / (dfact(I - i) * dfact(J - j));
+  mbar[(size_t)c * S + k] += acc;
+}
+
+// ------------------------------------------------------ M2L adjoint ----
+// Mbar_s[i,j] += sum_pairs sum_{I,J} (-1)^{i+j}/(I!J!) G[i+I, j+J] Lbar_t[I,J]
+
+// reference variant: the pairs naming s as source (reverse CSR of the live
+// pairs), each with its key's derivative table
+template <int P>
+__global__ void __launch_bounds__(256) k_vjp_m2l_ref(const uint32_t* __restrict__ roff,
+                                                     const uint32_t* __restrict__ rpair, uint32_t n_src,
+                                                     const uint32_t* __restrict__ tgt_of_pair,
+                                                     const uint32_t* __restrict__ key_of_pair,
+                                                     const double* __restrict__ gtab, const double* __restrict__ lbar,
+                                                     double* __restrict__ mbar) {
+  constexpr int S = num_moments(P), ST = 2 * P + 1, GS = ST * ST;
+  const uint32_t s = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (s >= n_src) return;
+  for (int k = lane; k < S; k += 64) {
+    int i, j;
+    cij<P>(k, i, j);
+    const double sg = ((i + j) & 1) ? -1.0 : 1.0;
+    double acc = 0.0;
+    for (uint32_t r = roff[s]; r < roff[s + 1]; ++r) {
+      const uint32_t e = rpair[r];
+      const double* G = gtab + (size_t)key_of_pair[e] * GS;
+      const double* Lb = lbar + (size_t)tgt_of_pair[e] * S;
+#pragma unroll
+      for (int I = 0; I <= P; ++I)
+#pragma unroll
+        for (int J = 0; J <= P - I; ++J)
+          acc += sg / (dfact(I) * dfact(J)) * G[(i + I + j + J) * ST + (j + J)] * Lb[moment_index(I, J)];
+    }
+    mbar[(size_t)s * S + k] = acc;
+  }
+}
+
+// corrected variant: lists are symmetric; exact derivatives per pair, built
+// in LDS by the same recurrence as the forward (D = derivatives of 1/R, G = -D)
+template <int P>
+__global__ void __launch_bounds__(256) k_vjp_m2l_exact(const uint32_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ dat, uint32_t nn,
+                                                       const double* __restrict__ comx,
+                                                       const double* __restrict__ comy, double eps2,
+                                                       const double* __restrict__ lbar, double* __restrict__ mbar) {
+  constexpr int S = num_moments(P), ST = 2 * P + 1;
+  __shared__ double sD[4][ST * ST];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t s = blockIdx.x * 4 + w;
+  if (s >= nn) return;
+  double* D = sD[w];
+  double acc[(S + 63) / 64];
+#pragma unroll
+  for (int q = 0; q < (S + 63) / 64; ++q) acc[q] = 0.0;
+  for (uint32_t r = off[s]; r < off[s + 1]; ++r) {
+    const uint32_t t = dat[r];
+    const double x = comx[t] - comx[s], y = comy[t] - comy[s];
+    const double r2 = x * x + y * y + eps2, ir2 = 1.0 / r2;
+    if (lane == 0) D[0] = 1.0 / sqrt(r2);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (int n = 1; n <= 2 * P; ++n) {
+      for (int k = lane; k <= n; k += 64) {
+        int l = n - k;
+        double v;
+        if (k >= 1) {
+          v = -(2.0 * k - 1.0) * x * D[(n - 1) * ST + l];
+          if (k >= 2) v -= (double)(k - 1) * (k - 1) * D[(n - 2) * ST + l];
+          if (l >= 1) v -= 2.0 * l * y * D[(n - 1) * ST + (l - 1)];
+          if (l >= 2) v -= (double)l * (l - 1) * D[(n - 2) * ST + (l - 2)];
+        } else {
+          v = -(2.0 * l - 1.0) * y * D[(n - 1) * ST + (l - 1)];
+          if (l >= 2) v -= (double)(l - 1) * (l - 1) * D[(n - 2) * ST + (l - 2)];
+        }
+        D[n * ST + l] = v * ir2;
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    const double* Lb = lbar + (size_t)t * S;
+#pragma unroll
+    for (int q = 0; q < (S + 63) / 64; ++q) {
+      int k = lane + 64 * q;
+      if (k < S) {
+        int i, j;
+        cij<P>(k, i, j);
+        const double sg = ((i + j) & 1) ? 1.0 : -1.0;  // (-1)^{i+j} * (G = -D)
+        double a = 0.0;
+#pragma unroll
+        for (int I = 0; I <= P; ++I)
+#pragma unroll
+          for (int J = 0; J <= P - I; ++J)
+            a += D[(i + I + j + J) * ST + (j + J)] * Lb[moment_index(I, J)] / (dfact(I) * dfact(J));
+        acc[q] += sg * a;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+#pragma unroll
+  for (int q = 0; q < (S + 63) / 64; ++q) {
+    int k = lane + 64 * q;
+    if (k < S) mbar[(size_t)s * S + k] = acc[q];
+  }
+}
+
+// reverse CSR of the live reference pairs: rpair lists, per source node, the
+// pair indices e (ascending) whose source is that node
+__global__ void k_rev_count(const uint32_t* __restrict__ dat, uint32_t n_pairs, uint32_t* __restrict__ cnt) {
+  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n_pairs) atomicAdd(&cnt[dat[e]], 1u);
+}
+__global__ void k_rev_fill(const uint32_t* __restrict__ dat, uint32_t n_pairs, const uint32_t* __restrict__ roff,
+                           uint32_t* __restrict__ cursor, uint32_t* __restrict__ rpair) {
+  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n_pairs) {
+    uint32_t s = dat[e];
+    rpair[roff[s] + atomicAdd(&cursor[s], 1u)] = e;
+  }
+}
+__global__ void k_rev_sort(uint32_t n, const uint32_t* __restrict__ off, uint32_t* __restrict__ dat) {
+  uint32_t sgi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sgi >= n) return;
+  for (uint32_t i = off[sgi] + 1; i < off[sgi + 1]; ++i) {
+    uint32_t v = dat[i], j = i;
+    while (j > off[sgi] && dat[j - 1] > v) {
+      dat[j] = dat[j - 1];
+      --j;
+    }
+    dat[j] = v;
+  }
+}
+
+template <int P>
+static int vjp_impl(galax_fmm* h, const double* x, const double* y, const double* m, const double* axb,
+                    const double* ayb, double* xb, double* yb, double* mb) {
+  DevTree& t = h->tree;
+  hipStream_t s = h->stream;
+  constexpr int S = num_moments(P);
+  const bool corrected = h->cfg.algo == GALAX_ALGO_CORRECTED;
+  const size_t nn = t.n_nodes, nl = t.n_leaves, n = t.n_bodies;
+  const double eps2 = h->cfg.softening * h->cfg.softening;
+  const double* comx = t.com_x.as<double>();
+  const double* comy = t.com_y.as<double>();
+  GX_CHECK(dbuf_reserve(h->vj_lbar, nn * S * 8, s));
+  GX_CHECK(dbuf_reserve(h->vj_mbar, nn * S * 8, s));
+  GX_CHECK(dbuf_reserve(h->vj_leaf_of, n * 4, s));
+  double* lbar = h->vj_lbar.as<double>();
+  double* mbar = h->vj_mbar.as<double>();
+  GX_HIP(hipMemsetAsync(lbar, 0, nn * S * 8, s));
+  GX_HIP(hipMemsetAsync(mbar, 0, nn * S * 8, s));
+  const uint32_t* goff = corrected ? h->c_gat_off.as<uint32_t>() : t.gat_off.as<uint32_t>();
+  const uint32_t* gdat = corrected ? h->c_gat_dat.as<uint32_t>() : t.gat_dat.as<uint32_t>();
+  // near field
+  k_vjp_near<<<(unsigned)nl, 64, 0, s>>>(x, y, m, axb, ayb, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(),
+                                         goff, gdat, eps2, xb, yb, mb);
+  GX_HIP(hipGetLastError());
+  // L2P adjoint -> Lbar of leaves
+  k_vjp_l2p_coef<P><<<nblk(nl, 4), 256, 0, s>>>(x, y, axb, ayb, t.body_start.as<uint32_t>(),
+                                                t.body_end.as<uint32_t>(), comx, comy, (uint32_t)nl, lbar);
+  GX_HIP(hipGetLastError());
+  // L2L adjoint, bottom-up (reference: level 1 only)
+  const uint32_t top = corrected ? t.n_levels : std::min<uint32_t>(t.n_levels, 2);
+  for (uint32_t l = 1; l < top; ++l) {
+    uint32_t lo = t.level_offset[l], hi = t.level_offset[l + 1];
+    k_vjp_l2l<P><<<nblk((size_t)(hi - lo) * S), 256, 0, s>>>(t.child_first.as<uint32_t>(),
+                                                             t.child_count.as<uint8_t>(), comx, comy, lo, hi, lbar);
+    GX_HIP(hipGetLastError());
+  }
+  // M2L adjoint
+  if (corrected) {
+    k_vjp_m2l_exact<P><<<nblk(nn, 4), 256, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(),
+                                                   (uint32_t)nn, comx, comy, eps2, lbar, mbar);
+  } else if (h->n_live_pairs > 0) {
+    const uint32_t live_hi = t.level_offset[top];
+    if (!h->vj_rev_valid) {
+      GX_CHECK(dbuf_reserve(h->vj_roff, (size_t)(live_hi + 1) * 4, s));
+      GX_CHECK(dbuf_reserve(h->vj_rpair, h->n_live_pairs * 4, s));
+      GX_CHECK(dbuf_reserve(h->ws.a, (size_t)(live_hi + 1) * 4, s));
+      uint32_t* roff = h->vj_roff.as<uint32_t>();
+      GX_HIP(hipMemsetAsync(roff, 0, (size_t)(live_hi + 1) * 4, s));
+      GX_HIP(hipMemsetAsync(h->ws.a.p, 0, (size_t)(live_hi + 1) * 4, s));
+      k_rev_count<<<nblk(h->n_live_pairs), 256, 0, s>>>(t.m2l_dat.as<uint32_t>(), (uint32_t)h->n_live_pairs, roff);
+      GX_HIP(hipGetLastError());
+      GX_CHECK(scan_u32(h, roff, roff, live_hi + 1, false, false, nullptr));
+      k_rev_fill<<<nblk(h->n_live_pairs), 256, 0, s>>>(t.m2l_dat.as<uint32_t>(), (uint32_t)h->n_live_pairs, roff,
+                                                       h->ws.a.as<uint32_t>(), h->vj_rpair.as<uint32_t>());
+      k_rev_sort<<<nblk(live_hi), 256, 0, s>>>(live_hi, roff, h->vj_rpair.as<uint32_t>());
+      GX_HIP(hipGetLastError());
+      h->vj_rev_valid = true;
+    }
+    k_vjp_m2l_ref<P><<<nblk(live_hi, 4), 256, 0, s>>>(h->vj_roff.as<uint32_t>(), h->vj_rpair.as<uint32_t>(), live_hi,
+                                                      h->tmp_u32.as<uint32_t>(), h->key_of_pair.as<uint32_t>(),
+                                                      h->gtab.as<double>(), lbar, mbar);
+  }
+  GX_HIP(hipGetLastError());
+  // M2M adjoint, top-down
+  for (uint32_t l = top; l-- > 1;) {
+    uint32_t lo = t.level_offset[l - 1], hi = t.level_offset[l];
+    k_vjp_m2m<P><<<nblk((size_t)(hi - lo) * S), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, lo, hi, mbar);
+    GX_HIP(hipGetLastError());
+  }
+  // far-field Hessian + P2M adjoint per body
+  k_leaf_of<<<nblk(nl), 256, 0, s>>>(t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), (uint32_t)nl,
+                                     h->vj_leaf_of.as<uint32_t>());
+  k_vjp_body<P><<<nblk(n), 256, 0, s>>>(x, y, m, axb, ayb, h->vj_leaf_of.as<uint32_t>(), comx, comy,
+                                        h->loc.as<double>(), mbar, n, xb, yb, mb);
+  GX_HIP(hipGetLastError());
+  GX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// forward (recomputing M and L from the given positions) then the adjoint passes
+int vjp(galax_fmm* h, const double* d_x, const double* d_y, const double* d_m, const double* d_axb,
+        const double* d_ayb, double* d_xb, double* d_yb, double* d_mb) {
+  if (h->cfg.precision != GALAX_PREC_FP64)
+    return set_error(GALAX_ERR_UNSUPPORTED, "the VJP is provided in FP64 (GALAX_PREC_FP64) only");
+  size_t n = h->tree.n_bodies;
+  GX_CHECK(dbuf_reserve(h->vj_fwd, std::max<size_t>(n, 1) * 16, h->stream));
+  double* fax = h->vj_fwd.as<double>();
+  GX_CHECK(evaluate(h, const_cast<double*>(d_x), const_cast<double*>(d_y), const_cast<double*>(d_m), fax, fax + n,
+                    nullptr));
+  switch (h->cfg.p) {
+#define GX_CASE(PP) \
+  case PP:          \
+    return vjp_impl<PP>(h, d_x, d_y, d_m, d_axb, d_ayb, d_xb, d_yb, d_mb);
+    GX_CASE(0) GX_CASE(1) GX_CASE(2) GX_CASE(3) GX_CASE(4) GX_CASE(5) GX_CASE(6) GX_CASE(7) GX_CASE(8)
+    GX_CASE(9) GX_CASE(10) GX_CASE(11) GX_CASE(12) GX_CASE(13) GX_CASE(14) GX_CASE(15) GX_CASE(16)
+#undef GX_CASE
+  }
+  return set_error(GALAX_ERR_INVALID, "p out of range");
+}
+
+}  // namespace gx

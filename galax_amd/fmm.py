@@ -100,6 +100,7 @@ def lib():
         "galax_fmm_last_timings": [vp, C.POINTER(_Timings)],
         "galax_fmm_kdk": [vp, C.POINTER(_Bodies), d, C.c_uint64, C.c_uint64, vp],
         "galax_fmm_vjp": [vp, C.POINTER(_Bodies), vp, vp, vp, vp, vp],
+        "galax_fmm_vjp_device": [vp] * 9,
         "galax_direct_sum": [vp, vp, vp, sz, d, vp, sz, vp, vp, vp],
         "galax_fmm_validate": [vp, C.POINTER(_Bodies), vp, sz, vp, vp, vp],
     }
@@ -154,7 +155,7 @@ class BodiesSoA:
     @classmethod
     def from_dict(cls, d: dict, with_phi=True):
         n = len(d["x"])
-        g = lambda k: np.ascontiguousarray(d[k], np.float64) if k in d else np.zeros(n)
+        g = lambda k: np.array(d[k], np.float64, copy=True) if k in d else np.zeros(n)
         return cls(g("x"), g("y"), g("vx"), g("vy"), g("m"), g("ax"), g("ay"), np.zeros(n) if with_phi else None)
 
     def __len__(self):
@@ -337,6 +338,11 @@ class GpuContext:
         _check(lib().galax_fmm_vjp(self.h, C.byref(cb), _ptr(_f64(ax_bar, n)), _ptr(_f64(ay_bar, n)), _ptr(xb),
                                    _ptr(yb), _ptr(mb)))
         return xb, yb, mb
+
+    def vjp_device(self, x, y, m, ax_bar, ay_bar, x_bar, y_bar, m_bar):
+        p = lambda t: None if t is None else (t.data_ptr() if hasattr(t, "data_ptr") else int(t))
+        _check(lib().galax_fmm_vjp_device(self.h, p(x), p(y), p(m), p(ax_bar), p(ay_bar), p(x_bar), p(y_bar),
+                                          p(m_bar)))
 
     def validate(self, b: BodiesSoA, samples: np.ndarray):
         """validate_fmm (galax-core/src/lib.rs:1503-1564) with given samples:

@@ -187,10 +187,17 @@ GALAX_API int galax_fmm_kdk(galax_fmm_t* h, galax_bodies_t* bodies, double dt, u
 
 /* Vector-Jacobian product of the accelerations: given cotangents ax_bar, ay_bar
  * (tree body order), returns x_bar, y_bar, m_bar = J^T (ax_bar, ay_bar) for
- * the map (x, y, m) -> (ax, ay) with the tree topology and lists frozen
- * (no reference counterpart; for checking through torch.autograd). */
+ * the map (x, y, m) -> (ax, ay) on the installed static tree (topology, lists,
+ * M2L keys and node centres frozen, as the reference freezes them after
+ * Tree::build). Recomputes the forward expansions first. FP64 contexts only.
+ * No reference counterpart; for differentiable simulation and for checking
+ * through torch.autograd. */
 GALAX_API int galax_fmm_vjp(galax_fmm_t* h, const galax_bodies_t* bodies, const double* ax_bar, const double* ay_bar,
-                  double* x_bar, double* y_bar, double* m_bar);
+                            double* x_bar, double* y_bar, double* m_bar);
+
+/* As galax_fmm_vjp on device arrays (x, y, m are sanitized in place). */
+GALAX_API int galax_fmm_vjp_device(galax_fmm_t* h, double* d_x, double* d_y, double* d_m, const double* d_ax_bar,
+                                   const double* d_ay_bar, double* d_x_bar, double* d_y_bar, double* d_m_bar);
 
 /* ~compute_n2_force (galax-core/src/lib.rs:1685-1721) / GpuContext::n2_debug
  * (galax-gpu/src/lib.rs:670-714) in FP64 on the device, for the targets listed
