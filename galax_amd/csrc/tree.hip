@@ -608,57 +608,6 @@ int build_tree_constrained(galax_fmm* h, size_t n) {
 
 // ------------------------------------------------------------ lists ----
 
-// Same-level neighbours: for each node P, the ids Q != P on P's level whose
-// boxes touch P's box, in ascending id order. The traversal descends from the
-// root through subtree boxes; a left-to-right preorder reaches the nodes of
-// one level in ascending id order because parents are created in the order
-// of their consecutive children.
-template <bool FILL>
-__global__ void k_same_level_nbrs(NodeArrays a, uint32_t lo, uint32_t hi, uint32_t root, const double* __restrict__ ux0,
-                                  const double* __restrict__ ux1, const double* __restrict__ uy0,
-                                  const double* __restrict__ uy1, uint32_t* __restrict__ cnt_or_off,
-                                  uint32_t* __restrict__ out, uint32_t* __restrict__ err) {
-  uint32_t P = lo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (P >= hi) return;
-  const uint8_t lvP = a.lv[P];
-  const double cx = a.cx[P], cy = a.cy[P], hw = a.hw[P];
-  const double tol = 1e-15 + 1e-12 * (fabs(cx) + fabs(cy) + hw + 1.0);
-  const double qx0 = cx - hw - tol, qx1 = cx + hw + tol, qy0 = cy - hw - tol, qy1 = cy + hw + tol;
-  uint32_t base = FILL ? cnt_or_off[P] : 0u;
-  uint32_t cnt = 0;
-  uint32_t stack[DFS_STACK];
-  int sp = 0;
-  stack[sp++] = root;
-  while (sp > 0) {
-    uint32_t nd = stack[--sp];
-    if (a.lv[nd] == lvP) {
-      if (nd != P) {
-        double dx = fabs(a.cx[nd] - cx), dy = fabs(a.cy[nd] - cy);
-        double hs = hw + a.hw[nd];
-        bool t;
-        if (lvP == 0)  // build_p2p_lists form, galax-core/src/lib.rs:1380-1383
-          t = dx <= hs + 1e-15 && dy <= hs + 1e-15;
-        else  // InteractionLists::build form, galax-core/src/lib.rs:990-995
-          t = !(dx > hs + 1e-15 || dy > hs + 1e-15);
-        if (t) {
-          if (FILL) out[base + cnt] = nd;
-          ++cnt;
-        }
-      }
-      continue;
-    }
-    if (ux0[nd] > qx1 || ux1[nd] < qx0 || uy0[nd] > qy1 || uy1[nd] < qy0) continue;
-    uint32_t cf = a.cf[nd];
-    for (int c = (int)a.cc[nd] - 1; c >= 0; --c) {
-      if (sp < DFS_STACK)
-        stack[sp++] = cf + (uint32_t)c;
-      else
-        atomicOr(err, 1u);
-    }
-  }
-  if (!FILL) cnt_or_off[P] = cnt;
-}
-
 __device__ __forceinline__ bool touch_child(const NodeArrays& a, uint32_t child, uint32_t node) {
   // galax-core/src/lib.rs:1000-1004
   double cdx = fabs(a.cx[child] - a.cx[node]), cdy = fabs(a.cy[child] - a.cy[node]);
@@ -666,52 +615,66 @@ __device__ __forceinline__ bool touch_child(const NodeArrays& a, uint32_t child,
   return cdx <= hs + 1e-15 && cdy <= hs + 1e-15;
 }
 
-// M2L list of node c: children of the parent's same-level neighbours (ascending
-// neighbour id, then child order) that do not touch c, capped at 64.
-template <bool FILL>
-__global__ void k_m2l_lists(NodeArrays a, uint32_t nn, const uint32_t* __restrict__ nb_off,
-                            const uint32_t* __restrict__ nb_dat, uint32_t* __restrict__ cnt_or_off,
-                            uint32_t* __restrict__ out) {
-  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nn) return;
-  uint32_t P = a.par[c];
-  uint32_t cnt = 0, base = FILL ? cnt_or_off[c] : 0u;
-  if (P != NONE) {
-    for (uint32_t k = nb_off[P]; k < nb_off[P + 1] && cnt < LIST_CAP; ++k) {
-      uint32_t Q = nb_dat[k];
-      uint32_t cf = a.cf[Q];
-      for (uint32_t q = cf; q < cf + a.cc[Q]; ++q) {
-        if (touch_child(a, q, c)) continue;
-        if (cnt < LIST_CAP) {
-          if (FILL) out[base + cnt] = q;
-          ++cnt;
-        }
+// M2L lists, as InteractionLists::build (galax-core/src/lib.rs:972-1016): for
+// node c with parent P, scan the nodes pn of P's level in ascending id, skip
+// pn == P and those whose box does not touch P's (the reference's test, with
+// its `>` form), and append every child q of pn whose box does not touch c,
+// until 64 entries. One thread per parent P fills the rows of all of P's
+// children in one scan of the level; rows are 64 slots wide, like the
+// reference's [u32; 64], and compacted afterwards. The level is scanned
+// directly: parent half-widths double every round, so at the upper levels
+// nearly every box touches every other and pruning by box gains nothing.
+__global__ void k_m2l_rows(NodeArrays a, uint32_t plo, uint32_t phi, uint32_t* __restrict__ rows,
+                           uint32_t* __restrict__ lens) {
+  uint32_t P = plo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (P >= phi) return;
+  const uint32_t cf = a.cf[P];
+  const uint32_t cc = a.cc[P];
+  const double pcx = a.cx[P], pcy = a.cy[P], phw = a.hw[P];
+  uint32_t cnt[4] = {0, 0, 0, 0};
+  uint32_t full = 0;
+  for (uint32_t pn = plo; pn < phi && full < cc; ++pn) {
+    if (pn == P) continue;
+    const double dx = fabs(a.cx[pn] - pcx), dy = fabs(a.cy[pn] - pcy);
+    const double h_sum = phw + a.hw[pn];
+    if (dx > h_sum + 1e-15 || dy > h_sum + 1e-15) continue;
+    const uint32_t qf = a.cf[pn], qc = a.cc[pn];
+    for (uint32_t q = qf; q < qf + qc; ++q) {
+      for (uint32_t k = 0; k < cc; ++k) {
+        if (cnt[k] >= LIST_CAP) continue;
+        if (touch_child(a, q, cf + k)) continue;
+        rows[(size_t)(cf + k) * LIST_CAP + cnt[k]] = q;
+        if (++cnt[k] == LIST_CAP) ++full;
       }
     }
   }
-  if (!FILL) cnt_or_off[c] = cnt;
+  for (uint32_t k = 0; k < cc; ++k) lens[cf + k] = cnt[k];
 }
 
-// P2P list of leaf a: a itself, then its touching leaves in ascending id, capped at 64.
-template <bool FILL>
-__global__ void k_p2p_lists(uint32_t n_leaves, uint32_t nn, const uint32_t* __restrict__ nb_off,
-                            const uint32_t* __restrict__ nb_dat, uint32_t* __restrict__ cnt_or_off,
-                            uint32_t* __restrict__ out) {
+// P2P lists, as build_p2p_lists (galax-core/src/lib.rs:1371-1389): the leaf
+// itself, then every other leaf in ascending id whose box touches it, until 64.
+__global__ void k_p2p_rows(NodeArrays a, uint32_t n_leaves, uint32_t* __restrict__ rows, uint32_t* __restrict__ lens) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_leaves) return;
+  const double cx = a.cx[i], cy = a.cy[i], hw = a.hw[i];
+  uint32_t* row = rows + (size_t)i * LIST_CAP;
+  row[0] = i;
+  uint32_t cnt = 1;
+  for (uint32_t b = 0; b < n_leaves && cnt < LIST_CAP; ++b) {
+    if (b == i) continue;
+    const double dx = fabs(cx - a.cx[b]), dy = fabs(cy - a.cy[b]);
+    const double hs = hw + a.hw[b];
+    if (dx <= hs + 1e-15 && dy <= hs + 1e-15) row[cnt++] = b;
+  }
+  lens[i] = cnt;
+}
+
+__global__ void k_rows_compact(const uint32_t* __restrict__ rows, const uint32_t* __restrict__ off, uint32_t nn,
+                               uint32_t* __restrict__ dat) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = (uint32_t)(t / LIST_CAP), k = (uint32_t)(t % LIST_CAP);
   if (i >= nn) return;
-  if (i >= n_leaves) {
-    if (!FILL) cnt_or_off[i] = 0;
-    return;
-  }
-  uint32_t nnb = nb_off[i + 1] - nb_off[i];
-  uint32_t cnt = 1 + (nnb < LIST_CAP - 1 ? nnb : LIST_CAP - 1);
-  if (!FILL) {
-    cnt_or_off[i] = cnt;
-    return;
-  }
-  uint32_t base = cnt_or_off[i];
-  out[base] = i;
-  for (uint32_t k = 1; k < cnt; ++k) out[base + k] = nb_dat[nb_off[i] + k - 1];
+  if (k < off[i + 1] - off[i]) dat[off[i] + k] = rows[t];
 }
 
 // Symmetric P2P gather lists from the reference lists: leaf t gathers from
@@ -826,55 +789,37 @@ int build_lists_reference(galax_fmm* h) {
     t.m2l_total = t.p2p_total = t.gat_total = 0;
     return 0;
   }
-  GX_CHECK(compute_uboxes(h));
-  double* u = h->ws.h.as<double>();
   NodeArrays a = node_arrays(h);
-  // same-level neighbours of every node
+  GX_CHECK(dbuf_reserve(h->ws.d, nn * LIST_CAP * sizeof(uint32_t), s));
   GX_CHECK(dbuf_reserve(h->ws.c, (nn + 1) * sizeof(uint32_t), s));
-  GX_CHECK(dbuf_reserve(h->flag, 64, s));
-  uint32_t* err = h->flag.as<uint32_t>();
-  GX_HIP(hipMemsetAsync(err, 0, sizeof(uint32_t), s));
-  uint32_t* nb_off = h->ws.c.as<uint32_t>();
-  // the count kernels write n entries; entry n must be 0 before the scan
-  GX_HIP(hipMemsetAsync(nb_off, 0, (nn + 1) * sizeof(uint32_t), s));
-  k_same_level_nbrs<false><<<nblk(nn, 128), 128, 0, s>>>(a, 0, (uint32_t)nn, (uint32_t)(nn - 1), u, u + nn,
-                                                         u + 2 * nn, u + 3 * nn, nb_off, nullptr, err);
-  GX_HIP(hipGetLastError());
-  uint32_t nb_total = 0;
-  GX_CHECK(scan_u32(h, nb_off, nb_off, nn + 1, false, false, &nb_total));
-  GX_CHECK(dbuf_reserve(h->ws.d, (size_t)nb_total * sizeof(uint32_t) + 4, s));
-  uint32_t* nb_dat = h->ws.d.as<uint32_t>();
-  k_same_level_nbrs<true><<<nblk(nn, 128), 128, 0, s>>>(a, 0, (uint32_t)nn, (uint32_t)(nn - 1), u, u + nn,
-                                                        u + 2 * nn, u + 3 * nn, nb_off, nb_dat, err);
-  GX_HIP(hipGetLastError());
-  uint32_t herr = 0;
-  GX_HIP(hipMemcpyAsync(&herr, err, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  GX_HIP(hipStreamSynchronize(s));
-  if (herr) return set_error(GALAX_ERR_INVALID, "neighbour traversal stack overflow (tree too deep)");
-  // M2L lists
+  uint32_t* rows = h->ws.d.as<uint32_t>();
+  uint32_t* lens = h->ws.c.as<uint32_t>();
+  // M2L lists: parents level by level
+  GX_HIP(hipMemsetAsync(lens, 0, (nn + 1) * sizeof(uint32_t), s));
+  for (uint32_t l = 1; l < t.n_levels; ++l) {
+    uint32_t lo = t.level_offset[l], hi = t.level_offset[l + 1];
+    k_m2l_rows<<<nblk(hi - lo, 128), 128, 0, s>>>(a, lo, hi, rows, lens);
+    GX_HIP(hipGetLastError());
+  }
   GX_CHECK(dbuf_reserve(t.m2l_off, (nn + 1) * sizeof(uint32_t), s));
-  uint32_t* moff = t.m2l_off.as<uint32_t>();
-  GX_HIP(hipMemsetAsync(moff, 0, (nn + 1) * sizeof(uint32_t), s));
-  k_m2l_lists<false><<<nblk(nn), 256, 0, s>>>(a, (uint32_t)nn, nb_off, nb_dat, moff, nullptr);
-  GX_HIP(hipGetLastError());
   uint32_t mt = 0;
-  GX_CHECK(scan_u32(h, moff, moff, nn + 1, false, false, &mt));
+  GX_CHECK(scan_u32(h, lens, t.m2l_off.as<uint32_t>(), nn + 1, false, false, &mt));
   t.m2l_total = mt;
   GX_CHECK(dbuf_reserve(t.m2l_dat, (size_t)mt * sizeof(uint32_t) + 4, s));
-  k_m2l_lists<true><<<nblk(nn), 256, 0, s>>>(a, (uint32_t)nn, nb_off, nb_dat, moff, t.m2l_dat.as<uint32_t>());
+  k_rows_compact<<<nblk(nn * LIST_CAP), 256, 0, s>>>(rows, t.m2l_off.as<uint32_t>(), (uint32_t)nn,
+                                                     t.m2l_dat.as<uint32_t>());
   GX_HIP(hipGetLastError());
-  // P2P lists
+  // P2P lists (leaves only; other rows empty)
+  GX_HIP(hipMemsetAsync(lens, 0, (nn + 1) * sizeof(uint32_t), s));
+  k_p2p_rows<<<nblk(t.n_leaves, 128), 128, 0, s>>>(a, (uint32_t)t.n_leaves, rows, lens);
+  GX_HIP(hipGetLastError());
   GX_CHECK(dbuf_reserve(t.p2p_off, (nn + 1) * sizeof(uint32_t), s));
-  uint32_t* poff = t.p2p_off.as<uint32_t>();
-  GX_HIP(hipMemsetAsync(poff, 0, (nn + 1) * sizeof(uint32_t), s));
-  k_p2p_lists<false><<<nblk(nn), 256, 0, s>>>((uint32_t)t.n_leaves, (uint32_t)nn, nb_off, nb_dat, poff, nullptr);
-  GX_HIP(hipGetLastError());
   uint32_t pt = 0;
-  GX_CHECK(scan_u32(h, poff, poff, nn + 1, false, false, &pt));
+  GX_CHECK(scan_u32(h, lens, t.p2p_off.as<uint32_t>(), nn + 1, false, false, &pt));
   t.p2p_total = pt;
   GX_CHECK(dbuf_reserve(t.p2p_dat, (size_t)pt * sizeof(uint32_t) + 4, s));
-  k_p2p_lists<true><<<nblk(nn), 256, 0, s>>>((uint32_t)t.n_leaves, (uint32_t)nn, nb_off, nb_dat, poff,
-                                             t.p2p_dat.as<uint32_t>());
+  k_rows_compact<<<nblk(t.n_leaves * LIST_CAP), 256, 0, s>>>(rows, t.p2p_off.as<uint32_t>(), (uint32_t)t.n_leaves,
+                                                             t.p2p_dat.as<uint32_t>());
   GX_HIP(hipGetLastError());
   GX_CHECK(build_gather(h));
   h->keys_valid = false;

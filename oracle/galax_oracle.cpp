@@ -1216,6 +1216,37 @@ ORC_API void orc_n2_force(const double* x, const double* y, const double* m, siz
   }
 }
 
+// Adjoint of compute_n2_force (no reference counterpart; the closed form of
+// the derivative of the direct sum): for cotangents (axb, ayb),
+//   xb_j = sum_{i!=j} K(x_i - x_j) (m_j a_i - m_i a_j),  K = d/dd [d / (|d|^2+eps^2)^{3/2}]
+//   mb_j = -sum_{i!=j} a_i . (x_i - x_j) / (|x_i - x_j|^2 + eps^2)^{3/2}
+ORC_API void orc_n2_vjp(const double* x, const double* y, const double* m, size_t n, double eps, const double* axb,
+                        const double* ayb, double* xb, double* yb, double* mb, int threads) {
+  double eps2 = eps * eps;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads > 0 ? threads : 1)
+#endif
+  for (long jj = 0; jj < (long)n; ++jj) {
+    size_t j = (size_t)jj;
+    double gx = 0.0, gy = 0.0, gm = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+      if (i == j) continue;
+      double dx = x[i] - x[j], dy = y[i] - y[j];
+      double r2 = dx * dx + dy * dy + eps2;
+      double inv = 1.0 / std::sqrt(r2);
+      double inv3 = inv * inv * inv, inv5 = inv3 * inv * inv;
+      double vx = m[j] * axb[i] - m[i] * axb[j], vy = m[j] * ayb[i] - m[i] * ayb[j];
+      double dv = dx * vx + dy * vy;
+      gx += vx * inv3 - 3.0 * dx * dv * inv5;
+      gy += vy * inv3 - 3.0 * dy * dv * inv5;
+      gm -= (axb[i] * dx + ayb[i] * dy) * inv3;
+    }
+    xb[j] = gx;
+    yb[j] = gy;
+    mb[j] = gm;
+  }
+}
+
 // Diagnostics — galax-core/src/lib.rs:1637-1683. out[6] = {KE, PE heuristic,
 // px, py, Lz, -} computed sequentially in index order.
 ORC_API void orc_diagnostics(const double* x, const double* y, const double* vx, const double* vy, const double* m,

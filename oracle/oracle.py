@@ -78,6 +78,7 @@ def lib():
     L.orc_compute_fmm_force.argtypes = [_vp] * 6 + [_sz, _vp, _vp, _vp, C.c_int, _d, C.c_int, C.c_int, _vp]
     L.orc_n2_force.argtypes = [_vp, _vp, _vp, _sz, _d, _vp, _sz, _vp, _vp, _vp, C.c_int]
     L.orc_diagnostics.argtypes = [_vp] * 7 + [_sz, _vp]
+    L.orc_n2_vjp.argtypes = [_vp, _vp, _vp, _sz, _d, _vp, _vp, _vp, _vp, _vp, C.c_int]
     L.orc_simulate.argtypes = [_vp] * 7 + [_sz, _vp, _vp, _vp, C.c_int, _d, _d, _u64, C.c_int, C.c_int]
     L.orc_default_softening.restype = _d
     L.orc_default_softening.argtypes = [_vp, _vp, _sz]
@@ -334,6 +335,15 @@ def n2_force(x, y, m, eps, targets=None, threads=1, want_phi=False):
     phi = np.zeros(nt) if want_phi else None
     lib().orc_n2_force(_p(x), _p(y), _p(m), n, eps, _p(idx), nt, _p(ax), _p(ay), _p(phi), threads)
     return ax, ay, phi
+
+
+def n2_vjp(x, y, m, eps, axb, ayb, threads=1):
+    """Closed-form adjoint of the direct sum: returns (x_bar, y_bar, m_bar)."""
+    x, y, m, axb, ayb = (_f64(a) for a in (x, y, m, axb, ayb))
+    n = len(x)
+    xb, yb, mb = np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().orc_n2_vjp(_p(x), _p(y), _p(m), n, eps, _p(axb), _p(ayb), _p(xb), _p(yb), _p(mb), threads)
+    return xb, yb, mb
 
 
 def diagnostics(x, y, vx, vy, m, ax, ay):

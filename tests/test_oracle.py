@@ -624,3 +624,30 @@ def test_default_softening_and_dt():  # galax-cli/src/main.rs:182-190, 232-244
     assert O.default_softening(x, y) == pytest.approx(0.1 * (5 + 4 + 3) / 3)
     assert O.default_dt(np.array([3.0]), np.array([4.0])) == pytest.approx(0.5 / math.sqrt(5))
     assert O.default_dt(np.array([0.0]), np.array([0.0])) == 0.01
+
+
+def test_n2_vjp_closed_form_matches_central_differences():
+    """The direct-sum adjoint (the checker of the VJP) against central
+    differences of the direct sum itself."""
+    r = ic.SplitMix64(55)
+    n = 40
+    x, y = 10 * r.uniform(n), 10 * r.uniform(n)
+    m = 0.5 + r.uniform(n)
+    eps = 0.3
+    axb, ayb = r.normal(n), r.normal(n)
+    xb, yb, mb = O.n2_vjp(x, y, m, eps, axb, ayb)
+    h = 1e-6
+
+    def f(xx, yy, mm):
+        ax, ay, _ = O.n2_force(xx, yy, mm, eps)
+        return float(np.dot(ax, axb) + np.dot(ay, ayb))
+
+    for k in range(0, n, 7):
+        for arr, g in ((x, xb), (y, yb), (m, mb)):
+            old = arr[k]
+            arr[k] = old + h
+            fp = f(x, y, m)
+            arr[k] = old - h
+            fm = f(x, y, m)
+            arr[k] = old
+            assert (fp - fm) / (2 * h) == pytest.approx(g[k], rel=1e-6, abs=1e-8)
