@@ -26,6 +26,8 @@ class FmmForce(torch.autograd.Function):
         xs, ys, ms = x.detach().clone(), y.detach().clone(), m.detach().clone()
         if x.is_cuda:
             ax, ay = torch.empty_like(xs), torch.empty_like(xs)
+            # the library runs on its own stream: torch's pending work must be done
+            torch.cuda.current_stream(x.device).synchronize()
             gctx.evaluate_device(xs, ys, ms, ax, ay, None)
         else:
             b = fmm.BodiesSoA(xs.numpy(), ys.numpy(), np.zeros(len(xs)), np.zeros(len(xs)), ms.numpy(),
@@ -45,6 +47,7 @@ class FmmForce(torch.autograd.Function):
         if xs.is_cuda:
             x2, y2, m2 = xs.clone(), ys.clone(), ms.clone()
             xb, yb, mb = torch.empty_like(xs), torch.empty_like(xs), torch.empty_like(xs)
+            torch.cuda.current_stream(xs.device).synchronize()
             gctx.vjp_device(x2, y2, m2, gax, gay, xb, yb, mb)
         else:
             b = fmm.BodiesSoA(xs.numpy().copy(), ys.numpy().copy(), np.zeros(len(xs)), np.zeros(len(xs)),

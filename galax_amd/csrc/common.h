@@ -53,6 +53,15 @@ struct DBuf {
 int dbuf_reserve(DBuf& b, size_t bytes, hipStream_t s);
 void dbuf_free(DBuf& b);
 
+// P2P work items over gather lists: item i covers target leaf tgt[i] and the
+// gather entries [g0[i], g1[i]). A target with several items writes partial
+// near fields to slots slot[i] and a combine pass sums them in item order;
+// a target with one item (slot NONE) writes its result directly.
+struct P2PItems {
+  DBuf tgt, g0, g1, slot, part_off, multi;
+  size_t n_items = 0, n_multi = 0, n_part = 0;
+};
+
 // Device-side tree in structure-of-arrays form. Node ids are level-ordered
 // (leaves first, root last) and each node's children are consecutive ids
 // [child_first, child_first + child_count), as Tree::build creates them.
@@ -72,12 +81,14 @@ struct DevTree {
   DBuf m2l_off, m2l_dat, p2p_off, p2p_dat, gat_off, gat_dat;
   size_t m2l_total = 0, p2p_total = 0, gat_total = 0;
   uint64_t p2p_interactions = 0;  // directed body-body pairs of the gather lists
+  P2PItems items;
   size_t capacity = 0;  // node capacity of the node arrays
 };
 
 struct Workspace {
   DBuf a, b, c, d, e, f, g, h;  // scratch reused across build stages
   DBuf scan[6];                 // partial sums of the multi-level scans
+  DBuf sort_ka, sort_kb, sort_va, sort_vb, sort_hist;  // generic radix sort
 };
 
 }  // namespace gx
@@ -109,6 +120,8 @@ struct galax_fmm {
   gx::DBuf c_m2l_off, c_m2l_dat, c_gat_off, c_gat_dat, c_src_pairs;
   size_t c_m2l_total = 0, c_gat_total = 0;
   uint64_t c_p2p_interactions = 0;
+  gx::P2PItems c_items;
+  gx::DBuf p2p_part;  // partial near fields of targets split over several items
   bool corrected_ready = false;
   galax_timings_t timings{};
   hipEvent_t ev[12]{};
@@ -132,6 +145,11 @@ int install_tree_host(galax_fmm* h, const galax_tree_view_t* t, const galax_list
 int build_lists_corrected(galax_fmm* h, const double* d_x, const double* d_y);
 int ensure_node_capacity(galax_fmm* h, size_t nodes);
 int count_interactions(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, uint64_t* out);
+int radix_sort_bits(galax_fmm* h, uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb, size_t n, int bits);
+// P2P work items: each target leaf's gather list in chunks of at most
+// P2P_ITEM_LEAVES source leaves (balances leaves with thousands of sources).
+constexpr uint32_t P2P_ITEM_LEAVES = 8;
+int build_p2p_items(galax_fmm* h, const uint32_t* goff, uint32_t n_leaves, P2PItems& it);
 
 // ---------------------------------------------------------- evaluation ----
 int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, double* d_ay, double* d_phi);

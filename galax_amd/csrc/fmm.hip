@@ -512,38 +512,44 @@ struct Near<float> {
   static __device__ __forceinline__ float rsq(float v) { return __builtin_amdgcn_rsqf(v); }
 };
 
-// One 64-lane wave per target leaf. Lanes 0-31 and 32-63 hold the same 32
-// target bodies and split the sources of every staged chunk (even / odd), so
-// no lane idles at n_max = 32; the halves are combined by one xor-shuffle.
-// Far field: the leaf's local expansion at each body (FP64, exact). Near field:
-// the leaf's gather list, sources staged through LDS as (x, y) relative to the
-// leaf COM plus m, in T precision.
+// One 64-lane wave per work item (a target leaf and at most P2P_ITEM_LEAVES of
+// its gather list). Lanes 0-31 and 32-63 hold the same 32 target bodies and
+// split the sources of every staged chunk (even / odd), so no lane idles at
+// n_max = 32; the halves are combined by one xor-shuffle. The first item of a
+// target adds the far field: the leaf's local expansion at each body (FP64,
+// exact). Near field: sources staged through LDS as (x, y) relative to the
+// leaf COM plus m, in T precision. Items of a target with one item write the
+// result; otherwise each item writes a partial to its slot.
 template <int P, typename T, bool PHI>
 __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, const double* __restrict__ y,
                                                 const double* __restrict__ m, const uint32_t* __restrict__ bs,
                                                 const uint32_t* __restrict__ be, const double* __restrict__ comx,
                                                 const double* __restrict__ comy, const double* __restrict__ loc,
                                                 const uint32_t* __restrict__ goff, const uint32_t* __restrict__ gdat,
-                                                double eps2d, double* __restrict__ ax, double* __restrict__ ay,
-                                                double* __restrict__ phi) {
+                                                const uint32_t* __restrict__ itgt, const uint32_t* __restrict__ ig0,
+                                                const uint32_t* __restrict__ ig1, const uint32_t* __restrict__ islot,
+                                                uint32_t n_max, double eps2d, double* __restrict__ ax,
+                                                double* __restrict__ ay, double* __restrict__ phi,
+                                                double* __restrict__ part) {
   constexpr int S = num_moments(P);
   typedef typename Near<T>::V2 V2;
   __shared__ V2 sxy[64];
   __shared__ T sm[64];
-  const uint32_t leaf = blockIdx.x;
+  const uint32_t item = blockIdx.x;
+  const uint32_t leaf = itgt[item], g0 = ig0[item], g1 = ig1[item], slot = islot[item];
+  const bool first = g0 == goff[leaf];
   const int lane = threadIdx.x;
   const int tl = lane & 31, half = lane >> 5;
   const uint32_t ts = bs[leaf], te = be[leaf];
   const double ox = comx[leaf], oy = comy[leaf];
   const T eps2 = (T)eps2d;
-  const uint32_t g0 = goff[leaf], g1 = goff[leaf + 1];
   for (uint32_t tb = ts; tb < te; tb += 32) {
     const uint32_t i = tb + tl;
     const bool valid = i < te;
     const double xi = valid ? x[i] : ox, yi = valid ? y[i] : oy;
     const T xr = (T)(xi - ox), yr = (T)(yi - oy);
     double fx = 0.0, fy = 0.0, fp = 0.0;
-    if (valid && half == 0) l2p_body<P>(loc + (size_t)leaf * S, xi - ox, yi - oy, &fx, &fy, &fp);
+    if (first && valid && half == 0) l2p_body<P>(loc + (size_t)leaf * S, xi - ox, yi - oy, &fx, &fy, &fp);
     T accx = 0, accy = 0, accp = 0;
     for (uint32_t g = g0; g < g1; ++g) {
       const uint32_t sl = gdat[g];
@@ -580,10 +586,43 @@ __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, co
     accy += __shfl_xor(accy, 32, 64);
     if (PHI) accp += __shfl_xor(accp, 32, 64);
     if (valid && half == 0) {
-      ax[i] = (0.0 - fx) + (double)accx;
-      ay[i] = (0.0 - fy) + (double)accy;
-      if (PHI) phi[i] = fp + (double)accp;
+      const double rx = (0.0 - fx) + (double)accx;
+      const double ry = (0.0 - fy) + (double)accy;
+      const double rp = fp + (double)accp;
+      if (slot == NONE) {
+        ax[i] = rx;
+        ay[i] = ry;
+        if (PHI) phi[i] = rp;
+      } else {
+        double* pp = part + ((size_t)slot * n_max + (i - ts)) * 3;
+        pp[0] = rx;
+        pp[1] = ry;
+        pp[2] = rp;
+      }
     }
+  }
+}
+
+// targets split over several items: sum their partials in item order
+__global__ void k_p2p_combine(const uint32_t* __restrict__ multi, uint32_t n_multi, const uint32_t* __restrict__ goff,
+                              const uint32_t* __restrict__ part_off, const uint32_t* __restrict__ bs,
+                              const uint32_t* __restrict__ be, uint32_t n_max, const double* __restrict__ part,
+                              double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ phi) {
+  const uint32_t q = blockIdx.x;
+  if (q >= n_multi) return;
+  const uint32_t t = multi[q];
+  const uint32_t nit = (goff[t + 1] - goff[t] + P2P_ITEM_LEAVES - 1) / P2P_ITEM_LEAVES;
+  for (uint32_t b = bs[t] + threadIdx.x; b < be[t]; b += blockDim.x) {
+    double sx = 0.0, sy = 0.0, sp = 0.0;
+    for (uint32_t k = 0; k < nit; ++k) {
+      const double* pp = part + ((size_t)(part_off[t] + k) * n_max + (b - bs[t])) * 3;
+      sx += pp[0];
+      sy += pp[1];
+      sp += pp[2];
+    }
+    ax[b] = sx;
+    ay[b] = sy;
+    if (phi) phi[b] = sp;
   }
 }
 
@@ -835,11 +874,15 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   GX_HIP(hipEventRecord(ev[6], s));
   const uint32_t* goff = corrected ? h->c_gat_off.as<uint32_t>() : t.gat_off.as<uint32_t>();
   const uint32_t* gdat = corrected ? h->c_gat_dat.as<uint32_t>() : t.gat_dat.as<uint32_t>();
+  const P2PItems& it = corrected ? h->c_items : t.items;
+  const uint32_t nmax = h->cfg.n_max;
+  GX_CHECK(dbuf_reserve(h->p2p_part, std::max<size_t>(it.n_part, 1) * nmax * 3 * 8, s));
   const bool fp32_near = h->cfg.precision != GALAX_PREC_FP64;
-#define GX_LAUNCH_P2P(T, PH)                                                                                       \
-  k_l2p_p2p<P, T, PH><<<(unsigned)nl, 64, 0, s>>>(d_x, d_y, d_m, t.body_start.as<uint32_t>(),                    \
-                                                   t.body_end.as<uint32_t>(), comx, comy, loc, goff, gdat, eps * eps, \
-                                                   d_ax, d_ay, d_phi)
+#define GX_LAUNCH_P2P(T, PH)                                                                                 \
+  k_l2p_p2p<P, T, PH><<<(unsigned)it.n_items, 64, 0, s>>>(                                                  \
+      d_x, d_y, d_m, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), comx, comy, loc, goff, gdat,     \
+      it.tgt.as<uint32_t>(), it.g0.as<uint32_t>(), it.g1.as<uint32_t>(), it.slot.as<uint32_t>(), nmax, eps * eps, \
+      d_ax, d_ay, d_phi, h->p2p_part.as<double>())
   if (fp32_near) {
     if (d_phi)
       GX_LAUNCH_P2P(float, true);
@@ -850,6 +893,14 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
       GX_LAUNCH_P2P(double, true);
     else
       GX_LAUNCH_P2P(double, false);
+  }
+  GX_HIP(hipGetLastError());
+  if (it.n_multi) {
+    k_p2p_combine<<<(unsigned)it.n_multi, 64, 0, s>>>(it.multi.as<uint32_t>(), (uint32_t)it.n_multi, goff,
+                                                       it.part_off.as<uint32_t>(), t.body_start.as<uint32_t>(),
+                                                       t.body_end.as<uint32_t>(), nmax, h->p2p_part.as<double>(), d_ax,
+                                                       d_ay, d_phi);
+    GX_HIP(hipGetLastError());
   }
 #undef GX_LAUNCH_P2P
   GX_HIP(hipGetLastError());

@@ -130,17 +130,21 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t* __res
   }
 }
 
-// Sorts keys (in a) with payload vals (in va); result in place. kb/vb scratch.
-static int radix_sort(galax_fmm* h, uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb, size_t n) {
+// Sorts keys (in ka) with payload vals (in va) on the low `bits` key bits;
+// result in place. kb/vb are scratch of n entries each.
+int radix_sort_bits(galax_fmm* h, uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb, size_t n, int bits) {
   if (n <= 1) return 0;
   hipStream_t s = h->stream;
   uint32_t nb = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
-  GX_CHECK(dbuf_reserve(h->ws.d, (size_t)RS_RADIX * nb * sizeof(uint32_t), s));
-  uint32_t* hist = h->ws.d.as<uint32_t>();
+  GX_CHECK(dbuf_reserve(h->ws.sort_hist, (size_t)RS_RADIX * nb * sizeof(uint32_t), s));
+  uint32_t* hist = h->ws.sort_hist.as<uint32_t>();
+  int passes = (bits + RS_BITS - 1) / RS_BITS;
+  if (passes & 1) ++passes;  // an even number of passes leaves the result in (ka, va)
   uint64_t *ki = ka, *ko = kb;
   uint32_t *vi = va, *vo = vb;
-  for (int pass = 0; pass < 6; ++pass) {
+  for (int pass = 0; pass < passes; ++pass) {
     int shift = pass * RS_BITS;
+    if (shift >= 64) shift = 63;  // padding pass: every digit equal, order kept
     k_rs_hist<<<nb, RS_THREADS, 0, s>>>(ki, n, shift, hist, nb);
     GX_HIP(hipGetLastError());
     GX_CHECK(scan_u32(h, hist, hist, (size_t)RS_RADIX * nb, false, false, nullptr));
@@ -149,8 +153,11 @@ static int radix_sort(galax_fmm* h, uint64_t* ka, uint32_t* va, uint64_t* kb, ui
     std::swap(ki, ko);
     std::swap(vi, vo);
   }
-  // 6 passes: the result is back in (ka, va)
   return 0;
+}
+
+static int radix_sort(galax_fmm* h, uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb, size_t n) {
+  return radix_sort_bits(h, ka, va, kb, vb, n, 2 * MORTON_BITS);
 }
 
 // ---------------------------------------------------------- permute ----
@@ -712,20 +719,49 @@ __global__ void k_gather_fill(uint32_t n_leaves, const uint32_t* __restrict__ of
   }
 }
 
-__global__ void k_sort_segments(uint32_t n_seg, const uint32_t* __restrict__ off, uint32_t* __restrict__ dat,
-                                uint32_t skip_front) {
-  uint32_t sgi = blockIdx.x * blockDim.x + threadIdx.x;
-  if (sgi >= n_seg) return;
-  uint32_t s = off[sgi] + skip_front, e = off[sgi + 1];
-  for (uint32_t i = s + 1; i < e; ++i) {
-    uint32_t v = dat[i];
-    uint32_t j = i;
-    while (j > s && dat[j - 1] > v) {
-      dat[j] = dat[j - 1];
-      --j;
+
+// Symmetric P2P gather lists from the reference lists: leaf t gathers from
+// itself, from b in list(t) with b > t, and from every a < t with t in list(a)
+// (galax-core/src/lib.rs:1415-1437 computes pair (a,b) iff b in list(a), a < b).
+// Every directed pair (t, s) becomes a key t * 2^w + s; one radix sort gives
+// the CSR in ascending (t, s) order. A leaf can sit in the lists of thousands
+// of smaller leaves (a large outer box touching a dense core), so the lists
+// are unbounded and are not sorted per segment.
+__global__ void k_pair_count(uint32_t n_leaves, const uint32_t* __restrict__ off, const uint32_t* __restrict__ dat,
+                             uint32_t* __restrict__ cnt) {
+  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n_leaves) return;
+  uint32_t c = 1;  // self
+  for (uint32_t k = off[a]; k < off[a + 1]; ++k) c += dat[k] > a ? 2u : 0u;
+  cnt[a] = c;
+}
+
+__global__ void k_pair_emit(uint32_t n_leaves, const uint32_t* __restrict__ off, const uint32_t* __restrict__ dat,
+                            const uint32_t* __restrict__ pos, int w, uint64_t* __restrict__ keys,
+                            uint32_t* __restrict__ vals) {
+  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n_leaves) return;
+  uint32_t p = pos[a];
+  keys[p] = ((uint64_t)a << w) | a;
+  vals[p++] = 0;
+  for (uint32_t k = off[a]; k < off[a + 1]; ++k) {
+    uint32_t b = dat[k];
+    if (b > a) {
+      keys[p] = ((uint64_t)a << w) | b;
+      vals[p++] = 0;
+      keys[p] = ((uint64_t)b << w) | a;
+      vals[p++] = 0;
     }
-    dat[j] = v;
   }
+}
+
+__global__ void k_pair_split(const uint64_t* __restrict__ keys, uint32_t n, int w, uint32_t* __restrict__ dat,
+                             uint32_t* __restrict__ cnt) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t k = keys[i];
+  dat[i] = (uint32_t)(k & ((1ull << w) - 1));
+  atomicAdd(&cnt[(uint32_t)(k >> w)], 1u);
 }
 
 // Directed body-body interactions of the gather lists (self pairs excluded).
@@ -756,28 +792,104 @@ int count_interactions(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat,
   return 0;
 }
 
+static int bits_for(uint64_t n) {
+  int w = 1;
+  while (w < 32 && (1ull << w) < n) ++w;
+  return w;
+}
+
+__global__ void k_items_count(const uint32_t* __restrict__ goff, uint32_t nl, uint32_t* __restrict__ nit,
+                              uint32_t* __restrict__ npart, uint32_t* __restrict__ ismulti) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nl) return;
+  uint32_t len = goff[t + 1] - goff[t];
+  uint32_t n = (len + P2P_ITEM_LEAVES - 1) / P2P_ITEM_LEAVES;
+  if (n == 0) n = 1;
+  nit[t] = n;
+  npart[t] = n > 1 ? n : 0u;
+  ismulti[t] = n > 1 ? 1u : 0u;
+}
+
+__global__ void k_items_fill(const uint32_t* __restrict__ goff, uint32_t nl, const uint32_t* __restrict__ ioff,
+                             const uint32_t* __restrict__ poff, const uint32_t* __restrict__ moff,
+                             uint32_t* __restrict__ tgt, uint32_t* __restrict__ g0, uint32_t* __restrict__ g1,
+                             uint32_t* __restrict__ slot, uint32_t* __restrict__ multi) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nl) return;
+  uint32_t n = ioff[t + 1] - ioff[t];
+  for (uint32_t k = 0; k < n; ++k) {
+    uint32_t i = ioff[t] + k;
+    uint32_t a = goff[t] + k * P2P_ITEM_LEAVES;
+    tgt[i] = t;
+    g0[i] = a;
+    g1[i] = min(a + P2P_ITEM_LEAVES, goff[t + 1]);
+    slot[i] = n > 1 ? poff[t] + k : NONE;
+  }
+  if (n > 1) multi[moff[t]] = t;
+}
+
+int build_p2p_items(galax_fmm* h, const uint32_t* goff, uint32_t nl, P2PItems& it) {
+  hipStream_t s = h->stream;
+  GX_CHECK(dbuf_reserve(it.part_off, (size_t)(nl + 1) * 4, s));
+  GX_CHECK(dbuf_reserve(h->ws.f, (size_t)(nl + 1) * 8, s));
+  uint32_t* ioff = h->ws.f.as<uint32_t>();
+  uint32_t* moff = ioff + nl + 1;
+  uint32_t* poff = it.part_off.as<uint32_t>();
+  GX_HIP(hipMemsetAsync(ioff, 0, (size_t)(nl + 1) * 8, s));
+  GX_HIP(hipMemsetAsync(poff, 0, (size_t)(nl + 1) * 4, s));
+  k_items_count<<<nblk(nl), 256, 0, s>>>(goff, nl, ioff, poff, moff);
+  GX_HIP(hipGetLastError());
+  uint32_t ni = 0, np = 0, nm = 0;
+  GX_CHECK(scan_u32(h, ioff, ioff, nl + 1, false, false, &ni));
+  GX_CHECK(scan_u32(h, poff, poff, nl + 1, false, false, &np));
+  GX_CHECK(scan_u32(h, moff, moff, nl + 1, false, false, &nm));
+  it.n_items = ni;
+  it.n_part = np;
+  it.n_multi = nm;
+  GX_CHECK(dbuf_reserve(it.tgt, (size_t)ni * 4 + 4, s));
+  GX_CHECK(dbuf_reserve(it.g0, (size_t)ni * 4 + 4, s));
+  GX_CHECK(dbuf_reserve(it.g1, (size_t)ni * 4 + 4, s));
+  GX_CHECK(dbuf_reserve(it.slot, (size_t)ni * 4 + 4, s));
+  GX_CHECK(dbuf_reserve(it.multi, (size_t)nm * 4 + 4, s));
+  k_items_fill<<<nblk(nl), 256, 0, s>>>(goff, nl, ioff, poff, moff, it.tgt.as<uint32_t>(), it.g0.as<uint32_t>(),
+                                        it.g1.as<uint32_t>(), it.slot.as<uint32_t>(), it.multi.as<uint32_t>());
+  GX_HIP(hipGetLastError());
+  GX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
 static int build_gather(galax_fmm* h) {
   DevTree& t = h->tree;
   hipStream_t s = h->stream;
   uint32_t nl = (uint32_t)t.n_leaves;
-  GX_CHECK(dbuf_reserve(t.gat_off, (nl + 1) * sizeof(uint32_t), s));
   GX_CHECK(dbuf_reserve(h->ws.e, (nl + 1) * sizeof(uint32_t), s));
-  uint32_t* goff = t.gat_off.as<uint32_t>();
-  GX_HIP(hipMemsetAsync(goff, 0, (nl + 1) * sizeof(uint32_t), s));
-  k_gather_count<<<nblk(nl), 256, 0, s>>>(nl, t.p2p_off.as<uint32_t>(), t.p2p_dat.as<uint32_t>(), goff);
+  uint32_t* cnt = h->ws.e.as<uint32_t>();
+  GX_HIP(hipMemsetAsync(cnt, 0, (nl + 1) * sizeof(uint32_t), s));
+  k_pair_count<<<nblk(nl), 256, 0, s>>>(nl, t.p2p_off.as<uint32_t>(), t.p2p_dat.as<uint32_t>(), cnt);
   GX_HIP(hipGetLastError());
   uint32_t total = 0;
-  GX_CHECK(scan_u32(h, goff, goff, nl + 1, false, false, &total));
+  GX_CHECK(scan_u32(h, cnt, cnt, nl + 1, false, false, &total));
   t.gat_total = total;
+  const int w = bits_for(nl);
+  GX_CHECK(dbuf_reserve(h->ws.sort_ka, (size_t)total * 8 + 8, s));
+  GX_CHECK(dbuf_reserve(h->ws.sort_kb, (size_t)total * 8 + 8, s));
+  GX_CHECK(dbuf_reserve(h->ws.sort_va, (size_t)total * 4 + 4, s));
+  GX_CHECK(dbuf_reserve(h->ws.sort_vb, (size_t)total * 4 + 4, s));
+  uint64_t* keys = h->ws.sort_ka.as<uint64_t>();
+  k_pair_emit<<<nblk(nl), 256, 0, s>>>(nl, t.p2p_off.as<uint32_t>(), t.p2p_dat.as<uint32_t>(), cnt, w, keys,
+                                       h->ws.sort_va.as<uint32_t>());
+  GX_HIP(hipGetLastError());
+  GX_CHECK(radix_sort_bits(h, keys, h->ws.sort_va.as<uint32_t>(), h->ws.sort_kb.as<uint64_t>(),
+                           h->ws.sort_vb.as<uint32_t>(), total, 2 * w));
+  GX_CHECK(dbuf_reserve(t.gat_off, (nl + 1) * sizeof(uint32_t), s));
   GX_CHECK(dbuf_reserve(t.gat_dat, (size_t)total * sizeof(uint32_t) + 4, s));
-  uint32_t* cur = h->ws.e.as<uint32_t>();
-  GX_HIP(hipMemsetAsync(cur, 0, (nl + 1) * sizeof(uint32_t), s));
-  k_gather_fill<<<nblk(nl), 256, 0, s>>>(nl, t.p2p_off.as<uint32_t>(), t.p2p_dat.as<uint32_t>(), goff, cur,
-                                         t.gat_dat.as<uint32_t>());
+  uint32_t* goff = t.gat_off.as<uint32_t>();
+  GX_HIP(hipMemsetAsync(goff, 0, (nl + 1) * sizeof(uint32_t), s));
+  k_pair_split<<<nblk(total), 256, 0, s>>>(keys, total, w, t.gat_dat.as<uint32_t>(), goff);
   GX_HIP(hipGetLastError());
-  k_sort_segments<<<nblk(nl), 256, 0, s>>>(nl, goff, t.gat_dat.as<uint32_t>(), 1);
-  GX_HIP(hipGetLastError());
+  GX_CHECK(scan_u32(h, goff, goff, nl + 1, false, false, nullptr));
   GX_CHECK(count_interactions(h, goff, t.gat_dat.as<uint32_t>(), &t.p2p_interactions));
+  GX_CHECK(build_p2p_items(h, goff, nl, t.items));
   return 0;
 }
 
@@ -864,6 +976,9 @@ int install_tree_host(galax_fmm* h, const galax_tree_view_t* tv, const galax_lis
     if (k == 0) {
       ++nl;
       nb = std::max<size_t>(nb, be[i]);
+      if (be[i] - bs[i] > h->cfg.n_max)
+        return set_error(GALAX_ERR_INVALID, "leaf %zu holds %u bodies, more than n_max = %u", i, be[i] - bs[i],
+                         h->cfg.n_max);
     }
   }
   for (size_t i = 1; i < nn; ++i)

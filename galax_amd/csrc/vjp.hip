@@ -48,16 +48,20 @@ __device__ __forceinline__ double dbinom(int n, int k) {
 
 // ------------------------------------------------------------- near ----
 
-// one wave per target leaf j-block; lanes 0-31 / 32-63 split the sources
+// one wave per P2P work item (target leaf and a chunk of its gather list);
+// lanes 0-31 / 32-63 split the sources; items of split targets write partials
 __global__ void __launch_bounds__(64) k_vjp_near(const double* __restrict__ x, const double* __restrict__ y,
                                                  const double* __restrict__ m, const double* __restrict__ axb,
                                                  const double* __restrict__ ayb, const uint32_t* __restrict__ bs,
-                                                 const uint32_t* __restrict__ be, const uint32_t* __restrict__ goff,
-                                                 const uint32_t* __restrict__ gdat, double eps2,
-                                                 double* __restrict__ xb, double* __restrict__ yb,
-                                                 double* __restrict__ mb) {
+                                                 const uint32_t* __restrict__ be, const uint32_t* __restrict__ gdat,
+                                                 const uint32_t* __restrict__ itgt, const uint32_t* __restrict__ ig0,
+                                                 const uint32_t* __restrict__ ig1, const uint32_t* __restrict__ islot,
+                                                 uint32_t n_max, double eps2, double* __restrict__ xb,
+                                                 double* __restrict__ yb, double* __restrict__ mb,
+                                                 double* __restrict__ part) {
   __shared__ double sx[64], sy[64], sm[64], sax[64], say[64];
-  const uint32_t leaf = blockIdx.x;
+  const uint32_t item = blockIdx.x;
+  const uint32_t leaf = itgt[item], g0 = ig0[item], g1 = ig1[item], slot = islot[item];
   const int lane = threadIdx.x, tl = lane & 31, half = lane >> 5;
   const uint32_t ts = bs[leaf], te = be[leaf];
   for (uint32_t tb = ts; tb < te; tb += 32) {
@@ -66,7 +70,7 @@ __global__ void __launch_bounds__(64) k_vjp_near(const double* __restrict__ x, c
     const double xj = valid ? x[j] : 0.0, yj = valid ? y[j] : 0.0, mj = valid ? m[j] : 0.0;
     const double abx = valid ? axb[j] : 0.0, aby = valid ? ayb[j] : 0.0;
     double gx_ = 0.0, gy_ = 0.0, gm_ = 0.0;
-    for (uint32_t g = goff[leaf]; g < goff[leaf + 1]; ++g) {
+    for (uint32_t g = g0; g < g1; ++g) {
       const uint32_t sl = gdat[g];
       const uint32_t ss = bs[sl], se = be[sl];
       for (uint32_t sb = ss; sb < se; sb += 64) {
@@ -102,10 +106,40 @@ __global__ void __launch_bounds__(64) k_vjp_near(const double* __restrict__ x, c
     gy_ += __shfl_xor(gy_, 32, 64);
     gm_ += __shfl_xor(gm_, 32, 64);
     if (valid && half == 0) {
-      xb[j] = gx_;
-      yb[j] = gy_;
-      mb[j] = gm_;
+      if (slot == NONE) {
+        xb[j] = gx_;
+        yb[j] = gy_;
+        mb[j] = gm_;
+      } else {
+        double* pp = part + ((size_t)slot * n_max + (j - ts)) * 3;
+        pp[0] = gx_;
+        pp[1] = gy_;
+        pp[2] = gm_;
+      }
     }
+  }
+}
+
+__global__ void k_vjp_near_combine(const uint32_t* __restrict__ multi, uint32_t n_multi,
+                                   const uint32_t* __restrict__ goff, const uint32_t* __restrict__ part_off,
+                                   const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be, uint32_t n_max,
+                                   const double* __restrict__ part, double* __restrict__ xb, double* __restrict__ yb,
+                                   double* __restrict__ mb) {
+  const uint32_t q = blockIdx.x;
+  if (q >= n_multi) return;
+  const uint32_t t = multi[q];
+  const uint32_t nit = (goff[t + 1] - goff[t] + P2P_ITEM_LEAVES - 1) / P2P_ITEM_LEAVES;
+  for (uint32_t b = bs[t] + threadIdx.x; b < be[t]; b += blockDim.x) {
+    double sx = 0.0, sy = 0.0, sm = 0.0;
+    for (uint32_t k = 0; k < nit; ++k) {
+      const double* pp = part + ((size_t)(part_off[t] + k) * n_max + (b - bs[t])) * 3;
+      sx += pp[0];
+      sy += pp[1];
+      sm += pp[2];
+    }
+    xb[b] = sx;
+    yb[b] = sy;
+    mb[b] = sm;
   }
 }
 
@@ -344,30 +378,15 @@ __global__ void __launch_bounds__(256) k_vjp_m2l_exact(const uint32_t* __restric
 }
 
 // reverse CSR of the live reference pairs: rpair lists, per source node, the
-// pair indices e (ascending) whose source is that node
-__global__ void k_rev_count(const uint32_t* __restrict__ dat, uint32_t n_pairs, uint32_t* __restrict__ cnt) {
+// pair indices e (ascending) whose source is that node; built by one radix
+// sort of the keys source * 2^w + e (a node can be the source of many pairs)
+__global__ void k_rev_keys(const uint32_t* __restrict__ dat, uint32_t n_pairs, int w, uint64_t* __restrict__ keys,
+                           uint32_t* __restrict__ vals, uint32_t* __restrict__ cnt) {
   uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < n_pairs) atomicAdd(&cnt[dat[e]], 1u);
-}
-__global__ void k_rev_fill(const uint32_t* __restrict__ dat, uint32_t n_pairs, const uint32_t* __restrict__ roff,
-                           uint32_t* __restrict__ cursor, uint32_t* __restrict__ rpair) {
-  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < n_pairs) {
-    uint32_t s = dat[e];
-    rpair[roff[s] + atomicAdd(&cursor[s], 1u)] = e;
-  }
-}
-__global__ void k_rev_sort(uint32_t n, const uint32_t* __restrict__ off, uint32_t* __restrict__ dat) {
-  uint32_t sgi = blockIdx.x * blockDim.x + threadIdx.x;
-  if (sgi >= n) return;
-  for (uint32_t i = off[sgi] + 1; i < off[sgi + 1]; ++i) {
-    uint32_t v = dat[i], j = i;
-    while (j > off[sgi] && dat[j - 1] > v) {
-      dat[j] = dat[j - 1];
-      --j;
-    }
-    dat[j] = v;
-  }
+  if (e >= n_pairs) return;
+  keys[e] = ((uint64_t)dat[e] << w) | e;
+  vals[e] = e;
+  atomicAdd(&cnt[dat[e]], 1u);
 }
 
 template <int P>
@@ -391,9 +410,21 @@ static int vjp_impl(galax_fmm* h, const double* x, const double* y, const double
   const uint32_t* goff = corrected ? h->c_gat_off.as<uint32_t>() : t.gat_off.as<uint32_t>();
   const uint32_t* gdat = corrected ? h->c_gat_dat.as<uint32_t>() : t.gat_dat.as<uint32_t>();
   // near field
-  k_vjp_near<<<(unsigned)nl, 64, 0, s>>>(x, y, m, axb, ayb, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(),
-                                         goff, gdat, eps2, xb, yb, mb);
+  const P2PItems& it = corrected ? h->c_items : t.items;
+  const uint32_t nmax = h->cfg.n_max;
+  GX_CHECK(dbuf_reserve(h->p2p_part, std::max<size_t>(it.n_part, 1) * nmax * 3 * 8, s));
+  k_vjp_near<<<(unsigned)it.n_items, 64, 0, s>>>(x, y, m, axb, ayb, t.body_start.as<uint32_t>(),
+                                                 t.body_end.as<uint32_t>(), gdat, it.tgt.as<uint32_t>(),
+                                                 it.g0.as<uint32_t>(), it.g1.as<uint32_t>(), it.slot.as<uint32_t>(),
+                                                 nmax, eps2, xb, yb, mb, h->p2p_part.as<double>());
   GX_HIP(hipGetLastError());
+  if (it.n_multi) {
+    k_vjp_near_combine<<<(unsigned)it.n_multi, 64, 0, s>>>(it.multi.as<uint32_t>(), (uint32_t)it.n_multi, goff,
+                                                            it.part_off.as<uint32_t>(), t.body_start.as<uint32_t>(),
+                                                            t.body_end.as<uint32_t>(), nmax, h->p2p_part.as<double>(),
+                                                            xb, yb, mb);
+    GX_HIP(hipGetLastError());
+  }
   // L2P adjoint -> Lbar of leaves
   k_vjp_l2p_coef<P><<<nblk(nl, 4), 256, 0, s>>>(x, y, axb, ayb, t.body_start.as<uint32_t>(),
                                                 t.body_end.as<uint32_t>(), comx, comy, (uint32_t)nl, lbar);
@@ -413,19 +444,24 @@ static int vjp_impl(galax_fmm* h, const double* x, const double* y, const double
   } else if (h->n_live_pairs > 0) {
     const uint32_t live_hi = t.level_offset[top];
     if (!h->vj_rev_valid) {
+      const uint32_t ne = (uint32_t)h->n_live_pairs;
+      int w = 1;
+      while (w < 32 && (1ull << w) < ne) ++w;
+      int ws_ = 1;
+      while (ws_ < 32 && (1ull << ws_) < live_hi) ++ws_;
       GX_CHECK(dbuf_reserve(h->vj_roff, (size_t)(live_hi + 1) * 4, s));
-      GX_CHECK(dbuf_reserve(h->vj_rpair, h->n_live_pairs * 4, s));
-      GX_CHECK(dbuf_reserve(h->ws.a, (size_t)(live_hi + 1) * 4, s));
+      GX_CHECK(dbuf_reserve(h->vj_rpair, (size_t)ne * 4 + 4, s));
+      GX_CHECK(dbuf_reserve(h->ws.sort_ka, (size_t)ne * 8 + 8, s));
+      GX_CHECK(dbuf_reserve(h->ws.sort_kb, (size_t)ne * 8 + 8, s));
+      GX_CHECK(dbuf_reserve(h->ws.sort_vb, (size_t)ne * 4 + 4, s));
       uint32_t* roff = h->vj_roff.as<uint32_t>();
       GX_HIP(hipMemsetAsync(roff, 0, (size_t)(live_hi + 1) * 4, s));
-      GX_HIP(hipMemsetAsync(h->ws.a.p, 0, (size_t)(live_hi + 1) * 4, s));
-      k_rev_count<<<nblk(h->n_live_pairs), 256, 0, s>>>(t.m2l_dat.as<uint32_t>(), (uint32_t)h->n_live_pairs, roff);
+      k_rev_keys<<<nblk(ne), 256, 0, s>>>(t.m2l_dat.as<uint32_t>(), ne, w, h->ws.sort_ka.as<uint64_t>(),
+                                          h->vj_rpair.as<uint32_t>(), roff);
       GX_HIP(hipGetLastError());
+      GX_CHECK(radix_sort_bits(h, h->ws.sort_ka.as<uint64_t>(), h->vj_rpair.as<uint32_t>(),
+                               h->ws.sort_kb.as<uint64_t>(), h->ws.sort_vb.as<uint32_t>(), ne, w + ws_));
       GX_CHECK(scan_u32(h, roff, roff, live_hi + 1, false, false, nullptr));
-      k_rev_fill<<<nblk(h->n_live_pairs), 256, 0, s>>>(t.m2l_dat.as<uint32_t>(), (uint32_t)h->n_live_pairs, roff,
-                                                       h->ws.a.as<uint32_t>(), h->vj_rpair.as<uint32_t>());
-      k_rev_sort<<<nblk(live_hi), 256, 0, s>>>(live_hi, roff, h->vj_rpair.as<uint32_t>());
-      GX_HIP(hipGetLastError());
       h->vj_rev_valid = true;
     }
     k_vjp_m2l_ref<P><<<nblk(live_hi, 4), 256, 0, s>>>(h->vj_roff.as<uint32_t>(), h->vj_rpair.as<uint32_t>(), live_hi,
