@@ -44,7 +44,8 @@ def main():
     ap.add_argument("--err-samples", type=int, default=10_000)
     ap.add_argument("--cpu-sample-n", type=int, default=1 << 18)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--form", default="mixed_fp32", choices=["fp64", "mixed_fp32", "mixed_bf16x3"],
+    ap.add_argument("--vjp-steps", type=int, default=3)
+    ap.add_argument("--form", default="mixed_fp32", choices=["fp64", "fp64_exact", "mixed_fp32", "mixed_bf16x3"],
                     help="the form whose throughput is `value`")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -59,7 +60,7 @@ def main():
     d, spec = ic.make(args.workload)
     n = len(d["x"])
     algo = fmm.ALGO_CORRECTED if args.algo == "corrected" else fmm.ALGO_REFERENCE
-    forms = [("fp64", fmm.PREC_FP64), ("mixed_fp32", fmm.PREC_MIXED_FP32),
+    forms = [("fp64", fmm.PREC_FP64), ("fp64_exact", fmm.PREC_FP64_EXACT), ("mixed_fp32", fmm.PREC_MIXED_FP32),
              ("mixed_bf16x3", fmm.PREC_MIXED_BF16X3)]
     headline = args.form
     res = {}
@@ -102,8 +103,24 @@ def main():
         ms = 1e3 * t_total / args.steps
         passes = {k: v / args.steps for k, v in passes.items()}
         log(f"[bench] {name}: {ms:.3f} ms/step (build {t_build:.3f}s, first eval {t_first:.3f}s); passes {passes}")
+        vjp_ms = None
+        if name in ("fp64", headline) and args.vjp_steps > 0:
+            # VJP of the accelerations (forward recomputed inside, adjoint passes in FP64)
+            g = torch.Generator(device="cpu").manual_seed(7)
+            axb = torch.randn(n, generator=g, dtype=torch.float64).to(dev)
+            ayb = torch.randn(n, generator=g, dtype=torch.float64).to(dev)
+            xb, yb, mb = torch.empty_like(x), torch.empty_like(x), torch.empty_like(x)
+            ctx.vjp_device(x, y, m, axb, ayb, xb, yb, mb)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.vjp_steps):
+                ctx.vjp_device(x, y, m, axb, ayb, xb, yb, mb)
+            torch.cuda.synchronize()
+            vjp_ms = 1e3 * (time.perf_counter() - t0) / args.vjp_steps
+            log(f"[bench] {name}: vjp {vjp_ms:.3f} ms ({vjp_ms / ms:.2f}x the forward)")
+            del axb, ayb, xb, yb, mb
         res[name] = dict(ms=ms, passes=passes, t_build=t_build, t_first=t_first, tm=tm, info=info,
-                         ax=ax.cpu().numpy(), ay=ay.cpu().numpy(), bodies=b)
+                         ax=ax.cpu().numpy(), ay=ay.cpu().numpy(), bodies=b, vjp_ms=vjp_ms)
         del ctx, x, y, m, ax, ay, phi
         torch.cuda.empty_cache()
 
@@ -136,7 +153,7 @@ def main():
     I = tm.p2p_interactions
     n_leaves = info["n_leaves"]
     dominant = max(("l2p_p2p", "m2l", "p2m", "m2m", "l2l"), key=lambda k: passes.get(k, 0.0))
-    fp32 = headline != "fp64"
+    fp32 = headline.startswith("mixed")
     peak_c = FP32_PEAK_TFS if fp32 else FP64_PEAK_TFS
     flops = {"l2p_p2p": 17.0 * I + 4.0 * S * n, "m2l": 2.0 * S * S * tm.m2l_pairs_executed}
     bytes_ = {"p2m": 24.0 * n + n_leaves * S * 8.0, "m2m": 2.0 * info["n_nodes"] * S * 8.0,
@@ -175,14 +192,16 @@ def main():
         "cpu_baseline": cpu,
         "forms": {k: {"ms_per_step": r["ms"], "bodies_per_s": n / (r["ms"] * 1e-3), "passes_ms": r["passes"],
                       "t_build_s": r["t_build"], "t_first_eval_s": r["t_first"],
-                      "p2p_interactions_per_s": I / (r["passes"]["l2p_p2p"] * 1e-3), **errs[k]}
+                      "p2p_interactions_per_s": I / (r["passes"]["l2p_p2p"] * 1e-3),
+                      "vjp_ms": r["vjp_ms"], "vjp_over_fwd": r["vjp_ms"] / r["ms"] if r["vjp_ms"] else None,
+                      **errs[k]}
                   for k, r in res.items()},
         "tree": info,
         "p2p_interactions": I,
         "m2l_pairs_executed": tm.m2l_pairs_executed,
         "m2l_pairs_reference": tm.m2l_pairs_reference,
         "m2l_keys": tm.m2l_keys,
-        "vjp_over_fwd": None,
+        "vjp_over_fwd": H["vjp_ms"] / H["ms"] if H["vjp_ms"] else None,
     }
     print(json.dumps(out), flush=True)
 

@@ -245,7 +245,7 @@ int check_cfg(const galax_fmm_config_t* c) {
     return set_error(GALAX_ERR_INVALID, "softening must be finite and >= 0");
   if (c->tree_mode > GALAX_TREE_CONSTRAINED) return set_error(GALAX_ERR_INVALID, "unknown tree_mode");
   if (c->algo > GALAX_ALGO_CORRECTED) return set_error(GALAX_ERR_INVALID, "unknown algo");
-  if (c->precision > GALAX_PREC_MIXED_BF16X3) return set_error(GALAX_ERR_INVALID, "unknown precision");
+  if (c->precision > GALAX_PREC_FP64_EXACT) return set_error(GALAX_ERR_INVALID, "unknown precision");
   if (c->algo == GALAX_ALGO_CORRECTED && !(c->theta > 0.0 && c->theta < 1.0))
     return set_error(GALAX_ERR_INVALID, "theta must be in (0, 1) for the corrected variant");
   return 0;
@@ -260,9 +260,8 @@ int reserve_bodies(galax_fmm* h, size_t n) {
 
 int after_tree_installed(galax_fmm* h) {
   h->has_tree = true;
-  h->vj_rev_valid = false;
   h->keys_valid = false;
-  h->mx_valid = false;
+  h->mx_valid = h->mx_adj_valid = h->mx_t64_valid = false;
   h->corrected_ready = false;
   return 0;
 }
@@ -324,16 +323,29 @@ void galax_fmm_destroy(galax_fmm_t* h) {
   DBuf* bufs[] = {&h->x, &h->y, &h->m, &h->vx, &h->vy, &h->ax, &h->ay, &h->phi, &h->mp, &h->loc, &h->mp32,
                   &h->loc32, &h->gtab, &h->key_of_pair, &h->perm, &h->hash, &h->tmp_u32, &h->flag,
                   &h->c_m2l_off, &h->c_m2l_dat, &h->c_gat_off, &h->c_gat_dat, &h->c_src_pairs,
-                  &h->key_dxy, &h->mx_pos, &h->mx_src, &h->mx_tiles, &h->mx_that, &h->mx_sig, &h->mx_R,
-                  &h->vj_lbar, &h->vj_mbar, &h->vj_leaf_of, &h->vj_roff, &h->vj_rpair, &h->vj_fwd};
+                  &h->key_dxy, &h->mx_t64, &h->mx_that, &h->mx_sig,
+                  &h->vj_lbar, &h->vj_mbar, &h->vj_leaf_of, &h->vj_fwd, &h->vj_out,
+                  &h->mx_fwd.src, &h->mx_fwd.tile_off, &h->mx_fwd.step_key, &h->mx_fwd.step_first,
+                  &h->mx_fwd.step_cnt, &h->mx_fwd.item_tile, &h->mx_fwd.item_s0, &h->mx_fwd.item_s1,
+                  &h->mx_fwd.item_slot, &h->mx_fwd.multi_tile, &h->mx_fwd.multi_slot, &h->mx_fwd.multi_cnt,
+                  &h->mx_fwd.part, &h->mx_adj.src, &h->mx_adj.tile_off, &h->mx_adj.step_key,
+                  &h->mx_adj.step_first, &h->mx_adj.step_cnt, &h->mx_adj.item_tile, &h->mx_adj.item_s0,
+                  &h->mx_adj.item_s1, &h->mx_adj.item_slot, &h->mx_adj.multi_tile, &h->mx_adj.multi_slot,
+                  &h->mx_adj.multi_cnt, &h->mx_adj.part};
   for (DBuf* b : bufs) dbuf_free(*b);
   DevTree& t = h->tree;
   DBuf* tb[] = {&t.center_x, &t.center_y, &t.half_width, &t.mass, &t.com_x, &t.com_y, &t.body_start, &t.body_end,
                 &t.child_first, &t.child_count, &t.parent, &t.level, &t.keys, &t.m2l_off, &t.m2l_dat, &t.p2p_off,
                 &t.p2p_dat, &t.gat_off, &t.gat_dat};
   for (DBuf* b : tb) dbuf_free(*b);
+  for (P2PItems* it : {&t.items, &h->c_items}) {
+    DBuf* ib[] = {&it->tgt, &it->g0, &it->g1, &it->slot, &it->part_off, &it->multi};
+    for (DBuf* b : ib) dbuf_free(*b);
+  }
+  dbuf_free(h->p2p_part);
   Workspace& w = h->ws;
-  DBuf* wb[] = {&w.a, &w.b, &w.c, &w.d, &w.e, &w.f, &w.g, &w.h};
+  DBuf* wb[] = {&w.a, &w.b, &w.c, &w.d, &w.e, &w.f, &w.g, &w.h, &w.sort_ka, &w.sort_kb, &w.sort_va, &w.sort_vb,
+                &w.sort_hist};
   for (DBuf* b : wb) dbuf_free(*b);
   for (DBuf& b : w.scan) dbuf_free(b);
   for (hipEvent_t& e : h->ev)
@@ -423,7 +435,7 @@ int galax_fmm_set_softening(galax_fmm_t* h, double softening) {
     return set_error(GALAX_ERR_INVALID, "softening must be finite and >= 0");
   h->cfg.softening = softening;
   h->keys_valid = false;  // derivative tables depend on eps
-  h->mx_valid = false;
+  h->mx_valid = h->mx_adj_valid = h->mx_t64_valid = false;
   return 0;
 }
 
@@ -571,8 +583,9 @@ int galax_fmm_vjp(galax_fmm_t* h, const galax_bodies_t* b, const double* axb, co
   GX_CHECK(up(s, h->m, b->m, n));
   GX_CHECK(up(s, h->ax, axb, n));  // the body buffers hold the cotangents and adjoints
   GX_CHECK(up(s, h->ay, ayb, n));
-  GX_CHECK(dbuf_reserve(h->ws.b, std::max<size_t>(n, 1) * 24, s));
-  double* out = h->ws.b.as<double>();
+  // a buffer of its own: the passes run inside reuse the workspace
+  GX_CHECK(dbuf_reserve(h->vj_out, std::max<size_t>(n, 1) * 24, s));
+  double* out = h->vj_out.as<double>();
   GX_CHECK(prepare_eval(h, h->x.as<double>(), h->y.as<double>()));
   GX_CHECK(gx::vjp(h, h->x.as<double>(), h->y.as<double>(), h->m.as<double>(), h->ax.as<double>(),
                    h->ay.as<double>(), out, out + n, out + 2 * n));

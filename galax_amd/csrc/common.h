@@ -62,6 +62,20 @@ struct P2PItems {
   size_t n_items = 0, n_multi = 0, n_part = 0;
 };
 
+// M2L work in tiles of 16 rows (targets, or sources for the adjoint). The live
+// pairs are grouped by (row, key); a tile's steps are the distinct keys of its
+// groups, step_first/step_cnt give each of its 16 columns' group as a range
+// of `src` (the other end of each pair, in list order). Work items cover at
+// most MX_ITEM_STEPS steps of one tile; tiles with several items write
+// partial results to slots that a combine pass sums in item order.
+struct M2LTiles {
+  DBuf src, tile_off, step_key, step_first, step_cnt;
+  DBuf item_tile, item_s0, item_s1, item_slot;  // per work item
+  DBuf multi_tile, multi_slot, multi_cnt;       // per tile with several items
+  DBuf part;                                    // partial results, 16 x S doubles per slot
+  size_t n_rows = 0, ntiles = 0, nsteps = 0, nitems = 0, nmulti = 0, nslots = 0;
+};
+
 // Device-side tree in structure-of-arrays form. Node ids are level-ordered
 // (leaves first, root last) and each node's children are consecutive ids
 // [child_first, child_first + child_count), as Tree::build creates them.
@@ -105,13 +119,13 @@ struct galax_fmm {
   // expansions and M2L key tables
   gx::DBuf mp, loc, mp32, loc32, gtab, key_of_pair, perm, hash, tmp_u32, flag;
   gx::DBuf key_dxy;  // representative offset (dx, dy) per M2L key
-  // mixed-precision M2L: pairs grouped by key into tiles of 16
-  gx::DBuf mx_pos, mx_src, mx_tiles, mx_that, mx_sig, mx_R;
-  size_t mx_npad = 0, mx_ntiles = 0;
-  bool mx_valid = false;
+  // M2L on the matrix cores (m2l_mfma.hip): forward tiles over targets,
+  // adjoint tiles over sources, and the per-key operators
+  gx::M2LTiles mx_fwd, mx_adj;
+  gx::DBuf mx_t64, mx_that, mx_sig;
+  bool mx_valid = false, mx_adj_valid = false, mx_t64_valid = false;
   // adjoint (VJP) buffers
-  gx::DBuf vj_lbar, vj_mbar, vj_leaf_of, vj_roff, vj_rpair, vj_fwd;
-  bool vj_rev_valid = false;
+  gx::DBuf vj_lbar, vj_mbar, vj_leaf_of, vj_fwd, vj_out;
   size_t n_keys = 0;
   size_t n_live_targets = 0, n_live_pairs = 0;
   bool keys_valid = false;
@@ -148,8 +162,12 @@ int count_interactions(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat,
 int radix_sort_bits(galax_fmm* h, uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb, size_t n, int bits);
 // P2P work items: each target leaf's gather list in chunks of at most
 // P2P_ITEM_LEAVES source leaves (balances leaves with thousands of sources).
-constexpr uint32_t P2P_ITEM_LEAVES = 8;
+constexpr uint32_t P2P_ITEM_LEAVES = 32;
 int build_p2p_items(galax_fmm* h, const uint32_t* goff, uint32_t n_leaves, P2PItems& it);
+// Sums the partial near-field slots of targets split over several items into
+// (o0, o1, o2) per body (o2 may be null); three doubles per body and slot.
+hipError_t p2p_combine(const P2PItems& it, const uint32_t* goff, const uint32_t* bs, const uint32_t* be,
+                       uint32_t n_max, const double* part, double* o0, double* o1, double* o2, hipStream_t s);
 
 // ---------------------------------------------------------- evaluation ----
 int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, double* d_ay, double* d_phi);
@@ -157,6 +175,10 @@ int vjp(galax_fmm* h, const double* d_x, const double* d_y, const double* d_m, c
         const double* d_ayb, double* d_xb, double* d_yb, double* d_mb);
 int m2l_mfma_setup(galax_fmm* h);
 int m2l_mfma_run(galax_fmm* h, const double* mp, double* loc);
+// Adjoint of the reference M2L on the matrix cores (FP64):
+// mbar_s = sum over pairs (t, s) of T_K^T lbar_t, for sources s < n_live_targets.
+int m2l_adjoint_setup(galax_fmm* h);
+int m2l_adjoint_run(galax_fmm* h, const double* lbar, double* mbar);
 int direct_sum_device(hipStream_t s, const double* x, const double* y, const double* m, size_t n, double eps,
                       const uint64_t* targets, size_t nt, double* ax, double* ay, double* phi);
 int kdk_device(galax_fmm* h, double dt, uint64_t n_steps, uint64_t every, galax_diag_t* diag);

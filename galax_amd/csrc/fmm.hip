@@ -84,8 +84,12 @@ __global__ void k_sanitize(double* __restrict__ x, double* __restrict__ y, doubl
 
 // --------------------------------------------------------------- P2M ----
 
-// exact: one wave per leaf, one lane per coefficient, bodies in order
-// (galax-core/src/lib.rs:854-885; coeff = mass*dx^i*dy^j/(i!*j!), non-finite -> 0).
+// exact: one wave per leaf, bodies in order (galax-core/src/lib.rs:854-885;
+// coeff = mass*dx^i*dy^j/(i!*j!), non-finite -> 0). The wave first stages a
+// chunk of the leaf's (dx, dy, m) in LDS (dx = x - com computed once per body,
+// the same value every lane would compute), then each lane sums its
+// coefficient over the chunk in body order. With S <= 32 the two half-waves
+// take two leaves.
 template <int P>
 __global__ void __launch_bounds__(256) k_p2m(const double* __restrict__ x, const double* __restrict__ y,
                                              const double* __restrict__ m, const uint32_t* __restrict__ bs,
@@ -94,24 +98,50 @@ __global__ void __launch_bounds__(256) k_p2m(const double* __restrict__ x, const
                                              double* __restrict__ mp) {
 #pragma clang fp contract(off)
   constexpr int S = num_moments(P);
-  const uint32_t leaf = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (leaf >= n_leaves) return;
-  const uint32_t s = bs[leaf], e = be[leaf];
-  const double cx = comx[leaf], cy = comy[leaf];
-  for (int k = lane; k < S; k += 64) {
-    int i, j;
-    coef_ij<P>(k, i, j);
+  constexpr int LPW = S <= 32 ? 2 : 1;  // leaves per wave
+  constexpr int LW = 64 / LPW;          // lanes per leaf = staging chunk
+  __shared__ double sdx[4][64], sdy[4][64], smb[4][64];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int sub = lane / LW, sl = lane % LW;
+  const uint32_t leaf = (blockIdx.x * 4 + w) * LPW + sub;
+  const bool live = leaf < n_leaves;
+  const uint32_t s = live ? bs[leaf] : 0u, e = live ? be[leaf] : 0u;
+  const double cx = live ? comx[leaf] : 0.0, cy = live ? comy[leaf] : 0.0;
+  // chunks run in lockstep over the block (barriers): its longest leaf sets the count
+  __shared__ uint32_t blen[4];
+  uint32_t len = e - s;
+  if (LPW == 2) len = max(len, (uint32_t)__shfl_xor((int)len, 32, 64));
+  if (lane == 0) blen[w] = len;
+  __syncthreads();
+  len = max(max(blen[0], blen[1]), max(blen[2], blen[3]));
+  double* dxs = &sdx[w][sub * LW];
+  double* dys = &sdy[w][sub * LW];
+  double* ms = &smb[w][sub * LW];
+  for (int k0 = 0; k0 < S; k0 += LW) {
+    const int k = k0 + sl;
+    int i = 0, j = 0;
+    if (k < S) coef_ij<P>(k, i, j);
     const double div = c_tabs.fact[i] * c_tabs.fact[j];
     double acc = 0.0;
-    for (uint32_t b = s; b < e; ++b) {
-      double dx = x[b] - cx, dy = y[b] - cy, mb = m[b];
-      double px = 1.0, py = 1.0;
-      for (int q = 0; q < i; ++q) px = px * dx;
-      for (int q = 0; q < j; ++q) py = py * dy;
-      acc += mb * px * py / div;
+    for (uint32_t c = 0; c < len; c += LW) {
+      __syncthreads();
+      const uint32_t b = s + c + sl;
+      if (b < e) {
+        dxs[sl] = x[b] - cx;
+        dys[sl] = y[b] - cy;
+        ms[sl] = m[b];
+      }
+      __syncthreads();
+      const uint32_t n = e > s + c ? min((uint32_t)LW, e - s - c) : 0u;
+      for (uint32_t q = 0; q < n; ++q) {
+        const double dx = dxs[q], dy = dys[q], mb = ms[q];
+        double px = 1.0, py = 1.0;
+        for (int r = 0; r < i; ++r) px = px * dx;
+        for (int r = 0; r < j; ++r) py = py * dy;
+        acc += mb * px * py / div;
+      }
     }
-    mp[(size_t)leaf * S + k] = isfinite(acc) ? acc : 0.0;
+    if (live && k < S) mp[(size_t)leaf * S + k] = isfinite(acc) ? acc : 0.0;
   }
 }
 
@@ -233,30 +263,44 @@ __device__ __forceinline__ bool pack_key(double dx, double dy, uint64_t* out) {
   return true;
 }
 
-// one thread per target: insert each pair's key, keeping the smallest pair index
-__global__ void k_keys_insert(const uint32_t* __restrict__ off, const uint32_t* __restrict__ dat, uint32_t n_targets,
-                              const double* __restrict__ comx, const double* __restrict__ comy,
-                              unsigned long long* __restrict__ hkeys, uint32_t* __restrict__ hrep, uint64_t mask,
-                              uint32_t* __restrict__ tgt_of_pair, uint32_t* __restrict__ err) {
+// target of each live pair (CSR row index)
+__global__ void k_pair_targets(const uint32_t* __restrict__ off, uint32_t n_targets, uint32_t* __restrict__ tgt_of_pair) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_targets) return;
-  for (uint32_t e = off[t]; e < off[t + 1]; ++e) {
-    uint32_t s = dat[e];
-    tgt_of_pair[e] = t;
-    uint64_t key;
-    if (!pack_key(comx[t] - comx[s], comy[t] - comy[s], &key)) {
-      atomicOr(err, 1u);
-      continue;
+  for (uint32_t e = off[t]; e < off[t + 1]; ++e) tgt_of_pair[e] = t;
+}
+
+// One thread per pair. The representative of a key is its smallest pair index
+// (first touch in list order). A slot already holding the key and an index
+// <= e needs no atomic: slots only ever take a key once and indices only
+// decrease, so a plain read that shows both is final for this pair.
+__global__ void k_keys_insert(const uint32_t* __restrict__ dat, const uint32_t* __restrict__ tgt_of_pair,
+                              uint32_t n_pairs, const double* __restrict__ comx, const double* __restrict__ comy,
+                              unsigned long long* __restrict__ hkeys, uint32_t* __restrict__ hrep, uint64_t mask,
+                              uint32_t* __restrict__ err) {
+  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_pairs) return;
+  const uint32_t t = tgt_of_pair[e], s = dat[e];
+  uint64_t key;
+  if (!pack_key(comx[t] - comx[s], comy[t] - comy[s], &key)) {
+    atomicOr(err, 1u);
+    return;
+  }
+  uint64_t slot = mix64(key) & mask;
+  for (;;) {
+    const unsigned long long cur = __atomic_load_n(&hkeys[slot], __ATOMIC_RELAXED);
+    if (cur == key) {
+      if (__atomic_load_n(&hrep[slot], __ATOMIC_RELAXED) > e) atomicMin(&hrep[slot], e);
+      return;
     }
-    uint64_t slot = mix64(key) & mask;
-    for (;;) {
-      unsigned long long prev = atomicCAS(&hkeys[slot], (unsigned long long)EMPTY_KEY, (unsigned long long)key);
+    if (cur == EMPTY_KEY) {
+      const unsigned long long prev = atomicCAS(&hkeys[slot], (unsigned long long)EMPTY_KEY, (unsigned long long)key);
       if (prev == EMPTY_KEY || prev == key) {
         atomicMin(&hrep[slot], e);
-        break;
+        return;
       }
-      slot = (slot + 1) & mask;
     }
+    slot = (slot + 1) & mask;
   }
 }
 
@@ -503,23 +547,46 @@ template <typename T>
 struct Near;
 template <>
 struct Near<double> {
-  typedef double2 V2;
+  typedef double4 V4;
   static __device__ __forceinline__ double rsq(double v) { return rsqrt(v); }
 };
 template <>
 struct Near<float> {
-  typedef float2 V2;
+  typedef float4 V4;
   static __device__ __forceinline__ float rsq(float v) { return __builtin_amdgcn_rsqf(v); }
 };
 
+// Near-field sum of one target body over staged sources [j0, n) step 2.
+// MASK: the chunk holds the target's own leaf; slot `self` is the body itself
+// and contributes nothing (its inverse distance is forced to 0).
+template <typename T, bool PHI, bool MASK>
+__device__ __forceinline__ void near_sum(const typename Near<T>::V4* __restrict__ src, int j0, int n, int self, T xr,
+                                         T yr, T eps2, T& accx, T& accy, T& accp) {
+#pragma unroll 4
+  for (int jj = j0; jj < n; jj += 2) {
+    const typename Near<T>::V4 sj = src[jj];
+    const T dx = sj.x - xr, dy = sj.y - yr;
+    const T r2 = fma(dx, dx, fma(dy, dy, eps2));
+    T inv = Near<T>::rsq(r2);
+    if (MASK) inv = jj == self ? (T)0 : inv;
+    const T f = sj.z * (inv * inv * inv);
+    accx = fma(f, dx, accx);
+    accy = fma(f, dy, accy);
+    if (PHI) accp = fma(-sj.z, inv, accp);
+  }
+}
+
 // One 64-lane wave per work item (a target leaf and at most P2P_ITEM_LEAVES of
-// its gather list). Lanes 0-31 and 32-63 hold the same 32 target bodies and
-// split the sources of every staged chunk (even / odd), so no lane idles at
-// n_max = 32; the halves are combined by one xor-shuffle. The first item of a
-// target adds the far field: the leaf's local expansion at each body (FP64,
-// exact). Near field: sources staged through LDS as (x, y) relative to the
-// leaf COM plus m, in T precision. Items of a target with one item write the
-// result; otherwise each item writes a partial to its slot.
+// its gather list). The item's sources are staged through LDS in chunks of
+// P2P_CHUNK bodies as (x, y) relative to the target leaf's COM plus m, in T
+// precision. Lanes 0-31 and 32-63 hold the same 32 target bodies and take the
+// even / odd sources of a chunk; the halves are combined by one xor-shuffle.
+// Only the chunks that hold the target leaf itself (found among the item's
+// source leaves by ballot) run with the self-exclusion mask. The first item of
+// a target also adds the far field: the leaf's local expansion at each body
+// (FP64, exact). Targets with one item write the result; otherwise each item
+// writes a partial to its slot.
+constexpr int P2P_CHUNK = 256;
 template <int P, typename T, bool PHI>
 __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, const double* __restrict__ y,
                                                 const double* __restrict__ m, const uint32_t* __restrict__ bs,
@@ -532,9 +599,10 @@ __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, co
                                                 double* __restrict__ ay, double* __restrict__ phi,
                                                 double* __restrict__ part) {
   constexpr int S = num_moments(P);
-  typedef typename Near<T>::V2 V2;
-  __shared__ V2 sxy[64];
-  __shared__ T sm[64];
+  typedef typename Near<T>::V4 V4;
+  static_assert(P2P_ITEM_LEAVES <= 64, "one lane per source leaf of an item");
+  __shared__ V4 src[P2P_CHUNK];
+  __shared__ uint32_t l_ss[P2P_ITEM_LEAVES], l_off[P2P_ITEM_LEAVES];
   const uint32_t item = blockIdx.x;
   const uint32_t leaf = itgt[item], g0 = ig0[item], g1 = ig1[item], slot = islot[item];
   const bool first = g0 == goff[leaf];
@@ -543,6 +611,34 @@ __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, co
   const uint32_t ts = bs[leaf], te = be[leaf];
   const double ox = comx[leaf], oy = comy[leaf];
   const T eps2 = (T)eps2d;
+  // source leaves of the item: start and offset in the item's source order
+  const int nlv = (int)(g1 - g0);
+  uint32_t cnt = 0, ss = 0;
+  bool is_self = false;
+  if (lane < nlv) {
+    const uint32_t sl = gdat[g0 + lane];
+    ss = bs[sl];
+    cnt = be[sl] - ss;
+    is_self = sl == leaf;
+  }
+  uint32_t inc = cnt;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t v = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += v;
+  }
+  if (lane < nlv) {
+    l_ss[lane] = ss;
+    l_off[lane] = inc - cnt;
+  }
+  const uint32_t total = __shfl(inc, 63, 64);
+  // source-order range [self_lo, self_hi) of the target leaf's own bodies
+  const unsigned long long sm_ = __ballot(is_self);
+  uint32_t self_lo = 0, self_hi = 0;
+  if (sm_) {
+    self_lo = __shfl(inc - cnt, __ffsll((long long)sm_) - 1, 64);
+    self_hi = self_lo + (te - ts);
+  }
   for (uint32_t tb = ts; tb < te; tb += 32) {
     const uint32_t i = tb + tl;
     const bool valid = i < te;
@@ -551,35 +647,34 @@ __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, co
     double fx = 0.0, fy = 0.0, fp = 0.0;
     if (first && valid && half == 0) l2p_body<P>(loc + (size_t)leaf * S, xi - ox, yi - oy, &fx, &fy, &fp);
     T accx = 0, accy = 0, accp = 0;
-    for (uint32_t g = g0; g < g1; ++g) {
-      const uint32_t sl = gdat[g];
-      const uint32_t ss = bs[sl], se = be[sl];
-      for (uint32_t sb = ss; sb < se; sb += 64) {
-        const uint32_t cnt = min(64u, se - sb);
-        __syncthreads();
-        if ((uint32_t)lane < cnt) {
-          V2 v;
-          v.x = (T)(x[sb + lane] - ox);
-          v.y = (T)(y[sb + lane] - oy);
-          sxy[lane] = v;
-          sm[lane] = (T)m[sb + lane];
+    for (uint32_t c0 = 0; c0 < total; c0 += P2P_CHUNK) {
+      const int n = (int)min((uint32_t)P2P_CHUNK, total - c0);
+      __syncthreads();
+      for (int pp = lane; pp < n; pp += 64) {
+        const uint32_t q = c0 + (uint32_t)pp;
+        int lo = 0, hi = nlv - 1;  // last source leaf with l_off <= q
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (l_off[mid] <= q)
+            lo = mid;
+          else
+            hi = mid - 1;
         }
-        __syncthreads();
-        if (valid) {
-          const int self = (int)i - (int)sb;  // skip j == i
-          for (int jj = half; jj < (int)cnt; jj += 2) {
-            if (jj == self) continue;
-            const V2 sj = sxy[jj];
-            const T dx = sj.x - xr, dy = sj.y - yr;
-            const T r2 = fma(dx, dx, fma(dy, dy, eps2));
-            const T inv = Near<T>::rsq(r2);
-            const T mj = sm[jj];
-            const T f = mj * (inv * inv * inv);
-            accx = fma(f, dx, accx);
-            accy = fma(f, dy, accy);
-            if (PHI) accp = fma(-mj, inv, accp);
-          }
-        }
+        const uint32_t g = l_ss[lo] + (q - l_off[lo]);
+        V4 v;
+        v.x = (T)(x[g] - ox);
+        v.y = (T)(y[g] - oy);
+        v.z = (T)m[g];
+        v.w = (T)0;
+        src[pp] = v;
+      }
+      __syncthreads();
+      if (valid) {
+        if (c0 < self_hi && c0 + (uint32_t)n > self_lo)
+          near_sum<T, PHI, true>(src, half, n, (int)(self_lo + (i - ts)) - (int)c0, xr, yr, eps2, accx, accy,
+                                 accp);
+        else
+          near_sum<T, PHI, false>(src, half, n, 0, xr, yr, eps2, accx, accy, accp);
       }
     }
     accx += __shfl_xor(accx, 32, 64);
@@ -603,27 +698,58 @@ __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, co
   }
 }
 
-// targets split over several items: sum their partials in item order
-__global__ void k_p2p_combine(const uint32_t* __restrict__ multi, uint32_t n_multi, const uint32_t* __restrict__ goff,
-                              const uint32_t* __restrict__ part_off, const uint32_t* __restrict__ bs,
-                              const uint32_t* __restrict__ be, uint32_t n_max, const double* __restrict__ part,
-                              double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ phi) {
+// Targets split over several items: sum their partials. 256 threads = 8
+// stripes x 32 bodies; stripe s sums items s, s+8, ... in order, then the 8
+// stripe sums are added in stripe order (a fixed order: deterministic).
+__global__ void __launch_bounds__(256) k_p2p_combine(const uint32_t* __restrict__ multi, uint32_t n_multi,
+                                                     const uint32_t* __restrict__ goff,
+                                                     const uint32_t* __restrict__ part_off,
+                                                     const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be,
+                                                     uint32_t n_max, const double* __restrict__ part,
+                                                     double* __restrict__ ax, double* __restrict__ ay,
+                                                     double* __restrict__ phi) {
+  __shared__ double red[8][32][3];
   const uint32_t q = blockIdx.x;
-  if (q >= n_multi) return;
   const uint32_t t = multi[q];
   const uint32_t nit = (goff[t + 1] - goff[t] + P2P_ITEM_LEAVES - 1) / P2P_ITEM_LEAVES;
-  for (uint32_t b = bs[t] + threadIdx.x; b < be[t]; b += blockDim.x) {
+  const uint32_t po = part_off[t], b0 = bs[t], b1 = be[t];
+  const int bl = threadIdx.x & 31, stripe = threadIdx.x >> 5;
+  for (uint32_t tb = b0; tb < b1; tb += 32) {
+    const uint32_t b = tb + bl;
     double sx = 0.0, sy = 0.0, sp = 0.0;
-    for (uint32_t k = 0; k < nit; ++k) {
-      const double* pp = part + ((size_t)(part_off[t] + k) * n_max + (b - bs[t])) * 3;
-      sx += pp[0];
-      sy += pp[1];
-      sp += pp[2];
+    if (b < b1) {
+      for (uint32_t k = stripe; k < nit; k += 8) {
+        const double* pp = part + ((size_t)(po + k) * n_max + (b - b0)) * 3;
+        sx += pp[0];
+        sy += pp[1];
+        sp += pp[2];
+      }
     }
-    ax[b] = sx;
-    ay[b] = sy;
-    if (phi) phi[b] = sp;
+    red[stripe][bl][0] = sx;
+    red[stripe][bl][1] = sy;
+    red[stripe][bl][2] = sp;
+    __syncthreads();
+    if (stripe == 0 && b < b1) {
+      double rx = 0.0, ry = 0.0, rp = 0.0;
+      for (int s8 = 0; s8 < 8; ++s8) {
+        rx += red[s8][bl][0];
+        ry += red[s8][bl][1];
+        rp += red[s8][bl][2];
+      }
+      ax[b] = rx;
+      ay[b] = ry;
+      if (phi) phi[b] = rp;
+    }
+    __syncthreads();
   }
+}
+
+hipError_t p2p_combine(const P2PItems& it, const uint32_t* goff, const uint32_t* bs, const uint32_t* be,
+                       uint32_t n_max, const double* part, double* o0, double* o1, double* o2, hipStream_t s) {
+  if (!it.n_multi) return hipSuccess;
+  k_p2p_combine<<<(unsigned)it.n_multi, 256, 0, s>>>(it.multi.as<uint32_t>(), (uint32_t)it.n_multi, goff,
+                                                     it.part_off.as<uint32_t>(), bs, be, n_max, part, o0, o1, o2);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------ direct sum ----
@@ -774,8 +900,9 @@ static int m2l_keys_setup(galax_fmm* h, uint32_t n_targets) {
   GX_HIP(hipMemsetAsync(err, 0, 4, s));
   const double* comx = t.com_x.as<double>();
   const double* comy = t.com_y.as<double>();
-  k_keys_insert<<<nblk(n_targets), 256, 0, s>>>(t.m2l_off.as<uint32_t>(), t.m2l_dat.as<uint32_t>(), n_targets, comx,
-                                                comy, hkeys, hrep, mask, h->tmp_u32.as<uint32_t>(), err);
+  k_pair_targets<<<nblk(n_targets), 256, 0, s>>>(t.m2l_off.as<uint32_t>(), n_targets, h->tmp_u32.as<uint32_t>());
+  k_keys_insert<<<nblk(n_pairs), 256, 0, s>>>(t.m2l_dat.as<uint32_t>(), h->tmp_u32.as<uint32_t>(), n_pairs, comx, comy,
+                                              hkeys, hrep, mask, err);
   GX_HIP(hipGetLastError());
   uint32_t herr = 0;
   GX_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s));
@@ -833,7 +960,7 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   hipEvent_t* ev = h->ev;
   GX_HIP(hipEventRecord(ev[1], s));
   {
-    k_p2m<P><<<nblk(nl, 4), 256, 0, s>>>(d_x, d_y, d_m, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), comx,
+    k_p2m<P><<<nblk(nl, num_moments(P) <= 32 ? 8 : 4), 256, 0, s>>>(d_x, d_y, d_m, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), comx,
                                          comy, (uint32_t)nl, mp);
     GX_HIP(hipGetLastError());
   }
@@ -852,7 +979,7 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
     k_m2l_exact<P><<<nblk(nn, 4), 256, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(),
                                                (uint32_t)nn, comx, comy, eps * eps, mp, loc);
     GX_HIP(hipGetLastError());
-  } else if (live_hi > 0 && h->cfg.precision != GALAX_PREC_FP64) {
+  } else if (live_hi > 0 && h->cfg.precision != GALAX_PREC_FP64_EXACT) {
     GX_CHECK(m2l_mfma_run(h, mp, loc));
   } else if (live_hi > 0) {
     k_m2l_ref<P><<<nblk(live_hi, 4), 256, 0, s>>>(t.m2l_off.as<uint32_t>(), t.m2l_dat.as<uint32_t>(),
@@ -877,7 +1004,8 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   const P2PItems& it = corrected ? h->c_items : t.items;
   const uint32_t nmax = h->cfg.n_max;
   GX_CHECK(dbuf_reserve(h->p2p_part, std::max<size_t>(it.n_part, 1) * nmax * 3 * 8, s));
-  const bool fp32_near = h->cfg.precision != GALAX_PREC_FP64;
+  const bool fp32_near =
+      h->cfg.precision == GALAX_PREC_MIXED_FP32 || h->cfg.precision == GALAX_PREC_MIXED_BF16X3;
 #define GX_LAUNCH_P2P(T, PH)                                                                                 \
   k_l2p_p2p<P, T, PH><<<(unsigned)it.n_items, 64, 0, s>>>(                                                  \
       d_x, d_y, d_m, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), comx, comy, loc, goff, gdat,     \
@@ -895,13 +1023,8 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
       GX_LAUNCH_P2P(double, false);
   }
   GX_HIP(hipGetLastError());
-  if (it.n_multi) {
-    k_p2p_combine<<<(unsigned)it.n_multi, 64, 0, s>>>(it.multi.as<uint32_t>(), (uint32_t)it.n_multi, goff,
-                                                       it.part_off.as<uint32_t>(), t.body_start.as<uint32_t>(),
-                                                       t.body_end.as<uint32_t>(), nmax, h->p2p_part.as<double>(), d_ax,
-                                                       d_ay, d_phi);
-    GX_HIP(hipGetLastError());
-  }
+  GX_HIP(p2p_combine(it, goff, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), nmax,
+                     h->p2p_part.as<double>(), d_ax, d_ay, d_phi, s));
 #undef GX_LAUNCH_P2P
   GX_HIP(hipGetLastError());
   GX_HIP(hipEventRecord(ev[7], s));
@@ -922,9 +1045,9 @@ int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, 
   if (!corrected && !h->keys_valid) {
     uint32_t top = std::min<uint32_t>(t.n_levels, 2);
     GX_CHECK(m2l_keys_setup(h, t.level_offset[top]));
-    h->mx_valid = false;
+    h->mx_valid = h->mx_adj_valid = h->mx_t64_valid = false;
   }
-  if (!corrected && h->cfg.precision != GALAX_PREC_FP64 && !h->mx_valid) GX_CHECK(m2l_mfma_setup(h));
+  if (!corrected && h->cfg.precision != GALAX_PREC_FP64_EXACT && !h->mx_valid) GX_CHECK(m2l_mfma_setup(h));
   GX_CHECK(dbuf_reserve(h->flag, 64, s));
   uint32_t* changed = h->flag.as<uint32_t>() + 4;
   GX_HIP(hipEventRecord(h->ev[0], s));

@@ -1,21 +1,39 @@
-// m2l_mfma.hip — M2L of the reference variant as per-key batched dense
-// contractions on the matrix cores (mixed-precision forms).
+// m2l_mfma.hip — M2L of the reference variant as batched dense contractions
+// on the matrix cores.
 //
 // galax-core's M2L (galax-core/src/lib.rs:1199-1259) uses, for every pair of a
 // target node t and source node s, the derivative table of the first pair with
 // the same key (dx as i64, dy as i64). The operator of a pair therefore
-// depends on its key only:
+// depends on its key K only:
 //     L_t[I,J] += sum_{i,j} (-1)^{i+j} / (I! J!) * G_K[i+I, j+J] * M_s[i,j]
-// so pairs sorted by key form one dense product per key: T_K (S x S) times a
-// block of multipole columns. Here every key gets a scaled FP32 operator
-//     T^_K[IJ][ij] = (-1)^{i+j}/(I!J!) G_K[i+I, j+J] * sig^(i+j) * sig^(I+J+1)
-// with sig the representative's softened distance, columns are
-// M^_ij = M_ij * sig^-(i+j), and results are unscaled by sig^-(I+J+1). Tiles of
-// 16 pairs of one key run on v_mfma_f32_16x16x4_f32 (FP32 form) or on
-// v_mfma_f32_16x16x32_bf16 with a 3-term bf16 split of both operands (BF16x3
-// form). Each pair's result column is written to its own slot and a second
-// kernel sums each target's pairs in list order (FP64), so results do not
-// depend on scheduling.
+// i.e. L_t += T_K M_s with T_K an S x S matrix. Two consequences are used:
+//  * the sources of one target that share a key are summed first (a group):
+//    L_t += T_K (sum_s M_s);
+//  * 16 consecutive targets form a tile whose columns are those targets; the
+//    tile steps through the distinct keys of its groups, and at each step one
+//    16-column product T_K B runs on the matrix cores, B's column c holding
+//    the group sum of target c for key K (zero when c has no such group).
+// Results accumulate in registers: no per-pair result buffer and no gather
+// pass. A work item is at most MX_ITEM_STEPS steps of one tile, run by one
+// block whose four waves take the steps round-robin and are summed in wave
+// order through LDS; a tile with several items writes partials that a combine
+// pass sums in item order. Results therefore do not depend on scheduling.
+// Blocks are mapped so that each XCD (and its L2) takes a contiguous range of
+// items, i.e. of neighbouring tiles that share sources and keys.
+//
+// The adjoint (VJP) uses the same machinery with rows = sources and the
+// transposed FP64 operator: mbar_s += T_K^T (sum of lbar_t over its group).
+//
+// Forms:
+//  * FP64: T_K in FP64 on v_mfma_f64_16x16x4_f64, accumulated across steps in
+//    the matrix accumulators.
+//  * MIXED_FP32: a scaled FP32 operator per key
+//        T^_K[IJ][ij] = T_K[IJ][ij] * sig^(i+j) * sig^(I+J+1),
+//    sig the representative's softened distance; columns M^_ij = M_ij *
+//    sig^-(i+j); each step's product (v_mfma_f32_16x16x4_f32) is unscaled by
+//    sig^-(I+J+1) and added to an FP64 accumulator.
+//  * MIXED_BF16X3: as MIXED_FP32 with both operands split into three bf16
+//    terms and six v_mfma_f32_16x16x32_bf16 products per k-block.
 #include <algorithm>
 #include <cmath>
 
@@ -24,19 +42,28 @@
 namespace gx {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef double d4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
+enum { MX_F64 = 0, MX_F32 = 1, MX_BF16X3 = 2 };
+
 static inline unsigned nblk(size_t n, unsigned bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+static int bits_of(uint64_t n) {  // bits to hold values 0 .. n-1 (at least 1)
+  int w = 1;
+  while (w < 63 && (1ull << w) < n) ++w;
+  return w;
+}
 
 __host__ __device__ constexpr int row_blocks(int p) { return (num_moments(p) + 15) / 16; }
 
 template <int P>
-__device__ __forceinline__ int order_of(int k) {
+__host__ __device__ __forceinline__ int order_of(int k) {
   int n = 0;
 #pragma unroll
   for (int q = 0; q <= P; ++q)
     if (k >= (q + 1) * (q + 2) / 2) n = q + 1;
-  return n;
+  return n < P ? n : P;
 }
 
 template <int P>
@@ -46,53 +73,133 @@ __device__ __forceinline__ void coef_ij2(int k, int& i, int& j) {
   i = n - j;
 }
 
+// Contraction index carried by slot e (0 <= e < KL) of lane quarter kq.
+// 16x16x4 forms: lane quarter kq holds the KS = K/4 consecutive indices
+// kq*KS + e, one per instruction. 16x16x32 bf16: instruction c takes 8
+// consecutive indices c*32 + 8*kq + (0..7) from each lane.
+template <int P, int MODE>
+struct Lanes {
+  static constexpr int S = num_moments(P), RB = row_blocks(P), K = 16 * RB;
+  static constexpr int KC = (K + 31) / 32;
+  static constexpr int KL = MODE == MX_BF16X3 ? KC * 8 : K / 4;
+  static __device__ __forceinline__ int k_of(int e, int kq) {
+    return MODE == MX_BF16X3 ? (e >> 3) * 32 + 8 * kq + (e & 7) : kq * (K / 4) + e;
+  }
+};
+
 // ------------------------------------------------------------- setup ----
 
-__global__ void k_mx_count(const uint32_t* __restrict__ key_of_pair, uint32_t n_pairs, uint32_t* __restrict__ cnt) {
-  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < n_pairs) atomicAdd(&cnt[key_of_pair[e]], 1u);
+constexpr uint32_t MX_ITEM_STEPS = 16;  // steps per work item
+constexpr uint32_t MX_GROUP = 8;        // pairs per group (one column of one step)
+
+// row of each live pair: its target (CSR row) or its source
+__global__ void k_mx_pair_rows(const uint32_t* __restrict__ off, uint32_t nt, uint32_t* __restrict__ tgt) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  for (uint32_t e = off[t]; e < off[t + 1]; ++e) tgt[e] = t;
 }
 
-__global__ void k_mx_pad(const uint32_t* __restrict__ cnt, uint32_t nk, uint32_t* __restrict__ padded,
-                         uint32_t* __restrict__ ntile) {
-  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nk) return;
-  uint32_t t = (cnt[k] + 15) / 16;
-  ntile[k] = t;
-  padded[k] = t * 16;
+__global__ void k_mx_pair_keys(const uint32_t* __restrict__ row_of_pair, const uint32_t* __restrict__ key_of_pair,
+                               size_t ne, uint32_t n_rows, int kb, unsigned long long* __restrict__ ka,
+                               uint32_t* __restrict__ va, uint32_t* __restrict__ err) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const uint32_t r = row_of_pair[e];
+  if (r >= n_rows) atomicOr(err, 1u);
+  ka[e] = ((unsigned long long)min(r, n_rows - 1) << kb) | key_of_pair[e];
+  va[e] = (uint32_t)e;
 }
 
-// positions: padded segment per key; any order inside a segment gives the same
-// values (each pair's column is computed independently)
-__global__ void k_mx_place(const uint32_t* __restrict__ key_of_pair, const uint32_t* __restrict__ dat,
-                           uint32_t n_pairs, const uint32_t* __restrict__ koffp, uint32_t* __restrict__ cursor,
-                           uint32_t* __restrict__ pos_of_pair, uint32_t* __restrict__ src_of_pos) {
-  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n_pairs) return;
-  uint32_t k = key_of_pair[e];
-  uint32_t pos = koffp[k] + atomicAdd(&cursor[k], 1u);
-  pos_of_pair[e] = pos;
-  src_of_pos[pos] = dat[e];
+__global__ void k_mx_run_flags(const unsigned long long* __restrict__ k, size_t n, int shift,
+                               uint32_t* __restrict__ f) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = (i == 0 || (k[i] >> shift) != (k[i - 1] >> shift)) ? 1u : 0u;
 }
 
-__global__ void k_mx_tiles(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ koffp,
-                           const uint32_t* __restrict__ toff, uint32_t nk, uint32_t* __restrict__ tiles) {
-  uint32_t k = blockIdx.x;
-  if (k >= nk) return;
-  uint32_t nt = (cnt[k] + 15) / 16;
-  for (uint32_t j = threadIdx.x; j < nt; j += blockDim.x) {
-    uint32_t tt = toff[k] + j;
-    tiles[3 * tt] = k;
-    tiles[3 * tt + 1] = koffp[k] + 16 * j;
-    tiles[3 * tt + 2] = min(16u, cnt[k] - 16 * j);
+// run starts of equal (row, key) in the sorted pairs, as their own index
+// (a max-scan then gives each pair the start of its run)
+__global__ void k_mx_run_starts(const unsigned long long* __restrict__ k, size_t n, uint32_t* __restrict__ f) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = (i == 0 || k[i] != k[i - 1]) ? (uint32_t)i : 0u;
+}
+
+// a group is a chunk of at most MX_GROUP pairs of one (row, key) run
+__global__ void k_mx_chunk_flags(const uint32_t* __restrict__ run_start, size_t n, uint32_t* __restrict__ f) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = ((i - run_start[i]) % MX_GROUP == 0) ? 1u : 0u;
+}
+
+// pairs sorted by (row, key, list position) -> the other end of each pair in
+// that order, and groups (chunks of one (row, key) run) with their tile
+// composite (tile, key, chunk rank, column) for the second sort.
+// gid: exclusive scan of the group starts.
+__global__ void k_mx_groups(const unsigned long long* __restrict__ ka, const uint32_t* __restrict__ va,
+                            const uint32_t* __restrict__ run_start, const uint32_t* __restrict__ gid, size_t ne,
+                            const uint32_t* __restrict__ other_of_pair, int kb, int rkb,
+                            uint32_t* __restrict__ src_sorted, uint32_t* __restrict__ gfirst,
+                            unsigned long long* __restrict__ g2k, uint32_t* __restrict__ g2v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne) return;
+  src_sorted[i] = other_of_pair[va[i]];
+  const uint32_t off = (uint32_t)i - run_start[i];
+  if (off % MX_GROUP == 0) {
+    const uint32_t g = gid[i];
+    const unsigned long long t = ka[i] >> kb, key = ka[i] & ((1ull << kb) - 1), rank = off / MX_GROUP;
+    gfirst[g] = (uint32_t)i;
+    g2k[g] = ((t >> 4) << (kb + rkb + 4)) | (key << (rkb + 4)) | (rank << 4) | (t & 15);
+    g2v[g] = g;
   }
 }
 
-// scaled operator of each key (FP32, (16 RB)^2 row-major, zero padded) and
-// the key's inverse scale powers sig^-n, n = 0..P+1 (FP64)
+// sid: inclusive scan of the step-start flags over the sorted groups
+__global__ void k_mx_steps(const unsigned long long* __restrict__ g2k, const uint32_t* __restrict__ g2v,
+                           const uint32_t* __restrict__ sid, size_t ng, const uint32_t* __restrict__ gfirst, int kb,
+                           int rkb, uint32_t* __restrict__ step_key, uint32_t* __restrict__ step_first,
+                           uint32_t* __restrict__ step_cnt, uint32_t* __restrict__ tile_cnt) {
+  size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ng) return;
+  const unsigned long long c = g2k[j];
+  const uint32_t st = sid[j] - 1, col = (uint32_t)(c & 15), g = g2v[j];
+  step_first[(size_t)st * 16 + col] = gfirst[g];
+  step_cnt[(size_t)st * 16 + col] = gfirst[g + 1] - gfirst[g];
+  if (j == 0 || (c >> 4) != (g2k[j - 1] >> 4)) {
+    step_key[st] = (uint32_t)((c >> (rkb + 4)) & ((1ull << kb) - 1));
+    atomicAdd(&tile_cnt[c >> (kb + rkb + 4)], 1u);
+  }
+}
+
+__global__ void k_mx_set_u32(uint32_t* __restrict__ p, uint32_t v) { *p = v; }
+
+// FP64 operator of each key: (16 RB)^2 row-major, zero padded
 template <int P>
-__global__ void k_mx_operators(const double* __restrict__ gtab, const double* __restrict__ dxy, uint32_t nk,
-                               double eps2, float* __restrict__ that, double* __restrict__ sig_inv) {
+__global__ void k_mx_operators64(const double* __restrict__ gtab, uint32_t nk, double* __restrict__ t64) {
+  constexpr int S = num_moments(P), K = 16 * row_blocks(P), ST = 2 * P + 1, GS = ST * ST;
+  uint32_t key = blockIdx.x;
+  if (key >= nk) return;
+  const double* G = gtab + (size_t)key * GS;
+  double fact[P + 1];
+  fact[0] = 1.0;
+  for (int q = 1; q <= P; ++q) fact[q] = fact[q - 1] * q;
+  for (int t = threadIdx.x; t < K * K; t += blockDim.x) {
+    int r = t / K, c = t % K;
+    double v = 0.0;
+    if (r < S && c < S) {
+      int I, J, i, j;
+      coef_ij2<P>(r, I, J);
+      coef_ij2<P>(c, i, j);
+      double w = (((i + j) & 1) ? -1.0 : 1.0) / (fact[I] * fact[J]);
+      v = w * G[(i + I + j + J) * ST + (j + J)];
+      if (!isfinite(v)) v = 0.0;
+    }
+    t64[(size_t)key * K * K + t] = v;
+  }
+}
+
+// scaled FP32 operator of each key and the key's inverse scale powers sig^-n,
+// n = 0..P+1 (FP64)
+template <int P>
+__global__ void k_mx_operators32(const double* __restrict__ gtab, const double* __restrict__ dxy, uint32_t nk,
+                                 double eps2, float* __restrict__ that, double* __restrict__ sig_inv) {
   constexpr int S = num_moments(P), K = 16 * row_blocks(P), ST = 2 * P + 1, GS = ST * ST;
   uint32_t key = blockIdx.x;
   if (key >= nk) return;
@@ -125,53 +232,6 @@ __global__ void k_mx_operators(const double* __restrict__ gtab, const double* __
 
 // ----------------------------------------------------------- products ----
 
-// One wave per tile of 16 pairs. Operand layouts of v_mfma_f32_16x16x4_f32:
-// A (16x4): lane l holds A[l%16][l/16]; B (4x16): lane l holds B[l/16][l%16];
-// D (16x16): lane l holds D[4*(l/16)+j][l%16], j = 0..3.
-template <int P>
-__global__ void __launch_bounds__(256) k_m2l_mfma_f32(const uint32_t* __restrict__ tiles, uint32_t ntiles,
-                                                      const uint32_t* __restrict__ src_of_pos,
-                                                      const float* __restrict__ that,
-                                                      const double* __restrict__ sig_inv,
-                                                      const double* __restrict__ mp, float* __restrict__ R) {
-  constexpr int S = num_moments(P), RB = row_blocks(P), K = 16 * RB, KS = K / 4;
-  const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63, col = lane & 15, kq = lane >> 4;
-  if (tile >= ntiles) return;
-  const uint32_t key = tiles[3 * tile], base = tiles[3 * tile + 1], cnt = tiles[3 * tile + 2];
-  const uint32_t pos = base + col;
-  const bool live = (uint32_t)col < cnt;
-  const uint32_t src = live ? src_of_pos[pos] : 0u;
-  const double* sg = sig_inv + (size_t)key * (P + 2);
-  float b[KS];
-#pragma unroll
-  for (int ks = 0; ks < KS; ++ks) {
-    const int k = ks * 4 + kq;
-    float v = 0.f;
-    if (live && k < S) {
-      double mv = mp[(size_t)src * S + k];
-      if (!isfinite(mv)) mv = 0.0;
-      v = (float)(mv * sg[order_of<P>(k)]);
-    }
-    b[ks] = v;
-  }
-  const float* T = that + (size_t)key * K * K;
-#pragma unroll
-  for (int rb = 0; rb < RB; ++rb) {
-    f4 acc = {0.f, 0.f, 0.f, 0.f};
-    const float* Trow = T + (size_t)(rb * 16 + col) * K;
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(Trow[ks * 4 + kq], b[ks], acc, 0, 0, 0);
-    if (live) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int kk = rb * 16 + 4 * kq + j;
-        if (kk < S) R[(size_t)pos * S + kk] = acc[j] * (float)sg[order_of<P>(kk) + 1];
-      }
-    }
-  }
-}
-
 // 3-term bf16 split x = hi + mid + lo
 __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l) {
   h = (__bf16)x;
@@ -181,163 +241,417 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
   l = (__bf16)r;
 }
 
-// BF16x3 form. Operand layouts of v_mfma_f32_16x16x32_bf16: A (16x32): lane l
-// holds A[l%16][8*(l/16) + 0..7]; B (32x16): lane l holds B[8*(l/16) + 0..7][l%16];
-// D as above. Products kept: hh, hm, mh, hl, lh, mm (the terms of order <= 2
-// in the split), accumulated in FP32.
-template <int P>
-__global__ void __launch_bounds__(256) k_m2l_mfma_bf16x3(const uint32_t* __restrict__ tiles, uint32_t ntiles,
-                                                         const uint32_t* __restrict__ src_of_pos,
-                                                         const float* __restrict__ that,
-                                                         const double* __restrict__ sig_inv,
-                                                         const double* __restrict__ mp, float* __restrict__ R) {
-  constexpr int S = num_moments(P), RB = row_blocks(P), K = 16 * RB, KC = (K + 31) / 32;
-  const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63, col = lane & 15, kq = lane >> 4;
-  if (tile >= ntiles) return;
-  const uint32_t key = tiles[3 * tile], base = tiles[3 * tile + 1], cnt = tiles[3 * tile + 2];
-  const uint32_t pos = base + col;
-  const bool live = (uint32_t)col < cnt;
-  const uint32_t src = live ? src_of_pos[pos] : 0u;
-  const double* sg = sig_inv + (size_t)key * (P + 2);
-  bf8 bh[KC], bm[KC], bl[KC];
+struct TileArgs {
+  const uint32_t *item_tile, *item_s0, *item_s1, *item_slot;
+  uint32_t nitems, per_xcd, n_rows;
+  const uint32_t *step_key, *step_first, *step_cnt, *src;
+  const double* t64;
+  const float* that;
+  const double* sig_inv;
+  const double* in;  // multipoles (forward) or lbar (adjoint), S per node
+  double* out;       // locals (forward) or mbar (adjoint), S per row
+  double* part;      // partials of tiles with several items: [slot][16][S]
+};
+
+// One block (4 waves) per work item; wave w takes steps s0 + w, s0 + w + 4, ...
+// Operand layouts (lane l, column/row c = l % 16, quarter kq = l / 16):
+//  v_mfma_*_16x16x4: A lane l holds A[c][slot kq]; B lane l holds B[slot kq][c];
+//  v_mfma_f32_16x16x32_bf16: 8 consecutive slots 8 kq + 0..7 per lane;
+//  D (16x16): v_mfma_f32_*: lane l holds D[4 kq + j][c], j = 0..3;
+//             v_mfma_f64_16x16x4_f64: lane l holds D[kq + 4 j][c] (rows interleaved).
+// TR (adjoint, FP64 only): A = T_K^T.
+template <int P, int MODE, bool TR>
+__global__ void __launch_bounds__(256) k_m2l_tiles(TileArgs a) {
+  typedef Lanes<P, MODE> LN;
+  constexpr int S = LN::S, RB = LN::RB, K = LN::K, KL = LN::KL, KS = K / 4;
+  static_assert(!TR || MODE == MX_F64, "the transposed (adjoint) product is FP64");
+  __shared__ double red[4][16][17];
+  // XCD-contiguous item ranges: block b runs on XCD b % 8
+  const uint32_t item = (blockIdx.x % 8) * a.per_xcd + blockIdx.x / 8;
+  if (item >= a.nitems) return;  // uniform over the block
+  const uint32_t tile = a.item_tile[item], s0 = a.item_s0[item], s1 = a.item_s1[item], slot = a.item_slot[item];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 15, kq = lane >> 4;
+  d4 L[RB];
 #pragma unroll
-  for (int kc = 0; kc < KC; ++kc)
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int k = kc * 32 + kq * 8 + e;
-      float v = 0.f;
-      if (live && k < S) {
-        double mv = mp[(size_t)src * S + k];
-        if (!isfinite(mv)) mv = 0.0;
-        v = (float)(mv * sg[order_of<P>(k)]);
-      }
-      __bf16 h, m, l;
-      split3(v, h, m, l);
-      bh[kc][e] = h;
-      bm[kc][e] = m;
-      bl[kc][e] = l;
+  for (int rb = 0; rb < RB; ++rb) L[rb] = (d4){0.0, 0.0, 0.0, 0.0};
+  uint32_t st = s0 + w;
+  uint32_t key_n = 0, f_n = 0, c_n = 0;
+  if (st < s1) {
+    key_n = a.step_key[st];
+    f_n = a.step_first[(size_t)st * 16 + col];
+    c_n = a.step_cnt[(size_t)st * 16 + col];
+  }
+  for (; st < s1; st += 4) {
+    const uint32_t key = key_n, f = f_n, cnt = c_n;
+    if (st + 4 < s1) {  // next step's record in flight during this one
+      key_n = a.step_key[st + 4];
+      f_n = a.step_first[(size_t)(st + 4) * 16 + col];
+      c_n = a.step_cnt[(size_t)(st + 4) * 16 + col];
     }
-  const float* T = that + (size_t)key * K * K;
+    double bs[KL];
 #pragma unroll
-  for (int rb = 0; rb < RB; ++rb) {
-    f4 acc = {0.f, 0.f, 0.f, 0.f};
-    const float* Trow = T + (size_t)(rb * 16 + col) * K;
+    for (int e = 0; e < KL; ++e) bs[e] = 0.0;
+    for (uint32_t q = 0; q < cnt; ++q) {
+      const double* row = a.in + (size_t)a.src[f + q] * S;
 #pragma unroll
-    for (int kc = 0; kc < KC; ++kc) {
-      bf8 ah, am, al;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int k = kc * 32 + kq * 8 + e;
-        float v = k < K ? Trow[k] : 0.f;
-        __bf16 h, m, l;
-        split3(v, h, m, l);
-        ah[e] = h;
-        am[e] = m;
-        al[e] = l;
+      for (int e = 0; e < KL; ++e) {
+        const int k = LN::k_of(e, kq);
+        if (k < S) {
+          double v = row[k];
+          bs[e] += isfinite(v) ? v : 0.0;
+        }
       }
-      // small terms first
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm[kc], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[kc], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[kc], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh[kc], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm[kc], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[kc], acc, 0, 0, 0);
     }
-    if (live) {
+    if (MODE == MX_F64) {
+      const double* T = a.t64 + (size_t)key * K * K;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int kk = rb * 16 + 4 * kq + j;
-        if (kk < S) R[(size_t)pos * S + kk] = acc[j] * (float)sg[order_of<P>(kk) + 1];
+      for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+        for (int e = 0; e < KS; ++e) {
+          const double av = TR ? T[(size_t)(kq * KS + e) * K + rb * 16 + col] : T[(size_t)(rb * 16 + col) * K + kq * KS + e];
+          L[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bs[e], L[rb], 0, 0, 0);
+        }
+      }
+    } else {
+      const double* sg = a.sig_inv + (size_t)key * (P + 2);
+      const float* T = a.that + (size_t)key * K * K;
+      if (MODE == MX_F32) {
+        float b[KS];
+#pragma unroll
+        for (int e = 0; e < KS; ++e) b[e] = (float)(bs[e] * sg[order_of<P>(LN::k_of(e, kq))]);
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+          const float* Tr = T + (size_t)(rb * 16 + col) * K + kq * KS;
+          f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int e = 0; e < KS; ++e) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(Tr[e], b[e], acc, 0, 0, 0);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) L[rb][j] += (double)acc[j] * sg[order_of<P>(rb * 16 + 4 * kq + j) + 1];
+        }
+      } else {
+        constexpr int KC = LN::KC;
+        bf8 bh[KC], bm[KC], bl[KC];
+#pragma unroll
+        for (int e = 0; e < KL; ++e) {
+          const float v = (float)(bs[e] * sg[order_of<P>(LN::k_of(e, kq))]);
+          __bf16 hh, mm, ll;
+          split3(v, hh, mm, ll);
+          bh[e >> 3][e & 7] = hh;
+          bm[e >> 3][e & 7] = mm;
+          bl[e >> 3][e & 7] = ll;
+        }
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+          const float* Tr = T + (size_t)(rb * 16 + col) * K;
+          f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int kc = 0; kc < KC; ++kc) {
+            bf8 ah, am, al;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+              const int k = kc * 32 + kq * 8 + e;
+              __bf16 hh, mm, ll;
+              split3(k < K ? Tr[k] : 0.f, hh, mm, ll);
+              ah[e] = hh;
+              am[e] = mm;
+              al[e] = ll;
+            }
+            // small terms first
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm[kc], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[kc], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[kc], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh[kc], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm[kc], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[kc], acc, 0, 0, 0);
+          }
+#pragma unroll
+          for (int j = 0; j < 4; ++j) L[rb][j] += (double)acc[j] * sg[order_of<P>(rb * 16 + 4 * kq + j) + 1];
+        }
       }
     }
   }
+  // sum the four waves' partial results in wave order
+  const uint32_t t = tile * 16 + col;
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) red[w][MODE == MX_F64 ? kq + 4 * j : 4 * kq + j][col] = L[rb][j];
+    __syncthreads();
+    if (w == 0) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = 4 * kq + j, row = rb * 16 + r;
+        const double v = ((red[0][r][col] + red[1][r][col]) + red[2][r][col]) + red[3][r][col];
+        if (row < S) {
+          if (slot == NONE) {
+            if (t < a.n_rows) a.out[(size_t)t * S + row] = v;
+          } else {
+            a.part[((size_t)slot * 16 + col) * S + row] = v;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
 }
 
-// Sum of each target's pair results in list order (FP64); one wave per target.
-template <int P>
-__global__ void __launch_bounds__(256) k_m2l_gather(const uint32_t* __restrict__ off,
-                                                    const uint32_t* __restrict__ pos_of_pair, uint32_t n_targets,
-                                                    const float* __restrict__ R, double* __restrict__ loc) {
-  constexpr int S = num_moments(P);
-  const uint32_t t = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (t >= n_targets) return;
-  for (int k = lane; k < S; k += 64) {
-    double acc = 0.0;
-    for (uint32_t e = off[t]; e < off[t + 1]; ++e) acc += (double)R[(size_t)pos_of_pair[e] * S + k];
-    loc[(size_t)t * S + k] = acc;
+// tiles with several items: sum their partials in item order
+__global__ void __launch_bounds__(256) k_m2l_tiles_combine(const uint32_t* __restrict__ multi_tile,
+                                                           const uint32_t* __restrict__ multi_slot,
+                                                           const uint32_t* __restrict__ multi_cnt, int S,
+                                                           uint32_t n_rows, const double* __restrict__ part,
+                                                           double* __restrict__ out) {
+  const uint32_t q = blockIdx.x;
+  const uint32_t tile = multi_tile[q], slot0 = multi_slot[q], nit = multi_cnt[q];
+  for (int idx = threadIdx.x; idx < 16 * S; idx += blockDim.x) {
+    const int col = idx / S, row = idx % S;
+    const uint32_t t = tile * 16 + col;
+    if (t >= n_rows) continue;
+    double v = 0.0;
+    for (uint32_t k = 0; k < nit; ++k) v += part[((size_t)(slot0 + k) * 16 + col) * S + row];
+    out[(size_t)t * S + row] = v;
   }
 }
 
 // ------------------------------------------------------- host drivers ----
 
+static int upload_u32(DBuf& b, const std::vector<uint32_t>& v, hipStream_t s) {
+  GX_CHECK(dbuf_reserve(b, std::max<size_t>(v.size(), 1) * 4, s));
+  if (!v.empty()) GX_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * 4, hipMemcpyHostToDevice, s));
+  return 0;
+}
+
+// Builds tiles over n_rows rows from the live pairs: row_of_pair[e] names the
+// row of pair e, other_of_pair[e] the node whose coefficients it reads.
+static int build_tiles(galax_fmm* h, const uint32_t* row_of_pair, const uint32_t* other_of_pair, uint32_t n_rows,
+                       M2LTiles& T) {
+  hipStream_t s = h->stream;
+  const uint32_t nk = (uint32_t)h->n_keys, ne = (uint32_t)h->n_live_pairs;
+  const uint32_t ntiles = (n_rows + 15) / 16;
+  T.n_rows = n_rows;
+  T.ntiles = ntiles;
+  T.nsteps = 0;
+  GX_CHECK(dbuf_reserve(T.tile_off, (size_t)(ntiles + 1) * 4, s));
+  uint32_t* tile_off = T.tile_off.as<uint32_t>();
+  GX_HIP(hipMemsetAsync(tile_off, 0, (size_t)(ntiles + 1) * 4, s));
+  if (ne > 0 && n_rows > 0) {
+    const int kb = bits_of(nk);
+    const int tb = bits_of(n_rows);
+    if (kb + tb > 59) return set_error(GALAX_ERR_UNSUPPORTED, "M2L tiles: too many keys and rows");
+    Workspace& ws = h->ws;
+    // 1. pairs sorted by (row, key), list order kept inside a group
+    GX_CHECK(dbuf_reserve(ws.sort_ka, (size_t)ne * 8, s));
+    GX_CHECK(dbuf_reserve(ws.sort_kb, (size_t)ne * 8, s));
+    GX_CHECK(dbuf_reserve(ws.sort_va, (size_t)ne * 4, s));
+    GX_CHECK(dbuf_reserve(ws.sort_vb, (size_t)ne * 4, s));
+    GX_CHECK(dbuf_reserve(h->flag, 64, s));
+    auto* ka = ws.sort_ka.as<unsigned long long>();
+    auto* kb2 = ws.sort_kb.as<unsigned long long>();
+    uint32_t* va = ws.sort_va.as<uint32_t>();
+    uint32_t* vb = ws.sort_vb.as<uint32_t>();
+    uint32_t* err = h->flag.as<uint32_t>() + 12;
+    GX_HIP(hipMemsetAsync(err, 0, 4, s));
+    k_mx_pair_keys<<<nblk(ne), 256, 0, s>>>(row_of_pair, h->key_of_pair.as<uint32_t>(), ne, n_rows, kb, ka, va, err);
+    GX_HIP(hipGetLastError());
+    uint32_t herr = 0;
+    GX_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s));
+    GX_HIP(hipStreamSynchronize(s));
+    if (herr) return set_error(GALAX_ERR_INVALID, "M2L tiles: a pair's row lies outside the live nodes");
+    GX_CHECK(radix_sort_bits(h, reinterpret_cast<uint64_t*>(ka), va, reinterpret_cast<uint64_t*>(kb2), vb, ne,
+                             kb + tb));
+    // 2. groups: chunks of at most MX_GROUP pairs of each (row, key) run
+    GX_CHECK(dbuf_reserve(ws.a, (size_t)ne * 4, s));
+    GX_CHECK(dbuf_reserve(ws.h, (size_t)ne * 4, s));
+    uint32_t* gid = ws.a.as<uint32_t>();
+    uint32_t* run_start = ws.h.as<uint32_t>();
+    k_mx_run_starts<<<nblk(ne), 256, 0, s>>>(ka, ne, run_start);
+    GX_HIP(hipGetLastError());
+    GX_CHECK(scan_u32(h, run_start, run_start, ne, true, true, nullptr));
+    k_mx_chunk_flags<<<nblk(ne), 256, 0, s>>>(run_start, ne, gid);
+    GX_HIP(hipGetLastError());
+    uint32_t ng = 0;  // exclusive: a group start i is group gid[i]
+    GX_CHECK(scan_u32(h, gid, gid, ne, false, false, &ng));
+    const int rkb = bits_of((ne + MX_GROUP - 1) / MX_GROUP);
+    if (bits_of(ntiles) + kb + rkb + 4 > 64) return set_error(GALAX_ERR_UNSUPPORTED, "M2L tiles: key space too wide");
+    GX_CHECK(dbuf_reserve(T.src, (size_t)ne * 4, s));
+    GX_CHECK(dbuf_reserve(ws.b, (size_t)(ng + 1) * 4, s));
+    GX_CHECK(dbuf_reserve(ws.c, (size_t)ng * 8, s));
+    GX_CHECK(dbuf_reserve(ws.d, (size_t)ng * 8, s));
+    GX_CHECK(dbuf_reserve(ws.e, (size_t)ng * 4, s));
+    GX_CHECK(dbuf_reserve(ws.g, (size_t)ng * 4, s));
+    uint32_t* gfirst = ws.b.as<uint32_t>();
+    auto* g2k = ws.c.as<unsigned long long>();
+    auto* g2kb = ws.d.as<unsigned long long>();
+    uint32_t* g2v = ws.e.as<uint32_t>();
+    uint32_t* g2vb = ws.g.as<uint32_t>();
+    k_mx_groups<<<nblk(ne), 256, 0, s>>>(ka, va, run_start, gid, ne, other_of_pair, kb, rkb, T.src.as<uint32_t>(),
+                                         gfirst, g2k, g2v);
+    k_mx_set_u32<<<1, 1, 0, s>>>(gfirst + ng, ne);
+    GX_HIP(hipGetLastError());
+    // 3. groups sorted by (tile, key, rank, column): runs of equal
+    // (tile, key, rank) are steps
+    GX_CHECK(radix_sort_bits(h, reinterpret_cast<uint64_t*>(g2k), g2v, reinterpret_cast<uint64_t*>(g2kb), g2vb, ng,
+                             bits_of(ntiles) + kb + rkb + 4));
+    uint32_t* sid = ws.h.as<uint32_t>();  // run_start is no longer needed
+    k_mx_run_flags<<<nblk(ng), 256, 0, s>>>(g2k, ng, 4, sid);
+    GX_HIP(hipGetLastError());
+    uint32_t ns = 0;  // inclusive: entry j belongs to step sid[j] - 1
+    GX_CHECK(scan_u32(h, sid, sid, ng, true, false, &ns));
+    T.nsteps = ns;
+    GX_CHECK(dbuf_reserve(T.step_key, (size_t)ns * 4, s));
+    GX_CHECK(dbuf_reserve(T.step_first, (size_t)ns * 64, s));
+    GX_CHECK(dbuf_reserve(T.step_cnt, (size_t)ns * 64, s));
+    GX_HIP(hipMemsetAsync(T.step_first.p, 0, (size_t)ns * 64, s));
+    GX_HIP(hipMemsetAsync(T.step_cnt.p, 0, (size_t)ns * 64, s));
+    k_mx_steps<<<nblk(ng), 256, 0, s>>>(g2k, g2v, sid, ng, gfirst, kb, rkb, T.step_key.as<uint32_t>(),
+                                        T.step_first.as<uint32_t>(), T.step_cnt.as<uint32_t>(), tile_off);
+    GX_HIP(hipGetLastError());
+  }
+  uint32_t tot = 0;
+  GX_CHECK(scan_u32(h, tile_off, tile_off, ntiles, false, false, &tot));
+  k_mx_set_u32<<<1, 1, 0, s>>>(tile_off + ntiles, tot);
+  GX_HIP(hipGetLastError());
+  if (tot != T.nsteps) return set_error(GALAX_ERR_DEVICE, "M2L tiles: %u steps counted, %zu emitted", tot, T.nsteps);
+  // 4. work items (host): at most MX_ITEM_STEPS steps each, one per empty tile
+  std::vector<uint32_t> off(ntiles + 1);
+  GX_HIP(hipMemcpyAsync(off.data(), tile_off, (ntiles + 1) * 4, hipMemcpyDeviceToHost, s));
+  GX_HIP(hipStreamSynchronize(s));
+  std::vector<uint32_t> it_tile, it_s0, it_s1, it_slot, m_tile, m_slot, m_cnt;
+  uint32_t nslots = 0;
+  for (uint32_t tt = 0; tt < ntiles; ++tt) {
+    const uint32_t a0 = off[tt], a1 = off[tt + 1];
+    const uint32_t nit = std::max<uint32_t>(1, (a1 - a0 + MX_ITEM_STEPS - 1) / MX_ITEM_STEPS);
+    if (nit > 1) {
+      m_tile.push_back(tt);
+      m_slot.push_back(nslots);
+      m_cnt.push_back(nit);
+    }
+    for (uint32_t k = 0; k < nit; ++k) {
+      it_tile.push_back(tt);
+      it_s0.push_back(std::min(a1, a0 + k * MX_ITEM_STEPS));
+      it_s1.push_back(std::min(a1, a0 + (k + 1) * MX_ITEM_STEPS));
+      it_slot.push_back(nit > 1 ? nslots + k : NONE);
+    }
+    if (nit > 1) nslots += nit;
+  }
+  T.nitems = it_tile.size();
+  T.nmulti = m_tile.size();
+  T.nslots = nslots;
+  GX_CHECK(upload_u32(T.item_tile, it_tile, s));
+  GX_CHECK(upload_u32(T.item_s0, it_s0, s));
+  GX_CHECK(upload_u32(T.item_s1, it_s1, s));
+  GX_CHECK(upload_u32(T.item_slot, it_slot, s));
+  GX_CHECK(upload_u32(T.multi_tile, m_tile, s));
+  GX_CHECK(upload_u32(T.multi_slot, m_slot, s));
+  GX_CHECK(upload_u32(T.multi_cnt, m_cnt, s));
+  GX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+template <int P>
+static int operators64(galax_fmm* h) {
+  constexpr int K = 16 * row_blocks(P);
+  const uint32_t nk = (uint32_t)h->n_keys;
+  if (nk) {
+    GX_CHECK(dbuf_reserve(h->mx_t64, (size_t)nk * K * K * 8, h->stream));
+    k_mx_operators64<P><<<nk, 256, 0, h->stream>>>(h->gtab.as<double>(), nk, h->mx_t64.as<double>());
+    GX_HIP(hipGetLastError());
+  }
+  h->mx_t64_valid = true;
+  return 0;
+}
+
 template <int P>
 static int setup_impl(galax_fmm* h) {
   constexpr int K = 16 * row_blocks(P);
   hipStream_t s = h->stream;
-  const uint32_t nk = (uint32_t)h->n_keys, ne = (uint32_t)h->n_live_pairs;
-  h->mx_ntiles = 0;
-  if (ne == 0) {
-    h->mx_valid = true;
-    return 0;
+  const uint32_t nk = (uint32_t)h->n_keys, ne = (uint32_t)h->n_live_pairs, nt = (uint32_t)h->n_live_targets;
+  GX_CHECK(dbuf_reserve(h->ws.f, (size_t)std::max<uint32_t>(ne, 1) * 4, s));
+  uint32_t* tgt = h->ws.f.as<uint32_t>();
+  if (nt) {
+    k_mx_pair_rows<<<nblk(nt), 256, 0, s>>>(h->tree.m2l_off.as<uint32_t>(), nt, tgt);
+    GX_HIP(hipGetLastError());
   }
-  GX_CHECK(dbuf_reserve(h->ws.a, (size_t)(4 * nk + 8) * sizeof(uint32_t), s));
-  uint32_t* cnt = h->ws.a.as<uint32_t>();
-  uint32_t* koffp = cnt + nk + 1;
-  uint32_t* toff = koffp + nk + 1;
-  uint32_t* cursor = toff + nk + 1;
-  GX_HIP(hipMemsetAsync(cnt, 0, (size_t)(nk + 1) * 4, s));
-  GX_HIP(hipMemsetAsync(cursor, 0, (size_t)(nk + 1) * 4, s));
-  k_mx_count<<<nblk(ne), 256, 0, s>>>(h->key_of_pair.as<uint32_t>(), ne, cnt);
-  k_mx_pad<<<nblk(nk), 256, 0, s>>>(cnt, nk, koffp, toff);
-  GX_HIP(hipGetLastError());
-  uint32_t npad = 0, ntiles = 0;
-  GX_CHECK(scan_u32(h, koffp, koffp, nk, false, false, &npad));
-  GX_CHECK(scan_u32(h, toff, toff, nk, false, false, &ntiles));
-  h->mx_npad = npad;
-  h->mx_ntiles = ntiles;
-  GX_CHECK(dbuf_reserve(h->mx_pos, (size_t)ne * 4, s));
-  GX_CHECK(dbuf_reserve(h->mx_src, (size_t)npad * 4, s));
-  GX_CHECK(dbuf_reserve(h->mx_tiles, (size_t)ntiles * 12, s));
-  k_mx_place<<<nblk(ne), 256, 0, s>>>(h->key_of_pair.as<uint32_t>(), h->tree.m2l_dat.as<uint32_t>(), ne, koffp,
-                                      cursor, h->mx_pos.as<uint32_t>(), h->mx_src.as<uint32_t>());
-  k_mx_tiles<<<nk, 64, 0, s>>>(cnt, koffp, toff, nk, h->mx_tiles.as<uint32_t>());
-  GX_HIP(hipGetLastError());
-  GX_CHECK(dbuf_reserve(h->mx_that, (size_t)nk * K * K * 4, s));
-  GX_CHECK(dbuf_reserve(h->mx_sig, (size_t)nk * (P + 2) * 8, s));
-  k_mx_operators<P><<<nk, 256, 0, s>>>(h->gtab.as<double>(), h->key_dxy.as<double>(), nk,
-                                       h->cfg.softening * h->cfg.softening, h->mx_that.as<float>(),
-                                       h->mx_sig.as<double>());
-  GX_HIP(hipGetLastError());
-  GX_CHECK(dbuf_reserve(h->mx_R, (size_t)npad * num_moments(P) * 4, s));
+  GX_CHECK(build_tiles(h, tgt, h->tree.m2l_dat.as<uint32_t>(), nt, h->mx_fwd));
+  if (h->cfg.precision == GALAX_PREC_FP64) {
+    GX_CHECK(operators64<P>(h));
+  } else if (nk) {
+    GX_CHECK(dbuf_reserve(h->mx_that, (size_t)nk * K * K * 4, s));
+    GX_CHECK(dbuf_reserve(h->mx_sig, (size_t)nk * (P + 2) * 8, s));
+    k_mx_operators32<P><<<nk, 256, 0, s>>>(h->gtab.as<double>(), h->key_dxy.as<double>(), nk,
+                                           h->cfg.softening * h->cfg.softening, h->mx_that.as<float>(),
+                                           h->mx_sig.as<double>());
+    GX_HIP(hipGetLastError());
+  }
   GX_HIP(hipStreamSynchronize(s));
   h->mx_valid = true;
   return 0;
 }
 
 template <int P>
-static int run_impl(galax_fmm* h, const double* mp, double* loc, bool bf16x3) {
+static int adjoint_setup_impl(galax_fmm* h) {
   hipStream_t s = h->stream;
-  if (h->mx_ntiles) {
-    if (bf16x3)
-      k_m2l_mfma_bf16x3<P><<<nblk(h->mx_ntiles, 4), 256, 0, s>>>(h->mx_tiles.as<uint32_t>(),
-                                                                  (uint32_t)h->mx_ntiles, h->mx_src.as<uint32_t>(),
-                                                                  h->mx_that.as<float>(), h->mx_sig.as<double>(), mp,
-                                                                  h->mx_R.as<float>());
-    else
-      k_m2l_mfma_f32<P><<<nblk(h->mx_ntiles, 4), 256, 0, s>>>(h->mx_tiles.as<uint32_t>(), (uint32_t)h->mx_ntiles,
-                                                               h->mx_src.as<uint32_t>(), h->mx_that.as<float>(),
-                                                               h->mx_sig.as<double>(), mp, h->mx_R.as<float>());
+  const uint32_t ne = (uint32_t)h->n_live_pairs, nt = (uint32_t)h->n_live_targets;
+  GX_CHECK(dbuf_reserve(h->ws.f, (size_t)std::max<uint32_t>(ne, 1) * 4, s));
+  uint32_t* tgt = h->ws.f.as<uint32_t>();
+  if (nt) {
+    k_mx_pair_rows<<<nblk(nt), 256, 0, s>>>(h->tree.m2l_off.as<uint32_t>(), nt, tgt);
     GX_HIP(hipGetLastError());
   }
-  uint32_t nt = (uint32_t)h->n_live_targets;
-  if (nt) {
-    k_m2l_gather<P><<<nblk(nt, 4), 256, 0, s>>>(h->tree.m2l_off.as<uint32_t>(), h->mx_pos.as<uint32_t>(), nt,
-                                                h->mx_R.as<float>(), loc);
+  // rows are sources; every source of a live pair is a live node (same level)
+  GX_CHECK(build_tiles(h, h->tree.m2l_dat.as<uint32_t>(), tgt, nt, h->mx_adj));
+  if (!h->mx_t64_valid) GX_CHECK(operators64<P>(h));
+  GX_HIP(hipStreamSynchronize(s));
+  h->mx_adj_valid = true;
+  return 0;
+}
+
+template <int P, int MODE, bool TR>
+static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
+  hipStream_t s = h->stream;
+  if (!T.nitems) return 0;
+  constexpr int S = num_moments(P);
+  if (T.nslots) GX_CHECK(dbuf_reserve(T.part, T.nslots * 16 * S * 8, s));
+  TileArgs a;
+  a.item_tile = T.item_tile.as<uint32_t>();
+  a.item_s0 = T.item_s0.as<uint32_t>();
+  a.item_s1 = T.item_s1.as<uint32_t>();
+  a.item_slot = T.item_slot.as<uint32_t>();
+  a.nitems = (uint32_t)T.nitems;
+  a.per_xcd = (uint32_t)((T.nitems + 7) / 8);
+  a.n_rows = (uint32_t)T.n_rows;
+  a.step_key = T.step_key.as<uint32_t>();
+  a.step_first = T.step_first.as<uint32_t>();
+  a.step_cnt = T.step_cnt.as<uint32_t>();
+  a.src = T.src.as<uint32_t>();
+  a.t64 = h->mx_t64.as<double>();
+  a.that = h->mx_that.as<float>();
+  a.sig_inv = h->mx_sig.as<double>();
+  a.in = in;
+  a.out = out;
+  a.part = T.part.as<double>();
+  k_m2l_tiles<P, MODE, TR><<<8 * a.per_xcd, 256, 0, s>>>(a);
+  GX_HIP(hipGetLastError());
+  if (T.nmulti) {
+    k_m2l_tiles_combine<<<(unsigned)T.nmulti, 256, 0, s>>>(T.multi_tile.as<uint32_t>(), T.multi_slot.as<uint32_t>(),
+                                                           T.multi_cnt.as<uint32_t>(), S, (uint32_t)T.n_rows,
+                                                           T.part.as<double>(), out);
     GX_HIP(hipGetLastError());
   }
   return 0;
+}
+
+template <int P>
+static int run_impl(galax_fmm* h, const double* mp, double* loc) {
+  switch (h->cfg.precision) {
+    case GALAX_PREC_FP64:
+      return run_tiles<P, MX_F64, false>(h, h->mx_fwd, mp, loc);
+    case GALAX_PREC_MIXED_FP32:
+      return run_tiles<P, MX_F32, false>(h, h->mx_fwd, mp, loc);
+    case GALAX_PREC_MIXED_BF16X3:
+      return run_tiles<P, MX_BF16X3, false>(h, h->mx_fwd, mp, loc);
+  }
+  return set_error(GALAX_ERR_INVALID, "M2L tiles: precision %u has no tiled form", h->cfg.precision);
 }
 
 #define GX_P_CASES(F) \
@@ -355,11 +669,32 @@ int m2l_mfma_setup(galax_fmm* h) {
 }
 
 int m2l_mfma_run(galax_fmm* h, const double* mp, double* loc) {
-  const bool bf = h->cfg.precision == GALAX_PREC_MIXED_BF16X3;
   switch (h->cfg.p) {
 #define GX_CASE(PP) \
   case PP:          \
-    return run_impl<PP>(h, mp, loc, bf);
+    return run_impl<PP>(h, mp, loc);
+    GX_P_CASES(GX_CASE)
+#undef GX_CASE
+  }
+  return set_error(GALAX_ERR_INVALID, "p out of range");
+}
+
+int m2l_adjoint_setup(galax_fmm* h) {
+  switch (h->cfg.p) {
+#define GX_CASE(PP) \
+  case PP:          \
+    return adjoint_setup_impl<PP>(h);
+    GX_P_CASES(GX_CASE)
+#undef GX_CASE
+  }
+  return set_error(GALAX_ERR_INVALID, "p out of range");
+}
+
+int m2l_adjoint_run(galax_fmm* h, const double* lbar, double* mbar) {
+  switch (h->cfg.p) {
+#define GX_CASE(PP) \
+  case PP:          \
+    return run_tiles<PP, MX_F64, true>(h, h->mx_adj, lbar, mbar);
     GX_P_CASES(GX_CASE)
 #undef GX_CASE
   }

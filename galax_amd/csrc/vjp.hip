@@ -120,54 +120,66 @@ __global__ void __launch_bounds__(64) k_vjp_near(const double* __restrict__ x, c
   }
 }
 
-__global__ void k_vjp_near_combine(const uint32_t* __restrict__ multi, uint32_t n_multi,
-                                   const uint32_t* __restrict__ goff, const uint32_t* __restrict__ part_off,
-                                   const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be, uint32_t n_max,
-                                   const double* __restrict__ part, double* __restrict__ xb, double* __restrict__ yb,
-                                   double* __restrict__ mb) {
-  const uint32_t q = blockIdx.x;
-  if (q >= n_multi) return;
-  const uint32_t t = multi[q];
-  const uint32_t nit = (goff[t + 1] - goff[t] + P2P_ITEM_LEAVES - 1) / P2P_ITEM_LEAVES;
-  for (uint32_t b = bs[t] + threadIdx.x; b < be[t]; b += blockDim.x) {
-    double sx = 0.0, sy = 0.0, sm = 0.0;
-    for (uint32_t k = 0; k < nit; ++k) {
-      const double* pp = part + ((size_t)(part_off[t] + k) * n_max + (b - bs[t])) * 3;
-      sx += pp[0];
-      sy += pp[1];
-      sm += pp[2];
-    }
-    xb[b] = sx;
-    yb[b] = sy;
-    mb[b] = sm;
-  }
-}
-
 // ------------------------------------------------------ L2P adjoint ----
 
-// Lbar of each leaf: lane per coefficient, bodies in order
+// Lbar of each leaf: Lbar[IJ] = -sum_b a_bar_b . grad(u^I v^J)(u_b). One wave
+// per leaf; chunks of 32 bodies stage their power tables u^0..u^P, v^0..v^P and
+// cotangents in LDS, then each lane sums its coefficient over the chunk.
 template <int P>
 __global__ void __launch_bounds__(256) k_vjp_l2p_coef(const double* __restrict__ x, const double* __restrict__ y,
                                                       const double* __restrict__ axb, const double* __restrict__ ayb,
                                                       const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be,
                                                       const double* __restrict__ comx, const double* __restrict__ comy,
                                                       uint32_t n_leaves, double* __restrict__ lbar) {
-  constexpr int S = num_moments(P);
-  const uint32_t leaf = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (leaf >= n_leaves) return;
-  for (int k = lane; k < S; k += 64) {
-    int I, J;
-    cij<P>(k, I, J);
+  constexpr int S = num_moments(P), CH = 32;
+  __shared__ double pu[4][CH][P + 1], pv[4][CH][P + 1], gx_[4][CH], gy_[4][CH];
+  __shared__ uint32_t blen[4];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t leaf = blockIdx.x * 4 + w;
+  const bool live = leaf < n_leaves;
+  const uint32_t s0 = live ? bs[leaf] : 0u, s1 = live ? be[leaf] : 0u;
+  const double cx = live ? comx[leaf] : 0.0, cy = live ? comy[leaf] : 0.0;
+  if (lane == 0) blen[w] = s1 - s0;
+  __syncthreads();
+  const uint32_t len = max(max(blen[0], blen[1]), max(blen[2], blen[3]));
+  for (int k0 = 0; k0 < S; k0 += 64) {
+    const int k = k0 + lane;
+    int I = 0, J = 0;
+    if (k < S) cij<P>(k, I, J);
     double acc = 0.0;
-    for (uint32_t b = bs[leaf]; b < be[leaf]; ++b) {
-      const double u = x[b] - comx[leaf], v = y[b] - comy[leaf];
-      double gu = 0.0, gv = 0.0;
-      if (I >= 1) gu = (double)I * pow(u, I - 1) * pow(v, J);
-      if (J >= 1) gv = (double)J * pow(u, I) * pow(v, J - 1);
-      acc -= axb[b] * gu + ayb[b] * gv;
+    for (uint32_t c = 0; c < len; c += CH) {
+      __syncthreads();
+      if (lane < CH) {
+        const uint32_t b = s0 + c + lane;
+        double u = 0.0, v = 0.0, gxv = 0.0, gyv = 0.0;
+        if (b < s1) {
+          u = x[b] - cx;
+          v = y[b] - cy;
+          gxv = axb[b];
+          gyv = ayb[b];
+        }
+        double a = 1.0, bb = 1.0;
+#pragma unroll
+        for (int q = 0; q <= P; ++q) {
+          pu[w][lane][q] = a;
+          pv[w][lane][q] = bb;
+          a *= u;
+          bb *= v;
+        }
+        gx_[w][lane] = gxv;
+        gy_[w][lane] = gyv;
+      }
+      __syncthreads();
+      const uint32_t n = s1 > s0 + c ? min((uint32_t)CH, s1 - s0 - c) : 0u;
+      if (k < S) {
+        for (uint32_t q = 0; q < n; ++q) {
+          const double gu = (double)I * pu[w][q][I > 0 ? I - 1 : 0] * pv[w][q][J];
+          const double gv = (double)J * pu[w][q][I] * pv[w][q][J > 0 ? J - 1 : 0];
+          acc -= gx_[w][q] * gu + gy_[w][q] * gv;
+        }
+      }
     }
-    lbar[(size_t)leaf * S + k] = acc;
+    if (live && k < S) lbar[(size_t)leaf * S + k] = acc;
   }
 }
 
@@ -276,38 +288,6 @@ __global__ void k_vjp_m2m(const uint32_t* __restrict__ par, const double* __rest
 // ------------------------------------------------------ M2L adjoint ----
 // Mbar_s[i,j] += sum_pairs sum_{I,J} (-1)^{i+j}/(I!J!) G[i+I, j+J] Lbar_t[I,J]
 
-// reference variant: the pairs naming s as source (reverse CSR of the live
-// pairs), each with its key's derivative table
-template <int P>
-__global__ void __launch_bounds__(256) k_vjp_m2l_ref(const uint32_t* __restrict__ roff,
-                                                     const uint32_t* __restrict__ rpair, uint32_t n_src,
-                                                     const uint32_t* __restrict__ tgt_of_pair,
-                                                     const uint32_t* __restrict__ key_of_pair,
-                                                     const double* __restrict__ gtab, const double* __restrict__ lbar,
-                                                     double* __restrict__ mbar) {
-  constexpr int S = num_moments(P), ST = 2 * P + 1, GS = ST * ST;
-  const uint32_t s = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (s >= n_src) return;
-  for (int k = lane; k < S; k += 64) {
-    int i, j;
-    cij<P>(k, i, j);
-    const double sg = ((i + j) & 1) ? -1.0 : 1.0;
-    double acc = 0.0;
-    for (uint32_t r = roff[s]; r < roff[s + 1]; ++r) {
-      const uint32_t e = rpair[r];
-      const double* G = gtab + (size_t)key_of_pair[e] * GS;
-      const double* Lb = lbar + (size_t)tgt_of_pair[e] * S;
-#pragma unroll
-      for (int I = 0; I <= P; ++I)
-#pragma unroll
-        for (int J = 0; J <= P - I; ++J)
-          acc += sg / (dfact(I) * dfact(J)) * G[(i + I + j + J) * ST + (j + J)] * Lb[moment_index(I, J)];
-    }
-    mbar[(size_t)s * S + k] = acc;
-  }
-}
-
 // corrected variant: lists are symmetric; exact derivatives per pair, built
 // in LDS by the same recurrence as the forward (D = derivatives of 1/R, G = -D)
 template <int P>
@@ -377,18 +357,6 @@ __global__ void __launch_bounds__(256) k_vjp_m2l_exact(const uint32_t* __restric
   }
 }
 
-// reverse CSR of the live reference pairs: rpair lists, per source node, the
-// pair indices e (ascending) whose source is that node; built by one radix
-// sort of the keys source * 2^w + e (a node can be the source of many pairs)
-__global__ void k_rev_keys(const uint32_t* __restrict__ dat, uint32_t n_pairs, int w, uint64_t* __restrict__ keys,
-                           uint32_t* __restrict__ vals, uint32_t* __restrict__ cnt) {
-  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n_pairs) return;
-  keys[e] = ((uint64_t)dat[e] << w) | e;
-  vals[e] = e;
-  atomicAdd(&cnt[dat[e]], 1u);
-}
-
 template <int P>
 static int vjp_impl(galax_fmm* h, const double* x, const double* y, const double* m, const double* axb,
                     const double* ayb, double* xb, double* yb, double* mb) {
@@ -418,13 +386,8 @@ static int vjp_impl(galax_fmm* h, const double* x, const double* y, const double
                                                  it.g0.as<uint32_t>(), it.g1.as<uint32_t>(), it.slot.as<uint32_t>(),
                                                  nmax, eps2, xb, yb, mb, h->p2p_part.as<double>());
   GX_HIP(hipGetLastError());
-  if (it.n_multi) {
-    k_vjp_near_combine<<<(unsigned)it.n_multi, 64, 0, s>>>(it.multi.as<uint32_t>(), (uint32_t)it.n_multi, goff,
-                                                            it.part_off.as<uint32_t>(), t.body_start.as<uint32_t>(),
-                                                            t.body_end.as<uint32_t>(), nmax, h->p2p_part.as<double>(),
-                                                            xb, yb, mb);
-    GX_HIP(hipGetLastError());
-  }
+  GX_HIP(p2p_combine(it, goff, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), nmax,
+                     h->p2p_part.as<double>(), xb, yb, mb, s));
   // L2P adjoint -> Lbar of leaves
   k_vjp_l2p_coef<P><<<nblk(nl, 4), 256, 0, s>>>(x, y, axb, ayb, t.body_start.as<uint32_t>(),
                                                 t.body_end.as<uint32_t>(), comx, comy, (uint32_t)nl, lbar);
@@ -442,31 +405,9 @@ static int vjp_impl(galax_fmm* h, const double* x, const double* y, const double
     k_vjp_m2l_exact<P><<<nblk(nn, 4), 256, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(),
                                                    (uint32_t)nn, comx, comy, eps2, lbar, mbar);
   } else if (h->n_live_pairs > 0) {
-    const uint32_t live_hi = t.level_offset[top];
-    if (!h->vj_rev_valid) {
-      const uint32_t ne = (uint32_t)h->n_live_pairs;
-      int w = 1;
-      while (w < 32 && (1ull << w) < ne) ++w;
-      int ws_ = 1;
-      while (ws_ < 32 && (1ull << ws_) < live_hi) ++ws_;
-      GX_CHECK(dbuf_reserve(h->vj_roff, (size_t)(live_hi + 1) * 4, s));
-      GX_CHECK(dbuf_reserve(h->vj_rpair, (size_t)ne * 4 + 4, s));
-      GX_CHECK(dbuf_reserve(h->ws.sort_ka, (size_t)ne * 8 + 8, s));
-      GX_CHECK(dbuf_reserve(h->ws.sort_kb, (size_t)ne * 8 + 8, s));
-      GX_CHECK(dbuf_reserve(h->ws.sort_vb, (size_t)ne * 4 + 4, s));
-      uint32_t* roff = h->vj_roff.as<uint32_t>();
-      GX_HIP(hipMemsetAsync(roff, 0, (size_t)(live_hi + 1) * 4, s));
-      k_rev_keys<<<nblk(ne), 256, 0, s>>>(t.m2l_dat.as<uint32_t>(), ne, w, h->ws.sort_ka.as<uint64_t>(),
-                                          h->vj_rpair.as<uint32_t>(), roff);
-      GX_HIP(hipGetLastError());
-      GX_CHECK(radix_sort_bits(h, h->ws.sort_ka.as<uint64_t>(), h->vj_rpair.as<uint32_t>(),
-                               h->ws.sort_kb.as<uint64_t>(), h->ws.sort_vb.as<uint32_t>(), ne, w + ws_));
-      GX_CHECK(scan_u32(h, roff, roff, live_hi + 1, false, false, nullptr));
-      h->vj_rev_valid = true;
-    }
-    k_vjp_m2l_ref<P><<<nblk(live_hi, 4), 256, 0, s>>>(h->vj_roff.as<uint32_t>(), h->vj_rpair.as<uint32_t>(), live_hi,
-                                                      h->tmp_u32.as<uint32_t>(), h->key_of_pair.as<uint32_t>(),
-                                                      h->gtab.as<double>(), lbar, mbar);
+    // mbar_s = sum over live pairs (t, s) of T_K^T lbar_t, on the matrix cores
+    if (!h->mx_adj_valid) GX_CHECK(m2l_adjoint_setup(h));
+    GX_CHECK(m2l_adjoint_run(h, lbar, mbar));
   }
   GX_HIP(hipGetLastError());
   // M2M adjoint, top-down
@@ -485,11 +426,10 @@ static int vjp_impl(galax_fmm* h, const double* x, const double* y, const double
   return 0;
 }
 
-// forward (recomputing M and L from the given positions) then the adjoint passes
+// forward (recomputing M and L from the given positions, in the handle's
+// precision form) then the adjoint passes, all in FP64
 int vjp(galax_fmm* h, const double* d_x, const double* d_y, const double* d_m, const double* d_axb,
         const double* d_ayb, double* d_xb, double* d_yb, double* d_mb) {
-  if (h->cfg.precision != GALAX_PREC_FP64)
-    return set_error(GALAX_ERR_UNSUPPORTED, "the VJP is provided in FP64 (GALAX_PREC_FP64) only");
   size_t n = h->tree.n_bodies;
   GX_CHECK(dbuf_reserve(h->vj_fwd, std::max<size_t>(n, 1) * 16, h->stream));
   double* fax = h->vj_fwd.as<double>();

@@ -31,7 +31,7 @@ LIB_PATH = os.path.join(HERE, "libgalax_fmm.so")
 
 ALGO_REFERENCE, ALGO_CORRECTED = 0, 1
 TREE_PLAIN, TREE_CONSTRAINED = 0, 1
-PREC_FP64, PREC_MIXED_FP32, PREC_MIXED_BF16X3 = 0, 1, 2
+PREC_FP64, PREC_MIXED_FP32, PREC_MIXED_BF16X3, PREC_FP64_EXACT = 0, 1, 2, 3
 
 STATUS = {0: "OK", -1: "InitFailed", -2: "ShaderCompilation", -3: "BufferAllocation", -4: "InvalidArgument",
           -5: "DeviceError", -6: "Unsupported", -7: "ThresholdExceeded"}

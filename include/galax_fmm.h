@@ -62,9 +62,12 @@ enum {
 };
 
 enum {
-  GALAX_PREC_FP64 = 0,       /* everything in FP64 */
-  GALAX_PREC_MIXED_FP32 = 1, /* FP32 expansions, FP32 MFMA M2L, FP32 P2P on leaf-relative coordinates */
-  GALAX_PREC_MIXED_BF16X3 = 2 /* as MIXED_FP32 with M2L as a 3-term split BF16 MFMA product */
+  GALAX_PREC_FP64 = 0,          /* everything in FP64; M2L on FP64 MFMA (grouped by cache key) */
+  GALAX_PREC_MIXED_FP32 = 1,    /* M2L products on FP32 MFMA, P2P in FP32 on leaf-relative coordinates;
+                                   expansions, geometry and sums in FP64 */
+  GALAX_PREC_MIXED_BF16X3 = 2,  /* as MIXED_FP32 with M2L as a 3-term split BF16 MFMA product */
+  GALAX_PREC_FP64_EXACT = 3     /* FP64 with galax-core's M2L summation order (per pair, on the VALU):
+                                   expansions bit-identical to the reference's arithmetic */
 };
 
 /* Mirrors GpuConfig (galax-gpu/src/lib.rs:22-40) plus the tree bounds that

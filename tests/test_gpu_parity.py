@@ -123,12 +123,13 @@ def _rel_diff(a, b):
     return np.hypot(a[0] - b[0], a[1] - b[1]) / den
 
 
+@pytest.mark.parametrize("prec", [fmm.PREC_FP64, fmm.PREC_FP64_EXACT])
 @pytest.mark.parametrize("p", [0, 2, 4, 6, 8])
-def test_forces_match_oracle_reference(p):
+def test_forces_match_oracle_reference(p, prec):
     d = ic.plummer(12000, seed=201)
     bounds = (-50.0, 50.0, -50.0, 50.0)
     eps = 0.02
-    ctx, b, _ = _gpu_build(d, p=p, eps=eps, bounds=bounds)
+    ctx, b, _ = _gpu_build(d, p=p, eps=eps, bounds=bounds, precision=prec)
     ctx.evaluate(b)
     ot, od, _ = _oracle_build(d, 32, bounds)
     ax, ay, phi = O.compute_fmm_force(od["x"], od["y"], od["m"], ot, O.m2l_lists(ot), O.p2p_lists(ot), p, eps,
@@ -271,6 +272,22 @@ def test_errors_are_codes_not_crashes():
 
 
 # ------------------------------------------------------ mixed precision ----
+
+@pytest.mark.parametrize("p", [2, 6, 8])
+def test_fp64_mfma_m2l_matches_exact_order(p):
+    """The FP64 form (M2L grouped by key on FP64 MFMA) against FP64_EXACT
+    (galax-core's per-pair order): the same operators summed in another order."""
+    d = ic.plummer(50_000, seed=303)
+    bounds = (-50.0, 50.0, -50.0, 50.0)
+    out = {}
+    for prec in (fmm.PREC_FP64, fmm.PREC_FP64_EXACT):
+        ctx, b, _ = _gpu_build(d, p=p, eps=0.02, bounds=bounds, precision=prec)
+        ctx.evaluate(b)
+        out[prec] = (b.ax.copy(), b.ay.copy(), b.phi.copy())
+    a, e = out[fmm.PREC_FP64], out[fmm.PREC_FP64_EXACT]
+    rd = _rel_diff(a[:2], e[:2])
+    assert rd.max() <= 1e-10, rd.max()
+    np.testing.assert_allclose(a[2], e[2], rtol=1e-11, atol=0.0)
 
 def _forms_run(d, p, eps, bounds, algo=fmm.ALGO_REFERENCE, n_max=32):
     out = {}
