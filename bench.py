@@ -45,6 +45,11 @@ def main():
     ap.add_argument("--cpu-sample-n", type=int, default=1 << 18)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--vjp-steps", type=int, default=3)
+    ap.add_argument("--forms", default="fp64,fp64_exact,mixed_fp32,mixed_bf16x3",
+                    help="comma-separated forms to run (the headline form and fp64 are always run)")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="after the forms, copy a buffer of this many MiB on the device (a kernel of known "
+                         "traffic that scales hardware byte counters)")
     ap.add_argument("--form", default="mixed_fp32", choices=["fp64", "fp64_exact", "mixed_fp32", "mixed_bf16x3"],
                     help="the form whose throughput is `value`")
     args = ap.parse_args()
@@ -60,9 +65,11 @@ def main():
     d, spec = ic.make(args.workload)
     n = len(d["x"])
     algo = fmm.ALGO_CORRECTED if args.algo == "corrected" else fmm.ALGO_REFERENCE
-    forms = [("fp64", fmm.PREC_FP64), ("fp64_exact", fmm.PREC_FP64_EXACT), ("mixed_fp32", fmm.PREC_MIXED_FP32),
-             ("mixed_bf16x3", fmm.PREC_MIXED_BF16X3)]
+    all_forms = [("fp64", fmm.PREC_FP64), ("fp64_exact", fmm.PREC_FP64_EXACT), ("mixed_fp32", fmm.PREC_MIXED_FP32),
+                 ("mixed_bf16x3", fmm.PREC_MIXED_BF16X3)]
     headline = args.form
+    wanted = set(args.forms.split(",")) | {"fp64", headline}
+    forms = [f for f in all_forms if f[0] in wanted]
     res = {}
     eps = None
     for name, prec in forms:
@@ -75,6 +82,7 @@ def main():
         t0 = time.perf_counter()
         ctx.build_tree(b)
         t_build = time.perf_counter() - t0
+        build_ms = ctx.build_timings()
         info = ctx.tree_info()
         if eps is None:
             eps = fmm.default_softening(b.x, b.y)  # galax-cli/src/main.rs:182-190, on the sorted bodies
@@ -120,9 +128,18 @@ def main():
             log(f"[bench] {name}: vjp {vjp_ms:.3f} ms ({vjp_ms / ms:.2f}x the forward)")
             del axb, ayb, xb, yb, mb
         res[name] = dict(ms=ms, passes=passes, t_build=t_build, t_first=t_first, tm=tm, info=info,
-                         ax=ax.cpu().numpy(), ay=ay.cpu().numpy(), bodies=b, vjp_ms=vjp_ms)
+                         ax=ax.cpu().numpy(), ay=ay.cpu().numpy(), bodies=b, vjp_ms=vjp_ms, build_ms=build_ms)
         del ctx, x, y, m, ax, ay, phi
         torch.cuda.empty_cache()
+
+    if args.copy_probe_mb > 0:
+        src = torch.empty(args.copy_probe_mb << 20, dtype=torch.uint8, device=dev).fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: {args.copy_probe_mb} MiB read + {args.copy_probe_mb} MiB written")
+        del src, dst
 
     b = res["fp64"]["bodies"]
     rs = ic.SplitMix64(4242)
@@ -146,27 +163,15 @@ def main():
         errs[name] = e
     log(f"[bench] errors {errs}")
 
-    # roofline of the headline form's dominant kernel (algorithmic flops or bytes per launch)
+    # roofline of the headline form's dominant pass (algorithmic flops or bytes per launch)
     H = res[headline]
-    passes, tm, info = H["passes"], H["tm"], H["info"]
-    S = _num_moments(args.p)
+    stages = stage_rooflines(H, n, args.p, headline)
+    dominant = max(("l2p_p2p", "m2l", "p2m", "m2m", "l2l"), key=lambda k: stages.get(k, {}).get("ms", 0.0))
+    st = stages[dominant]
+    roof = {"kernel": dominant, "bound": st["bound"], "achieved": st["achieved"], "peak": st["peak"],
+            "unit": st["unit"], "frac": st["frac"], "traffic": measured_traffic(args, headline, dominant)}
+    tm, info = H["tm"], H["info"]
     I = tm.p2p_interactions
-    n_leaves = info["n_leaves"]
-    dominant = max(("l2p_p2p", "m2l", "p2m", "m2m", "l2l"), key=lambda k: passes.get(k, 0.0))
-    fp32 = headline.startswith("mixed")
-    peak_c = FP32_PEAK_TFS if fp32 else FP64_PEAK_TFS
-    flops = {"l2p_p2p": 17.0 * I + 4.0 * S * n, "m2l": 2.0 * S * S * tm.m2l_pairs_executed}
-    bytes_ = {"p2m": 24.0 * n + n_leaves * S * 8.0, "m2m": 2.0 * info["n_nodes"] * S * 8.0,
-              "l2l": 2.0 * n_leaves * S * 8.0}
-    t_dom = passes[dominant] * 1e-3
-    if dominant in flops:
-        achieved = flops[dominant] / t_dom / 1e12
-        roof = {"kernel": dominant, "bound": "valu" if dominant == "l2p_p2p" else "mfma", "achieved": achieved,
-                "peak": peak_c, "unit": "TFLOP/s", "frac": achieved / peak_c, "traffic": None}
-    else:
-        achieved = bytes_[dominant] / t_dom / 1e9
-        roof = {"kernel": dominant, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -183,7 +188,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if fp32 else "f64",
+        "dtype": "f32" if headline.startswith("mixed") else "f64",
         "data": "synthetic (seeded SplitMix64 2D Plummer sphere)",
         "config": {"workload": args.workload, "N": n, "p": args.p, "n_max": spec["n_max"], "algo": args.algo,
                    "form": headline, "softening": eps, "bounds": list(spec["bounds"]),
@@ -196,7 +201,10 @@ def main():
                       "vjp_ms": r["vjp_ms"], "vjp_over_fwd": r["vjp_ms"] / r["ms"] if r["vjp_ms"] else None,
                       **errs[k]}
                   for k, r in res.items()},
+        "stages": stages,
+        "build_ms": H["build_ms"],
         "tree": info,
+        "m2l_tiles": {"groups": tm.m2l_groups, "steps": tm.m2l_steps, "items": tm.m2l_items},
         "p2p_interactions": I,
         "m2l_pairs_executed": tm.m2l_pairs_executed,
         "m2l_pairs_reference": tm.m2l_pairs_reference,
@@ -204,6 +212,61 @@ def main():
         "vjp_over_fwd": H["vjp_ms"] / H["ms"] if H["vjp_ms"] else None,
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(args, form, stage):
+    """HBM bytes per launch of the stage's main kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/traffic.json), when they
+    were taken on this workload, p and form; else None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if (t.get("workload"), t.get("p"), t.get("form")) != (args.workload, args.p, form):
+        return None
+    mode = {"fp64": 0, "mixed_fp32": 1, "mixed_bf16x3": 2}.get(form)
+    prec = "float" if form.startswith("mixed") else "double"
+    name = {"m2l": f"gx::k_m2l_tiles<{args.p}, {mode}, false>", "l2p_p2p": f"gx::k_l2p_p2p<{args.p}, {prec}, true>"}.get(stage)
+    k = t["kernels"].get(name)
+    return k["traffic_bytes"] if k else None
+
+
+def stage_rooflines(r, n, p, form):
+    """Achieved rate of each pass and build stage against the MI355X peak that
+    bounds it, from algorithmic bytes or flops per launch (N bodies, L leaves,
+    V nodes, S coefficients, E executed M2L pairs, I P2P interactions; w = 8:
+    expansions and bodies are FP64 in every form)."""
+    passes, tm, info, b = r["passes"], r["tm"], r["info"], r["build_ms"]
+    S = _num_moments(p)
+    L, V, E, I = info["n_leaves"], info["n_nodes"], tm.m2l_pairs_executed, tm.p2p_interactions
+    mixed = form.startswith("mixed")
+    out = {}
+
+    def hbm(name, ms, nbytes):
+        if ms and ms > 0:
+            a = nbytes / (ms * 1e-3) / 1e9
+            out[name] = {"ms": ms, "bound": "hbm", "bytes": nbytes, "achieved": a, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": a / HBM_PEAK_GBS}
+
+    def comp(name, ms, flops, bound, peak):
+        if ms and ms > 0:
+            a = flops / (ms * 1e-3) / 1e12
+            out[name] = {"ms": ms, "bound": bound, "flops": flops, "achieved": a, "peak": peak, "unit": "TFLOP/s",
+                         "frac": a / peak}
+
+    hbm("sort", b["sort"], 24.0 * n + 6 * 2 * 12.0 * n)      # Morton keys + 6 radix passes of (key, index)
+    hbm("permute", b["permute"], 2 * 56.0 * n)               # gather of the 7 body arrays
+    hbm("tree", b["tree"], 24.0 * n + V * 2 * 64.0)          # leaf geometry + parent rounds
+    hbm("p2m", passes.get("p2m"), 24.0 * n + L * S * 8.0)
+    hbm("m2m", passes.get("m2m"), 2.0 * L * S * 8.0)         # reference variant: level-1 parents of the leaves
+    hbm("l2l", passes.get("l2l"), 2.0 * L * S * 8.0)
+    # the main kernel of each pass alone when it was timed (no combine pass)
+    comp("m2l", passes.get("m2l_kernel") or passes.get("m2l"), 2.0 * S * S * E, "mfma",
+         FP32_PEAK_TFS if mixed else FP64_PEAK_TFS)
+    comp("l2p_p2p", passes.get("p2p_kernel") or passes.get("l2p_p2p"), 17.0 * I + 4.0 * S * n, "valu",
+         FP32_PEAK_TFS if mixed else FP64_PEAK_TFS)
+    return out
 
 
 def cpu_baseline(args, spec, log):

@@ -350,15 +350,46 @@ void galax_fmm_destroy(galax_fmm_t* h) {
   for (DBuf& b : w.scan) dbuf_free(b);
   for (hipEvent_t& e : h->ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t& e : h->bev)
+    if (e) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
 
 static int build_common(galax_fmm_t* h, size_t n) {
-  int rc = h->cfg.tree_mode == GALAX_TREE_CONSTRAINED ? build_tree_constrained(h, n) : build_tree_plain(h, n);
+  for (hipEvent_t& e : h->bev)
+    if (!e) GX_HIP(hipEventCreate(&e));
+  h->build_timed = true;
+  const bool constrained = h->cfg.tree_mode == GALAX_TREE_CONSTRAINED;
+  GX_CHECK(build_mark(h, 0));
+  if (constrained || n == 0) {  // no sort or permutation stage
+    GX_CHECK(build_mark(h, 1));
+    GX_CHECK(build_mark(h, 2));
+  }
+  int rc = constrained ? build_tree_constrained(h, n) : build_tree_plain(h, n);
   if (rc) return rc;
+  GX_CHECK(build_mark(h, 3));
   GX_CHECK(build_lists_reference(h));
+  if (h->tree.n_nodes == 0) GX_CHECK(build_mark(h, 4));
+  GX_CHECK(build_mark(h, 5));
+  GX_HIP(hipStreamSynchronize(h->stream));
   return after_tree_installed(h);
+}
+
+int galax_fmm_last_build_timings(galax_fmm_t* h, galax_build_timings_t* out) {
+  if (!h || !out) return set_error(GALAX_ERR_INVALID, "NULL argument");
+  *out = galax_build_timings_t{};
+  if (!h->build_timed || h->tree.n_bodies == 0) return 0;
+  float ms[5], tot = 0.f;
+  for (int i = 0; i < 5; ++i) GX_HIP(hipEventElapsedTime(&ms[i], h->bev[i], h->bev[i + 1]));
+  GX_HIP(hipEventElapsedTime(&tot, h->bev[0], h->bev[5]));
+  out->sort = ms[0];
+  out->permute = ms[1];
+  out->tree = ms[2];
+  out->lists = ms[3];
+  out->gather = ms[4];
+  out->total = tot;
+  return 0;
 }
 
 int galax_fmm_build_tree(galax_fmm_t* h, galax_bodies_t* b, uint64_t* perm_out) {
@@ -424,6 +455,7 @@ int galax_fmm_build_tree_device(galax_fmm_t* h, double* d_x, double* d_y, double
 int galax_fmm_upload_tree(galax_fmm_t* h, const galax_tree_view_t* tree, const galax_lists_view_t* m2l,
                           const galax_lists_view_t* p2p) {
   if (!h || !tree || !m2l || !p2p) return set_error(GALAX_ERR_INVALID, "NULL argument");
+  h->build_timed = false;  // an uploaded tree has no device build stages
   GX_CHECK(install_tree_host(h, tree, m2l, p2p));
   if (h->tree.n_bodies > h->max_n) return set_error(GALAX_ERR_INVALID, "tree holds more bodies than max_n");
   return after_tree_installed(h);

@@ -73,7 +73,7 @@ struct M2LTiles {
   DBuf item_tile, item_s0, item_s1, item_slot;  // per work item
   DBuf multi_tile, multi_slot, multi_cnt;       // per tile with several items
   DBuf part;                                    // partial results, 16 x S doubles per slot
-  size_t n_rows = 0, ntiles = 0, nsteps = 0, nitems = 0, nmulti = 0, nslots = 0;
+  size_t n_rows = 0, ntiles = 0, ngroups = 0, nsteps = 0, nitems = 0, nmulti = 0, nslots = 0;
 };
 
 // Device-side tree in structure-of-arrays form. Node ids are level-ordered
@@ -139,6 +139,8 @@ struct galax_fmm {
   bool corrected_ready = false;
   galax_timings_t timings{};
   hipEvent_t ev[12]{};
+  hipEvent_t bev[6]{};  // build stages
+  bool build_timed = false;
 };
 
 namespace gx {
@@ -154,6 +156,9 @@ int scan_u32(galax_fmm* h, const uint32_t* d_in, uint32_t* d_out, size_t n, bool
 int build_tree_plain(galax_fmm* h, size_t n);             // bodies already in h->x/y/... unsorted
 int build_tree_constrained(galax_fmm* h, size_t n);       // bodies in h->x/y/m, current order
 int build_lists_reference(galax_fmm* h);                  // M2L + P2P lists + gather CSR
+// Records build-stage event i (0..5) on the handle's stream when the handle
+// has its build events (created by the API entry points).
+int build_mark(galax_fmm* h, int i);
 int install_tree_host(galax_fmm* h, const galax_tree_view_t* t, const galax_lists_view_t* m2l,
                       const galax_lists_view_t* p2p);
 int build_lists_corrected(galax_fmm* h, const double* d_x, const double* d_y);

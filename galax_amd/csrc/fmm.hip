@@ -556,19 +556,22 @@ struct Near<float> {
   static __device__ __forceinline__ float rsq(float v) { return __builtin_amdgcn_rsqf(v); }
 };
 
-// Near-field sum of one target body over staged sources [j0, n) step 2.
+// Near-field sum of one target body over the staged sources half, half + 2,
+// ... < n (n even: the chunk is padded with a far zero-mass source). The loop
+// index is the same for both half-waves, so it lives in scalar registers.
 // MASK: the chunk holds the target's own leaf; slot `self` is the body itself
 // and contributes nothing (its inverse distance is forced to 0).
 template <typename T, bool PHI, bool MASK>
-__device__ __forceinline__ void near_sum(const typename Near<T>::V4* __restrict__ src, int j0, int n, int self, T xr,
-                                         T yr, T eps2, T& accx, T& accy, T& accp) {
+__device__ __forceinline__ void near_sum(const typename Near<T>::V4* __restrict__ src, int half, int n, int self,
+                                         T xr, T yr, T eps2, T& accx, T& accy, T& accp) {
+  const typename Near<T>::V4* sh = src + half;
 #pragma unroll 4
-  for (int jj = j0; jj < n; jj += 2) {
-    const typename Near<T>::V4 sj = src[jj];
+  for (int jj = 0; jj < n; jj += 2) {
+    const typename Near<T>::V4 sj = sh[jj];
     const T dx = sj.x - xr, dy = sj.y - yr;
     const T r2 = fma(dx, dx, fma(dy, dy, eps2));
     T inv = Near<T>::rsq(r2);
-    if (MASK) inv = jj == self ? (T)0 : inv;
+    if (MASK) inv = jj + half == self ? (T)0 : inv;
     const T f = sj.z * (inv * inv * inv);
     accx = fma(f, dx, accx);
     accy = fma(f, dy, accy);
@@ -601,7 +604,7 @@ __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, co
   constexpr int S = num_moments(P);
   typedef typename Near<T>::V4 V4;
   static_assert(P2P_ITEM_LEAVES <= 64, "one lane per source leaf of an item");
-  __shared__ V4 src[P2P_CHUNK];
+  __shared__ V4 src[P2P_CHUNK + 1];
   __shared__ uint32_t l_ss[P2P_ITEM_LEAVES], l_off[P2P_ITEM_LEAVES];
   const uint32_t item = blockIdx.x;
   const uint32_t leaf = itgt[item], g0 = ig0[item], g1 = ig1[item], slot = islot[item];
@@ -668,13 +671,22 @@ __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, co
         v.w = (T)0;
         src[pp] = v;
       }
+      if (lane == 0 && (n & 1)) {  // pad to even: zero mass far away (r^2 overflows, 1/r = 0)
+        V4 v;
+        v.x = (T)3e19;
+        v.y = (T)3e19;
+        v.z = (T)0;
+        v.w = (T)0;
+        src[n] = v;
+      }
       __syncthreads();
       if (valid) {
+        const int ne = (n + 1) & ~1;
         if (c0 < self_hi && c0 + (uint32_t)n > self_lo)
-          near_sum<T, PHI, true>(src, half, n, (int)(self_lo + (i - ts)) - (int)c0, xr, yr, eps2, accx, accy,
+          near_sum<T, PHI, true>(src, half, ne, (int)(self_lo + (i - ts)) - (int)c0, xr, yr, eps2, accx, accy,
                                  accp);
         else
-          near_sum<T, PHI, false>(src, half, n, 0, xr, yr, eps2, accx, accy, accp);
+          near_sum<T, PHI, false>(src, half, ne, 0, xr, yr, eps2, accx, accy, accp);
       }
     }
     accx += __shfl_xor(accx, 32, 64);
@@ -1011,6 +1023,7 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
       d_x, d_y, d_m, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), comx, comy, loc, goff, gdat,     \
       it.tgt.as<uint32_t>(), it.g0.as<uint32_t>(), it.g1.as<uint32_t>(), it.slot.as<uint32_t>(), nmax, eps * eps, \
       d_ax, d_ay, d_phi, h->p2p_part.as<double>())
+  GX_HIP(hipEventRecord(ev[10], s));
   if (fp32_near) {
     if (d_phi)
       GX_LAUNCH_P2P(float, true);
@@ -1023,6 +1036,7 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
       GX_LAUNCH_P2P(double, false);
   }
   GX_HIP(hipGetLastError());
+  GX_HIP(hipEventRecord(ev[11], s));
   GX_HIP(p2p_combine(it, goff, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), nmax,
                      h->p2p_part.as<double>(), d_ax, d_ay, d_phi, s));
 #undef GX_LAUNCH_P2P
@@ -1084,6 +1098,15 @@ int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, 
   tm.m2l_pairs_reference = corrected ? h->c_m2l_total : t.m2l_total;
   tm.m2l_keys = corrected ? 0 : h->n_keys;
   tm.p2p_interactions = corrected ? h->c_p2p_interactions : t.p2p_interactions;
+  const bool tiled = !corrected && h->cfg.precision != GALAX_PREC_FP64_EXACT;
+  tm.m2l_groups = tiled ? h->mx_fwd.ngroups : 0;
+  tm.m2l_steps = tiled ? h->mx_fwd.nsteps : 0;
+  tm.m2l_items = tiled ? h->mx_fwd.nitems : 0;
+  float km = 0.f, kp = 0.f;
+  if (tiled && h->mx_fwd.nitems && h->n_live_targets) GX_HIP(hipEventElapsedTime(&km, h->ev[8], h->ev[9]));
+  GX_HIP(hipEventElapsedTime(&kp, h->ev[10], h->ev[11]));
+  tm.m2l_kernel = km;
+  tm.p2p_kernel = kp;
   return 0;
 }
 

@@ -408,6 +408,7 @@ __global__ void __launch_bounds__(256) k_m2l_tiles_combine(const uint32_t* __res
     const uint32_t t = tile * 16 + col;
     if (t >= n_rows) continue;
     double v = 0.0;
+#pragma unroll 8
     for (uint32_t k = 0; k < nit; ++k) v += part[((size_t)(slot0 + k) * 16 + col) * S + row];
     out[(size_t)t * S + row] = v;
   }
@@ -430,6 +431,7 @@ static int build_tiles(galax_fmm* h, const uint32_t* row_of_pair, const uint32_t
   const uint32_t ntiles = (n_rows + 15) / 16;
   T.n_rows = n_rows;
   T.ntiles = ntiles;
+  T.ngroups = 0;
   T.nsteps = 0;
   GX_CHECK(dbuf_reserve(T.tile_off, (size_t)(ntiles + 1) * 4, s));
   uint32_t* tile_off = T.tile_off.as<uint32_t>();
@@ -471,6 +473,7 @@ static int build_tiles(galax_fmm* h, const uint32_t* row_of_pair, const uint32_t
     GX_HIP(hipGetLastError());
     uint32_t ng = 0;  // exclusive: a group start i is group gid[i]
     GX_CHECK(scan_u32(h, gid, gid, ne, false, false, &ng));
+    T.ngroups = ng;
     const int rkb = bits_of((ne + MX_GROUP - 1) / MX_GROUP);
     if (bits_of(ntiles) + kb + rkb + 4 > 64) return set_error(GALAX_ERR_UNSUPPORTED, "M2L tiles: key space too wide");
     GX_CHECK(dbuf_reserve(T.src, (size_t)ne * 4, s));
@@ -630,8 +633,10 @@ static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
   a.in = in;
   a.out = out;
   a.part = T.part.as<double>();
+  if (!TR) GX_HIP(hipEventRecord(h->ev[8], s));
   k_m2l_tiles<P, MODE, TR><<<8 * a.per_xcd, 256, 0, s>>>(a);
   GX_HIP(hipGetLastError());
+  if (!TR) GX_HIP(hipEventRecord(h->ev[9], s));
   if (T.nmulti) {
     k_m2l_tiles_combine<<<(unsigned)T.nmulti, 256, 0, s>>>(T.multi_tile.as<uint32_t>(), T.multi_slot.as<uint32_t>(),
                                                            T.multi_cnt.as<uint32_t>(), S, (uint32_t)T.n_rows,

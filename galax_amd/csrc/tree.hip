@@ -413,6 +413,7 @@ int build_tree_plain(galax_fmm* h, size_t n) {
                                    va);
   GX_HIP(hipGetLastError());
   GX_CHECK(radix_sort(h, ka, va, kb, vb, n));
+  GX_CHECK(build_mark(h, 1));
   // permutation: new[i] = old[va[i]]; kept for galax_fmm_build_tree's perm_out
   GX_CHECK(dbuf_reserve(h->perm, n * sizeof(uint32_t), s));
   GX_HIP(hipMemcpyAsync(h->perm.p, va, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
@@ -425,6 +426,7 @@ int build_tree_plain(galax_fmm* h, size_t n) {
     GX_HIP(hipGetLastError());
     GX_HIP(hipMemcpyAsync(b->p, tmp, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
+  GX_CHECK(build_mark(h, 2));
   // Re-encoding the permuted bodies reproduces the sorted keys exactly (the
   // encoding is a pure function of the body), so ka already holds them.
   size_t n_leaves = (n + c.n_max - 1) / c.n_max;
@@ -933,6 +935,7 @@ int build_lists_reference(galax_fmm* h) {
   k_rows_compact<<<nblk(t.n_leaves * LIST_CAP), 256, 0, s>>>(rows, t.p2p_off.as<uint32_t>(), (uint32_t)t.n_leaves,
                                                              t.p2p_dat.as<uint32_t>());
   GX_HIP(hipGetLastError());
+  GX_CHECK(build_mark(h, 4));
   GX_CHECK(build_gather(h));
   h->keys_valid = false;
   return 0;

@@ -173,4 +173,9 @@ int scan_u32(galax_fmm* h, const uint32_t* d_in, uint32_t* d_out, size_t n, bool
   return 0;
 }
 
+int build_mark(galax_fmm* h, int i) {
+  if (h->build_timed) GX_HIP(hipEventRecord(h->bev[i], h->stream));
+  return 0;
+}
+
 }  // namespace gx

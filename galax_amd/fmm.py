@@ -72,7 +72,12 @@ class _Timings(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("sanitize", "p2m", "m2m", "m2l_setup", "m2l", "l2l", "l2p_p2p",
                                           "total")] + \
                [(k, C.c_uint64) for k in ("m2l_pairs_executed", "m2l_pairs_reference", "m2l_keys",
-                                          "p2p_interactions")]
+                                          "p2p_interactions", "m2l_groups", "m2l_steps", "m2l_items")] + \
+               [(k, C.c_double) for k in ("m2l_kernel", "p2p_kernel")]
+
+
+class _BuildTimings(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("sort", "permute", "tree", "lists", "gather", "total")]
 
 
 _lib = None
@@ -98,6 +103,7 @@ def lib():
         "galax_fmm_evaluate": [vp, C.POINTER(_Bodies)],
         "galax_fmm_evaluate_device": [vp] * 7,
         "galax_fmm_last_timings": [vp, C.POINTER(_Timings)],
+        "galax_fmm_last_build_timings": [vp, C.POINTER(_BuildTimings)],
         "galax_fmm_kdk": [vp, C.POINTER(_Bodies), d, C.c_uint64, C.c_uint64, vp],
         "galax_fmm_vjp": [vp, C.POINTER(_Bodies), vp, vp, vp, vp, vp],
         "galax_fmm_vjp_device": [vp] * 9,
@@ -195,6 +201,9 @@ class Timings:
     m2l_pairs_reference: int = 0
     m2l_keys: int = 0
     p2p_interactions: int = 0
+    m2l_groups: int = 0
+    m2l_steps: int = 0
+    m2l_items: int = 0
 
 
 class GpuContext:
@@ -318,9 +327,16 @@ class GpuContext:
     def timings(self) -> Timings:
         t = _Timings()
         _check(lib().galax_fmm_last_timings(self.h, C.byref(t)))
-        names = ("sanitize", "p2m", "m2m", "m2l_setup", "m2l", "l2l", "l2p_p2p", "total")
+        names = ("sanitize", "p2m", "m2m", "m2l_setup", "m2l", "l2l", "l2p_p2p", "total", "m2l_kernel", "p2p_kernel")
         return Timings({k: getattr(t, k) for k in names}, t.m2l_pairs_executed, t.m2l_pairs_reference, t.m2l_keys,
-                       t.p2p_interactions)
+                       t.p2p_interactions, t.m2l_groups, t.m2l_steps, t.m2l_items)
+
+    def build_timings(self) -> dict:
+        """Device milliseconds of the last tree build per stage (sort, permute,
+        tree, lists, gather, total)."""
+        t = _BuildTimings()
+        _check(lib().galax_fmm_last_build_timings(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in _BuildTimings._fields_}
 
     def simulate(self, b: BodiesSoA, dt: float, n_steps: int, energy_every: int = 0):
         """simulate / leapfrog_kdk (galax-integrate/src/lib.rs:158-237) on the

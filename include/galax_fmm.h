@@ -124,7 +124,20 @@ typedef struct galax_fmm galax_fmm_t;
 typedef struct {
   double sanitize, p2m, m2m, m2l_setup, m2l, l2l, l2p_p2p, total;
   uint64_t m2l_pairs_executed, m2l_pairs_reference, m2l_keys, p2p_interactions;
+  /* matrix-core M2L work units: (target, key) groups of at most 8 pairs,
+   * 16-column products (steps) and work items; 0 for FP64_EXACT / CORRECTED */
+  uint64_t m2l_groups, m2l_steps, m2l_items;
+  /* the main kernel of the M2L and of the L2P+P2P pass alone (no combine) */
+  double m2l_kernel, p2p_kernel;
 } galax_timings_t;
+
+/* Device times of the last tree build, milliseconds (HIP events on the
+ * handle's stream): Morton keys + radix sort, permutation of the body arrays,
+ * leaves + parent rounds, the reference M2L/P2P lists, and the gather lists +
+ * P2P work items. sort and permute are 0 for GALAX_TREE_CONSTRAINED. */
+typedef struct {
+  double sort, permute, tree, lists, gather, total;
+} galax_build_timings_t;
 
 /* ~GpuContext::new (galax-gpu/src/lib.rs:268-390): validates the config,
  * selects the device current on the calling thread, creates the stream and
@@ -176,8 +189,13 @@ GALAX_API int galax_fmm_evaluate(galax_fmm_t* h, galax_bodies_t* bodies);
 GALAX_API int galax_fmm_evaluate_device(galax_fmm_t* h, double* d_x, double* d_y, double* d_m, double* d_ax, double* d_ay,
                               double* d_phi);
 
-/* Device times and work counts of the last evaluate. */
+/* Device times and work counts of the last evaluate. No reference counterpart
+ * (galax-bench times whole calls on the host, galax-bench/src/main.rs:7-17). */
 GALAX_API int galax_fmm_last_timings(galax_fmm_t* h, galax_timings_t* out);
+
+/* Device times of the last galax_fmm_build_tree[_device]. No reference
+ * counterpart (as above). */
+GALAX_API int galax_fmm_last_build_timings(galax_fmm_t* h, galax_build_timings_t* out);
 
 /* ~galax-integrate simulate / leapfrog_kdk_gpu (galax-integrate/src/lib.rs:158-237,
  * 292-368): n_steps kick-drift-kick steps on the device (positions and
