@@ -52,9 +52,17 @@ def main():
                          "traffic that scales hardware byte counters)")
     ap.add_argument("--form", default="mixed_fp32", choices=["fp64", "fp64_exact", "mixed_fp32", "mixed_bf16x3"],
                     help="the form whose throughput is `value`")
+    ap.add_argument("--config", default=None, choices=["c1", "c2", "c3", "c4", "c5"],
+                    help="instead of the headline line, measure one of the workload configurations "
+                         "(uniform10k, plummer1m p=4 + 100 KDK steps, disk4m p-sweep, galaxy262k VJP, "
+                         "clustered16m) and print its JSON line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         sys.exit("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)")
+    if args.config:
+        import bench_configs
+        print(json.dumps(bench_configs.run(args.config, args)), flush=True)
+        return
 
     import torch
 

@@ -1,0 +1,394 @@
+"""Measurements of the workload configurations beside the headline one
+(`python bench.py --config c1 ... c5`). Each returns one JSON-able dict.
+
+  c1  uniform10k: p=4, n_max=32, FP64; forces vs the exact N^2 sum and the
+      --validate metric next to the oracle's; 10 KDK steps with
+      dt = 0.5/sqrt(max|a|); the CLI bounds [-50,50]^2 as a stress case.
+  c2  plummer1m at p=4 (p=6 is the headline): forms, errors vs an FP64
+      direct sum on 10,000 samples; 100 KDK steps with energy and momentum
+      drift: the reference algorithm at galax-cli's dt on the static tree,
+      the corrected variant (theta = 0.3) at dt = 0.01 on the static tree and
+      with the tree rebuilt every 10 steps.
+  c3  disk4m: p = 2, 4, 6, 8 x {FP64, MIXED_FP32, MIXED_BF16X3}: RMS / max
+      relative force error vs the FP64 direct sum on 10,000 samples, forces/s;
+      and the corrected variant at p = 4, 6, 8 (FP64, MIXED_FP32).
+  c4  galaxy262k, both algorithms: VJP with the FP32 and FP64 forwards;
+      central differences (h = 1e-6, FP64) on 64 random coordinates; a
+      50-step leapfrog (dt = 1e-4) with an MSE loss, checkpointed every 10
+      steps.
+  c5  clustered16m: n_max=64, p=8: one evaluation per form and one VJP; stage
+      rooflines; the tree against the oracle's CPU build, bit for bit.
+"""
+from __future__ import annotations
+
+import math
+import sys
+import time
+
+import numpy as np
+
+import bench as B
+
+
+def _log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _errors(ax, ay, ex, ey, si):
+    emag = np.maximum(np.hypot(ex, ey), 1e-300)
+    vec = np.hypot(ax[si] - ex, ay[si] - ey) / emag
+    mag = np.abs(np.hypot(ax[si], ay[si]) - np.hypot(ex, ey)) / np.maximum(np.hypot(ex, ey), 1e-20)
+    return {"rms_vec": float(np.sqrt(np.mean(vec ** 2))), "max_vec": float(vec.max()),
+            "validate_max_mag": float(mag.max()), "validate_mean_mag": float(mag.mean())}
+
+
+def _samples(n, k=10_000, seed=4242):
+    from galax_amd import ic
+    r = ic.SplitMix64(seed)
+    return np.unique((r.next_u64(k) % np.uint64(n)).astype(np.uint64))
+
+
+def _eval_form(fmm, torch, d, spec, p, prec, eps=None, steps=5, warmup=2, algo=0, theta=0.3):
+    """Builds a context for one form, evaluates on the device; returns a dict
+    with times, passes, the bodies in tree order and the accelerations."""
+    n = len(d["x"])
+    dev = torch.device("cuda:0")
+    ctx = fmm.GpuContext(fmm.GpuConfig(max_n=n, p=p, eps=0.1, n_max=spec["n_max"], bounds=spec["bounds"],
+                                       precision=prec, algo=algo, theta=theta))
+    b = fmm.BodiesSoA.from_dict(d)
+    t0 = time.perf_counter()
+    ctx.build_tree(b)
+    t_build = time.perf_counter() - t0
+    if eps is None:
+        eps = fmm.default_softening(b.x, b.y)
+    ctx.set_softening(eps)
+    x, y, m = (torch.from_numpy(a).to(dev) for a in (b.x, b.y, b.mass))
+    ax, ay, phi = torch.empty_like(x), torch.empty_like(x), torch.empty_like(x)
+    torch.cuda.synchronize()  # the library runs on its own stream
+    for _ in range(warmup + 1):
+        ctx.evaluate_device(x, y, m, ax, ay, phi)
+    torch.cuda.synchronize()
+    passes = {}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.evaluate_device(x, y, m, ax, ay, phi)
+        for k, v in ctx.timings().passes_ms.items():
+            passes[k] = passes.get(k, 0.0) + v / steps
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    return dict(ctx=ctx, b=b, eps=eps, ms=ms, passes=passes, t_build=t_build, tm=ctx.timings(),
+                build_ms=ctx.build_timings(), info=ctx.tree_info(), x=x, y=y, m=m,
+                ax=ax.cpu().numpy(), ay=ay.cpu().numpy(), phi=phi.cpu().numpy())
+
+
+def c1(args, fmm, torch, ic):
+    from oracle import oracle as O
+    d, spec = ic.make("uniform10k")
+    out = {"workload": "uniform10k", "N": len(d["x"]), "p": 4, "n_max": 32}
+    for bounds_name, bounds in (("data", spec["bounds"]), ("cli", (-50.0, 50.0, -50.0, 50.0))):
+        sp = dict(spec, bounds=bounds)
+        r = _eval_form(fmm, torch, d, sp, 4, fmm.PREC_FP64)
+        b, n = r["b"], len(r["b"])
+        si = np.arange(n, dtype=np.uint64)
+        ex, ey, _ = fmm.compute_n2_force(b.x, b.y, b.mass, r["eps"])
+        e = _errors(r["ax"], r["ay"], ex, ey, si.astype(np.int64))
+        # the oracle on the same sorted bodies: its --validate metric beside ours
+        od = {k: np.array(d[k], np.float64) for k in ("x", "y", "vx", "vy", "m", "ax", "ay")}
+        ot = O.tree_build(od, 32, bounds)
+        oax, oay, _ = O.compute_fmm_force(od["x"], od["y"], od["m"], ot, O.m2l_lists(ot), O.p2p_lists(ot), 4,
+                                          r["eps"], threads=1)  # sequential cache first-touch, as on the device
+        eo = _errors(oax, oay, ex, ey, si.astype(np.int64))
+        rd = np.hypot(r["ax"] - oax, r["ay"] - oay) / np.maximum(np.hypot(oax, oay), 1e-300)
+        res = {"softening": r["eps"], "ms": r["ms"], "bodies_per_s": n / (r["ms"] * 1e-3), "errors": e,
+               "oracle_errors": eo, "median_rel_vs_oracle": float(np.median(rd)),
+               "validate_within_5pct": bool(abs(e["validate_max_mag"] - eo["validate_max_mag"])
+                                            <= 0.05 * eo["validate_max_mag"]
+                                            and abs(e["validate_mean_mag"] - eo["validate_mean_mag"])
+                                            <= 0.05 * eo["validate_mean_mag"])}
+        if bounds_name == "data":
+            # 10 KDK steps, dt from the initial accelerations (galax-cli/src/main.rs:232-244)
+            b.ax[:], b.ay[:] = r["ax"], r["ay"]
+            dt = fmm.default_dt(b.ax, b.ay)
+            diags = r["ctx"].simulate(b, dt, 10, energy_every=1)
+            e0 = diags[0]["kinetic"] + diags[0]["potential"]
+            e1 = diags[-1]["kinetic"] + diags[-1]["potential"]
+            res["kdk"] = {"dt": dt, "steps": 10, "energy_first": e0, "energy_last": e1,
+                          "rel_energy_drift": abs(e1 - e0) / max(abs(e0), 1e-300),
+                          "momentum_last": [diags[-1]["momentum_x"], diags[-1]["momentum_y"]]}
+        out["bounds_" + bounds_name] = res
+        del r
+    return out
+
+
+def c2(args, fmm, torch, ic):
+    d, spec = ic.make("plummer1m")
+    n = len(d["x"])
+    out = {"workload": "plummer1m", "N": n, "p": 4, "n_max": 32, "forms": {}}
+    res = {}
+    for name, prec in (("fp64", fmm.PREC_FP64), ("mixed_fp32", fmm.PREC_MIXED_FP32),
+                       ("mixed_bf16x3", fmm.PREC_MIXED_BF16X3)):
+        r = _eval_form(fmm, torch, d, spec, 4, prec, steps=10, warmup=3)
+        _log(f"[c2] {name}: {r['ms']:.3f} ms")
+        res[name] = r
+    b = res["fp64"]["b"]
+    si = _samples(n)
+    ex, ey, _ = fmm.compute_n2_force(b.x, b.y, b.mass, res["fp64"]["eps"], targets=si)
+    ref = res["fp64"]
+    for name, r in res.items():
+        f = {"ms_per_step": r["ms"], "bodies_per_s": n / (r["ms"] * 1e-3), "passes_ms": r["passes"],
+             "errors_vs_direct": _errors(r["ax"], r["ay"], ex, ey, si.astype(np.int64))}
+        if name != "fp64":
+            d64 = np.hypot(r["ax"] - ref["ax"], r["ay"] - ref["ay"]) / np.maximum(np.hypot(ref["ax"], ref["ay"]), 1e-300)
+            f["rms_vs_fp64"] = float(np.sqrt(np.mean(d64 ** 2)))
+            f["speedup_vs_fp64"] = ref["ms"] / r["ms"]
+        out["forms"][name] = f
+    # 100 KDK steps on the device, diagnostics every 10 (energy from the potentials).
+    # The reference algorithm's forces carry its own O(1) errors (max ~80x), so at
+    # galax-cli's dt = 0.5/sqrt(max|a|) its orbits diverge; the corrected
+    # variant shows the integrator's drift.
+    def kdk(r, dt, tag):
+        bb = fmm.BodiesSoA(r["b"].x.copy(), r["b"].y.copy(), r["b"].vx.copy(), r["b"].vy.copy(), r["b"].mass.copy(),
+                           r["ax"].copy(), r["ay"].copy(), None)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        diags = r["ctx"].simulate(bb, dt, 100, energy_every=10)
+        t = time.perf_counter() - t0
+        e = [dg["kinetic"] + dg["potential"] for dg in diags]
+        px = [dg["momentum_x"] for dg in diags]
+        py = [dg["momentum_y"] for dg in diags]
+        lz = [dg["angular_momentum"] for dg in diags]
+        rec = {"dt": dt, "seconds": t, "ms_per_step": 1e3 * t / 100, "energy": e,
+               "rel_energy_drift": abs(e[-1] - e[0]) / max(abs(e[0]), 1e-300) if math.isfinite(e[-1]) else None,
+               "momentum": [px, py], "angular_momentum": lz, "finite": bool(np.isfinite(bb.x).all())}
+        _log(f"[c2] {tag}: 100 KDK steps dt={dt:.3g} {t:.2f} s, energy drift {rec['rel_energy_drift']}")
+        return rec
+    for name in ("fp64", "mixed_fp32"):
+        r = res[name]
+        out["forms"][name]["kdk100_cli_dt"] = kdk(r, fmm.default_dt(r["ax"], r["ay"]), name + " reference")
+    def kdk_rebuild(r, dt, every, tag):
+        # the tree (and the corrected lists) rebuilt every `every` steps
+        ctx = r["ctx"]
+        bb = fmm.BodiesSoA(r["b"].x.copy(), r["b"].y.copy(), r["b"].vx.copy(), r["b"].vy.copy(), r["b"].mass.copy(),
+                           r["ax"].copy(), r["ay"].copy(), None)
+        e, px, py = [], [], []
+        t0 = time.perf_counter()
+        for blk in range(100 // every):
+            if blk:
+                ctx.build_tree(bb)
+                ctx.evaluate(bb)
+            dg = ctx.simulate(bb, dt, every, energy_every=every)[-1]
+            e.append(dg["kinetic"] + dg["potential"])
+            px.append(dg["momentum_x"])
+            py.append(dg["momentum_y"])
+        t = time.perf_counter() - t0
+        rec = {"dt": dt, "rebuild_every": every, "seconds_with_rebuilds": t, "energy": e,
+               "rel_energy_drift": abs(e[-1] - e[0]) / max(abs(e[0]), 1e-300) if math.isfinite(e[-1]) else None,
+               "momentum": [px, py]}
+        _log(f"[c2] {tag}: 100 KDK steps dt={dt:.3g} rebuilt every {every}: {t:.2f} s, energy drift {rec['rel_energy_drift']}")
+        return rec
+    out["corrected"] = {}
+    for name, prec in (("fp64", fmm.PREC_FP64), ("mixed_fp32", fmm.PREC_MIXED_FP32)):
+        r = _eval_form(fmm, torch, d, spec, 4, prec, steps=5, warmup=2, algo=fmm.ALGO_CORRECTED, theta=0.3)
+        f = {"ms_per_step": r["ms"], "bodies_per_s": n / (r["ms"] * 1e-3), "passes_ms": r["passes"],
+             "errors_vs_direct": _errors(r["ax"], r["ay"], ex, ey, si.astype(np.int64)),
+             "kdk100_static_dt_0.01": kdk(r, 0.01, name + " corrected, static tree"),
+             "kdk100_rebuild10_dt_0.01": kdk_rebuild(r, 0.01, 10, name + " corrected")}
+        out["corrected"][name] = f
+        del r
+    return out
+
+
+def c3(args, fmm, torch, ic):
+    d, spec = ic.make("disk4m")
+    n = len(d["x"])
+    out = {"workload": "disk4m", "N": n, "n_max": 32, "sweep": []}
+    si = _samples(n)
+    ex = ey = None
+    for p in (2, 4, 6, 8):
+        ref = None
+        for name, prec in (("fp64", fmm.PREC_FP64), ("mixed_fp32", fmm.PREC_MIXED_FP32),
+                           ("mixed_bf16x3", fmm.PREC_MIXED_BF16X3)):
+            r = _eval_form(fmm, torch, d, spec, p, prec, steps=5, warmup=2)
+            if ex is None:
+                b = r["b"]
+                ex, ey, _ = fmm.compute_n2_force(b.x, b.y, b.mass, r["eps"], targets=si)
+            row = {"p": p, "form": name, "ms_per_step": r["ms"], "bodies_per_s": n / (r["ms"] * 1e-3),
+                   "errors_vs_direct": _errors(r["ax"], r["ay"], ex, ey, si.astype(np.int64)),
+                   "passes_ms": r["passes"]}
+            if ref is None:
+                ref = r
+            else:
+                d64 = np.hypot(r["ax"] - ref["ax"], r["ay"] - ref["ay"]) / np.maximum(np.hypot(ref["ax"], ref["ay"]), 1e-300)
+                row["rms_vs_fp64"] = float(np.sqrt(np.mean(d64 ** 2)))
+                row["max_vs_fp64"] = float(d64.max())
+                row["speedup_vs_fp64"] = ref["ms"] / r["ms"]
+            _log(f"[c3] p={p} {name}: {r['ms']:.3f} ms, rms vs direct {row['errors_vs_direct']['rms_vec']:.3e}")
+            out["sweep"].append(row)
+            if r is not ref:
+                del r
+        del ref
+    # the corrected variant (theta = 0.3): the FMM's own truncation error per p
+    out["corrected_sweep"] = []
+    for p in (4, 6, 8):
+        ref = None
+        for name, prec in (("fp64", fmm.PREC_FP64), ("mixed_fp32", fmm.PREC_MIXED_FP32)):
+            r = _eval_form(fmm, torch, d, spec, p, prec, steps=3, warmup=1, algo=fmm.ALGO_CORRECTED, theta=0.3)
+            row = {"p": p, "form": name, "theta": 0.3, "ms_per_step": r["ms"], "bodies_per_s": n / (r["ms"] * 1e-3),
+                   "errors_vs_direct": _errors(r["ax"], r["ay"], ex, ey, si.astype(np.int64)), "passes_ms": r["passes"]}
+            if ref is None:
+                ref = r
+            else:
+                d64 = np.hypot(r["ax"] - ref["ax"], r["ay"] - ref["ay"]) / np.maximum(np.hypot(ref["ax"], ref["ay"]), 1e-300)
+                row["rms_vs_fp64"] = float(np.sqrt(np.mean(d64 ** 2)))
+                row["speedup_vs_fp64"] = ref["ms"] / r["ms"]
+            _log(f"[c3] corrected p={p} {name}: {r['ms']:.3f} ms, rms vs direct {row['errors_vs_direct']['rms_vec']:.3e}")
+            out["corrected_sweep"].append(row)
+            if r is not ref:
+                del r
+        del ref
+    return out
+
+
+def c4(args, fmm, torch, ic):
+    from galax_amd import autograd as GA
+    d, spec = ic.make("galaxy262k")
+    n = len(d["x"])
+    dev = torch.device("cuda:0")
+    out = {"workload": "galaxy262k", "N": n, "p": 6, "n_max": 32}
+    runs = [(a + "_" + name, prec, algo) for a, algo in (("reference", fmm.ALGO_REFERENCE), ("corrected", fmm.ALGO_CORRECTED))
+            for name, prec in (("mixed_fp32", fmm.PREC_MIXED_FP32), ("fp64", fmm.PREC_FP64))]
+    for name, prec, algo in runs:
+        r = _eval_form(fmm, torch, d, spec, 6, prec, steps=10, warmup=3, algo=algo, theta=0.3)
+        ctx, b = r["ctx"], r["b"]
+        g = ic.SplitMix64(11)
+        axb = torch.tensor(g.normal(n), device=dev)
+        ayb = torch.tensor(g.normal(n), device=dev)
+        xb, yb, mb = (torch.empty_like(r["x"]) for _ in range(3))
+        torch.cuda.synchronize()
+        ctx.vjp_device(r["x"], r["y"], r["m"], axb, ayb, xb, yb, mb)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            ctx.vjp_device(r["x"], r["y"], r["m"], axb, ayb, xb, yb, mb)
+        torch.cuda.synchronize()
+        vjp_ms = 1e3 * (time.perf_counter() - t0) / 5
+        res = {"forward_ms": r["ms"], "vjp_ms": vjp_ms, "vjp_over_fwd": vjp_ms / r["ms"]}
+        if name.endswith("fp64"):
+            # central differences of sum(a . abar) in FP64 on 64 random coordinates (frozen tree)
+            gx, gy, gm = xb.cpu().numpy(), yb.cpu().numpy(), mb.cpu().numpy()
+            abx, aby = axb.cpu().numpy(), ayb.cpu().numpy()
+            rr = ic.SplitMix64(12)
+            idx = (rr.next_u64(64) % np.uint64(n)).astype(np.int64)
+            which = (rr.next_u64(64) % np.uint64(3)).astype(np.int64)
+            h = 1e-6
+            worst = 0.0
+            xs, ys, ms_ = r["x"].clone(), r["y"].clone(), r["m"].clone()
+            fa, fb = torch.empty_like(xs), torch.empty_like(xs)
+            for k, wv in zip(idx, which):
+                arr = (xs, ys, ms_)[wv]
+                v0 = float(arr[k])
+                vals = []
+                for sgn in (1, -1):
+                    arr[k] = v0 + sgn * h
+                    torch.cuda.synchronize()
+                    ctx.evaluate_device(xs, ys, ms_, fa, fb, None)
+                    torch.cuda.synchronize()
+                    vals.append(float((fa * axb).sum() + (fb * ayb).sum()))
+                arr[k] = v0
+                fd = (vals[0] - vals[1]) / (2 * h)
+                gv = (gx, gy, gm)[wv][k]
+                scale = max(abs(fd), abs(gv), 1e-3 * max(np.abs(gx).max(), np.abs(gy).max()))
+                worst = max(worst, abs(fd - gv) / scale)
+            res["fd_worst_rel"] = worst
+            res["fd_coordinates"] = 64
+        # 50-step checkpointed leapfrog, MSE loss to a perturbed target
+        x0 = r["x"].clone().requires_grad_(True)
+        vx = torch.tensor(b.vx, device=dev)
+        vy = torch.tensor(b.vy, device=dev)
+        tg = ic.SplitMix64(13)
+        tx = r["x"] + 0.01 * torch.tensor(tg.normal(n), device=dev)
+        ty = r["y"] + 0.01 * torch.tensor(tg.normal(n), device=dev)
+        # dt small enough that bodies stay near their (static) leaves over 50 steps
+        dt = 1e-4
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xf, yf, _, _ = GA.leapfrog(x0, r["y"], vx, vy, r["m"], ctx, dt, 50, checkpoint_every=10)
+        loss = ((xf - tx) ** 2 + (yf - ty) ** 2).mean()
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        (gx0,) = torch.autograd.grad(loss, (x0,))
+        torch.cuda.synchronize()
+        t_all = time.perf_counter() - t0
+        res["leapfrog50"] = {"dt": dt, "loss": float(loss.detach()), "forward_s": t_fwd, "forward_backward_s": t_all,
+                             "grad_norm": float(gx0.norm())}
+        res["max_accel"] = float(np.hypot(r["ax"], r["ay"]).max())
+        _log(f"[c4] {name}: fwd {r['ms']:.3f} ms vjp {vjp_ms:.3f} ms leapfrog50 {t_all:.2f} s")
+        out[name] = res
+        del r, ctx
+    return out
+
+
+def c5(args, fmm, torch, ic):
+    from oracle import oracle as O
+    d, spec = ic.make("clustered16m")
+    n = len(d["x"])
+    out = {"workload": "clustered16m", "N": n, "p": 8, "n_max": 64, "bounds": list(spec["bounds"]), "forms": {}}
+    dev = torch.device("cuda:0")
+    for name, prec in (("fp64", fmm.PREC_FP64), ("mixed_fp32", fmm.PREC_MIXED_FP32)):
+        r = _eval_form(fmm, torch, d, spec, 8, prec, steps=3, warmup=1)
+        ctx = r["ctx"]
+        g = ic.SplitMix64(21)
+        axb = torch.tensor(g.normal(n), device=dev)
+        ayb = torch.tensor(g.normal(n), device=dev)
+        xb, yb, mb = (torch.empty_like(r["x"]) for _ in range(3))
+        torch.cuda.synchronize()
+        ctx.vjp_device(r["x"], r["y"], r["m"], axb, ayb, xb, yb, mb)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.vjp_device(r["x"], r["y"], r["m"], axb, ayb, xb, yb, mb)
+        torch.cuda.synchronize()
+        vjp_ms = 1e3 * (time.perf_counter() - t0)
+        hr = {"ms_per_step": r["ms"], "bodies_per_s": n / (r["ms"] * 1e-3), "passes_ms": r["passes"],
+              "build_ms": r["build_ms"], "t_build_s": r["t_build"], "vjp_ms": vjp_ms,
+              "vjp_over_fwd": vjp_ms / r["ms"], "stages": B.stage_rooflines(r, n, 8, name),
+              "tree": r["info"], "p2p_interactions": r["tm"].p2p_interactions,
+              "m2l_pairs_executed": r["tm"].m2l_pairs_executed}
+        if name == "fp64":
+            # the tree against the oracle's CPU build of the same bodies, bit for bit
+            gt, _, _ = ctx.export_tree()
+            od = {k: np.array(d[k], np.float64) for k in ("x", "y", "vx", "vy", "m", "ax", "ay")}
+            t0 = time.perf_counter()
+            ot = O.tree_build(od, 64, spec["bounds"])
+            hr["oracle_tree_build_s"] = time.perf_counter() - t0
+            oe = ot.export()
+            same = True
+            for k in oe:
+                a_, b_ = np.asarray(gt[k]), np.asarray(oe[k])
+                if a_.shape != b_.shape:
+                    same = False
+                elif a_.dtype == np.float64:
+                    same = same and np.array_equal(a_.view(np.uint64), b_.view(np.uint64))
+                else:
+                    same = same and np.array_equal(a_.astype(np.int64), b_.astype(np.int64))
+            same = same and np.array_equal(r["b"].x.view(np.uint64), od["x"].view(np.uint64))
+            same = same and np.array_equal(r["b"].mass.view(np.uint64), od["m"].view(np.uint64))
+            hr["tree_bit_exact_vs_oracle"] = bool(same)
+            hr["n_nodes_oracle"] = len(oe["mass"])
+        _log(f"[c5] {name}: {r['ms']:.3f} ms, vjp {vjp_ms:.3f} ms")
+        out["forms"][name] = hr
+        del r, ctx
+    f64, mx = out["forms"]["fp64"], out["forms"]["mixed_fp32"]
+    mx["speedup_vs_fp64"] = f64["ms_per_step"] / mx["ms_per_step"]
+    return out
+
+
+def run(config, args):
+    import torch
+
+    from galax_amd import fmm, ic
+    fn = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c5": c5}[config]
+    t0 = time.perf_counter()
+    out = fn(args, fmm, torch, ic)
+    out["config"] = config
+    out["seconds"] = time.perf_counter() - t0
+    return out

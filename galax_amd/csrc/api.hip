@@ -326,10 +326,10 @@ void galax_fmm_destroy(galax_fmm_t* h) {
                   &h->key_dxy, &h->mx_t64, &h->mx_that, &h->mx_sig,
                   &h->vj_lbar, &h->vj_mbar, &h->vj_leaf_of, &h->vj_fwd, &h->vj_out,
                   &h->mx_fwd.src, &h->mx_fwd.tile_off, &h->mx_fwd.step_key, &h->mx_fwd.step_first,
-                  &h->mx_fwd.step_cnt, &h->mx_fwd.item_tile, &h->mx_fwd.item_s0, &h->mx_fwd.item_s1,
+                  &h->mx_fwd.step_cnt, &h->mx_fwd.step_ids, &h->mx_fwd.item_tile, &h->mx_fwd.item_s0, &h->mx_fwd.item_s1,
                   &h->mx_fwd.item_slot, &h->mx_fwd.multi_tile, &h->mx_fwd.multi_slot, &h->mx_fwd.multi_cnt,
                   &h->mx_fwd.part, &h->mx_adj.src, &h->mx_adj.tile_off, &h->mx_adj.step_key,
-                  &h->mx_adj.step_first, &h->mx_adj.step_cnt, &h->mx_adj.item_tile, &h->mx_adj.item_s0,
+                  &h->mx_adj.step_first, &h->mx_adj.step_cnt, &h->mx_adj.step_ids, &h->mx_adj.item_tile, &h->mx_adj.item_s0,
                   &h->mx_adj.item_s1, &h->mx_adj.item_slot, &h->mx_adj.multi_tile, &h->mx_adj.multi_slot,
                   &h->mx_adj.multi_cnt, &h->mx_adj.part};
   for (DBuf* b : bufs) dbuf_free(*b);

@@ -65,11 +65,12 @@ struct P2PItems {
 // M2L work in tiles of 16 rows (targets, or sources for the adjoint). The live
 // pairs are grouped by (row, key); a tile's steps are the distinct keys of its
 // groups, step_first/step_cnt give each of its 16 columns' group as a range
-// of `src` (the other end of each pair, in list order). Work items cover at
+// of `src` (the other end of each pair, in list order), and step_ids the same
+// source ids laid out [step][q][column] for the products. Work items cover at
 // most MX_ITEM_STEPS steps of one tile; tiles with several items write
 // partial results to slots that a combine pass sums in item order.
 struct M2LTiles {
-  DBuf src, tile_off, step_key, step_first, step_cnt;
+  DBuf src, tile_off, step_key, step_first, step_cnt, step_ids;
   DBuf item_tile, item_s0, item_s1, item_slot;  // per work item
   DBuf multi_tile, multi_slot, multi_cnt;       // per tile with several items
   DBuf part;                                    // partial results, 16 x S doubles per slot

@@ -90,7 +90,7 @@ struct Lanes {
 // ------------------------------------------------------------- setup ----
 
 constexpr uint32_t MX_ITEM_STEPS = 16;  // steps per work item
-constexpr uint32_t MX_GROUP = 8;        // pairs per group (one column of one step)
+constexpr uint32_t MX_GROUP = 4;        // pairs per group (one column of one step)
 
 // row of each live pair: its target (CSR row) or its source
 __global__ void k_mx_pair_rows(const uint32_t* __restrict__ off, uint32_t nt, uint32_t* __restrict__ tgt) {
@@ -170,6 +170,16 @@ __global__ void k_mx_steps(const unsigned long long* __restrict__ g2k, const uin
 
 __global__ void k_mx_set_u32(uint32_t* __restrict__ p, uint32_t v) { *p = v; }
 
+// source ids of every step, [step][q][column], NONE past a group's end
+__global__ void k_mx_step_ids(const uint32_t* __restrict__ step_first, const uint32_t* __restrict__ step_cnt,
+                              const uint32_t* __restrict__ src, size_t ns, uint32_t* __restrict__ ids) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns * 16) return;
+  const size_t st = i / 16, col = i % 16;
+  const uint32_t f = step_first[i], c = step_cnt[i];
+  for (uint32_t q = 0; q < MX_GROUP; ++q) ids[(st * MX_GROUP + q) * 16 + col] = q < c ? src[f + q] : NONE;
+}
+
 // FP64 operator of each key: (16 RB)^2 row-major, zero padded
 template <int P>
 __global__ void k_mx_operators64(const double* __restrict__ gtab, uint32_t nk, double* __restrict__ t64) {
@@ -244,7 +254,7 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
 struct TileArgs {
   const uint32_t *item_tile, *item_s0, *item_s1, *item_slot;
   uint32_t nitems, per_xcd, n_rows;
-  const uint32_t *step_key, *step_first, *step_cnt, *src;
+  const uint32_t *step_key, *step_ids;
   const double* t64;
   const float* that;
   const double* sig_inv;
@@ -275,30 +285,40 @@ __global__ void __launch_bounds__(256) k_m2l_tiles(TileArgs a) {
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb) L[rb] = (d4){0.0, 0.0, 0.0, 0.0};
   uint32_t st = s0 + w;
-  uint32_t key_n = 0, f_n = 0, c_n = 0;
+  // the step record (key, and the source ids of this lane's column) is
+  // loaded one step ahead, so each step waits only on its multipole rows
+  uint32_t key_n = 0, id_n[MX_GROUP];
+#pragma unroll
+  for (uint32_t q = 0; q < MX_GROUP; ++q) id_n[q] = NONE;
   if (st < s1) {
     key_n = a.step_key[st];
-    f_n = a.step_first[(size_t)st * 16 + col];
-    c_n = a.step_cnt[(size_t)st * 16 + col];
+#pragma unroll
+    for (uint32_t q = 0; q < MX_GROUP; ++q) id_n[q] = a.step_ids[((size_t)st * MX_GROUP + q) * 16 + col];
   }
   for (; st < s1; st += 4) {
-    const uint32_t key = key_n, f = f_n, cnt = c_n;
-    if (st + 4 < s1) {  // next step's record in flight during this one
+    const uint32_t key = key_n;
+    uint32_t id[MX_GROUP];
+#pragma unroll
+    for (uint32_t q = 0; q < MX_GROUP; ++q) id[q] = id_n[q];
+    if (st + 4 < s1) {
       key_n = a.step_key[st + 4];
-      f_n = a.step_first[(size_t)(st + 4) * 16 + col];
-      c_n = a.step_cnt[(size_t)(st + 4) * 16 + col];
+#pragma unroll
+      for (uint32_t q = 0; q < MX_GROUP; ++q) id_n[q] = a.step_ids[((size_t)(st + 4) * MX_GROUP + q) * 16 + col];
     }
     double bs[KL];
 #pragma unroll
     for (int e = 0; e < KL; ++e) bs[e] = 0.0;
-    for (uint32_t q = 0; q < cnt; ++q) {
-      const double* row = a.in + (size_t)a.src[f + q] * S;
 #pragma unroll
-      for (int e = 0; e < KL; ++e) {
-        const int k = LN::k_of(e, kq);
-        if (k < S) {
-          double v = row[k];
-          bs[e] += isfinite(v) ? v : 0.0;
+    for (uint32_t q = 0; q < MX_GROUP; ++q) {  // a group's sources in list order
+      if (id[q] != NONE) {
+        const double* row = a.in + (size_t)id[q] * S;
+#pragma unroll
+        for (int e = 0; e < KL; ++e) {
+          const int k = LN::k_of(e, kq);
+          if (k < S) {
+            const double v = row[k];
+            bs[e] += isfinite(v) ? v : 0.0;
+          }
         }
       }
     }
@@ -509,6 +529,10 @@ static int build_tiles(galax_fmm* h, const uint32_t* row_of_pair, const uint32_t
     k_mx_steps<<<nblk(ng), 256, 0, s>>>(g2k, g2v, sid, ng, gfirst, kb, rkb, T.step_key.as<uint32_t>(),
                                         T.step_first.as<uint32_t>(), T.step_cnt.as<uint32_t>(), tile_off);
     GX_HIP(hipGetLastError());
+    GX_CHECK(dbuf_reserve(T.step_ids, (size_t)ns * MX_GROUP * 16 * 4, s));
+    k_mx_step_ids<<<nblk((size_t)ns * 16), 256, 0, s>>>(T.step_first.as<uint32_t>(), T.step_cnt.as<uint32_t>(),
+                                                        T.src.as<uint32_t>(), ns, T.step_ids.as<uint32_t>());
+    GX_HIP(hipGetLastError());
   }
   uint32_t tot = 0;
   GX_CHECK(scan_u32(h, tile_off, tile_off, ntiles, false, false, &tot));
@@ -624,9 +648,7 @@ static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
   a.per_xcd = (uint32_t)((T.nitems + 7) / 8);
   a.n_rows = (uint32_t)T.n_rows;
   a.step_key = T.step_key.as<uint32_t>();
-  a.step_first = T.step_first.as<uint32_t>();
-  a.step_cnt = T.step_cnt.as<uint32_t>();
-  a.src = T.src.as<uint32_t>();
+  a.step_ids = T.step_ids.as<uint32_t>();
   a.t64 = h->mx_t64.as<double>();
   a.that = h->mx_that.as<float>();
   a.sig_inv = h->mx_sig.as<double>();

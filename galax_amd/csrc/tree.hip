@@ -624,58 +624,99 @@ __device__ __forceinline__ bool touch_child(const NodeArrays& a, uint32_t child,
   return cdx <= hs + 1e-15 && cdy <= hs + 1e-15;
 }
 
+// wave-wide inclusive prefix sum of one u32 per lane
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t u = __shfl_up(v, d, 64);
+    if (lane >= d) v += u;
+  }
+  return v;
+}
+
 // M2L lists, as InteractionLists::build (galax-core/src/lib.rs:972-1016): for
 // node c with parent P, scan the nodes pn of P's level in ascending id, skip
 // pn == P and those whose box does not touch P's (the reference's test, with
 // its `>` form), and append every child q of pn whose box does not touch c,
-// until 64 entries. One thread per parent P fills the rows of all of P's
-// children in one scan of the level; rows are 64 slots wide, like the
-// reference's [u32; 64], and compacted afterwards. The level is scanned
-// directly: parent half-widths double every round, so at the upper levels
-// nearly every box touches every other and pruning by box gains nothing.
-__global__ void k_m2l_rows(NodeArrays a, uint32_t plo, uint32_t phi, uint32_t* __restrict__ rows,
-                           uint32_t* __restrict__ lens) {
-  uint32_t P = plo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (P >= phi) return;
+// until 64 entries. One wave per parent P fills the rows of all of P's
+// children: its lanes test 64 consecutive candidates pn at a time and a
+// wave prefix sum places each lane's appends after those of lower ids, so
+// every row holds exactly the reference's first 64 entries in its order.
+// Rows are 64 slots wide, like the reference's [u32; 64], and compacted
+// afterwards. The level is scanned directly: parent half-widths double every
+// round, so at the upper levels nearly every box touches every other and
+// pruning by box gains nothing.
+__global__ void __launch_bounds__(256) k_m2l_rows(NodeArrays a, uint32_t plo, uint32_t phi,
+                                                  uint32_t* __restrict__ rows, uint32_t* __restrict__ lens) {
+  const uint32_t P = plo + blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (P >= phi) return;  // uniform over the wave
   const uint32_t cf = a.cf[P];
   const uint32_t cc = a.cc[P];
   const double pcx = a.cx[P], pcy = a.cy[P], phw = a.hw[P];
   uint32_t cnt[4] = {0, 0, 0, 0};
-  uint32_t full = 0;
-  for (uint32_t pn = plo; pn < phi && full < cc; ++pn) {
-    if (pn == P) continue;
-    const double dx = fabs(a.cx[pn] - pcx), dy = fabs(a.cy[pn] - pcy);
-    const double h_sum = phw + a.hw[pn];
-    if (dx > h_sum + 1e-15 || dy > h_sum + 1e-15) continue;
-    const uint32_t qf = a.cf[pn], qc = a.cc[pn];
-    for (uint32_t q = qf; q < qf + qc; ++q) {
-      for (uint32_t k = 0; k < cc; ++k) {
-        if (cnt[k] >= LIST_CAP) continue;
-        if (touch_child(a, q, cf + k)) continue;
-        rows[(size_t)(cf + k) * LIST_CAP + cnt[k]] = q;
-        if (++cnt[k] == LIST_CAP) ++full;
+  for (uint32_t base = plo; base < phi; base += 64) {
+    const uint32_t pn = base + (uint32_t)lane;
+    uint32_t qf = 0, qc = 0;
+    if (pn < phi && pn != P) {
+      const double dx = fabs(a.cx[pn] - pcx), dy = fabs(a.cy[pn] - pcy);
+      const double h_sum = phw + a.hw[pn];
+      if (!(dx > h_sum + 1e-15 || dy > h_sum + 1e-15)) {
+        qf = a.cf[pn];
+        qc = a.cc[pn];
       }
     }
+    bool done = true;
+    for (uint32_t k = 0; k < cc; ++k) {
+      uint32_t m = 0, c = 0;  // this lane's appendable children of pn for child k of P
+      for (uint32_t j = 0; j < qc; ++j)
+        if (!touch_child(a, qf + j, cf + k)) {
+          m |= 1u << j;
+          ++c;
+        }
+      const uint32_t incl = wave_incl_scan(c, lane);
+      const uint32_t total = __shfl(incl, 63, 64);
+      uint32_t pos = cnt[k] + incl - c;
+      for (uint32_t j = 0; j < qc; ++j)
+        if ((m >> j) & 1u) {
+          if (pos < LIST_CAP) rows[(size_t)(cf + k) * LIST_CAP + pos] = qf + j;
+          ++pos;
+        }
+      cnt[k] = min(LIST_CAP, cnt[k] + total);
+      done = done && cnt[k] == LIST_CAP;
+    }
+    if (done) break;  // every row is full (uniform: cnt is the same in all lanes)
   }
-  for (uint32_t k = 0; k < cc; ++k) lens[cf + k] = cnt[k];
+  if (lane == 0)
+    for (uint32_t k = 0; k < cc; ++k) lens[cf + k] = cnt[k];
 }
 
 // P2P lists, as build_p2p_lists (galax-core/src/lib.rs:1371-1389): the leaf
 // itself, then every other leaf in ascending id whose box touches it, until 64.
-__global__ void k_p2p_rows(NodeArrays a, uint32_t n_leaves, uint32_t* __restrict__ rows, uint32_t* __restrict__ lens) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_leaves) return;
+// One wave per leaf; 64 candidates per iteration, placed by a prefix sum.
+__global__ void __launch_bounds__(256) k_p2p_rows(NodeArrays a, uint32_t n_leaves, uint32_t* __restrict__ rows,
+                                                  uint32_t* __restrict__ lens) {
+  const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n_leaves) return;  // uniform over the wave
   const double cx = a.cx[i], cy = a.cy[i], hw = a.hw[i];
   uint32_t* row = rows + (size_t)i * LIST_CAP;
-  row[0] = i;
+  if (lane == 0) row[0] = i;
   uint32_t cnt = 1;
-  for (uint32_t b = 0; b < n_leaves && cnt < LIST_CAP; ++b) {
-    if (b == i) continue;
-    const double dx = fabs(cx - a.cx[b]), dy = fabs(cy - a.cy[b]);
-    const double hs = hw + a.hw[b];
-    if (dx <= hs + 1e-15 && dy <= hs + 1e-15) row[cnt++] = b;
+  for (uint32_t base = 0; base < n_leaves && cnt < LIST_CAP; base += 64) {
+    const uint32_t b = base + (uint32_t)lane;
+    uint32_t hit = 0;
+    if (b < n_leaves && b != i) {
+      const double dx = fabs(cx - a.cx[b]), dy = fabs(cy - a.cy[b]);
+      const double hs = hw + a.hw[b];
+      hit = (dx <= hs + 1e-15 && dy <= hs + 1e-15) ? 1u : 0u;
+    }
+    const uint32_t incl = wave_incl_scan(hit, lane);
+    const uint32_t pos = cnt + incl - hit;
+    if (hit && pos < LIST_CAP) row[pos] = b;
+    cnt = min(LIST_CAP, cnt + __shfl(incl, 63, 64));
   }
-  lens[i] = cnt;
+  if (lane == 0) lens[i] = cnt;
 }
 
 __global__ void k_rows_compact(const uint32_t* __restrict__ rows, const uint32_t* __restrict__ off, uint32_t nn,
@@ -912,7 +953,7 @@ int build_lists_reference(galax_fmm* h) {
   GX_HIP(hipMemsetAsync(lens, 0, (nn + 1) * sizeof(uint32_t), s));
   for (uint32_t l = 1; l < t.n_levels; ++l) {
     uint32_t lo = t.level_offset[l], hi = t.level_offset[l + 1];
-    k_m2l_rows<<<nblk(hi - lo, 128), 128, 0, s>>>(a, lo, hi, rows, lens);
+    k_m2l_rows<<<nblk(hi - lo, 4), 256, 0, s>>>(a, lo, hi, rows, lens);
     GX_HIP(hipGetLastError());
   }
   GX_CHECK(dbuf_reserve(t.m2l_off, (nn + 1) * sizeof(uint32_t), s));
@@ -925,7 +966,7 @@ int build_lists_reference(galax_fmm* h) {
   GX_HIP(hipGetLastError());
   // P2P lists (leaves only; other rows empty)
   GX_HIP(hipMemsetAsync(lens, 0, (nn + 1) * sizeof(uint32_t), s));
-  k_p2p_rows<<<nblk(t.n_leaves, 128), 128, 0, s>>>(a, (uint32_t)t.n_leaves, rows, lens);
+  k_p2p_rows<<<nblk(t.n_leaves, 4), 256, 0, s>>>(a, (uint32_t)t.n_leaves, rows, lens);
   GX_HIP(hipGetLastError());
   GX_CHECK(dbuf_reserve(t.p2p_off, (nn + 1) * sizeof(uint32_t), s));
   uint32_t pt = 0;
