@@ -63,18 +63,20 @@ struct P2PItems {
 };
 
 // M2L work in tiles of 16 rows (targets, or sources for the adjoint). The live
-// pairs are grouped by (row, key); a tile's steps are the distinct keys of its
-// groups, step_first/step_cnt give each of its 16 columns' group as a range
-// of `src` (the other end of each pair, in list order), and step_ids the same
-// source ids laid out [step][q][column] for the products. Work items cover at
+// pairs are grouped by (row, key) runs; a tile's steps are the distinct keys of
+// its groups. step_ids lists, per step, each of its 16 columns' sources
+// ([step][q][column], at most MX_GROUP): the other ends of the pairs (node ids),
+// or for a run longer than MX_GROUP one pre-summed row (id n_nodes + its index
+// in lr_first/lr_len, summed into `virt` each evaluation). Work items cover at
 // most MX_ITEM_STEPS steps of one tile; tiles with several items write
 // partial results to slots that a combine pass sums in item order.
 struct M2LTiles {
   DBuf src, tile_off, step_key, step_first, step_cnt, step_ids;
+  DBuf lr_first, lr_len, virt, hubs;  // long runs: source range; pre-summed rows; runs taking a block
   DBuf item_tile, item_s0, item_s1, item_slot;  // per work item
   DBuf multi_tile, multi_slot, multi_cnt;       // per tile with several items
   DBuf part;                                    // partial results, 16 x S doubles per slot
-  size_t n_rows = 0, ntiles = 0, ngroups = 0, nsteps = 0, nitems = 0, nmulti = 0, nslots = 0;
+  size_t n_rows = 0, ntiles = 0, ngroups = 0, nsteps = 0, nitems = 0, nmulti = 0, nslots = 0, nlong = 0, nhub = 0;
 };
 
 // Device-side tree in structure-of-arrays form. Node ids are level-ordered

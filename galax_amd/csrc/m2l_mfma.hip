@@ -90,7 +90,8 @@ struct Lanes {
 // ------------------------------------------------------------- setup ----
 
 constexpr uint32_t MX_ITEM_STEPS = 16;  // steps per work item
-constexpr uint32_t MX_GROUP = 4;        // pairs per group (one column of one step)
+constexpr uint32_t MX_GROUP = 4;        // longest run summed inside the product kernel; longer runs are pre-summed
+constexpr uint32_t MX_WAVE_RUN = 256;   // longest run pre-summed by one wave; longer runs take a block
 
 // row of each live pair: its target (CSR row) or its source
 __global__ void k_mx_pair_rows(const uint32_t* __restrict__ off, uint32_t nt, uint32_t* __restrict__ tgt) {
@@ -123,15 +124,15 @@ __global__ void k_mx_run_starts(const unsigned long long* __restrict__ k, size_t
   if (i < n) f[i] = (i == 0 || k[i] != k[i - 1]) ? (uint32_t)i : 0u;
 }
 
-// a group is a chunk of at most MX_GROUP pairs of one (row, key) run
+// a group is one (row, key) run: flag its first pair
 __global__ void k_mx_chunk_flags(const uint32_t* __restrict__ run_start, size_t n, uint32_t* __restrict__ f) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) f[i] = ((i - run_start[i]) % MX_GROUP == 0) ? 1u : 0u;
+  if (i < n) f[i] = (i == run_start[i]) ? 1u : 0u;
 }
 
 // pairs sorted by (row, key, list position) -> the other end of each pair in
-// that order, and groups (chunks of one (row, key) run) with their tile
-// composite (tile, key, chunk rank, column) for the second sort.
+// that order, and groups (one per (row, key) run) with their tile composite
+// (tile, key, rank 0, column) for the second sort.
 // gid: exclusive scan of the group starts.
 __global__ void k_mx_groups(const unsigned long long* __restrict__ ka, const uint32_t* __restrict__ va,
                             const uint32_t* __restrict__ run_start, const uint32_t* __restrict__ gid, size_t ne,
@@ -141,10 +142,9 @@ __global__ void k_mx_groups(const unsigned long long* __restrict__ ka, const uin
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ne) return;
   src_sorted[i] = other_of_pair[va[i]];
-  const uint32_t off = (uint32_t)i - run_start[i];
-  if (off % MX_GROUP == 0) {
+  if ((uint32_t)i == run_start[i]) {
     const uint32_t g = gid[i];
-    const unsigned long long t = ka[i] >> kb, key = ka[i] & ((1ull << kb) - 1), rank = off / MX_GROUP;
+    const unsigned long long t = ka[i] >> kb, key = ka[i] & ((1ull << kb) - 1), rank = 0;
     gfirst[g] = (uint32_t)i;
     g2k[g] = ((t >> 4) << (kb + rkb + 4)) | (key << (rkb + 4)) | (rank << 4) | (t & 15);
     g2v[g] = g;
@@ -153,15 +153,14 @@ __global__ void k_mx_groups(const unsigned long long* __restrict__ ka, const uin
 
 // sid: inclusive scan of the step-start flags over the sorted groups
 __global__ void k_mx_steps(const unsigned long long* __restrict__ g2k, const uint32_t* __restrict__ g2v,
-                           const uint32_t* __restrict__ sid, size_t ng, const uint32_t* __restrict__ gfirst, int kb,
-                           int rkb, uint32_t* __restrict__ step_key, uint32_t* __restrict__ step_first,
-                           uint32_t* __restrict__ step_cnt, uint32_t* __restrict__ tile_cnt) {
+                           const uint32_t* __restrict__ sid, size_t ng, int kb, int rkb,
+                           uint32_t* __restrict__ step_key, uint32_t* __restrict__ step_grp,
+                           uint32_t* __restrict__ tile_cnt) {
   size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= ng) return;
   const unsigned long long c = g2k[j];
   const uint32_t st = sid[j] - 1, col = (uint32_t)(c & 15), g = g2v[j];
-  step_first[(size_t)st * 16 + col] = gfirst[g];
-  step_cnt[(size_t)st * 16 + col] = gfirst[g + 1] - gfirst[g];
+  step_grp[(size_t)st * 16 + col] = g;
   if (j == 0 || (c >> 4) != (g2k[j - 1] >> 4)) {
     step_key[st] = (uint32_t)((c >> (rkb + 4)) & ((1ull << kb) - 1));
     atomicAdd(&tile_cnt[c >> (kb + rkb + 4)], 1u);
@@ -170,14 +169,110 @@ __global__ void k_mx_steps(const unsigned long long* __restrict__ g2k, const uin
 
 __global__ void k_mx_set_u32(uint32_t* __restrict__ p, uint32_t v) { *p = v; }
 
-// source ids of every step, [step][q][column], NONE past a group's end
-__global__ void k_mx_step_ids(const uint32_t* __restrict__ step_first, const uint32_t* __restrict__ step_cnt,
-                              const uint32_t* __restrict__ src, size_t ns, uint32_t* __restrict__ ids) {
+// groups of more than MX_GROUP pairs (long runs): index and source range
+__global__ void k_mx_long_flags(const uint32_t* __restrict__ gfirst, size_t ng, uint32_t* __restrict__ f) {
+  size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < ng) f[g] = gfirst[g + 1] - gfirst[g] > MX_GROUP ? 1u : 0u;
+}
+
+__global__ void k_mx_long_table(const uint32_t* __restrict__ gfirst, const uint32_t* __restrict__ lrid, size_t ng,
+                                uint32_t* __restrict__ lr_first, uint32_t* __restrict__ lr_len) {
+  size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const uint32_t c = gfirst[g + 1] - gfirst[g];
+  if (c > MX_GROUP) {
+    lr_first[lrid[g]] = gfirst[g];
+    lr_len[lrid[g]] = c;
+  }
+}
+
+__global__ void k_mx_hub_flags(const uint32_t* __restrict__ lr_len, size_t n, uint32_t* __restrict__ f) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) f[r] = lr_len[r] > MX_WAVE_RUN ? 1u : 0u;
+}
+
+__global__ void k_mx_hub_list(const uint32_t* __restrict__ lr_len, const uint32_t* __restrict__ pos, size_t n,
+                              uint32_t* __restrict__ hubs) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n && lr_len[r] > MX_WAVE_RUN) hubs[pos[r]] = (uint32_t)r;
+}
+
+// source ids of every step, [step][q][column], NONE past a group's end; a
+// long run's only entry is its pre-summed row, id n_in + its index
+__global__ void k_mx_step_ids(const uint32_t* __restrict__ step_grp, const uint32_t* __restrict__ gfirst,
+                              const uint32_t* __restrict__ lrid, const uint32_t* __restrict__ src, size_t ns,
+                              uint32_t n_in, uint32_t* __restrict__ ids) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ns * 16) return;
   const size_t st = i / 16, col = i % 16;
-  const uint32_t f = step_first[i], c = step_cnt[i];
-  for (uint32_t q = 0; q < MX_GROUP; ++q) ids[(st * MX_GROUP + q) * 16 + col] = q < c ? src[f + q] : NONE;
+  const uint32_t g = step_grp[i];
+  uint32_t f = 0, c = 0;
+  if (g != NONE) {
+    f = gfirst[g];
+    c = gfirst[g + 1] - f;
+  }
+  for (uint32_t q = 0; q < MX_GROUP; ++q) {
+    uint32_t v = NONE;
+    if (c > MX_GROUP)
+      v = q == 0 ? n_in + lrid[g] : NONE;
+    else if (q < c)
+      v = src[f + q];
+    ids[(st * MX_GROUP + q) * 16 + col] = v;
+  }
+}
+
+// Pre-sums of the long runs, every evaluation, in a fixed order. Runs of at
+// most MX_WAVE_RUN pairs: one wave per run, lane per coefficient, sources in
+// list order. Longer runs (sources listed by very many rows, as in the
+// adjoint): one block per run, the four waves take sources w, w + 4, ... and
+// are added in wave order.
+template <int P>
+__global__ void __launch_bounds__(256) k_m2l_run_sums_w(const uint32_t* __restrict__ lr_first,
+                                                        const uint32_t* __restrict__ lr_len, uint32_t nlr,
+                                                        const uint32_t* __restrict__ src,
+                                                        const double* __restrict__ in, double* __restrict__ virt) {
+  constexpr int S = num_moments(P);
+  const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= nlr) return;  // uniform over the wave
+  const uint32_t f = lr_first[r], len = lr_len[r];
+  if (len > MX_WAVE_RUN) return;
+  for (int k = lane; k < S; k += 64) {
+    double acc = 0.0;
+#pragma unroll 4
+    for (uint32_t i = 0; i < len; ++i) {
+      const double v = in[(size_t)src[f + i] * S + k];
+      acc += isfinite(v) ? v : 0.0;
+    }
+    virt[(size_t)r * S + k] = acc;
+  }
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) k_m2l_run_sums(const uint32_t* __restrict__ hubs,
+                                                      const uint32_t* __restrict__ lr_first,
+                                                      const uint32_t* __restrict__ lr_len,
+                                                      const uint32_t* __restrict__ src, const double* __restrict__ in,
+                                                      double* __restrict__ virt) {
+  constexpr int S = num_moments(P);
+  __shared__ double red[4][64];
+  const uint32_t r = hubs[blockIdx.x], f = lr_first[r], len = lr_len[r];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int k0 = 0; k0 < S; k0 += 64) {
+    const int k = k0 + lane;
+    double acc = 0.0;
+    if (k < S) {
+#pragma unroll 4
+      for (uint32_t i = (uint32_t)w; i < len; i += 4) {
+        const double v = in[(size_t)src[f + i] * S + k];
+        acc += isfinite(v) ? v : 0.0;
+      }
+    }
+    red[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && k < S) virt[(size_t)r * S + k] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+    __syncthreads();
+  }
 }
 
 // FP64 operator of each key: (16 RB)^2 row-major, zero padded
@@ -259,6 +354,8 @@ struct TileArgs {
   const float* that;
   const double* sig_inv;
   const double* in;  // multipoles (forward) or lbar (adjoint), S per node
+  const double* virt;  // pre-summed long runs, S per run; ids n_in + r
+  uint32_t n_in;
   double* out;       // locals (forward) or mbar (adjoint), S per row
   double* part;      // partials of tiles with several items: [slot][16][S]
 };
@@ -311,7 +408,7 @@ __global__ void __launch_bounds__(256) k_m2l_tiles(TileArgs a) {
 #pragma unroll
     for (uint32_t q = 0; q < MX_GROUP; ++q) {  // a group's sources in list order
       if (id[q] != NONE) {
-        const double* row = a.in + (size_t)id[q] * S;
+        const double* row = id[q] < a.n_in ? a.in + (size_t)id[q] * S : a.virt + (size_t)(id[q] - a.n_in) * S;
 #pragma unroll
         for (int e = 0; e < KL; ++e) {
           const int k = LN::k_of(e, kq);
@@ -451,6 +548,8 @@ static int build_tiles(galax_fmm* h, const uint32_t* row_of_pair, const uint32_t
   const uint32_t ntiles = (n_rows + 15) / 16;
   T.n_rows = n_rows;
   T.ntiles = ntiles;
+  T.nlong = 0;
+  T.nhub = 0;
   T.ngroups = 0;
   T.nsteps = 0;
   GX_CHECK(dbuf_reserve(T.tile_off, (size_t)(ntiles + 1) * 4, s));
@@ -481,7 +580,7 @@ static int build_tiles(galax_fmm* h, const uint32_t* row_of_pair, const uint32_t
     if (herr) return set_error(GALAX_ERR_INVALID, "M2L tiles: a pair's row lies outside the live nodes");
     GX_CHECK(radix_sort_bits(h, reinterpret_cast<uint64_t*>(ka), va, reinterpret_cast<uint64_t*>(kb2), vb, ne,
                              kb + tb));
-    // 2. groups: chunks of at most MX_GROUP pairs of each (row, key) run
+    // 2. groups: one per (row, key) run
     GX_CHECK(dbuf_reserve(ws.a, (size_t)ne * 4, s));
     GX_CHECK(dbuf_reserve(ws.h, (size_t)ne * 4, s));
     uint32_t* gid = ws.a.as<uint32_t>();
@@ -494,7 +593,7 @@ static int build_tiles(galax_fmm* h, const uint32_t* row_of_pair, const uint32_t
     uint32_t ng = 0;  // exclusive: a group start i is group gid[i]
     GX_CHECK(scan_u32(h, gid, gid, ne, false, false, &ng));
     T.ngroups = ng;
-    const int rkb = bits_of((ne + MX_GROUP - 1) / MX_GROUP);
+    const int rkb = 1;  // every (row, key) run is one group: its rank field is always 0
     if (bits_of(ntiles) + kb + rkb + 4 > 64) return set_error(GALAX_ERR_UNSUPPORTED, "M2L tiles: key space too wide");
     GX_CHECK(dbuf_reserve(T.src, (size_t)ne * 4, s));
     GX_CHECK(dbuf_reserve(ws.b, (size_t)(ng + 1) * 4, s));
@@ -522,16 +621,40 @@ static int build_tiles(galax_fmm* h, const uint32_t* row_of_pair, const uint32_t
     GX_CHECK(scan_u32(h, sid, sid, ng, true, false, &ns));
     T.nsteps = ns;
     GX_CHECK(dbuf_reserve(T.step_key, (size_t)ns * 4, s));
-    GX_CHECK(dbuf_reserve(T.step_first, (size_t)ns * 64, s));
-    GX_CHECK(dbuf_reserve(T.step_cnt, (size_t)ns * 64, s));
-    GX_HIP(hipMemsetAsync(T.step_first.p, 0, (size_t)ns * 64, s));
-    GX_HIP(hipMemsetAsync(T.step_cnt.p, 0, (size_t)ns * 64, s));
-    k_mx_steps<<<nblk(ng), 256, 0, s>>>(g2k, g2v, sid, ng, gfirst, kb, rkb, T.step_key.as<uint32_t>(),
-                                        T.step_first.as<uint32_t>(), T.step_cnt.as<uint32_t>(), tile_off);
+    GX_CHECK(dbuf_reserve(T.step_first, (size_t)ns * 64, s));  // group id per (step, column)
+    GX_HIP(hipMemsetAsync(T.step_first.p, 0xFF, (size_t)ns * 64, s));
+    k_mx_steps<<<nblk(ng), 256, 0, s>>>(g2k, g2v, sid, ng, kb, rkb, T.step_key.as<uint32_t>(),
+                                        T.step_first.as<uint32_t>(), tile_off);
     GX_HIP(hipGetLastError());
+    // long runs (more than MX_GROUP pairs) become one pre-summed row each
+    uint32_t* lrid = sid;  // the step ids are no longer needed
+    k_mx_long_flags<<<nblk(ng), 256, 0, s>>>(gfirst, ng, lrid);
+    GX_HIP(hipGetLastError());
+    uint32_t nlr = 0;
+    GX_CHECK(scan_u32(h, lrid, lrid, ng, false, false, &nlr));
+    T.nlong = nlr;
+    GX_CHECK(dbuf_reserve(T.lr_first, (size_t)std::max<uint32_t>(nlr, 1) * 4, s));
+    GX_CHECK(dbuf_reserve(T.lr_len, (size_t)std::max<uint32_t>(nlr, 1) * 4, s));
+    k_mx_long_table<<<nblk(ng), 256, 0, s>>>(gfirst, lrid, ng, T.lr_first.as<uint32_t>(), T.lr_len.as<uint32_t>());
+    GX_HIP(hipGetLastError());
+    // runs too long for one wave (hubs) get a block each
+    T.nhub = 0;
+    if (nlr) {
+      GX_CHECK(dbuf_reserve(ws.g, (size_t)nlr * 4, s));
+      uint32_t* hf = ws.g.as<uint32_t>();
+      k_mx_hub_flags<<<nblk(nlr), 256, 0, s>>>(T.lr_len.as<uint32_t>(), nlr, hf);
+      GX_HIP(hipGetLastError());
+      uint32_t nh = 0;
+      GX_CHECK(scan_u32(h, hf, hf, nlr, false, false, &nh));
+      T.nhub = nh;
+      GX_CHECK(dbuf_reserve(T.hubs, (size_t)std::max<uint32_t>(nh, 1) * 4, s));
+      k_mx_hub_list<<<nblk(nlr), 256, 0, s>>>(T.lr_len.as<uint32_t>(), hf, nlr, T.hubs.as<uint32_t>());
+      GX_HIP(hipGetLastError());
+    }
     GX_CHECK(dbuf_reserve(T.step_ids, (size_t)ns * MX_GROUP * 16 * 4, s));
-    k_mx_step_ids<<<nblk((size_t)ns * 16), 256, 0, s>>>(T.step_first.as<uint32_t>(), T.step_cnt.as<uint32_t>(),
-                                                        T.src.as<uint32_t>(), ns, T.step_ids.as<uint32_t>());
+    k_mx_step_ids<<<nblk((size_t)ns * 16), 256, 0, s>>>(T.step_first.as<uint32_t>(), gfirst, lrid,
+                                                        T.src.as<uint32_t>(), ns, (uint32_t)h->tree.n_nodes,
+                                                        T.step_ids.as<uint32_t>());
     GX_HIP(hipGetLastError());
   }
   uint32_t tot = 0;
@@ -649,6 +772,19 @@ static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
   a.n_rows = (uint32_t)T.n_rows;
   a.step_key = T.step_key.as<uint32_t>();
   a.step_ids = T.step_ids.as<uint32_t>();
+  a.n_in = (uint32_t)h->tree.n_nodes;
+  if (T.nlong) {
+    GX_CHECK(dbuf_reserve(T.virt, T.nlong * S * 8, s));
+    k_m2l_run_sums_w<P><<<nblk(T.nlong, 4), 256, 0, s>>>(T.lr_first.as<uint32_t>(), T.lr_len.as<uint32_t>(),
+                                                          (uint32_t)T.nlong, T.src.as<uint32_t>(), in,
+                                                          T.virt.as<double>());
+    if (T.nhub)
+      k_m2l_run_sums<P><<<(unsigned)T.nhub, 256, 0, s>>>(T.hubs.as<uint32_t>(), T.lr_first.as<uint32_t>(),
+                                                         T.lr_len.as<uint32_t>(), T.src.as<uint32_t>(), in,
+                                                         T.virt.as<double>());
+    GX_HIP(hipGetLastError());
+  }
+  a.virt = T.virt.as<double>();
   a.t64 = h->mx_t64.as<double>();
   a.that = h->mx_that.as<float>();
   a.sig_inv = h->mx_sig.as<double>();

@@ -48,8 +48,39 @@ __device__ __forceinline__ double dbinom(int n, int k) {
 
 // ------------------------------------------------------------- near ----
 
-// one wave per P2P work item (target leaf and a chunk of its gather list);
-// lanes 0-31 / 32-63 split the sources; items of split targets write partials
+// Adjoint of the near field for the targets j of one P2P work item (a target
+// leaf and at most P2P_ITEM_LEAVES of its gather list), as the forward P2P
+// kernel is organised: the item's sources are staged through LDS in chunks of
+// VJP_CHUNK as (x, y, m, abar_x) and abar_y; lanes 0-31 and 32-63 hold the
+// same 32 targets and take the even / odd sources (the chunk is padded to an
+// even length with a zero-mass, zero-cotangent source far away); only the
+// chunks holding the target leaf itself run with the self mask. With
+// d = x_i - x_j and v = m_j abar_i - m_i abar_j (i the source):
+//   xbar_j += v / r^3 - 3 d (d . v) / r^5,   mbar_j -= abar_i . d / r^3.
+constexpr int VJP_CHUNK = 256;
+template <bool MASK>
+__device__ __forceinline__ void vjp_near_sum(const double4* __restrict__ sa, const double* __restrict__ sb, int half,
+                                             int n, int self, double xj, double yj, double mj, double abx,
+                                             double aby, double eps2, double& gx, double& gy, double& gm) {
+  const double4* ah = sa + half;
+  const double* bh = sb + half;
+#pragma unroll 2
+  for (int jj = 0; jj < n; jj += 2) {
+    const double4 s = ah[jj];
+    const double sby = bh[jj];
+    const double dx = s.x - xj, dy = s.y - yj;
+    const double r2 = fma(dx, dx, fma(dy, dy, eps2));
+    double inv = rsqrt(r2);
+    if (MASK) inv = jj + half == self ? 0.0 : inv;
+    const double inv2 = inv * inv, inv3 = inv2 * inv, inv5 = inv3 * inv2;
+    const double vx = fma(mj, s.w, -s.z * abx), vy = fma(mj, sby, -s.z * aby);
+    const double t = 3.0 * fma(dx, vx, dy * vy) * inv5;
+    gx += fma(vx, inv3, -dx * t);
+    gy += fma(vy, inv3, -dy * t);
+    gm -= fma(s.w, dx, sby * dy) * inv3;
+  }
+}
+
 __global__ void __launch_bounds__(64) k_vjp_near(const double* __restrict__ x, const double* __restrict__ y,
                                                  const double* __restrict__ m, const double* __restrict__ axb,
                                                  const double* __restrict__ ayb, const uint32_t* __restrict__ bs,
@@ -59,47 +90,76 @@ __global__ void __launch_bounds__(64) k_vjp_near(const double* __restrict__ x, c
                                                  uint32_t n_max, double eps2, double* __restrict__ xb,
                                                  double* __restrict__ yb, double* __restrict__ mb,
                                                  double* __restrict__ part) {
-  __shared__ double sx[64], sy[64], sm[64], sax[64], say[64];
+  static_assert(P2P_ITEM_LEAVES <= 64, "one lane per source leaf of an item");
+  __shared__ double4 sa[VJP_CHUNK + 1];
+  __shared__ double sb[VJP_CHUNK + 1];
+  __shared__ uint32_t l_ss[P2P_ITEM_LEAVES], l_off[P2P_ITEM_LEAVES];
   const uint32_t item = blockIdx.x;
   const uint32_t leaf = itgt[item], g0 = ig0[item], g1 = ig1[item], slot = islot[item];
   const int lane = threadIdx.x, tl = lane & 31, half = lane >> 5;
   const uint32_t ts = bs[leaf], te = be[leaf];
+  // source leaves of the item: start and offset in the item's source order
+  const int nlv = (int)(g1 - g0);
+  uint32_t cnt = 0, ss = 0;
+  bool is_self = false;
+  if (lane < nlv) {
+    const uint32_t sl = gdat[g0 + lane];
+    ss = bs[sl];
+    cnt = be[sl] - ss;
+    is_self = sl == leaf;
+  }
+  uint32_t inc = cnt;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t v = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += v;
+  }
+  if (lane < nlv) {
+    l_ss[lane] = ss;
+    l_off[lane] = inc - cnt;
+  }
+  const uint32_t total = __shfl(inc, 63, 64);
+  const unsigned long long sm_ = __ballot(is_self);
+  uint32_t self_lo = 0, self_hi = 0;
+  if (sm_) {
+    self_lo = __shfl(inc - cnt, __ffsll((long long)sm_) - 1, 64);
+    self_hi = self_lo + (te - ts);
+  }
   for (uint32_t tb = ts; tb < te; tb += 32) {
     const uint32_t j = tb + tl;
     const bool valid = j < te;
     const double xj = valid ? x[j] : 0.0, yj = valid ? y[j] : 0.0, mj = valid ? m[j] : 0.0;
     const double abx = valid ? axb[j] : 0.0, aby = valid ? ayb[j] : 0.0;
     double gx_ = 0.0, gy_ = 0.0, gm_ = 0.0;
-    for (uint32_t g = g0; g < g1; ++g) {
-      const uint32_t sl = gdat[g];
-      const uint32_t ss = bs[sl], se = be[sl];
-      for (uint32_t sb = ss; sb < se; sb += 64) {
-        const uint32_t cnt = min(64u, se - sb);
-        __syncthreads();
-        if ((uint32_t)lane < cnt) {
-          sx[lane] = x[sb + lane];
-          sy[lane] = y[sb + lane];
-          sm[lane] = m[sb + lane];
-          sax[lane] = axb[sb + lane];
-          say[lane] = ayb[sb + lane];
+    for (uint32_t c0 = 0; c0 < total; c0 += VJP_CHUNK) {
+      const int n = (int)min((uint32_t)VJP_CHUNK, total - c0);
+      __syncthreads();
+      for (int pp = lane; pp < n; pp += 64) {
+        const uint32_t q = c0 + (uint32_t)pp;
+        int lo = 0, hi = nlv - 1;  // last source leaf with l_off <= q
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (l_off[mid] <= q)
+            lo = mid;
+          else
+            hi = mid - 1;
         }
-        __syncthreads();
-        if (valid) {
-          const int self = (int)j - (int)sb;
-          for (int ii = half; ii < (int)cnt; ii += 2) {
-            if (ii == self) continue;
-            const double dx = sx[ii] - xj, dy = sy[ii] - yj;  // d = x_i - x_j
-            const double r2 = dx * dx + dy * dy + eps2;
-            const double inv = 1.0 / sqrt(r2);
-            const double inv3 = inv * inv * inv, inv5 = inv3 * inv * inv;
-            const double mi = sm[ii];
-            const double vx = mj * sax[ii] - mi * abx, vy = mj * say[ii] - mi * aby;
-            const double dv = dx * vx + dy * vy;
-            gx_ += vx * inv3 - 3.0 * dx * dv * inv5;
-            gy_ += vy * inv3 - 3.0 * dy * dv * inv5;
-            gm_ -= (sax[ii] * dx + say[ii] * dy) * inv3;
-          }
-        }
+        const uint32_t g = l_ss[lo] + (q - l_off[lo]);
+        sa[pp] = make_double4(x[g], y[g], m[g], axb[g]);
+        sb[pp] = ayb[g];
+      }
+      if (lane == 0 && (n & 1)) {  // pad to even: zero mass and cotangent, far away
+        sa[n] = make_double4(3e19, 3e19, 0.0, 0.0);
+        sb[n] = 0.0;
+      }
+      __syncthreads();
+      if (valid) {
+        const int ne = (n + 1) & ~1;
+        if (c0 < self_hi && c0 + (uint32_t)n > self_lo)
+          vjp_near_sum<true>(sa, sb, half, ne, (int)(self_lo + (j - ts)) - (int)c0, xj, yj, mj, abx, aby, eps2, gx_,
+                             gy_, gm_);
+        else
+          vjp_near_sum<false>(sa, sb, half, ne, 0, xj, yj, mj, abx, aby, eps2, gx_, gy_, gm_);
       }
     }
     gx_ += __shfl_xor(gx_, 32, 64);
@@ -239,6 +299,24 @@ __global__ void k_leaf_of(const uint32_t* __restrict__ bs, const uint32_t* __res
 }
 
 // ------------------------------------------------------ L2L adjoint ----
+
+// factorials and binomials up to MAX_P, filled at compile time
+struct VTabs {
+  double fact[MAX_P + 1];
+  double binom[MAX_P + 1][MAX_P + 1];
+};
+constexpr VTabs make_vtabs() {
+  VTabs t{};
+  t.fact[0] = 1.0;
+  for (int n = 1; n <= MAX_P; ++n) t.fact[n] = t.fact[n - 1] * n;
+  for (int n = 0; n <= MAX_P; ++n) {
+    t.binom[n][0] = 1.0;
+    for (int k = 1; k <= n; ++k) t.binom[n][k] = t.binom[n][k - 1] * (n - k + 1) / k;
+  }
+  return t;
+}
+static __constant__ VTabs v_tabs = make_vtabs();
+
 // parents in [lo, hi): Lbar_P[i,j] += sum_c sum_{oi<=i, oj<=j} Lbar_c[oi,oj] C(i,oi) C(j,oj) dx^(i-oi) dy^(j-oj)
 template <int P>
 __global__ void k_vjp_l2l(const uint32_t* __restrict__ cf, const uint8_t* __restrict__ cc,
@@ -255,9 +333,16 @@ __global__ void k_vjp_l2l(const uint32_t* __restrict__ cf, const uint8_t* __rest
   for (uint32_t c = cf[p]; c < cf[p] + cc[p]; ++c) {
     const double dx = comx[c] - comx[p], dy = comy[c] - comy[p];
     const double* lc = lbar + (size_t)c * S;
-    for (int oi = 0; oi <= i; ++oi)
-      for (int oj = 0; oj <= j; ++oj)
-        acc += lc[moment_index(oi, oj)] * dbinom(i, oi) * dbinom(j, oj) * pow(dx, i - oi) * pow(dy, j - oj);
+    double xw = 1.0;  // dx^(i - oi)
+    for (int oi = i; oi >= 0; --oi) {
+      double row = 0.0, yw = 1.0;  // yw = dy^(j - oj)
+      for (int oj = j; oj >= 0; --oj) {
+        row += lc[moment_index(oi, oj)] * v_tabs.binom[j][oj] * yw;
+        yw *= dy;
+      }
+      acc += row * v_tabs.binom[i][oi] * xw;
+      xw *= dx;
+    }
   }
   lbar[(size_t)p * S + k] += acc;
 }
@@ -279,9 +364,15 @@ __global__ void k_vjp_m2m(const uint32_t* __restrict__ par, const double* __rest
   const double dx = comx[c] - comx[p], dy = comy[c] - comy[p];
   const double* mp_ = mbar + (size_t)p * S;
   double acc = 0.0;
-  for (int I = i; I <= P; ++I)
-    for (int J = j; J <= P - I; ++J)
-      acc += mp_[moment_index(I, J)] * pow(dx, I - i) * pow(dy, J - j) / (dfact(I - i) * dfact(J - j));
+  double xw = 1.0;  // dx^(I - i) / (I - i)!
+  for (int I = i; I <= P; ++I) {
+    double yw = 1.0;  // dy^(J - j) / (J - j)!
+    for (int J = j; J <= P - I; ++J) {
+      acc += mp_[moment_index(I, J)] * xw * yw;
+      yw *= dy / (J - j + 1);
+    }
+    xw *= dx / (I - i + 1);
+  }
   mbar[(size_t)c * S + k] += acc;
 }
 

@@ -124,7 +124,8 @@ typedef struct galax_fmm galax_fmm_t;
 typedef struct {
   double sanitize, p2m, m2m, m2l_setup, m2l, l2l, l2p_p2p, total;
   uint64_t m2l_pairs_executed, m2l_pairs_reference, m2l_keys, p2p_interactions;
-  /* matrix-core M2L work units: (target, key) groups of at most 8 pairs,
+  /* matrix-core M2L work units: (target, key) groups (runs of more than 4
+   * pairs are pre-summed into one row each evaluation),
    * 16-column products (steps) and work items; 0 for FP64_EXACT / CORRECTED */
   uint64_t m2l_groups, m2l_steps, m2l_items;
   /* the main kernel of the M2L and of the L2P+P2P pass alone (no combine) */
