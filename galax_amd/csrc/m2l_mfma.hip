@@ -231,13 +231,16 @@ __global__ void __launch_bounds__(256) k_m2l_run_sums_w(const uint32_t* __restri
                                                         const uint32_t* __restrict__ lr_len, uint32_t nlr,
                                                         const uint32_t* __restrict__ src,
                                                         const double* __restrict__ in, double* __restrict__ virt) {
+  // one wave per run, or one half-wave per run when S <= 32
   constexpr int S = num_moments(P);
-  const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  constexpr int RPW = S <= 32 ? 2 : 1, LW = 64 / RPW;
   const int lane = threadIdx.x & 63;
-  if (r >= nlr) return;  // uniform over the wave
+  const uint32_t r = (blockIdx.x * 4 + (threadIdx.x >> 6)) * RPW + lane / LW;
+  const int kl = lane % LW;
+  if (r >= nlr) return;
   const uint32_t f = lr_first[r], len = lr_len[r];
   if (len > MX_WAVE_RUN) return;
-  for (int k = lane; k < S; k += 64) {
+  for (int k = kl; k < S; k += LW) {
     double acc = 0.0;
 #pragma unroll 4
     for (uint32_t i = 0; i < len; ++i) {
@@ -775,7 +778,7 @@ static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
   a.n_in = (uint32_t)h->tree.n_nodes;
   if (T.nlong) {
     GX_CHECK(dbuf_reserve(T.virt, T.nlong * S * 8, s));
-    k_m2l_run_sums_w<P><<<nblk(T.nlong, 4), 256, 0, s>>>(T.lr_first.as<uint32_t>(), T.lr_len.as<uint32_t>(),
+    k_m2l_run_sums_w<P><<<nblk(T.nlong, S <= 32 ? 8 : 4), 256, 0, s>>>(T.lr_first.as<uint32_t>(), T.lr_len.as<uint32_t>(),
                                                           (uint32_t)T.nlong, T.src.as<uint32_t>(), in,
                                                           T.virt.as<double>());
     if (T.nhub)
