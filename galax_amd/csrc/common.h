@@ -178,7 +178,9 @@ hipError_t p2p_combine(const P2PItems& it, const uint32_t* goff, const uint32_t*
                        uint32_t n_max, const double* part, double* o0, double* o1, double* o2, hipStream_t s);
 
 // ---------------------------------------------------------- evaluation ----
-int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, double* d_ay, double* d_phi);
+// near = false computes the expansions only (P2M .. L2L), for the VJP
+int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, double* d_ay, double* d_phi,
+             bool near = true);
 int vjp(galax_fmm* h, const double* d_x, const double* d_y, const double* d_m, const double* d_axb,
         const double* d_ayb, double* d_xb, double* d_yb, double* d_mb);
 int m2l_mfma_setup(galax_fmm* h);

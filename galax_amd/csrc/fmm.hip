@@ -949,7 +949,7 @@ static int m2l_keys_setup(galax_fmm* h, uint32_t n_targets) {
 
 template <int P>
 static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, double* d_ay,
-                      double* d_phi) {
+                      double* d_phi, bool near) {
   DevTree& t = h->tree;
   hipStream_t s = h->stream;
   constexpr int S = num_moments(P);
@@ -1024,6 +1024,11 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
       it.tgt.as<uint32_t>(), it.g0.as<uint32_t>(), it.g1.as<uint32_t>(), it.slot.as<uint32_t>(), nmax, eps * eps, \
       d_ax, d_ay, d_phi, h->p2p_part.as<double>())
   GX_HIP(hipEventRecord(ev[10], s));
+  if (!near) {  // expansions only (the VJP's forward): no L2P / P2P
+    GX_HIP(hipEventRecord(ev[11], s));
+    GX_HIP(hipEventRecord(ev[7], s));
+    return 0;
+  }
   if (fp32_near) {
     if (d_phi)
       GX_LAUNCH_P2P(float, true);
@@ -1045,7 +1050,8 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   return 0;
 }
 
-int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, double* d_ay, double* d_phi) {
+int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, double* d_ay, double* d_phi,
+             bool near) {
   DevTree& t = h->tree;
   hipStream_t s = h->stream;
   const size_t n = t.n_bodies;
@@ -1072,7 +1078,7 @@ int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, 
   switch (h->cfg.p) {
 #define GX_CASE(PP)                                                  \
   case PP:                                                           \
-    rc = run_passes<PP>(h, d_x, d_y, d_m, d_ax, d_ay, d_phi);        \
+    rc = run_passes<PP>(h, d_x, d_y, d_m, d_ax, d_ay, d_phi, near);  \
     break;
     GX_P_CASES(GX_CASE)
 #undef GX_CASE

@@ -524,8 +524,9 @@ int vjp(galax_fmm* h, const double* d_x, const double* d_y, const double* d_m, c
   size_t n = h->tree.n_bodies;
   GX_CHECK(dbuf_reserve(h->vj_fwd, std::max<size_t>(n, 1) * 16, h->stream));
   double* fax = h->vj_fwd.as<double>();
+  // the adjoint needs the expansions (loc for the L2P Hessian), not the forces
   GX_CHECK(evaluate(h, const_cast<double*>(d_x), const_cast<double*>(d_y), const_cast<double*>(d_m), fax, fax + n,
-                    nullptr));
+                    nullptr, false));
   switch (h->cfg.p) {
 #define GX_CASE(PP) \
   case PP:          \
