@@ -472,9 +472,34 @@ __global__ void __launch_bounds__(256) k_m2l_exact(const uint32_t* __restrict__ 
 
 // --------------------------------------------------------------- L2L ----
 
-// exact: children in [lo, hi), one thread per (child, coefficient);
-// L_child[oi,oj] += sum_{i>=oi, j>=oj, i+j<=p} L_par[i,j] C(i,oi) C(j,oj) dx^(i-oi) dy^(j-oj)
-// (galax-core/src/lib.rs:1274-1299)
+// exact: L_child[oi,oj] += sum_{i>=oi, j>=oj, i+j<=p} L_par[i,j] C(i,oi) C(j,oj) dx^(i-oi) dy^(j-oj)
+// (galax-core/src/lib.rs:1274-1299). l2l_sum is the sum for one coefficient of
+// one child at offset (dx, dy) from its parent, in the reference's order.
+template <int P>
+__device__ __forceinline__ double l2l_sum(const double* __restrict__ lp, int oi, int oj, double dx, double dy) {
+#pragma clang fp contract(off)
+  double dxp[P + 1], dyp[P + 1];
+  dxp[0] = 1.0;
+  dyp[0] = 1.0;
+#pragma unroll
+  for (int q = 1; q <= P; ++q) {
+    dxp[q] = dxp[q - 1] * dx;
+    dyp[q] = dyp[q - 1] * dy;
+  }
+  double sum = 0.0;
+#pragma unroll
+  for (int i = 0; i <= P; ++i) {
+    if (i < oi) continue;
+#pragma unroll
+    for (int j = 0; j <= P - i; ++j) {
+      if (j < oj) continue;
+      sum += lp[moment_index(i, j)] * c_tabs.binom[i][oi] * c_tabs.binom[j][oj] * dxp[i - oi] * dyp[j - oj];
+    }
+  }
+  return sum;
+}
+
+// children in [lo, hi), one thread per (child, coefficient)
 template <int P>
 __global__ void k_l2l(const uint32_t* __restrict__ par, const double* __restrict__ comx,
                       const double* __restrict__ comy, uint32_t lo, uint32_t hi, double* __restrict__ loc) {
@@ -488,27 +513,8 @@ __global__ void k_l2l(const uint32_t* __restrict__ par, const double* __restrict
   if (p == NONE) return;
   int oi, oj;
   coef_ij<P>(k, oi, oj);
-  double dx = comx[c] - comx[p], dy = comy[c] - comy[p];
-  double dxp[P + 1], dyp[P + 1];
-  dxp[0] = 1.0;
-  dyp[0] = 1.0;
-#pragma unroll
-  for (int q = 1; q <= P; ++q) {
-    dxp[q] = dxp[q - 1] * dx;
-    dyp[q] = dyp[q - 1] * dy;
-  }
-  const double* lp = loc + (size_t)p * S;
-  double sum = 0.0;
-#pragma unroll
-  for (int i = 0; i <= P; ++i) {
-    if (i < oi) continue;
-#pragma unroll
-    for (int j = 0; j <= P - i; ++j) {
-      if (j < oj) continue;
-      sum += lp[moment_index(i, j)] * c_tabs.binom[i][oi] * c_tabs.binom[j][oj] * dxp[i - oi] * dyp[j - oj];
-    }
-  }
-  loc[(size_t)c * S + k] += sum;
+  const double dx = comx[c] - comx[p], dy = comy[c] - comy[p];
+  loc[(size_t)c * S + k] += l2l_sum<P>(loc + (size_t)p * S, oi, oj, dx, dy);
 }
 
 // ------------------------------------------------------ L2P + P2P ----
@@ -600,9 +606,11 @@ __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, co
                                                 const uint32_t* __restrict__ ig1, const uint32_t* __restrict__ islot,
                                                 uint32_t n_max, double eps2d, double* __restrict__ ax,
                                                 double* __restrict__ ay, double* __restrict__ phi,
-                                                double* __restrict__ part) {
+                                                double* __restrict__ part, const uint32_t* __restrict__ par,
+                                                int fuse_l2l) {
   constexpr int S = num_moments(P);
   typedef typename Near<T>::V4 V4;
+  __shared__ double sloc[S];
   static_assert(P2P_ITEM_LEAVES <= 64, "one lane per source leaf of an item");
   __shared__ V4 src[P2P_CHUNK + 1];
   __shared__ uint32_t l_ss[P2P_ITEM_LEAVES], l_off[P2P_ITEM_LEAVES];
@@ -642,13 +650,29 @@ __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, co
     self_lo = __shfl(inc - cnt, __ffsll((long long)sm_) - 1, 64);
     self_hi = self_lo + (te - ts);
   }
+  // the leaf's local expansion; with fuse_l2l, the last L2L (parent to leaf)
+  // is applied here, with the same arithmetic as k_l2l
+  if (first) {  // uniform over the block
+    const uint32_t pa = fuse_l2l ? par[leaf] : NONE;
+    const double ldx = pa != NONE ? ox - comx[pa] : 0.0, ldy = pa != NONE ? oy - comy[pa] : 0.0;
+    for (int k = lane; k < S; k += 64) {
+      double v = loc[(size_t)leaf * S + k];
+      if (pa != NONE) {
+        int oi, oj;
+        coef_ij<P>(k, oi, oj);
+        v += l2l_sum<P>(loc + (size_t)pa * S, oi, oj, ldx, ldy);
+      }
+      sloc[k] = v;
+    }
+    __syncthreads();
+  }
   for (uint32_t tb = ts; tb < te; tb += 32) {
     const uint32_t i = tb + tl;
     const bool valid = i < te;
     const double xi = valid ? x[i] : ox, yi = valid ? y[i] : oy;
     const T xr = (T)(xi - ox), yr = (T)(yi - oy);
     double fx = 0.0, fy = 0.0, fp = 0.0;
-    if (first && valid && half == 0) l2p_body<P>(loc + (size_t)leaf * S, xi - ox, yi - oy, &fx, &fy, &fp);
+    if (first && valid && half == 0) l2p_body<P>(sloc, xi - ox, yi - oy, &fx, &fy, &fp);
     T accx = 0, accy = 0, accp = 0;
     for (uint32_t c0 = 0; c0 < total; c0 += P2P_CHUNK) {
       const int n = (int)min((uint32_t)P2P_CHUNK, total - c0);
@@ -1000,13 +1024,18 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
     GX_HIP(hipGetLastError());
   }
   GX_HIP(hipEventRecord(ev[5], s));
+  // L2L top-down; the last level (parents to leaves) runs inside the L2P+P2P
+  // kernel when the forces are computed (the VJP's expansions-only forward
+  // needs the leaves' locals in memory)
+  const bool fuse_l2l = near && t.n_levels >= 2;
+  const uint32_t l2l_stop = fuse_l2l ? 2 : 1;
   if (corrected) {
-    for (uint32_t l = t.n_levels - 1; l >= 1; --l) {
+    for (uint32_t l = t.n_levels - 1; l >= l2l_stop; --l) {
       uint32_t lo = t.level_offset[l - 1], hi = t.level_offset[l];
       k_l2l<P><<<nblk((size_t)(hi - lo) * S), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, lo, hi, loc);
       GX_HIP(hipGetLastError());
     }
-  } else if (t.n_levels >= 2) {
+  } else if (t.n_levels >= 2 && !fuse_l2l) {
     k_l2l<P><<<nblk(nl * S), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, 0, (uint32_t)nl, loc);
     GX_HIP(hipGetLastError());
   }
@@ -1022,7 +1051,7 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   k_l2p_p2p<P, T, PH><<<(unsigned)it.n_items, 64, 0, s>>>(                                                  \
       d_x, d_y, d_m, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), comx, comy, loc, goff, gdat,     \
       it.tgt.as<uint32_t>(), it.g0.as<uint32_t>(), it.g1.as<uint32_t>(), it.slot.as<uint32_t>(), nmax, eps * eps, \
-      d_ax, d_ay, d_phi, h->p2p_part.as<double>())
+      d_ax, d_ay, d_phi, h->p2p_part.as<double>(), t.parent.as<uint32_t>(), fuse_l2l ? 1 : 0)
   GX_HIP(hipEventRecord(ev[10], s));
   if (!near) {  // expansions only (the VJP's forward): no L2P / P2P
     GX_HIP(hipEventRecord(ev[11], s));
