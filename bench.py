@@ -101,16 +101,17 @@ def main():
         ax, ay, phi = torch.empty_like(x), torch.empty_like(x), torch.empty_like(x)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        ctx.evaluate_device(x, y, m, ax, ay, phi)  # the first call also builds the static M2L key tables
+        ctx.evaluate_device(x, y, m, ax, ay)  # the first call also builds the static M2L key tables
         torch.cuda.synchronize()
         t_first = time.perf_counter() - t0
         for _ in range(args.warmup):
-            ctx.evaluate_device(x, y, m, ax, ay, phi)
+            ctx.evaluate_device(x, y, m, ax, ay)
         torch.cuda.synchronize()
+        # the timed step is compute_fmm_force's output: accelerations only
         passes = {}
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            ctx.evaluate_device(x, y, m, ax, ay, phi)
+            ctx.evaluate_device(x, y, m, ax, ay)
             tm = ctx.timings()
             for k, v in tm.passes_ms.items():
                 passes[k] = passes.get(k, 0.0) + v
@@ -119,6 +120,16 @@ def main():
         ms = 1e3 * t_total / args.steps
         passes = {k: v / args.steps for k, v in passes.items()}
         log(f"[bench] {name}: {ms:.3f} ms/step (build {t_build:.3f}s, first eval {t_first:.3f}s); passes {passes}")
+        # the same step with the potential as well (an extra output of this library)
+        nphi = max(1, args.steps // 2)
+        ctx.evaluate_device(x, y, m, ax, ay, phi)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(nphi):
+            ctx.evaluate_device(x, y, m, ax, ay, phi)
+        torch.cuda.synchronize()
+        ms_phi = 1e3 * (time.perf_counter() - t0) / nphi
+        log(f"[bench] {name}: with the potential {ms_phi:.3f} ms/step")
         vjp_ms = None
         if name in ("fp64", headline) and args.vjp_steps > 0:
             # VJP of the accelerations (forward recomputed inside, adjoint passes in FP64)
@@ -135,7 +146,7 @@ def main():
             vjp_ms = 1e3 * (time.perf_counter() - t0) / args.vjp_steps
             log(f"[bench] {name}: vjp {vjp_ms:.3f} ms ({vjp_ms / ms:.2f}x the forward)")
             del axb, ayb, xb, yb, mb
-        res[name] = dict(ms=ms, passes=passes, t_build=t_build, t_first=t_first, tm=tm, info=info,
+        res[name] = dict(ms=ms, ms_phi=ms_phi, passes=passes, t_build=t_build, t_first=t_first, tm=tm, info=info,
                          ax=ax.cpu().numpy(), ay=ay.cpu().numpy(), bodies=b, vjp_ms=vjp_ms, build_ms=build_ms)
         del ctx, x, y, m, ax, ay, phi
         torch.cuda.empty_cache()
@@ -173,7 +184,7 @@ def main():
 
     # roofline of the headline form's dominant pass (algorithmic flops or bytes per launch)
     H = res[headline]
-    stages = stage_rooflines(H, n, args.p, headline)
+    stages = stage_rooflines(H, n, args.p, headline, args.algo == "corrected")
     dominant = max(("l2p_p2p", "m2l", "p2m", "m2m", "l2l"), key=lambda k: stages.get(k, {}).get("ms", 0.0))
     st = stages[dominant]
     roof = {"kernel": dominant, "bound": st["bound"], "achieved": st["achieved"], "peak": st["peak"],
@@ -203,7 +214,8 @@ def main():
                    "theta": args.theta if args.algo == "corrected" else None},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "forms": {k: {"ms_per_step": r["ms"], "bodies_per_s": n / (r["ms"] * 1e-3), "passes_ms": r["passes"],
+        "forms": {k: {"ms_per_step": r["ms"], "bodies_per_s": n / (r["ms"] * 1e-3),
+                      "ms_per_step_with_potential": r["ms_phi"], "passes_ms": r["passes"],
                       "t_build_s": r["t_build"], "t_first_eval_s": r["t_first"],
                       "p2p_interactions_per_s": I / (r["passes"]["l2p_p2p"] * 1e-3),
                       "vjp_ms": r["vjp_ms"], "vjp_over_fwd": r["vjp_ms"] / r["ms"] if r["vjp_ms"] else None,
@@ -240,7 +252,7 @@ def measured_traffic(args, form, stage):
     return k["traffic_bytes"] if k else None
 
 
-def stage_rooflines(r, n, p, form):
+def stage_rooflines(r, n, p, form, algo_corrected=False):
     """Achieved rate of each pass and build stage against the MI355X peak that
     bounds it, from algorithmic bytes or flops per launch (N bodies, L leaves,
     V nodes, S coefficients, E executed M2L pairs, I P2P interactions; w = 8:
@@ -268,11 +280,12 @@ def stage_rooflines(r, n, p, form):
     hbm("tree", b["tree"], 24.0 * n + V * 2 * 64.0)          # leaf geometry + parent rounds
     hbm("p2m", passes.get("p2m"), 24.0 * n + L * S * 8.0)
     hbm("m2m", passes.get("m2m"), 2.0 * L * S * 8.0)         # reference variant: level-1 parents of the leaves
-    hbm("l2l", passes.get("l2l"), 2.0 * L * S * 8.0)
+    if algo_corrected:  # the reference variant's one L2L level runs inside the L2P+P2P kernel
+        hbm("l2l", passes.get("l2l"), 2.0 * V * S * 8.0)
     # the main kernel of each pass alone when it was timed (no combine pass)
     comp("m2l", passes.get("m2l_kernel") or passes.get("m2l"), 2.0 * S * S * E, "mfma",
          FP32_PEAK_TFS if mixed else FP64_PEAK_TFS)
-    comp("l2p_p2p", passes.get("p2p_kernel") or passes.get("l2p_p2p"), 17.0 * I + 4.0 * S * n, "valu",
+    comp("l2p_p2p", passes.get("p2p_kernel") or passes.get("l2p_p2p"), 14.0 * I + 4.0 * S * n, "valu",
          FP32_PEAK_TFS if mixed else FP64_PEAK_TFS)
     return out
 

@@ -476,25 +476,23 @@ __global__ void __launch_bounds__(256) k_m2l_exact(const uint32_t* __restrict__ 
 // (galax-core/src/lib.rs:1274-1299). l2l_sum is the sum for one coefficient of
 // one child at offset (dx, dy) from its parent, in the reference's order.
 template <int P>
-__device__ __forceinline__ double l2l_sum(const double* __restrict__ lp, int oi, int oj, double dx, double dy) {
+__device__ __forceinline__ double l2l_sum(const double* __restrict__ lp, const double* __restrict__ bt, int ld,
+                                          int oi, int oj, double dx, double dy) {
 #pragma clang fp contract(off)
-  double dxp[P + 1], dyp[P + 1];
-  dxp[0] = 1.0;
-  dyp[0] = 1.0;
-#pragma unroll
-  for (int q = 1; q <= P; ++q) {
-    dxp[q] = dxp[q - 1] * dx;
-    dyp[q] = dyp[q - 1] * dy;
-  }
-  double sum = 0.0;
-#pragma unroll
-  for (int i = 0; i <= P; ++i) {
-    if (i < oi) continue;
-#pragma unroll
-    for (int j = 0; j <= P - i; ++j) {
-      if (j < oj) continue;
-      sum += lp[moment_index(i, j)] * c_tabs.binom[i][oi] * c_tabs.binom[j][oj] * dxp[i - oi] * dyp[j - oj];
+  // rolled loops keep the register count low where this runs inside the
+  // L2P+P2P kernel; px, py are dx^(i-oi), dy^(j-oj) by repeated products;
+  // bt[n * ld + k] is the binomial C(n, k)
+  double sum = 0.0, px = 1.0;
+#pragma unroll 1
+  for (int i = oi; i <= P; ++i) {
+    const double bi = bt[i * ld + oi];
+    double py = 1.0;
+#pragma unroll 1
+    for (int j = oj; j <= P - i; ++j) {
+      sum += lp[moment_index(i, j)] * bi * bt[j * ld + oj] * px * py;
+      py = py * dy;
     }
+    px = px * dx;
   }
   return sum;
 }
@@ -514,13 +512,16 @@ __global__ void k_l2l(const uint32_t* __restrict__ par, const double* __restrict
   int oi, oj;
   coef_ij<P>(k, oi, oj);
   const double dx = comx[c] - comx[p], dy = comy[c] - comy[p];
-  loc[(size_t)c * S + k] += l2l_sum<P>(loc + (size_t)p * S, oi, oj, dx, dy);
+  loc[(size_t)c * S + k] += l2l_sum<P>(loc + (size_t)p * S, &c_tabs.binom[0][0], MAX_P + 1, oi, oj, dx, dy);
 }
 
 // ------------------------------------------------------ L2P + P2P ----
 
-// exact far field of one body (galax-core/src/lib.rs:1321-1345)
-template <int P>
+// exact far field of one body (galax-core/src/lib.rs:1321-1345). ROWS: the
+// rows i of L are walked by a rolled loop (the powers of dux by repeated
+// products, as in the table), so that at most one row of coefficients is in
+// registers at a time; same terms in the same order.
+template <int P, bool ROWS = false>
 __device__ __forceinline__ void l2p_body(const double* __restrict__ L, double dux, double duy, double* fx,
                                          double* fy, double* fphi) {
 #pragma clang fp contract(off)
@@ -533,28 +534,53 @@ __device__ __forceinline__ void l2p_body(const double* __restrict__ L, double du
     yp[q] = yp[q - 1] * duy;
   }
   double ax = 0.0, ay = 0.0, ph = 0.0;
+  if (ROWS) {
+    double xpi = 1.0, xpm = 1.0;  // dux^i, dux^(i-1)
+#pragma unroll 1
+    for (int i = 0; i <= P; ++i) {
 #pragma unroll
-  for (int i = 0; i <= P; ++i)
-#pragma unroll
-    for (int j = 0; j <= P - i; ++j) {
-      double c = L[moment_index(i, j)];
-      if (i >= 1) ax += c * (double)i * xp[i - 1] * yp[j];
-      if (j >= 1) ay += c * (double)j * xp[i] * yp[j - 1];
-      ph += c * xp[i] * yp[j];
+      for (int j = 0; j <= P; ++j) {
+        if (j <= P - i) {
+          double c = L[moment_index(i, j)];
+          if (i >= 1) ax += c * (double)i * xpm * yp[j];
+          if (j >= 1) ay += c * (double)j * xpi * yp[j - 1];
+          ph += c * xpi * yp[j];
+        }
+      }
+      xpm = xpi;
+      xpi = xpi * dux;
     }
+  } else {
+#pragma unroll
+    for (int i = 0; i <= P; ++i)
+#pragma unroll
+      for (int j = 0; j <= P - i; ++j) {
+        double c = L[moment_index(i, j)];
+        if (i >= 1) ax += c * (double)i * xp[i - 1] * yp[j];
+        if (j >= 1) ay += c * (double)j * xp[i] * yp[j - 1];
+        ph += c * xp[i] * yp[j];
+      }
+  }
   *fx = ax;
   *fy = ay;
   *fphi = ph;
 }
 
-// Near-field arithmetic type: FP64 (rsqrt: v_rsq_f64 + one refinement) or
-// FP32 (v_rsq_f32) on coordinates relative to the target leaf's COM.
+// Near-field arithmetic type: FP64 or FP32 (v_rsq_f32) on coordinates
+// relative to the target leaf's COM. FP64: the v_rsq_f64 estimate y with one
+// second-order refinement, y (1 + e/2 + 3e^2/8) with e = 1 - v y^2; no
+// fix-ups for v = 0 or inf (v >= eps^2 > 0 on every unmasked use; with eps = 0
+// two coincident bodies give a non-finite force, as 1/sqrt(0) does).
 template <typename T>
 struct Near;
 template <>
 struct Near<double> {
   typedef double4 V4;
-  static __device__ __forceinline__ double rsq(double v) { return rsqrt(v); }
+  static __device__ __forceinline__ double rsq(double v) {
+    const double y = __builtin_amdgcn_rsq(v);
+    const double e = fma(-v * y, y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
+  }
 };
 template <>
 struct Near<float> {
@@ -571,7 +597,8 @@ template <typename T, bool PHI, bool MASK>
 __device__ __forceinline__ void near_sum(const typename Near<T>::V4* __restrict__ src, int half, int n, int self,
                                          T xr, T yr, T eps2, T& accx, T& accy, T& accp) {
   const typename Near<T>::V4* sh = src + half;
-#pragma unroll 4
+  constexpr int U = sizeof(T) == 8 ? 2 : 4;
+#pragma unroll U
   for (int jj = 0; jj < n; jj += 2) {
     const typename Near<T>::V4 sj = sh[jj];
     const T dx = sj.x - xr, dy = sj.y - yr;
@@ -592,9 +619,10 @@ __device__ __forceinline__ void near_sum(const typename Near<T>::V4* __restrict_
 // even / odd sources of a chunk; the halves are combined by one xor-shuffle.
 // Only the chunks that hold the target leaf itself (found among the item's
 // source leaves by ballot) run with the self-exclusion mask. The first item of
-// a target also adds the far field: the leaf's local expansion at each body
-// (FP64, exact). Targets with one item write the result; otherwise each item
-// writes a partial to its slot.
+// a target also adds the far field in FP64: the leaf's local expansion at each
+// body and, with fuse_l2l, the parent's local about the parent's COM (in place
+// of the last L2L level). Targets with one item write the result; otherwise
+// each item writes a partial to its slot.
 constexpr int P2P_CHUNK = 256;
 template <int P, typename T, bool PHI>
 __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, const double* __restrict__ y,
@@ -610,7 +638,7 @@ __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, co
                                                 int fuse_l2l) {
   constexpr int S = num_moments(P);
   typedef typename Near<T>::V4 V4;
-  __shared__ double sloc[S];
+  __shared__ double sloc[S], spar[S];
   static_assert(P2P_ITEM_LEAVES <= 64, "one lane per source leaf of an item");
   __shared__ V4 src[P2P_CHUNK + 1];
   __shared__ uint32_t l_ss[P2P_ITEM_LEAVES], l_off[P2P_ITEM_LEAVES];
@@ -642,7 +670,7 @@ __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, co
     l_ss[lane] = ss;
     l_off[lane] = inc - cnt;
   }
-  const uint32_t total = __shfl(inc, 63, 64);
+  const uint32_t total = __builtin_amdgcn_readlane(inc, 63);  // uniform: scalar loop bounds below
   // source-order range [self_lo, self_hi) of the target leaf's own bodies
   const unsigned long long sm_ = __ballot(is_self);
   uint32_t self_lo = 0, self_hi = 0;
@@ -650,29 +678,36 @@ __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, co
     self_lo = __shfl(inc - cnt, __ffsll((long long)sm_) - 1, 64);
     self_hi = self_lo + (te - ts);
   }
-  // the leaf's local expansion; with fuse_l2l, the last L2L (parent to leaf)
-  // is applied here, with the same arithmetic as k_l2l
+  // far field (first item of the leaf only): the leaf's local, staged in LDS,
+  // evaluated at each body by lanes 0-31. With fuse_l2l the parent's local
+  // is staged too and lanes 32-63 evaluate it about the parent's COM: L2L is
+  // an exact re-expansion of the parent's polynomial, so the sum of the two
+  // equals L2P of the shifted local (up to rounding), and the separate L2L
+  // of the last level is not needed.
+  uint32_t pa = NONE;
+  double pcx = 0.0, pcy = 0.0;
   if (first) {  // uniform over the block
-    const uint32_t pa = fuse_l2l ? par[leaf] : NONE;
-    const double ldx = pa != NONE ? ox - comx[pa] : 0.0, ldy = pa != NONE ? oy - comy[pa] : 0.0;
+    pa = fuse_l2l ? par[leaf] : NONE;
+    if (pa != NONE) {
+      pcx = comx[pa];
+      pcy = comy[pa];
+    }
     for (int k = lane; k < S; k += 64) {
-      double v = loc[(size_t)leaf * S + k];
-      if (pa != NONE) {
-        int oi, oj;
-        coef_ij<P>(k, oi, oj);
-        v += l2l_sum<P>(loc + (size_t)pa * S, oi, oj, ldx, ldy);
-      }
-      sloc[k] = v;
+      sloc[k] = loc[(size_t)leaf * S + k];
+      if (pa != NONE) spar[k] = loc[(size_t)pa * S + k];
     }
     __syncthreads();
   }
+  const bool far_lane = first && (half == 0 || pa != NONE);
+  const double fcx = half ? pcx : ox, fcy = half ? pcy : oy;
+  const double* far_loc = half ? spar : sloc;
   for (uint32_t tb = ts; tb < te; tb += 32) {
     const uint32_t i = tb + tl;
     const bool valid = i < te;
     const double xi = valid ? x[i] : ox, yi = valid ? y[i] : oy;
     const T xr = (T)(xi - ox), yr = (T)(yi - oy);
     double fx = 0.0, fy = 0.0, fp = 0.0;
-    if (first && valid && half == 0) l2p_body<P>(sloc, xi - ox, yi - oy, &fx, &fy, &fp);
+    if (far_lane && valid) l2p_body<P, true>(far_loc, xi - fcx, yi - fcy, &fx, &fy, &fp);
     T accx = 0, accy = 0, accp = 0;
     for (uint32_t c0 = 0; c0 < total; c0 += P2P_CHUNK) {
       const int n = (int)min((uint32_t)P2P_CHUNK, total - c0);
@@ -704,18 +739,23 @@ __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, co
         src[n] = v;
       }
       __syncthreads();
-      if (valid) {
-        const int ne = (n + 1) & ~1;
-        if (c0 < self_hi && c0 + (uint32_t)n > self_lo)
-          near_sum<T, PHI, true>(src, half, ne, (int)(self_lo + (i - ts)) - (int)c0, xr, yr, eps2, accx, accy,
-                                 accp);
-        else
-          near_sum<T, PHI, false>(src, half, ne, 0, xr, yr, eps2, accx, accy, accp);
-      }
+      // every lane runs the sum (lanes past the leaf's end sit at its COM and
+      // their sums are dropped), so the loop needs no exec mask
+      const int ne = (n + 1) & ~1;
+      if (c0 < self_hi && c0 + (uint32_t)n > self_lo)
+        near_sum<T, PHI, true>(src, half, ne, (int)(self_lo + (i - ts)) - (int)c0, xr, yr, eps2, accx, accy,
+                               accp);
+      else
+        near_sum<T, PHI, false>(src, half, ne, 0, xr, yr, eps2, accx, accy, accp);
     }
     accx += __shfl_xor(accx, 32, 64);
     accy += __shfl_xor(accy, 32, 64);
     if (PHI) accp += __shfl_xor(accp, 32, 64);
+    if (pa != NONE) {  // uniform: add the parent's far field from lanes 32-63
+      fx += __shfl_xor(fx, 32, 64);
+      fy += __shfl_xor(fy, 32, 64);
+      if (PHI) fp += __shfl_xor(fp, 32, 64);
+    }
     if (valid && half == 0) {
       const double rx = (0.0 - fx) + (double)accx;
       const double ry = (0.0 - fy) + (double)accy;
