@@ -177,6 +177,28 @@ int build_p2p_items(galax_fmm* h, const uint32_t* goff, uint32_t n_leaves, P2PIt
 hipError_t p2p_combine(const P2PItems& it, const uint32_t* goff, const uint32_t* bs, const uint32_t* be,
                        uint32_t n_max, const double* part, double* o0, double* o1, double* o2, hipStream_t s);
 
+// Near-field arithmetic type: FP64 or FP32 (v_rsq_f32) on coordinates
+// relative to the target leaf's COM. FP64: the v_rsq_f64 estimate y with one
+// second-order refinement, y (1 + e/2 + 3e^2/8) with e = 1 - v y^2; no
+// fix-ups for v = 0 or inf (v >= eps^2 > 0 on every unmasked use; with eps = 0
+// two coincident bodies give a non-finite force, as 1/sqrt(0) does).
+template <typename T>
+struct Near;
+template <>
+struct Near<double> {
+  typedef double4 V4;
+  static __device__ __forceinline__ double rsq(double v) {
+    const double y = __builtin_amdgcn_rsq(v);
+    const double e = fma(-v * y, y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
+  }
+};
+template <>
+struct Near<float> {
+  typedef float4 V4;
+  static __device__ __forceinline__ float rsq(float v) { return __builtin_amdgcn_rsqf(v); }
+};
+
 // ---------------------------------------------------------- evaluation ----
 // near = false computes the expansions only (P2M .. L2L), for the VJP
 int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, double* d_ay, double* d_phi,

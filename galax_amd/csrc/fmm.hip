@@ -19,6 +19,7 @@ namespace gx {
 struct Tabs {
   double fact[MAX_P + 1];
   double binom[MAX_P + 1][MAX_P + 1];
+  double rfact2[MAX_P + 1][MAX_P + 1];  // RN(1 / (i! j!)), for div_exact
 };
 
 __constant__ Tabs c_tabs;
@@ -34,7 +35,18 @@ static Tabs host_tabs() {
   }
   t.fact[0] = 1.0;
   for (int i = 1; i <= MAX_P; ++i) t.fact[i] = t.fact[i - 1] * (double)i;
+  for (int i = 0; i <= MAX_P; ++i)
+    for (int j = 0; j <= MAX_P; ++j) t.rfact2[i][j] = 1.0 / (t.fact[i] * t.fact[j]);
   return t;
+}
+
+// a / d rounded to nearest, from r = RN(1 / d): q = RN(a r), the residual
+// a - q d is exact under FMA, and RN(q + (a - q d) r) = RN(a / d) (Markstein's
+// theorem; it holds for normal results, which is every use here). Three FP64
+// operations in place of a full division.
+__device__ __forceinline__ double div_exact(double a, double d, double r) {
+  const double q = a * r;
+  return fma(fma(-q, d, a), r, q);
 }
 
 static int upload_tabs(hipStream_t s) {
@@ -85,11 +97,11 @@ __global__ void k_sanitize(double* __restrict__ x, double* __restrict__ y, doubl
 // --------------------------------------------------------------- P2M ----
 
 // exact: one wave per leaf, bodies in order (galax-core/src/lib.rs:854-885;
-// coeff = mass*dx^i*dy^j/(i!*j!), non-finite -> 0). The wave first stages a
-// chunk of the leaf's (dx, dy, m) in LDS (dx = x - com computed once per body,
-// the same value every lane would compute), then each lane sums its
-// coefficient over the chunk in body order. With S <= 32 the two half-waves
-// take two leaves.
+// coeff = mass*dx^i*dy^j/(i!*j!), non-finite -> 0). The wave stages a chunk of
+// the leaf's bodies in LDS as m and the power tables dx^0..dx^P, dy^0..dy^P
+// (dx = x - com, powers by repeated products: the values every lane would
+// compute), then each lane sums its coefficient over the chunk in body order.
+// With S <= 32 the two half-waves take two leaves.
 template <int P>
 __global__ void __launch_bounds__(256) k_p2m(const double* __restrict__ x, const double* __restrict__ y,
                                              const double* __restrict__ m, const uint32_t* __restrict__ bs,
@@ -100,7 +112,7 @@ __global__ void __launch_bounds__(256) k_p2m(const double* __restrict__ x, const
   constexpr int S = num_moments(P);
   constexpr int LPW = S <= 32 ? 2 : 1;  // leaves per wave
   constexpr int LW = 64 / LPW;          // lanes per leaf = staging chunk
-  __shared__ double sdx[4][64], sdy[4][64], smb[4][64];
+  __shared__ double spx[4][64][P + 1], spy[4][64][P + 1], smb[4][64];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int sub = lane / LW, sl = lane % LW;
   const uint32_t leaf = (blockIdx.x * 4 + w) * LPW + sub;
@@ -114,31 +126,35 @@ __global__ void __launch_bounds__(256) k_p2m(const double* __restrict__ x, const
   if (lane == 0) blen[w] = len;
   __syncthreads();
   len = max(max(blen[0], blen[1]), max(blen[2], blen[3]));
-  double* dxs = &sdx[w][sub * LW];
-  double* dys = &sdy[w][sub * LW];
-  double* ms = &smb[w][sub * LW];
+  const int base = sub * LW;
   for (int k0 = 0; k0 < S; k0 += LW) {
     const int k = k0 + sl;
     int i = 0, j = 0;
     if (k < S) coef_ij<P>(k, i, j);
-    const double div = c_tabs.fact[i] * c_tabs.fact[j];
+    const double div = c_tabs.fact[i] * c_tabs.fact[j], rdiv = c_tabs.rfact2[i][j];
     double acc = 0.0;
     for (uint32_t c = 0; c < len; c += LW) {
       __syncthreads();
       const uint32_t b = s + c + sl;
       if (b < e) {
-        dxs[sl] = x[b] - cx;
-        dys[sl] = y[b] - cy;
-        ms[sl] = m[b];
+        const double dx = x[b] - cx, dy = y[b] - cy;
+        double px = 1.0, py = 1.0;
+        spx[w][base + sl][0] = 1.0;
+        spy[w][base + sl][0] = 1.0;
+#pragma unroll
+        for (int r = 1; r <= P; ++r) {
+          px = px * dx;
+          py = py * dy;
+          spx[w][base + sl][r] = px;
+          spy[w][base + sl][r] = py;
+        }
+        smb[w][base + sl] = m[b];
       }
       __syncthreads();
       const uint32_t n = e > s + c ? min((uint32_t)LW, e - s - c) : 0u;
       for (uint32_t q = 0; q < n; ++q) {
-        const double dx = dxs[q], dy = dys[q], mb = ms[q];
-        double px = 1.0, py = 1.0;
-        for (int r = 0; r < i; ++r) px = px * dx;
-        for (int r = 0; r < j; ++r) py = py * dy;
-        acc += mb * px * py / div;
+        const double t = smb[w][base + q] * spx[w][base + q][i] * spy[w][base + q][j];
+        acc += div_exact(t, div, rdiv);
       }
     }
     if (live && k < S) mp[(size_t)leaf * S + k] = isfinite(acc) ? acc : 0.0;
@@ -147,42 +163,87 @@ __global__ void __launch_bounds__(256) k_p2m(const double* __restrict__ x, const
 
 // --------------------------------------------------------------- M2M ----
 
-// exact: parents in [lo, hi), one thread per (parent, coefficient); children in
-// order, each child's shifted sum added to the parent (galax-core/src/lib.rs:894-932).
+// exact: parents in [lo, hi); children in order, each child's shifted sum
+// added to the parent (galax-core/src/lib.rs:894-932). One wave per parent,
+// or per two parents when S <= 32 (half-waves); lane = coefficient. For each
+// child the wave stages in LDS the child's multipole and the power tables
+// dx^a, dy^a (by repeated products, as each lane would compute them); the
+// factorial products and their reciprocals sit in LDS for the block.
 template <int P>
-__global__ void k_m2m(const uint32_t* __restrict__ cf, const uint8_t* __restrict__ cc, const double* __restrict__ comx,
-                      const double* __restrict__ comy, uint32_t lo, uint32_t hi, double* __restrict__ mp) {
+__global__ void __launch_bounds__(256) k_m2m(const uint32_t* __restrict__ cf, const uint8_t* __restrict__ cc,
+                                             const double* __restrict__ comx, const double* __restrict__ comy,
+                                             uint32_t lo, uint32_t hi, double* __restrict__ mp) {
 #pragma clang fp contract(off)
   constexpr int S = num_moments(P);
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)(hi - lo) * S) return;
-  uint32_t p = lo + (uint32_t)(t / S);
-  int k = (int)(t % S);
-  int oi, oj;
-  coef_ij<P>(k, oi, oj);
-  double acc = 0.0;
-  uint32_t c0 = cf[p];
-  for (uint32_t c = c0; c < c0 + cc[p]; ++c) {
-    double dx = comx[c] - comx[p], dy = comy[c] - comy[p];
-    double dxp[P + 1], dyp[P + 1];
-    dxp[0] = 1.0;
-    dyp[0] = 1.0;
+  constexpr int PPW = S <= 32 ? 2 : 1;  // parents per wave
+  constexpr int LW = 64 / PPW;          // lanes per parent
+  constexpr int Q = P + 1;
+  __shared__ double smc[4][PPW][S], sdx[4][PPW][Q], sdy[4][PPW][Q], sd[Q][Q], sr[Q][Q];
+  __shared__ uint32_t bmax[4];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int sub = lane / LW, sl = lane % LW;
+  for (int q = threadIdx.x; q < Q * Q; q += 256) {
+    sd[q / Q][q % Q] = c_tabs.fact[q / Q] * c_tabs.fact[q % Q];
+    sr[q / Q][q % Q] = c_tabs.rfact2[q / Q][q % Q];
+  }
+  const uint32_t p = lo + (blockIdx.x * 4 + w) * PPW + sub;
+  const bool live = p < hi;
+  const uint32_t c0 = live ? cf[p] : 0u, nc = live ? cc[p] : 0u;
+  const double px = live ? comx[p] : 0.0, py = live ? comy[p] : 0.0;
+  // children run in lockstep over the block: its largest child count sets the loop
+  uint32_t ncm = nc;
+  if (PPW == 2) ncm = max(ncm, (uint32_t)__shfl_xor((int)ncm, 32, 64));
+  if (lane == 0) bmax[w] = ncm;
+  __syncthreads();
+  ncm = max(max(bmax[0], bmax[1]), max(bmax[2], bmax[3]));
+  double acc[(S + LW - 1) / LW];
 #pragma unroll
-    for (int q = 1; q <= P; ++q) {
-      dxp[q] = dxp[q - 1] * dx;
-      dyp[q] = dyp[q - 1] * dy;
-    }
-    const double* mc = mp + (size_t)c * S;
-    double sum = 0.0;
-    for (int i = 0; i <= oi; ++i) {
-      for (int j = 0; j <= oj; ++j) {
-        if (i + j > P) continue;
-        sum += mc[moment_index(i, j)] * dxp[oi - i] * dyp[oj - j] / (c_tabs.fact[oi - i] * c_tabs.fact[oj - j]);
+  for (int r = 0; r < (S + LW - 1) / LW; ++r) acc[r] = 0.0;
+  for (uint32_t ci = 0; ci < ncm; ++ci) {
+    const bool has = ci < nc;
+    __syncthreads();
+    if (has) {
+      const uint32_t c = c0 + ci;
+      for (int k = sl; k < S; k += LW) smc[w][sub][k] = mp[(size_t)c * S + k];
+      if (sl < Q) {
+        const double dx = comx[c] - px, dy = comy[c] - py;
+        double ex = 1.0, ey = 1.0;
+        for (int r = 0; r < sl; ++r) {
+          ex = ex * dx;
+          ey = ey * dy;
+        }
+        sdx[w][sub][sl] = ex;
+        sdy[w][sub][sl] = ey;
       }
     }
-    acc += sum;
+    __syncthreads();
+    if (has) {
+      const double* mc = smc[w][sub];
+      const double* ex = sdx[w][sub];
+      const double* ey = sdy[w][sub];
+#pragma unroll
+      for (int r = 0; r < (S + LW - 1) / LW; ++r) {
+        const int k = sl + r * LW;
+        if (k < S) {
+          int oi, oj;
+          coef_ij<P>(k, oi, oj);
+          double sum = 0.0;
+          for (int i = 0; i <= oi; ++i)
+            for (int j = 0; j <= oj; ++j)
+              sum += div_exact(mc[moment_index(i, j)] * ex[oi - i] * ey[oj - j], sd[oi - i][oj - j],
+                               sr[oi - i][oj - j]);
+          acc[r] += sum;
+        }
+      }
+    }
   }
-  mp[(size_t)p * S + k] = acc;
+  if (live) {
+#pragma unroll
+    for (int r = 0; r < (S + LW - 1) / LW; ++r) {
+      const int k = sl + r * LW;
+      if (k < S) mp[(size_t)p * S + k] = acc[r];
+    }
+  }
 }
 
 // ----------------------------------------------- kernel derivatives ----
@@ -565,28 +626,6 @@ __device__ __forceinline__ void l2p_body(const double* __restrict__ L, double du
   *fy = ay;
   *fphi = ph;
 }
-
-// Near-field arithmetic type: FP64 or FP32 (v_rsq_f32) on coordinates
-// relative to the target leaf's COM. FP64: the v_rsq_f64 estimate y with one
-// second-order refinement, y (1 + e/2 + 3e^2/8) with e = 1 - v y^2; no
-// fix-ups for v = 0 or inf (v >= eps^2 > 0 on every unmasked use; with eps = 0
-// two coincident bodies give a non-finite force, as 1/sqrt(0) does).
-template <typename T>
-struct Near;
-template <>
-struct Near<double> {
-  typedef double4 V4;
-  static __device__ __forceinline__ double rsq(double v) {
-    const double y = __builtin_amdgcn_rsq(v);
-    const double e = fma(-v * y, y, 1.0);
-    return fma(y * e, fma(e, 0.375, 0.5), y);
-  }
-};
-template <>
-struct Near<float> {
-  typedef float4 V4;
-  static __device__ __forceinline__ float rsq(float v) { return __builtin_amdgcn_rsqf(v); }
-};
 
 // Near-field sum of one target body over the staged sources half, half + 2,
 // ... < n (n even: the chunk is padded with a far zero-mass source). The loop
@@ -1044,8 +1083,8 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   {
     for (uint32_t l = 1; l < top; ++l) {
       uint32_t lo = t.level_offset[l], hi = t.level_offset[l + 1];
-      k_m2m<P><<<nblk((size_t)(hi - lo) * S), 256, 0, s>>>(t.child_first.as<uint32_t>(), t.child_count.as<uint8_t>(),
-                                                          comx, comy, lo, hi, mp);
+      k_m2m<P><<<nblk(hi - lo, num_moments(P) <= 32 ? 8 : 4), 256, 0, s>>>(
+          t.child_first.as<uint32_t>(), t.child_count.as<uint8_t>(), comx, comy, lo, hi, mp);
       GX_HIP(hipGetLastError());
     }
   }

@@ -51,53 +51,60 @@ __device__ __forceinline__ double dbinom(int n, int k) {
 // Adjoint of the near field for the targets j of one P2P work item (a target
 // leaf and at most P2P_ITEM_LEAVES of its gather list), as the forward P2P
 // kernel is organised: the item's sources are staged through LDS in chunks of
-// VJP_CHUNK as (x, y, m, abar_x) and abar_y; lanes 0-31 and 32-63 hold the
-// same 32 targets and take the even / odd sources (the chunk is padded to an
-// even length with a zero-mass, zero-cotangent source far away); only the
-// chunks holding the target leaf itself run with the self mask. With
-// d = x_i - x_j and v = m_j abar_i - m_i abar_j (i the source):
+// VJP_CHUNK as (x, y, m, abar_x) and abar_y, in T precision with x, y relative
+// to the target leaf's COM; lanes 0-31 and 32-63 hold the same 32 targets and
+// take the even / odd sources (the chunk is padded to an even length with a
+// zero-mass, zero-cotangent source far away); only the chunks holding the
+// target leaf itself run with the self mask. With d = x_i - x_j and
+// v = m_j abar_i - m_i abar_j (i the source):
 //   xbar_j += v / r^3 - 3 d (d . v) / r^5,   mbar_j -= abar_i . d / r^3.
+// T = float for the mixed forms (the precision of their forward near field).
 constexpr int VJP_CHUNK = 256;
-template <bool MASK>
-__device__ __forceinline__ void vjp_near_sum(const double4* __restrict__ sa, const double* __restrict__ sb, int half,
-                                             int n, int self, double xj, double yj, double mj, double abx,
-                                             double aby, double eps2, double& gx, double& gy, double& gm) {
-  const double4* ah = sa + half;
-  const double* bh = sb + half;
+template <typename T, bool MASK>
+__device__ __forceinline__ void vjp_near_sum(const typename Near<T>::V4* __restrict__ sa, const T* __restrict__ sb,
+                                             int half, int n, int self, T xj, T yj, T mj, T abx, T aby, T eps2,
+                                             T& gx, T& gy, T& gm) {
+  const typename Near<T>::V4* ah = sa + half;
+  const T* bh = sb + half;
 #pragma unroll 2
   for (int jj = 0; jj < n; jj += 2) {
-    const double4 s = ah[jj];
-    const double sby = bh[jj];
-    const double dx = s.x - xj, dy = s.y - yj;
-    const double r2 = fma(dx, dx, fma(dy, dy, eps2));
-    double inv = rsqrt(r2);
-    if (MASK) inv = jj + half == self ? 0.0 : inv;
-    const double inv2 = inv * inv, inv3 = inv2 * inv, inv5 = inv3 * inv2;
-    const double vx = fma(mj, s.w, -s.z * abx), vy = fma(mj, sby, -s.z * aby);
-    const double t = 3.0 * fma(dx, vx, dy * vy) * inv5;
+    const typename Near<T>::V4 s = ah[jj];
+    const T sby = bh[jj];
+    const T dx = s.x - xj, dy = s.y - yj;
+    const T r2 = fma(dx, dx, fma(dy, dy, eps2));
+    T inv = Near<T>::rsq(r2);
+    if (MASK) inv = jj + half == self ? (T)0 : inv;
+    const T inv2 = inv * inv, inv3 = inv2 * inv, inv5 = inv3 * inv2;
+    const T vx = fma(mj, s.w, -s.z * abx), vy = fma(mj, sby, -s.z * aby);
+    const T t = (T)3 * fma(dx, vx, dy * vy) * inv5;
     gx += fma(vx, inv3, -dx * t);
     gy += fma(vy, inv3, -dy * t);
     gm -= fma(s.w, dx, sby * dy) * inv3;
   }
 }
 
+template <typename T>
 __global__ void __launch_bounds__(64) k_vjp_near(const double* __restrict__ x, const double* __restrict__ y,
                                                  const double* __restrict__ m, const double* __restrict__ axb,
                                                  const double* __restrict__ ayb, const uint32_t* __restrict__ bs,
-                                                 const uint32_t* __restrict__ be, const uint32_t* __restrict__ gdat,
+                                                 const uint32_t* __restrict__ be, const double* __restrict__ comx,
+                                                 const double* __restrict__ comy, const uint32_t* __restrict__ gdat,
                                                  const uint32_t* __restrict__ itgt, const uint32_t* __restrict__ ig0,
                                                  const uint32_t* __restrict__ ig1, const uint32_t* __restrict__ islot,
-                                                 uint32_t n_max, double eps2, double* __restrict__ xb,
+                                                 uint32_t n_max, double eps2d, double* __restrict__ xb,
                                                  double* __restrict__ yb, double* __restrict__ mb,
                                                  double* __restrict__ part) {
+  typedef typename Near<T>::V4 V4;
   static_assert(P2P_ITEM_LEAVES <= 64, "one lane per source leaf of an item");
-  __shared__ double4 sa[VJP_CHUNK + 1];
-  __shared__ double sb[VJP_CHUNK + 1];
+  __shared__ V4 sa[VJP_CHUNK + 1];
+  __shared__ T sb[VJP_CHUNK + 1];
   __shared__ uint32_t l_ss[P2P_ITEM_LEAVES], l_off[P2P_ITEM_LEAVES];
   const uint32_t item = blockIdx.x;
   const uint32_t leaf = itgt[item], g0 = ig0[item], g1 = ig1[item], slot = islot[item];
   const int lane = threadIdx.x, tl = lane & 31, half = lane >> 5;
   const uint32_t ts = bs[leaf], te = be[leaf];
+  const double ox = comx[leaf], oy = comy[leaf];
+  const T eps2 = (T)eps2d;
   // source leaves of the item: start and offset in the item's source order
   const int nlv = (int)(g1 - g0);
   uint32_t cnt = 0, ss = 0;
@@ -118,7 +125,7 @@ __global__ void __launch_bounds__(64) k_vjp_near(const double* __restrict__ x, c
     l_ss[lane] = ss;
     l_off[lane] = inc - cnt;
   }
-  const uint32_t total = __shfl(inc, 63, 64);
+  const uint32_t total = __builtin_amdgcn_readlane(inc, 63);  // uniform: scalar loop bounds below
   const unsigned long long sm_ = __ballot(is_self);
   uint32_t self_lo = 0, self_hi = 0;
   if (sm_) {
@@ -128,9 +135,10 @@ __global__ void __launch_bounds__(64) k_vjp_near(const double* __restrict__ x, c
   for (uint32_t tb = ts; tb < te; tb += 32) {
     const uint32_t j = tb + tl;
     const bool valid = j < te;
-    const double xj = valid ? x[j] : 0.0, yj = valid ? y[j] : 0.0, mj = valid ? m[j] : 0.0;
-    const double abx = valid ? axb[j] : 0.0, aby = valid ? ayb[j] : 0.0;
-    double gx_ = 0.0, gy_ = 0.0, gm_ = 0.0;
+    // lanes past the leaf's end sit at its COM with zero mass and cotangent
+    const T xj = valid ? (T)(x[j] - ox) : (T)0, yj = valid ? (T)(y[j] - oy) : (T)0;
+    const T mj = valid ? (T)m[j] : (T)0, abx = valid ? (T)axb[j] : (T)0, aby = valid ? (T)ayb[j] : (T)0;
+    T gx_ = 0, gy_ = 0, gm_ = 0;
     for (uint32_t c0 = 0; c0 < total; c0 += VJP_CHUNK) {
       const int n = (int)min((uint32_t)VJP_CHUNK, total - c0);
       __syncthreads();
@@ -145,36 +153,44 @@ __global__ void __launch_bounds__(64) k_vjp_near(const double* __restrict__ x, c
             hi = mid - 1;
         }
         const uint32_t g = l_ss[lo] + (q - l_off[lo]);
-        sa[pp] = make_double4(x[g], y[g], m[g], axb[g]);
-        sb[pp] = ayb[g];
+        V4 v;
+        v.x = (T)(x[g] - ox);
+        v.y = (T)(y[g] - oy);
+        v.z = (T)m[g];
+        v.w = (T)axb[g];
+        sa[pp] = v;
+        sb[pp] = (T)ayb[g];
       }
       if (lane == 0 && (n & 1)) {  // pad to even: zero mass and cotangent, far away
-        sa[n] = make_double4(3e19, 3e19, 0.0, 0.0);
-        sb[n] = 0.0;
+        V4 v;
+        v.x = (T)3e19;
+        v.y = (T)3e19;
+        v.z = (T)0;
+        v.w = (T)0;
+        sa[n] = v;
+        sb[n] = (T)0;
       }
       __syncthreads();
-      if (valid) {
-        const int ne = (n + 1) & ~1;
-        if (c0 < self_hi && c0 + (uint32_t)n > self_lo)
-          vjp_near_sum<true>(sa, sb, half, ne, (int)(self_lo + (j - ts)) - (int)c0, xj, yj, mj, abx, aby, eps2, gx_,
-                             gy_, gm_);
-        else
-          vjp_near_sum<false>(sa, sb, half, ne, 0, xj, yj, mj, abx, aby, eps2, gx_, gy_, gm_);
-      }
+      const int ne = (n + 1) & ~1;
+      if (c0 < self_hi && c0 + (uint32_t)n > self_lo)
+        vjp_near_sum<T, true>(sa, sb, half, ne, (int)(self_lo + (j - ts)) - (int)c0, xj, yj, mj, abx, aby, eps2,
+                              gx_, gy_, gm_);
+      else
+        vjp_near_sum<T, false>(sa, sb, half, ne, 0, xj, yj, mj, abx, aby, eps2, gx_, gy_, gm_);
     }
     gx_ += __shfl_xor(gx_, 32, 64);
     gy_ += __shfl_xor(gy_, 32, 64);
     gm_ += __shfl_xor(gm_, 32, 64);
     if (valid && half == 0) {
       if (slot == NONE) {
-        xb[j] = gx_;
-        yb[j] = gy_;
-        mb[j] = gm_;
+        xb[j] = (double)gx_;
+        yb[j] = (double)gy_;
+        mb[j] = (double)gm_;
       } else {
         double* pp = part + ((size_t)slot * n_max + (j - ts)) * 3;
-        pp[0] = gx_;
-        pp[1] = gy_;
-        pp[2] = gm_;
+        pp[0] = (double)gx_;
+        pp[1] = (double)gy_;
+        pp[2] = (double)gm_;
       }
     }
   }
@@ -472,10 +488,18 @@ static int vjp_impl(galax_fmm* h, const double* x, const double* y, const double
   const P2PItems& it = corrected ? h->c_items : t.items;
   const uint32_t nmax = h->cfg.n_max;
   GX_CHECK(dbuf_reserve(h->p2p_part, std::max<size_t>(it.n_part, 1) * nmax * 3 * 8, s));
-  k_vjp_near<<<(unsigned)it.n_items, 64, 0, s>>>(x, y, m, axb, ayb, t.body_start.as<uint32_t>(),
-                                                 t.body_end.as<uint32_t>(), gdat, it.tgt.as<uint32_t>(),
-                                                 it.g0.as<uint32_t>(), it.g1.as<uint32_t>(), it.slot.as<uint32_t>(),
-                                                 nmax, eps2, xb, yb, mb, h->p2p_part.as<double>());
+  const bool fp32_near =
+      h->cfg.precision == GALAX_PREC_MIXED_FP32 || h->cfg.precision == GALAX_PREC_MIXED_BF16X3;
+#define GX_LAUNCH_VJP_NEAR(T)                                                                                  \
+  k_vjp_near<T><<<(unsigned)it.n_items, 64, 0, s>>>(                                                          \
+      x, y, m, axb, ayb, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), t.com_x.as<double>(),          \
+      t.com_y.as<double>(), gdat, it.tgt.as<uint32_t>(), it.g0.as<uint32_t>(), it.g1.as<uint32_t>(),            \
+      it.slot.as<uint32_t>(), nmax, eps2, xb, yb, mb, h->p2p_part.as<double>())
+  if (fp32_near)
+    GX_LAUNCH_VJP_NEAR(float);
+  else
+    GX_LAUNCH_VJP_NEAR(double);
+#undef GX_LAUNCH_VJP_NEAR
   GX_HIP(hipGetLastError());
   GX_HIP(p2p_combine(it, goff, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), nmax,
                      h->p2p_part.as<double>(), xb, yb, mb, s));

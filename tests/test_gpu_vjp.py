@@ -51,6 +51,29 @@ def test_vjp_matches_central_differences(algo):
     assert worst < 1e-5, worst
 
 
+@pytest.mark.parametrize("algo", [fmm.ALGO_REFERENCE, fmm.ALGO_CORRECTED])
+def test_mixed_form_vjp_matches_fp64_vjp(algo):
+    """The mixed forms run the near-field adjoint in FP32 on leaf-relative
+    coordinates (their forward near field's precision) and the rest in FP64:
+    the cotangents must match the FP64 form's to 1e-5 in norm."""
+    d = ic.plummer(8192, seed=405)
+    r = ic.SplitMix64(11)
+    axb, ayb = r.normal(len(d["x"])), r.normal(len(d["x"]))
+    out = {}
+    for prec in (fmm.PREC_FP64, fmm.PREC_MIXED_FP32, fmm.PREC_MIXED_BF16X3):
+        n = len(d["x"])
+        ctx = fmm.GpuContext(fmm.GpuConfig(max_n=n, p=6, eps=0.05, n_max=32, bounds=(-50.0, 50.0, -50.0, 50.0),
+                                           algo=algo, theta=0.3, precision=prec))
+        b = fmm.BodiesSoA.from_dict(d)
+        ctx.build_tree(b)
+        _forward(ctx, b)
+        out[prec] = ctx.vjp(b, axb, ayb)
+    for prec in (fmm.PREC_MIXED_FP32, fmm.PREC_MIXED_BF16X3):
+        for g, g64 in zip(out[prec], out[fmm.PREC_FP64]):
+            rel = np.linalg.norm(g - g64) / np.linalg.norm(g64)
+            assert rel <= 1e-5, (prec, rel)
+
+
 def test_vjp_single_leaf_equals_direct_sum_adjoint():
     """With all bodies in one leaf the FMM is the direct sum: the VJP must equal
     the closed-form adjoint of the direct sum."""
