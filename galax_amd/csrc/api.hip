@@ -323,7 +323,7 @@ void galax_fmm_destroy(galax_fmm_t* h) {
   DBuf* bufs[] = {&h->x, &h->y, &h->m, &h->vx, &h->vy, &h->ax, &h->ay, &h->phi, &h->mp, &h->loc, &h->mp32,
                   &h->loc32, &h->gtab, &h->key_of_pair, &h->perm, &h->hash, &h->tmp_u32, &h->flag,
                   &h->c_m2l_off, &h->c_m2l_dat, &h->c_gat_off, &h->c_gat_dat, &h->c_src_pairs,
-                  &h->key_dxy, &h->mx_t64, &h->mx_that, &h->mx_sig,
+                  &h->key_dxy, &h->mx_t64, &h->mx_that, &h->mx_sig, &h->mx_zero,
                   &h->vj_lbar, &h->vj_mbar, &h->vj_leaf_of, &h->vj_fwd, &h->vj_out,
                   &h->mx_fwd.src, &h->mx_fwd.tile_off, &h->mx_fwd.step_key, &h->mx_fwd.step_first,
                   &h->mx_fwd.step_cnt, &h->mx_fwd.step_ids, &h->mx_fwd.item_tile, &h->mx_fwd.item_s0, &h->mx_fwd.item_s1,

@@ -125,7 +125,7 @@ struct galax_fmm {
   // M2L on the matrix cores (m2l_mfma.hip): forward tiles over targets,
   // adjoint tiles over sources, and the per-key operators
   gx::M2LTiles mx_fwd, mx_adj;
-  gx::DBuf mx_t64, mx_that, mx_sig;
+  gx::DBuf mx_t64, mx_that, mx_sig, mx_zero;  // mx_zero: one zero source row (missing ids read it)
   bool mx_valid = false, mx_adj_valid = false, mx_t64_valid = false;
   // adjoint (VJP) buffers
   gx::DBuf vj_lbar, vj_mbar, vj_leaf_of, vj_fwd, vj_out;

@@ -241,7 +241,7 @@ __global__ void __launch_bounds__(256) k_m2m(const uint32_t* __restrict__ cf, co
 #pragma unroll
     for (int r = 0; r < (S + LW - 1) / LW; ++r) {
       const int k = sl + r * LW;
-      if (k < S) mp[(size_t)p * S + k] = acc[r];
+      if (k < S) mp[(size_t)p * S + k] = isfinite(acc[r]) ? acc[r] : 0.0;  // non-finite -> 0 (galax-core/src/lib.rs:934-936)
     }
   }
 }

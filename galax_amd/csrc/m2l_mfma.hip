@@ -37,6 +37,8 @@
 #include <algorithm>
 #include <cmath>
 
+#include <type_traits>
+
 #include "common.h"
 
 namespace gx {
@@ -307,17 +309,19 @@ __global__ void k_mx_operators64(const double* __restrict__ gtab, uint32_t nk, d
 // n = 0..P+1 (FP64)
 template <int P>
 __global__ void k_mx_operators32(const double* __restrict__ gtab, const double* __restrict__ dxy, uint32_t nk,
-                                 double eps2, float* __restrict__ that, double* __restrict__ sig_inv) {
+                                 double eps2, float* __restrict__ that, int* __restrict__ sig_exp) {
   constexpr int S = num_moments(P), K = 16 * row_blocks(P), ST = 2 * P + 1, GS = ST * ST;
   uint32_t key = blockIdx.x;
   if (key >= nk) return;
   const double dx = dxy[2 * key], dy = dxy[2 * key + 1];
   double sig = sqrt(dx * dx + dy * dy + eps2);
   if (!(sig > 0.0) || !isfinite(sig)) sig = 1.0;
+  // the scale is the power of two 2^a nearest sig, so scaling is exact
+  constexpr int AMAX = 1000 / (2 * P + 2);
+  const int ae = min(max((int)rint(log2(sig)), -AMAX), AMAX);
   double pw[2 * P + 2];
-  pw[0] = 1.0;
-  for (int q = 1; q < 2 * P + 2; ++q) pw[q] = pw[q - 1] * sig;
-  if (threadIdx.x < P + 2) sig_inv[(size_t)key * (P + 2) + threadIdx.x] = 1.0 / pw[threadIdx.x];
+  for (int q = 0; q < 2 * P + 2; ++q) pw[q] = ldexp(1.0, ae * q);
+  if (threadIdx.x == 0) sig_exp[key] = ae;
   const double* G = gtab + (size_t)key * GS;
   double fact[P + 1];
   fact[0] = 1.0;
@@ -355,9 +359,10 @@ struct TileArgs {
   const uint32_t *step_key, *step_ids;
   const double* t64;
   const float* that;
-  const double* sig_inv;
+  const int* sig_exp;  // per key: operators scaled by 2^(a (n + m + 1)), n, m the orders
   const double* in;  // multipoles (forward) or lbar (adjoint), S per node
   const double* virt;  // pre-summed long runs, S per run; ids n_in + r
+  const double* zero;  // a row of zeros, read for missing ids
   uint32_t n_in;
   double* out;       // locals (forward) or mbar (adjoint), S per row
   double* part;      // partials of tiles with several items: [slot][16][S]
@@ -380,10 +385,28 @@ __global__ void __launch_bounds__(256) k_m2l_tiles(TileArgs a) {
   const uint32_t item = (blockIdx.x % 8) * a.per_xcd + blockIdx.x / 8;
   if (item >= a.nitems) return;  // uniform over the block
   const uint32_t tile = a.item_tile[item], s0 = a.item_s0[item], s1 = a.item_s1[item], slot = a.item_slot[item];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 15, kq = lane >> 4;
+  // w is wave-uniform: readfirstlane lets the step records and operator addresses live in scalar registers
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63, col = lane & 15, kq = lane >> 4;
   d4 L[RB];
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb) L[rb] = (d4){0.0, 0.0, 0.0, 0.0};
+  // per-lane orders of the B coefficients and of the D rows, 5 bits each
+  // (the power-of-two scales of the mixed forms)
+  constexpr int NOUT = RB * 4;
+  uint32_t ord_in[(KL + 5) / 6], ord_out[(NOUT + 5) / 6];
+#pragma unroll
+  for (int q = 0; q < (KL + 5) / 6; ++q) ord_in[q] = 0;
+#pragma unroll
+  for (int q = 0; q < (NOUT + 5) / 6; ++q) ord_out[q] = 0;
+  if (MODE != MX_F64) {
+#pragma unroll
+    for (int e = 0; e < KL; ++e) ord_in[e / 6] |= (uint32_t)order_of<P>(LN::k_of(e, kq)) << (5 * (e % 6));
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        ord_out[(rb * 4 + j) / 6] |= (uint32_t)(order_of<P>(rb * 16 + 4 * kq + j) + 1) << (5 * ((rb * 4 + j) % 6));
+  }
   uint32_t st = s0 + w;
   // the step record (key, and the source ids of this lane's column) is
   // loaded one step ahead, so each step waits only on its multipole rows
@@ -405,88 +428,121 @@ __global__ void __launch_bounds__(256) k_m2l_tiles(TileArgs a) {
 #pragma unroll
       for (uint32_t q = 0; q < MX_GROUP; ++q) id_n[q] = a.step_ids[((size_t)(st + 4) * MX_GROUP + q) * 16 + col];
     }
-    double bs[KL];
-#pragma unroll
-    for (int e = 0; e < KL; ++e) bs[e] = 0.0;
-#pragma unroll
-    for (uint32_t q = 0; q < MX_GROUP; ++q) {  // a group's sources in list order
-      if (id[q] != NONE) {
-        const double* row = id[q] < a.n_in ? a.in + (size_t)id[q] * S : a.virt + (size_t)(id[q] - a.n_in) * S;
-#pragma unroll
-        for (int e = 0; e < KL; ++e) {
-          const int k = LN::k_of(e, kq);
-          if (k < S) {
-            const double v = row[k];
-            bs[e] += isfinite(v) ? v : 0.0;
-          }
-        }
-      }
-    }
+    // every load of the step is issued before any is used: the operator rows
+    // (their address needs only the key), then the source rows (a missing id
+    // reads row 0 and is masked; padding coefficients k >= S read k = S - 1)
+    constexpr int KC = LN::KC;
+    constexpr int TW = MODE == MX_BF16X3 ? KC * 8 : KS;
+    typedef typename std::conditional<MODE == MX_F64, double, float>::type TT;
+    TT tv[RB][TW];
     if (MODE == MX_F64) {
       const double* T = a.t64 + (size_t)key * K * K;
 #pragma unroll
-      for (int rb = 0; rb < RB; ++rb) {
+      for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-        for (int e = 0; e < KS; ++e) {
-          const double av = TR ? T[(size_t)(kq * KS + e) * K + rb * 16 + col] : T[(size_t)(rb * 16 + col) * K + kq * KS + e];
-          L[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bs[e], L[rb], 0, 0, 0);
-        }
-      }
+        for (int e = 0; e < KS; ++e)
+          tv[rb][e] = (TT)(TR ? T[(size_t)(kq * KS + e) * K + rb * 16 + col] : T[(size_t)(rb * 16 + col) * K + kq * KS + e]);
     } else {
-      const double* sg = a.sig_inv + (size_t)key * (P + 2);
       const float* T = a.that + (size_t)key * K * K;
-      if (MODE == MX_F32) {
-        float b[KS];
 #pragma unroll
-        for (int e = 0; e < KS; ++e) b[e] = (float)(bs[e] * sg[order_of<P>(LN::k_of(e, kq))]);
+      for (int rb = 0; rb < RB; ++rb) {
+        const float* Tr = T + (size_t)(rb * 16 + col) * K;
+        if (MODE == MX_F32) {
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb) {
-          const float* Tr = T + (size_t)(rb * 16 + col) * K + kq * KS;
-          f4 acc = {0.f, 0.f, 0.f, 0.f};
+          for (int e = 0; e < KS; ++e) tv[rb][e] = (TT)Tr[kq * KS + e];
+        } else {
 #pragma unroll
-          for (int e = 0; e < KS; ++e) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(Tr[e], b[e], acc, 0, 0, 0);
-#pragma unroll
-          for (int j = 0; j < 4; ++j) L[rb][j] += (double)acc[j] * sg[order_of<P>(rb * 16 + 4 * kq + j) + 1];
-        }
-      } else {
-        constexpr int KC = LN::KC;
-        bf8 bh[KC], bm[KC], bl[KC];
-#pragma unroll
-        for (int e = 0; e < KL; ++e) {
-          const float v = (float)(bs[e] * sg[order_of<P>(LN::k_of(e, kq))]);
-          __bf16 hh, mm, ll;
-          split3(v, hh, mm, ll);
-          bh[e >> 3][e & 7] = hh;
-          bm[e >> 3][e & 7] = mm;
-          bl[e >> 3][e & 7] = ll;
-        }
-#pragma unroll
-        for (int rb = 0; rb < RB; ++rb) {
-          const float* Tr = T + (size_t)(rb * 16 + col) * K;
-          f4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-          for (int kc = 0; kc < KC; ++kc) {
-            bf8 ah, am, al;
+          for (int kc = 0; kc < KC; ++kc)
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
               const int k = kc * 32 + kq * 8 + e;
-              __bf16 hh, mm, ll;
-              split3(k < K ? Tr[k] : 0.f, hh, mm, ll);
-              ah[e] = hh;
-              am[e] = mm;
-              al[e] = ll;
+              tv[rb][kc * 8 + e] = (TT)(k < K ? Tr[k] : 0.f);
             }
-            // small terms first
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm[kc], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[kc], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[kc], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh[kc], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm[kc], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[kc], acc, 0, 0, 0);
-          }
-#pragma unroll
-          for (int j = 0; j < 4; ++j) L[rb][j] += (double)acc[j] * sg[order_of<P>(rb * 16 + 4 * kq + j) + 1];
         }
+      }
+    }
+    const int sx = MODE == MX_F64 ? 0 : a.sig_exp[key];
+    // missing ids and padding coefficients k >= S load nothing (inactive
+    // lanes make no cache accesses); the rows are finite (P2M and M2M map
+    // non-finite sums to 0)
+    double v[MX_GROUP][KL];
+#pragma unroll
+    for (uint32_t q = 0; q < MX_GROUP; ++q) {
+      const uint32_t iq = id[q];
+#pragma unroll
+      for (int e = 0; e < KL; ++e) v[q][e] = 0.0;
+      if (iq != NONE) {
+        const double* row = iq < a.n_in ? a.in + (size_t)iq * S : a.virt + (size_t)(iq - a.n_in) * S;
+#pragma unroll
+        for (int e = 0; e < KL; ++e) {
+          const int k = LN::k_of(e, kq);
+          if (k < S) v[q][e] = row[k];
+        }
+      }
+    }
+    double bs[KL];
+#pragma unroll
+    for (int e = 0; e < KL; ++e) {  // a group's sources in list order
+      bs[e] = v[0][e];
+#pragma unroll
+      for (uint32_t q = 1; q < MX_GROUP; ++q) bs[e] += v[q][e];
+    }
+    if (MODE == MX_F64) {
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+        for (int e = 0; e < KS; ++e)
+          L[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)tv[rb][e], bs[e], L[rb], 0, 0, 0);
+      }
+    } else if (MODE == MX_F32) {
+      float b[KS];
+#pragma unroll
+      for (int e = 0; e < KS; ++e) b[e] = (float)ldexp(bs[e], -sx * (int)((ord_in[e / 6] >> (5 * (e % 6))) & 31u));
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) {
+        f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int e = 0; e < KS; ++e) acc = __builtin_amdgcn_mfma_f32_16x16x4f32((float)tv[rb][e], b[e], acc, 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          L[rb][j] += ldexp((double)acc[j], -sx * (int)((ord_out[(rb * 4 + j) / 6] >> (5 * ((rb * 4 + j) % 6))) & 31u));
+      }
+    } else {
+      bf8 bh[KC], bm[KC], bl[KC];
+#pragma unroll
+      for (int e = 0; e < KL; ++e) {
+        const float vv = (float)ldexp(bs[e], -sx * (int)((ord_in[e / 6] >> (5 * (e % 6))) & 31u));
+        __bf16 hh, mm, ll;
+        split3(vv, hh, mm, ll);
+        bh[e >> 3][e & 7] = hh;
+        bm[e >> 3][e & 7] = mm;
+        bl[e >> 3][e & 7] = ll;
+      }
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) {
+        f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc) {
+          bf8 ah, am, al;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            __bf16 hh, mm, ll;
+            split3((float)tv[rb][kc * 8 + e], hh, mm, ll);
+            ah[e] = hh;
+            am[e] = mm;
+            al[e] = ll;
+          }
+          // small terms first
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm[kc], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[kc], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[kc], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh[kc], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm[kc], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[kc], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          L[rb][j] += ldexp((double)acc[j], -sx * (int)((ord_out[(rb * 4 + j) / 6] >> (5 * ((rb * 4 + j) % 6))) & 31u));
       }
     }
   }
@@ -730,10 +786,10 @@ static int setup_impl(galax_fmm* h) {
     GX_CHECK(operators64<P>(h));
   } else if (nk) {
     GX_CHECK(dbuf_reserve(h->mx_that, (size_t)nk * K * K * 4, s));
-    GX_CHECK(dbuf_reserve(h->mx_sig, (size_t)nk * (P + 2) * 8, s));
+    GX_CHECK(dbuf_reserve(h->mx_sig, (size_t)nk * 4, s));
     k_mx_operators32<P><<<nk, 256, 0, s>>>(h->gtab.as<double>(), h->key_dxy.as<double>(), nk,
                                            h->cfg.softening * h->cfg.softening, h->mx_that.as<float>(),
-                                           h->mx_sig.as<double>());
+                                           h->mx_sig.as<int>());
     GX_HIP(hipGetLastError());
   }
   GX_HIP(hipStreamSynchronize(s));
@@ -788,9 +844,14 @@ static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
     GX_HIP(hipGetLastError());
   }
   a.virt = T.virt.as<double>();
+  if (!h->mx_zero.p) {
+    GX_CHECK(dbuf_reserve(h->mx_zero, 16 * row_blocks(MAX_P) * 8, s));
+    GX_HIP(hipMemsetAsync(h->mx_zero.p, 0, 16 * row_blocks(MAX_P) * 8, s));
+  }
+  a.zero = h->mx_zero.as<double>();
   a.t64 = h->mx_t64.as<double>();
   a.that = h->mx_that.as<float>();
-  a.sig_inv = h->mx_sig.as<double>();
+  a.sig_exp = h->mx_sig.as<int>();
   a.in = in;
   a.out = out;
   a.part = T.part.as<double>();
