@@ -22,6 +22,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import os
+import time
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -34,7 +35,7 @@ TREE_PLAIN, TREE_CONSTRAINED = 0, 1
 PREC_FP64, PREC_MIXED_FP32, PREC_MIXED_BF16X3, PREC_FP64_EXACT = 0, 1, 2, 3
 
 STATUS = {0: "OK", -1: "InitFailed", -2: "ShaderCompilation", -3: "BufferAllocation", -4: "InvalidArgument",
-          -5: "DeviceError", -6: "Unsupported", -7: "ThresholdExceeded"}
+          -5: "DeviceError", -6: "Unsupported", -7: "ThresholdExceeded", -8: "Io", -9: "Format"}
 
 
 class GalaxError(RuntimeError):
@@ -109,6 +110,9 @@ def lib():
         "galax_fmm_vjp_device": [vp] * 9,
         "galax_direct_sum": [vp, vp, vp, sz, d, vp, sz, vp, vp, vp],
         "galax_fmm_validate": [vp, C.POINTER(_Bodies), vp, sz, vp, vp, vp],
+        "galax_snapshot_write": [C.c_char_p, C.POINTER(_Bodies), d],
+        "galax_snapshot_info": [C.c_char_p, vp, vp],
+        "galax_snapshot_read": [C.c_char_p, C.POINTER(_Bodies)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -417,3 +421,31 @@ def default_dt(ax: np.ndarray, ay: np.ndarray) -> float:
 
 def version() -> str:
     return lib().galax_fmm_version().decode()
+
+
+# ------------------------------------------------------------ snapshots ----
+
+def write_snapshot(path, bodies: BodiesSoA, timestamp: float | None = None) -> None:
+    """galax-io write_snapshot (galax-io/src/lib.rs:21-42): a GALX v1 file with
+    x, y, vx, vy, mass, ax, ay (phi is not stored). The reference stamps the
+    current time; pass `timestamp` to fix it."""
+    ts = time.time() if timestamp is None else float(timestamp)
+    cb = bodies._c()
+    _check(lib().galax_snapshot_write(os.fsencode(path), C.byref(cb), ts))
+
+
+def snapshot_info(path) -> tuple[int, float]:
+    """(body count, timestamp) of a GALX file."""
+    n, ts = C.c_size_t(0), C.c_double(0.0)
+    _check(lib().galax_snapshot_info(os.fsencode(path), C.byref(n), C.byref(ts)))
+    return n.value, ts.value
+
+
+def read_snapshot(path) -> BodiesSoA:
+    """galax-io read_snapshot (galax-io/src/lib.rs:53-84). Raises GalaxError with
+    code -9 (Format: wrong magic, short file), -6 (Unsupported version) or -8 (Io)."""
+    n, _ = snapshot_info(path)
+    b = BodiesSoA.new(n, with_phi=False)
+    cb = b._c()
+    _check(lib().galax_snapshot_read(os.fsencode(path), C.byref(cb)))
+    return b

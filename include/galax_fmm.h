@@ -42,7 +42,9 @@ typedef enum {
   GALAX_ERR_INVALID = -4,     /* bad argument, size above capacity, no tree yet */
   GALAX_ERR_DEVICE = -5,      /* a HIP call or kernel failed */
   GALAX_ERR_UNSUPPORTED = -6, /* a combination this build does not provide */
-  GALAX_ERR_THRESHOLD = -7    /* crosscheck error above its threshold */
+  GALAX_ERR_THRESHOLD = -7,   /* crosscheck error above its threshold */
+  GALAX_ERR_IO = -8,          /* SnapshotError::Io (galax-io/src/lib.rs:10-11) */
+  GALAX_ERR_FORMAT = -9       /* SnapshotError::BadMagic / UnexpectedEof (galax-io/src/lib.rs:12-17) */
 } galax_status_t;
 
 /* Algorithm variants (same tree, same kernels). */
@@ -211,9 +213,11 @@ GALAX_API int galax_fmm_kdk(galax_fmm_t* h, galax_bodies_t* bodies, double dt, u
  * (tree body order), returns x_bar, y_bar, m_bar = J^T (ax_bar, ay_bar) for
  * the map (x, y, m) -> (ax, ay) on the installed static tree (topology, lists,
  * M2L keys and node centres frozen, as the reference freezes them after
- * Tree::build). Recomputes the forward expansions first. FP64 contexts only.
- * No reference counterpart; for differentiable simulation and for checking
- * through torch.autograd. */
+ * Tree::build). Recomputes the forward expansions first, in the context's
+ * precision form. The adjoint passes run in FP64, except the near field of the
+ * mixed forms, which runs in FP32 on leaf-relative coordinates like their
+ * forward near field. No reference counterpart; for differentiable simulation
+ * and for checking through torch.autograd. */
 GALAX_API int galax_fmm_vjp(galax_fmm_t* h, const galax_bodies_t* bodies, const double* ax_bar, const double* ay_bar,
                             double* x_bar, double* y_bar, double* m_bar);
 
@@ -235,6 +239,23 @@ GALAX_API int galax_fmm_validate(galax_fmm_t* h, galax_bodies_t* bodies, const u
                        double* max_rel, double* mean_rel, double* rms_vec);
 
 GALAX_API void galax_fmm_destroy(galax_fmm_t* h);
+
+/* ~galax-io write_snapshot (galax-io/src/lib.rs:21-42): a GALX v1 file,
+ * little-endian: "GALX", u32 version 1, u64 n, f64 timestamp, then n f64 each
+ * of x, y, vx, vy, mass, ax, ay. The reference stamps the current time; here
+ * the caller passes it. NULL arrays are written as zeros (phi is not stored). */
+GALAX_API int galax_snapshot_write(const char* path, const galax_bodies_t* bodies, double timestamp);
+
+/* The header of a GALX file: body count and timestamp (either may be NULL).
+ * Errors as galax_snapshot_read. */
+GALAX_API int galax_snapshot_info(const char* path, size_t* n, double* timestamp);
+
+/* ~galax-io read_snapshot (galax-io/src/lib.rs:53-84): fills caller arrays of
+ * bodies->n entries, which must equal the file's count (galax_snapshot_info);
+ * NULL arrays are skipped, phi is untouched. GALAX_ERR_FORMAT for a wrong magic
+ * or a short file, GALAX_ERR_UNSUPPORTED for a version other than 1,
+ * GALAX_ERR_IO when the file cannot be opened or read. */
+GALAX_API int galax_snapshot_read(const char* path, galax_bodies_t* bodies);
 
 /* Text of the last error raised on the calling thread ("" if none). */
 GALAX_API const char* galax_fmm_last_error(void);

@@ -2,6 +2,8 @@
 oracle, on the GPU: bit-exact tree, permutation and lists; forces within the
 north-star tolerance (median relative difference <= 1e-6 in FP64); the
 reference's known-answer tests; edge cases."""
+import os
+
 import numpy as np
 import pytest
 
@@ -9,6 +11,7 @@ from galax_amd import fmm, ic
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 TREE_FIELDS = ("center_x", "center_y", "half_width", "mass", "com_x", "com_y", "body_start", "body_end",
                "children", "parent", "is_leaf")
@@ -64,8 +67,18 @@ def _assert_lists_equal(g_csr, o_lists):
         raise AssertionError(f"list entries differ at {len(bad)} places, first {bad[0]}: {gd[bad[0]]} vs {od[bad[0]]}")
 
 
+def _snap(name):
+    """Bodies of one of the reference's snapshots (post-step states of its own
+    runs, runs/*/snap_*.bin, copied under tests/golden/)."""
+    b = fmm.read_snapshot(os.path.join(GOLDEN, name))
+    return {"x": b.x, "y": b.y, "vx": b.vx, "vy": b.vy, "m": b.mass, "ax": b.ax, "ay": b.ay}
+
+
 CASES = [
     ("uniform", lambda: ic.uniform(6000, seed=101), 32, (0.0, 100.0, 0.0, 100.0)),
+    ("snap_demo", lambda: _snap("galx_demo_0010.bin"), 32, (-50.0, 50.0, -50.0, 50.0)),
+    ("snap_test", lambda: _snap("galx_test_0005.bin"), 8, (-50.0, 50.0, -50.0, 50.0)),
+    ("snap_viz_demo", lambda: _snap("galx_viz_demo_0010.bin"), 16, (-50.0, 50.0, -50.0, 50.0)),
     ("plummer", lambda: ic.plummer(8000, seed=102), 32, (-50.0, 50.0, -50.0, 50.0)),
     ("clamped_dupes", lambda: ic.uniform(5000, seed=103), 32, (-50.0, 50.0, -50.0, 50.0)),  # 75% clamp (CLI bounds)
     ("galaxy", lambda: ic.galaxy(4096, seed=104), 32, (-50.0, 50.0, -50.0, 50.0)),
@@ -138,6 +151,22 @@ def test_forces_match_oracle_reference(p, prec):
     assert np.median(rd) <= 1e-6, np.median(rd)
     assert np.max(rd) <= 1e-9, np.max(rd)
     np.testing.assert_allclose(b.phi, phi, rtol=1e-9, atol=1e-12 * np.abs(phi).max())
+
+
+@pytest.mark.parametrize("name", ["galx_demo_0010.bin", "galx_viz_demo_0010.bin"])
+def test_forces_on_reference_snapshots(name):
+    """Forces on post-step states of the reference's own runs (GALX fixtures)."""
+    d = _snap(name)
+    bounds = (-50.0, 50.0, -50.0, 50.0)
+    eps, p = 0.05, 4
+    ctx, b, _ = _gpu_build(d, p=p, eps=eps, bounds=bounds)
+    ctx.evaluate(b)
+    ot, od, _ = _oracle_build(d, 32, bounds)
+    ax, ay, _ = O.compute_fmm_force(od["x"], od["y"], od["m"], ot, O.m2l_lists(ot), O.p2p_lists(ot), p, eps,
+                                    want_phi=True)
+    rd = _rel_diff((b.ax, b.ay), (ax, ay))
+    assert np.median(rd) <= 1e-6, np.median(rd)
+    assert np.max(rd) <= 1e-9, np.max(rd)
 
 
 def test_uniform10k_forces_and_validate():
