@@ -548,7 +548,7 @@ int galax_fmm_evaluate(galax_fmm_t* h, galax_bodies_t* b) {
   GX_CHECK(evaluate(h, h->x.as<double>(), h->y.as<double>(), h->m.as<double>(), h->ax.as<double>(),
                     h->ay.as<double>(), b->phi ? h->phi.as<double>() : nullptr));
   uint32_t changed = 0;
-  GX_HIP(hipMemcpyAsync(&changed, h->flag.as<uint32_t>() + 4, 4, hipMemcpyDeviceToHost, s));
+  GX_HIP(hipMemcpyAsync(&changed, h->flag.as<uint32_t>() + 4 + h->san_cur, 4, hipMemcpyDeviceToHost, s));
   GX_CHECK(down(s, b->ax, h->ax, n));
   GX_CHECK(down(s, b->ay, h->ay, n));
   if (b->phi) GX_CHECK(down(s, b->phi, h->phi, n));
@@ -574,6 +574,12 @@ int galax_fmm_evaluate_device(galax_fmm_t* h, double* d_x, double* d_y, double* 
 int galax_fmm_last_timings(galax_fmm_t* h, galax_timings_t* out) {
   if (!h || !out) return set_error(GALAX_ERR_INVALID, "NULL argument");
   *out = h->timings;
+  return 0;
+}
+
+int galax_fmm_set_timing(galax_fmm_t* h, int enable) {
+  if (!h) return set_error(GALAX_ERR_INVALID, "NULL handle");
+  h->timing = enable != 0;
   return 0;
 }
 

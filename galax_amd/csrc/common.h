@@ -142,6 +142,9 @@ struct galax_fmm {
   bool corrected_ready = false;
   galax_timings_t timings{};
   hipEvent_t ev[12]{};
+  bool timing = true;     // record the per-pass events of evaluate (galax_fmm_set_timing)
+  uint32_t san_cur = 0;   // sanitize counter of the last evaluate: flag[4 + san_cur]
+  bool san_init = false;  // both sanitize counters zeroed
   hipEvent_t bev[6]{};  // build stages
   bool build_timed = false;
 };

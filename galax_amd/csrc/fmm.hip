@@ -76,9 +76,12 @@ static inline unsigned nblk(size_t n, unsigned bs = 256) { return (unsigned)((n 
 // ---------------------------------------------------------- sanitize ----
 
 // galax-core/src/lib.rs:1475-1482 (mutates its input)
+// counts the bodies it changes in *changed and zeroes *next, the counter of
+// the following evaluation (two counters in turn: no memset between passes)
 __global__ void k_sanitize(double* __restrict__ x, double* __restrict__ y, double* __restrict__ m, size_t n,
-                           uint32_t* __restrict__ changed) {
+                           uint32_t* __restrict__ changed, uint32_t* __restrict__ next) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *next = 0u;
   if (i >= n) return;
   double xi = x[i], yi = y[i], mi = m[i];
   bool c = false;
@@ -1073,13 +1076,13 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   const uint32_t top = corrected ? t.n_levels : std::min<uint32_t>(t.n_levels, 2);
   const uint32_t live_hi = t.level_offset[top];
   hipEvent_t* ev = h->ev;
-  GX_HIP(hipEventRecord(ev[1], s));
+  if (h->timing) GX_HIP(hipEventRecord(ev[1], s));
   {
     k_p2m<P><<<nblk(nl, num_moments(P) <= 32 ? 8 : 4), 256, 0, s>>>(d_x, d_y, d_m, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), comx,
                                          comy, (uint32_t)nl, mp);
     GX_HIP(hipGetLastError());
   }
-  GX_HIP(hipEventRecord(ev[2], s));
+  if (h->timing) GX_HIP(hipEventRecord(ev[2], s));
   {
     for (uint32_t l = 1; l < top; ++l) {
       uint32_t lo = t.level_offset[l], hi = t.level_offset[l + 1];
@@ -1088,8 +1091,8 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
       GX_HIP(hipGetLastError());
     }
   }
-  GX_HIP(hipEventRecord(ev[3], s));
-  GX_HIP(hipEventRecord(ev[4], s));
+  if (h->timing) GX_HIP(hipEventRecord(ev[3], s));
+  if (h->timing) GX_HIP(hipEventRecord(ev[4], s));
   if (corrected) {
     k_m2l_exact<P><<<nblk(nn, 4), 256, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(),
                                                (uint32_t)nn, comx, comy, eps * eps, mp, loc);
@@ -1102,7 +1105,7 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
                                                   loc);
     GX_HIP(hipGetLastError());
   }
-  GX_HIP(hipEventRecord(ev[5], s));
+  if (h->timing) GX_HIP(hipEventRecord(ev[5], s));
   // L2L top-down; the last level (parents to leaves) runs inside the L2P+P2P
   // kernel when the forces are computed (the VJP's expansions-only forward
   // needs the leaves' locals in memory)
@@ -1118,7 +1121,7 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
     k_l2l<P><<<nblk(nl * S), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, 0, (uint32_t)nl, loc);
     GX_HIP(hipGetLastError());
   }
-  GX_HIP(hipEventRecord(ev[6], s));
+  if (h->timing) GX_HIP(hipEventRecord(ev[6], s));
   const uint32_t* goff = corrected ? h->c_gat_off.as<uint32_t>() : t.gat_off.as<uint32_t>();
   const uint32_t* gdat = corrected ? h->c_gat_dat.as<uint32_t>() : t.gat_dat.as<uint32_t>();
   const P2PItems& it = corrected ? h->c_items : t.items;
@@ -1131,10 +1134,10 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
       d_x, d_y, d_m, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), comx, comy, loc, goff, gdat,     \
       it.tgt.as<uint32_t>(), it.g0.as<uint32_t>(), it.g1.as<uint32_t>(), it.slot.as<uint32_t>(), nmax, eps * eps, \
       d_ax, d_ay, d_phi, h->p2p_part.as<double>(), t.parent.as<uint32_t>(), fuse_l2l ? 1 : 0)
-  GX_HIP(hipEventRecord(ev[10], s));
+  if (h->timing) GX_HIP(hipEventRecord(ev[10], s));
   if (!near) {  // expansions only (the VJP's forward): no L2P / P2P
-    GX_HIP(hipEventRecord(ev[11], s));
-    GX_HIP(hipEventRecord(ev[7], s));
+    if (h->timing) GX_HIP(hipEventRecord(ev[11], s));
+    if (h->timing) GX_HIP(hipEventRecord(ev[7], s));
     return 0;
   }
   if (fp32_near) {
@@ -1149,12 +1152,12 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
       GX_LAUNCH_P2P(double, false);
   }
   GX_HIP(hipGetLastError());
-  GX_HIP(hipEventRecord(ev[11], s));
+  if (h->timing) GX_HIP(hipEventRecord(ev[11], s));
   GX_HIP(p2p_combine(it, goff, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), nmax,
                      h->p2p_part.as<double>(), d_ax, d_ay, d_phi, s));
 #undef GX_LAUNCH_P2P
   GX_HIP(hipGetLastError());
-  GX_HIP(hipEventRecord(ev[7], s));
+  if (h->timing) GX_HIP(hipEventRecord(ev[7], s));
   return 0;
 }
 
@@ -1177,10 +1180,16 @@ int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, 
   }
   if (!corrected && h->cfg.precision != GALAX_PREC_FP64_EXACT && !h->mx_valid) GX_CHECK(m2l_mfma_setup(h));
   GX_CHECK(dbuf_reserve(h->flag, 64, s));
-  uint32_t* changed = h->flag.as<uint32_t>() + 4;
-  GX_HIP(hipEventRecord(h->ev[0], s));
-  GX_HIP(hipMemsetAsync(changed, 0, 4, s));
-  k_sanitize<<<nblk(n), 256, 0, s>>>(d_x, d_y, d_m, n, changed);
+  if (!h->san_init) {
+    GX_HIP(hipMemsetAsync(h->flag.as<uint32_t>() + 4, 0, 8, s));
+    h->san_init = true;
+    h->san_cur = 1;
+  }
+  h->san_cur ^= 1u;
+  uint32_t* changed = h->flag.as<uint32_t>() + 4 + h->san_cur;
+  uint32_t* next = h->flag.as<uint32_t>() + 4 + (h->san_cur ^ 1u);
+  if (h->timing) GX_HIP(hipEventRecord(h->ev[0], s));
+  k_sanitize<<<nblk(n), 256, 0, s>>>(d_x, d_y, d_m, n, changed, next);
   GX_HIP(hipGetLastError());
   int rc = 0;
   switch (h->cfg.p) {
@@ -1195,9 +1204,11 @@ int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, 
   }
   GX_CHECK(rc);
   GX_HIP(hipStreamSynchronize(s));
-  float ms[8];
-  for (int i = 0; i < 7; ++i) GX_HIP(hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1]));
   galax_timings_t& tm = h->timings;
+  // pass times when the events were recorded (galax_fmm_set_timing), else 0
+  float ms[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (h->timing)
+    for (int i = 0; i < 7; ++i) GX_HIP(hipEventElapsedTime(&ms[i], h->ev[i], h->ev[i + 1]));
   tm.sanitize = ms[0];
   tm.p2m = ms[1];
   tm.m2m = ms[2];
@@ -1206,7 +1217,7 @@ int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, 
   tm.l2l = ms[5];
   tm.l2p_p2p = ms[6];
   float tot = 0.f;
-  GX_HIP(hipEventElapsedTime(&tot, h->ev[0], h->ev[7]));
+  if (h->timing) GX_HIP(hipEventElapsedTime(&tot, h->ev[0], h->ev[7]));
   tm.total = tot;
   tm.m2l_pairs_executed = corrected ? h->c_m2l_total : h->n_live_pairs;
   tm.m2l_pairs_reference = corrected ? h->c_m2l_total : t.m2l_total;
@@ -1217,8 +1228,9 @@ int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, 
   tm.m2l_steps = tiled ? h->mx_fwd.nsteps : 0;
   tm.m2l_items = tiled ? h->mx_fwd.nitems : 0;
   float km = 0.f, kp = 0.f;
-  if (tiled && h->mx_fwd.nitems && h->n_live_targets) GX_HIP(hipEventElapsedTime(&km, h->ev[8], h->ev[9]));
-  GX_HIP(hipEventElapsedTime(&kp, h->ev[10], h->ev[11]));
+  if (h->timing && tiled && h->mx_fwd.nitems && h->n_live_targets)
+    GX_HIP(hipEventElapsedTime(&km, h->ev[8], h->ev[9]));
+  if (h->timing) GX_HIP(hipEventElapsedTime(&kp, h->ev[10], h->ev[11]));
   tm.m2l_kernel = km;
   tm.p2p_kernel = kp;
   return 0;

@@ -855,10 +855,10 @@ static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
   a.in = in;
   a.out = out;
   a.part = T.part.as<double>();
-  if (!TR) GX_HIP(hipEventRecord(h->ev[8], s));
+  if (!TR && h->timing) GX_HIP(hipEventRecord(h->ev[8], s));
   k_m2l_tiles<P, MODE, TR><<<8 * a.per_xcd, 256, 0, s>>>(a);
   GX_HIP(hipGetLastError());
-  if (!TR) GX_HIP(hipEventRecord(h->ev[9], s));
+  if (!TR && h->timing) GX_HIP(hipEventRecord(h->ev[9], s));
   if (T.nmulti) {
     k_m2l_tiles_combine<<<(unsigned)T.nmulti, 256, 0, s>>>(T.multi_tile.as<uint32_t>(), T.multi_slot.as<uint32_t>(),
                                                            T.multi_cnt.as<uint32_t>(), S, (uint32_t)T.n_rows,

@@ -105,6 +105,7 @@ def lib():
         "galax_fmm_evaluate_device": [vp] * 7,
         "galax_fmm_last_timings": [vp, C.POINTER(_Timings)],
         "galax_fmm_last_build_timings": [vp, C.POINTER(_BuildTimings)],
+        "galax_fmm_set_timing": [vp, C.c_int],
         "galax_fmm_kdk": [vp, C.POINTER(_Bodies), d, C.c_uint64, C.c_uint64, vp],
         "galax_fmm_vjp": [vp, C.POINTER(_Bodies), vp, vp, vp, vp, vp],
         "galax_fmm_vjp_device": [vp] * 9,
@@ -334,6 +335,11 @@ class GpuContext:
         names = ("sanitize", "p2m", "m2m", "m2l_setup", "m2l", "l2l", "l2p_p2p", "total", "m2l_kernel", "p2p_kernel")
         return Timings({k: getattr(t, k) for k in names}, t.m2l_pairs_executed, t.m2l_pairs_reference, t.m2l_keys,
                        t.p2p_interactions, t.m2l_groups, t.m2l_steps, t.m2l_items)
+
+    def set_timing(self, enable: bool) -> None:
+        """Per-pass device timing events of evaluate on (default) or off; with
+        them off, timings() reports work counts and zero times."""
+        _check(lib().galax_fmm_set_timing(self.h, 1 if enable else 0))
 
     def build_timings(self) -> dict:
         """Device milliseconds of the last tree build per stage (sort, permute,

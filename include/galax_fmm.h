@@ -193,8 +193,15 @@ GALAX_API int galax_fmm_evaluate_device(galax_fmm_t* h, double* d_x, double* d_y
                               double* d_phi);
 
 /* Device times and work counts of the last evaluate. No reference counterpart
- * (galax-bench times whole calls on the host, galax-bench/src/main.rs:7-17). */
+ * (galax-bench times whole calls on the host, galax-bench/src/main.rs:7-17).
+ * The times are 0 when timing is off (galax_fmm_set_timing). */
 GALAX_API int galax_fmm_last_timings(galax_fmm_t* h, galax_timings_t* out);
+
+/* Turns the per-pass timing events of evaluate on (the default) or off. Each
+ * event separates two passes on the stream and costs a few microseconds of
+ * device time; a caller that does not read galax_fmm_last_timings can turn
+ * them off. No reference counterpart. */
+GALAX_API int galax_fmm_set_timing(galax_fmm_t* h, int enable);
 
 /* Device times of the last galax_fmm_build_tree[_device]. No reference
  * counterpart (as above). */
