@@ -180,6 +180,27 @@ int build_p2p_items(galax_fmm* h, const uint32_t* goff, uint32_t n_leaves, P2PIt
 hipError_t p2p_combine(const P2PItems& it, const uint32_t* goff, const uint32_t* bs, const uint32_t* be,
                        uint32_t n_max, const double* part, double* o0, double* o1, double* o2, hipStream_t s);
 
+// Body index of element w0 + lane of a P2P work item's source order (w0
+// uniform over the wave). The item's source leaf j covers elements
+// [offs_j, offs_j + count_j) and starts at body starts_j; lane j of the wave
+// holds offs_j and starts_j (j < nlv). jcur (uniform) is a leaf whose range
+// starts at or before w0; on return it is the leaf of element w0 + 63, the
+// start for the next window. The scan reads each candidate leaf's offset with
+// one readlane and stops at the first leaf past the window, so a window of 64
+// elements costs a few steps (no per-element search).
+__device__ __forceinline__ uint32_t item_source(uint32_t w0, int lane, int nlv, uint32_t offs, uint32_t starts,
+                                                uint32_t& jcur) {
+  const uint32_t q = w0 + (uint32_t)lane;
+  uint32_t jl = jcur;
+  for (uint32_t j = jcur + 1; j < (uint32_t)nlv; ++j) {
+    const uint32_t oj = __builtin_amdgcn_readlane(offs, j);
+    if (oj > w0 + 63u) break;
+    if (q >= oj) jl = j;
+  }
+  jcur = __builtin_amdgcn_readlane(jl, 63);
+  return (uint32_t)__shfl((int)starts, (int)jl, 64) + (q - (uint32_t)__shfl((int)offs, (int)jl, 64));
+}
+
 // Near-field arithmetic type: FP64 or FP32 (v_rsq_f32) on coordinates
 // relative to the target leaf's COM. FP64: the v_rsq_f64 estimate y with one
 // second-order refinement, y (1 + e/2 + 3e^2/8) with e = 1 - v y^2; no

@@ -683,7 +683,6 @@ __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, co
   __shared__ double sloc[S], spar[S];
   static_assert(P2P_ITEM_LEAVES <= 64, "one lane per source leaf of an item");
   __shared__ V4 src[P2P_CHUNK + 1];
-  __shared__ uint32_t l_ss[P2P_ITEM_LEAVES], l_off[P2P_ITEM_LEAVES];
   const uint32_t item = blockIdx.x;
   const uint32_t leaf = itgt[item], g0 = ig0[item], g1 = ig1[item], slot = islot[item];
   const bool first = g0 == goff[leaf];
@@ -708,10 +707,7 @@ __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, co
     const uint32_t v = __shfl_up(inc, d, 64);
     if (lane >= d) inc += v;
   }
-  if (lane < nlv) {
-    l_ss[lane] = ss;
-    l_off[lane] = inc - cnt;
-  }
+  const uint32_t offs = inc - cnt;  // lane j < nlv: source-order offset of leaf j (its first body: ss)
   const uint32_t total = __builtin_amdgcn_readlane(inc, 63);  // uniform: scalar loop bounds below
   // source-order range [self_lo, self_hi) of the target leaf's own bodies
   const unsigned long long sm_ = __ballot(is_self);
@@ -751,26 +747,21 @@ __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, co
     double fx = 0.0, fy = 0.0, fp = 0.0;
     if (far_lane && valid) l2p_body<P, true>(far_loc, xi - fcx, yi - fcy, &fx, &fy, &fp);
     T accx = 0, accy = 0, accp = 0;
+    uint32_t jcur = 0;
     for (uint32_t c0 = 0; c0 < total; c0 += P2P_CHUNK) {
       const int n = (int)min((uint32_t)P2P_CHUNK, total - c0);
       __syncthreads();
-      for (int pp = lane; pp < n; pp += 64) {
-        const uint32_t q = c0 + (uint32_t)pp;
-        int lo = 0, hi = nlv - 1;  // last source leaf with l_off <= q
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (l_off[mid] <= q)
-            lo = mid;
-          else
-            hi = mid - 1;
+      for (int w0 = 0; w0 < n; w0 += 64) {  // uniform: 64-element windows
+        const uint32_t g = item_source(c0 + (uint32_t)w0, lane, nlv, offs, ss, jcur);
+        const int pp = w0 + lane;
+        if (pp < n) {
+          V4 v;
+          v.x = (T)(x[g] - ox);
+          v.y = (T)(y[g] - oy);
+          v.z = (T)m[g];
+          v.w = (T)0;
+          src[pp] = v;
         }
-        const uint32_t g = l_ss[lo] + (q - l_off[lo]);
-        V4 v;
-        v.x = (T)(x[g] - ox);
-        v.y = (T)(y[g] - oy);
-        v.z = (T)m[g];
-        v.w = (T)0;
-        src[pp] = v;
       }
       if (lane == 0 && (n & 1)) {  // pad to even: zero mass far away (r^2 overflows, 1/r = 0)
         V4 v;

@@ -91,7 +91,7 @@ struct Lanes {
 
 // ------------------------------------------------------------- setup ----
 
-constexpr uint32_t MX_ITEM_STEPS = 16;  // steps per work item
+constexpr uint32_t MX_ITEM_STEPS = 32;  // steps per work item
 constexpr uint32_t MX_GROUP = 4;        // longest run summed inside the product kernel; longer runs are pre-summed
 constexpr uint32_t MX_WAVE_RUN = 256;   // longest run pre-summed by one wave; longer runs take a block
 

@@ -98,7 +98,6 @@ __global__ void __launch_bounds__(64) k_vjp_near(const double* __restrict__ x, c
   static_assert(P2P_ITEM_LEAVES <= 64, "one lane per source leaf of an item");
   __shared__ V4 sa[VJP_CHUNK + 1];
   __shared__ T sb[VJP_CHUNK + 1];
-  __shared__ uint32_t l_ss[P2P_ITEM_LEAVES], l_off[P2P_ITEM_LEAVES];
   const uint32_t item = blockIdx.x;
   const uint32_t leaf = itgt[item], g0 = ig0[item], g1 = ig1[item], slot = islot[item];
   const int lane = threadIdx.x, tl = lane & 31, half = lane >> 5;
@@ -121,10 +120,7 @@ __global__ void __launch_bounds__(64) k_vjp_near(const double* __restrict__ x, c
     const uint32_t v = __shfl_up(inc, d, 64);
     if (lane >= d) inc += v;
   }
-  if (lane < nlv) {
-    l_ss[lane] = ss;
-    l_off[lane] = inc - cnt;
-  }
+  const uint32_t offs = inc - cnt;  // lane j < nlv: source-order offset of leaf j (its first body: ss)
   const uint32_t total = __builtin_amdgcn_readlane(inc, 63);  // uniform: scalar loop bounds below
   const unsigned long long sm_ = __ballot(is_self);
   uint32_t self_lo = 0, self_hi = 0;
@@ -139,20 +135,14 @@ __global__ void __launch_bounds__(64) k_vjp_near(const double* __restrict__ x, c
     const T xj = valid ? (T)(x[j] - ox) : (T)0, yj = valid ? (T)(y[j] - oy) : (T)0;
     const T mj = valid ? (T)m[j] : (T)0, abx = valid ? (T)axb[j] : (T)0, aby = valid ? (T)ayb[j] : (T)0;
     T gx_ = 0, gy_ = 0, gm_ = 0;
+    uint32_t jcur = 0;
     for (uint32_t c0 = 0; c0 < total; c0 += VJP_CHUNK) {
       const int n = (int)min((uint32_t)VJP_CHUNK, total - c0);
       __syncthreads();
-      for (int pp = lane; pp < n; pp += 64) {
-        const uint32_t q = c0 + (uint32_t)pp;
-        int lo = 0, hi = nlv - 1;  // last source leaf with l_off <= q
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (l_off[mid] <= q)
-            lo = mid;
-          else
-            hi = mid - 1;
-        }
-        const uint32_t g = l_ss[lo] + (q - l_off[lo]);
+      for (int w0 = 0; w0 < n; w0 += 64) {  // uniform: 64-element windows
+        const uint32_t g = item_source(c0 + (uint32_t)w0, lane, nlv, offs, ss, jcur);
+        const int pp = w0 + lane;
+        if (pp >= n) continue;
         V4 v;
         v.x = (T)(x[g] - ox);
         v.y = (T)(y[g] - oy);
