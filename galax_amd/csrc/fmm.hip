@@ -1,6 +1,6 @@
 // fmm.hip — the FMM passes of compute_fmm_force (galax-core/src/lib.rs:1466-1499)
 // on the device, FP64:
-//   sanitize  :1475-1482     P2M  :848-887       M2M :890-938
+//   sanitize  :1475-1482 (in the P2M kernel)  P2M :848-887  M2M :890-938
 //   M2L       :1199-1259 (derivatives :1040-1151, plan :1153-1196)
 //   L2L       :1262-1302     L2P  :1308-1355     P2P :1395-1462
 // Functions marked "exact" run without FMA contraction and in the reference's
@@ -73,44 +73,23 @@ __device__ __forceinline__ void coef_ij(int k, int& i, int& j) {
 
 static inline unsigned nblk(size_t n, unsigned bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
-// ---------------------------------------------------------- sanitize ----
-
-// galax-core/src/lib.rs:1475-1482 (mutates its input)
-// counts the bodies it changes in *changed and zeroes *next, the counter of
-// the following evaluation (two counters in turn: no memset between passes)
-__global__ void k_sanitize(double* __restrict__ x, double* __restrict__ y, double* __restrict__ m, size_t n,
-                           uint32_t* __restrict__ changed, uint32_t* __restrict__ next) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *next = 0u;
-  if (i >= n) return;
-  double xi = x[i], yi = y[i], mi = m[i];
-  bool c = false;
-  if (fabs(xi) > 1e100 || fabs(yi) > 1e100 || !isfinite(xi) || !isfinite(yi)) {
-    x[i] = 0.0;
-    y[i] = 0.0;
-    c = true;
-  }
-  if (!isfinite(mi) || mi <= 0.0) {
-    m[i] = 1.0;
-    c = true;
-  }
-  if (c) atomicAdd(changed, 1u);
-}
-
 // --------------------------------------------------------------- P2M ----
 
 // exact: one wave per leaf, bodies in order (galax-core/src/lib.rs:854-885;
-// coeff = mass*dx^i*dy^j/(i!*j!), non-finite -> 0). The wave stages a chunk of
+// coeff = mass*dx^i*dy^j/(i!*j!), non-finite -> 0). The kernel also applies
+// compute_fmm_force's sanitize to every body (each body is in one leaf),
+// writing fixed values back and counting them in *changed; it zeroes *next,
+// the counter of the following evaluation (two counters in turn). The wave stages a chunk of
 // the leaf's bodies in LDS as m and the power tables dx^0..dx^P, dy^0..dy^P
 // (dx = x - com, powers by repeated products: the values every lane would
 // compute), then each lane sums its coefficient over the chunk in body order.
 // With S <= 32 the two half-waves take two leaves.
 template <int P>
-__global__ void __launch_bounds__(256) k_p2m(const double* __restrict__ x, const double* __restrict__ y,
-                                             const double* __restrict__ m, const uint32_t* __restrict__ bs,
-                                             const uint32_t* __restrict__ be, const double* __restrict__ comx,
-                                             const double* __restrict__ comy, uint32_t n_leaves,
-                                             double* __restrict__ mp) {
+__global__ void __launch_bounds__(256) k_p2m(double* __restrict__ x, double* __restrict__ y, double* __restrict__ m,
+                                             const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be,
+                                             const double* __restrict__ comx, const double* __restrict__ comy,
+                                             uint32_t n_leaves, double* __restrict__ mp,
+                                             uint32_t* __restrict__ changed, uint32_t* __restrict__ next) {
 #pragma clang fp contract(off)
   constexpr int S = num_moments(P);
   constexpr int LPW = S <= 32 ? 2 : 1;  // leaves per wave
@@ -118,6 +97,7 @@ __global__ void __launch_bounds__(256) k_p2m(const double* __restrict__ x, const
   __shared__ double spx[4][64][P + 1], spy[4][64][P + 1], smb[4][64];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int sub = lane / LW, sl = lane % LW;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *next = 0u;
   const uint32_t leaf = (blockIdx.x * 4 + w) * LPW + sub;
   const bool live = leaf < n_leaves;
   const uint32_t s = live ? bs[leaf] : 0u, e = live ? be[leaf] : 0u;
@@ -140,7 +120,25 @@ __global__ void __launch_bounds__(256) k_p2m(const double* __restrict__ x, const
       __syncthreads();
       const uint32_t b = s + c + sl;
       if (b < e) {
-        const double dx = x[b] - cx, dy = y[b] - cy;
+        // sanitize (galax-core/src/lib.rs:1475-1482), written back once
+        double xb = x[b], yb = y[b], mb = m[b];
+        bool fix = false;
+        if (fabs(xb) > 1e100 || fabs(yb) > 1e100 || !isfinite(xb) || !isfinite(yb)) {
+          xb = 0.0;
+          yb = 0.0;
+          fix = true;
+        }
+        if (!isfinite(mb) || mb <= 0.0) {
+          mb = 1.0;
+          fix = true;
+        }
+        if (fix && k0 == 0) {
+          x[b] = xb;
+          y[b] = yb;
+          m[b] = mb;
+          atomicAdd(changed, 1u);
+        }
+        const double dx = xb - cx, dy = yb - cy;
         double px = 1.0, py = 1.0;
         spx[w][base + sl][0] = 1.0;
         spy[w][base + sl][0] = 1.0;
@@ -151,7 +149,7 @@ __global__ void __launch_bounds__(256) k_p2m(const double* __restrict__ x, const
           spx[w][base + sl][r] = px;
           spy[w][base + sl][r] = py;
         }
-        smb[w][base + sl] = m[b];
+        smb[w][base + sl] = mb;
       }
       __syncthreads();
       const uint32_t n = e > s + c ? min((uint32_t)LW, e - s - c) : 0u;
@@ -1069,8 +1067,11 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   hipEvent_t* ev = h->ev;
   if (h->timing) GX_HIP(hipEventRecord(ev[1], s));
   {
-    k_p2m<P><<<nblk(nl, num_moments(P) <= 32 ? 8 : 4), 256, 0, s>>>(d_x, d_y, d_m, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), comx,
-                                         comy, (uint32_t)nl, mp);
+    uint32_t* changed = h->flag.as<uint32_t>() + 4 + h->san_cur;
+    uint32_t* next = h->flag.as<uint32_t>() + 4 + (h->san_cur ^ 1u);
+    k_p2m<P><<<nblk(nl, num_moments(P) <= 32 ? 8 : 4), 256, 0, s>>>(
+        d_x, d_y, d_m, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), comx, comy, (uint32_t)nl, mp, changed,
+        next);
     GX_HIP(hipGetLastError());
   }
   if (h->timing) GX_HIP(hipEventRecord(ev[2], s));
@@ -1176,12 +1177,8 @@ int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, 
     h->san_init = true;
     h->san_cur = 1;
   }
-  h->san_cur ^= 1u;
-  uint32_t* changed = h->flag.as<uint32_t>() + 4 + h->san_cur;
-  uint32_t* next = h->flag.as<uint32_t>() + 4 + (h->san_cur ^ 1u);
+  h->san_cur ^= 1u;  // the sanitize runs inside the P2M kernel
   if (h->timing) GX_HIP(hipEventRecord(h->ev[0], s));
-  k_sanitize<<<nblk(n), 256, 0, s>>>(d_x, d_y, d_m, n, changed, next);
-  GX_HIP(hipGetLastError());
   int rc = 0;
   switch (h->cfg.p) {
 #define GX_CASE(PP)                                                  \

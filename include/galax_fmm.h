@@ -122,7 +122,8 @@ typedef struct {
 typedef struct galax_fmm galax_fmm_t;
 
 /* Per-pass device times of the last evaluate, milliseconds (HIP events on the
- * handle's stream). */
+ * handle's stream). The sanitize of compute_fmm_force runs inside the P2M
+ * kernel: its time is part of p2m and `sanitize` is the gap before it. */
 typedef struct {
   double sanitize, p2m, m2m, m2l_setup, m2l, l2l, l2p_p2p, total;
   uint64_t m2l_pairs_executed, m2l_pairs_reference, m2l_keys, p2p_interactions;
