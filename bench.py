@@ -253,7 +253,9 @@ def measured_traffic(args, form, stage):
         return None
     mode = {"fp64": 0, "mixed_fp32": 1, "mixed_bf16x3": 2}.get(form)
     prec = "float" if form.startswith("mixed") else "double"
-    name = {"m2l": f"gx::k_m2l_tiles<{args.p}, {mode}, false>", "l2p_p2p": f"gx::k_l2p_p2p<{args.p}, {prec}, true>"}.get(stage)
+    # the timed step's kernels: accelerations only (no potential)
+    name = {"m2l": f"gx::k_m2l_tiles<{args.p}, {mode}, false>",
+            "l2p_p2p": f"gx::k_l2p_p2p<{args.p}, {prec}, false>"}.get(stage)
     k = t["kernels"].get(name)
     return k["traffic_bytes"] if k else None
 
