@@ -256,6 +256,30 @@ def test_sanitize_mutates_like_reference():
     assert np.isfinite(b.ax).all() and np.isfinite(b.ay).all()
 
 
+def test_sanitize_every_evaluation_and_timing_switch():
+    """The sanitize counters alternate between evaluations: a bad value in the
+    third of several evaluations is still fixed and written back; forces with
+    the timing events off equal those with them on, and the times read 0."""
+    d = ic.uniform(400, seed=207, hi=10.0)
+    bounds = (0.0, 10.0, 0.0, 10.0)
+    ctx, b, _ = _gpu_build(d, p=2, n_max=8, bounds=bounds)
+    ctx.evaluate(b)
+    ax0, ay0 = b.ax.copy(), b.ay.copy()
+    ctx.set_timing(False)
+    ctx.evaluate(b)
+    np.testing.assert_array_equal(b.ax, ax0)
+    np.testing.assert_array_equal(b.ay, ay0)
+    assert ctx.timings().passes_ms["total"] == 0.0 and ctx.timings().p2p_interactions > 0
+    for k in range(3):
+        b.mass[11 + k] = -2.0  # a bad mass in each of three more evaluations
+        ctx.evaluate(b)
+        assert b.mass[11 + k] == 1.0, k
+        assert np.isfinite(b.ax).all()
+    ctx.set_timing(True)
+    ctx.evaluate(b)
+    assert ctx.timings().passes_ms["total"] > 0.0
+
+
 @pytest.mark.parametrize("n", [1, 2, 31, 33])
 def test_small_n(n):
     d = ic.uniform(n, seed=206)
