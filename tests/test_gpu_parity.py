@@ -342,6 +342,38 @@ def test_fp64_mfma_m2l_matches_exact_order(p):
     assert rd.max() <= 1e-10, rd.max()
     np.testing.assert_allclose(a[2], e[2], rtol=1e-11, atol=0.0)
 
+@pytest.mark.parametrize("p", [12, 16])
+def test_high_orders_all_forms(p):
+    """p = 12 and 16 (the largest): the FP64 form (and at p = 12 the mixed
+    forms) against FP64_EXACT, and FP64_EXACT against the oracle, on a small
+    Plummer sphere. These orders run the register-heavy variants of every
+    kernel. (At p = 16 the scaled FP32 operators reach (2p)! ~ 3e35, at the
+    edge of FP32's range, so the mixed forms are checked at p = 12 only. There
+    the reference algorithm's key cache makes the high-order M2L terms large
+    and cancelling, so FP32 loses more than at p <= 8: measured 1.9e-5 RMS vs
+    FP64; the bound is 1e-4.)"""
+    d = ic.plummer(3000, seed=306)
+    bounds = (-50.0, 50.0, -50.0, 50.0)
+    eps = 0.02
+    out = {}
+    forms = (fmm.PREC_FP64_EXACT, fmm.PREC_FP64) + ((fmm.PREC_MIXED_FP32, fmm.PREC_MIXED_BF16X3) if p <= 12 else ())
+    for prec in forms:
+        ctx, b, _ = _gpu_build(d, p=p, eps=eps, bounds=bounds, precision=prec)
+        ctx.evaluate(b)
+        assert np.isfinite(b.ax).all() and np.isfinite(b.ay).all(), prec
+        out[prec] = (b.ax.copy(), b.ay.copy())
+    e = out[fmm.PREC_FP64_EXACT]
+    assert _rel_diff(out[fmm.PREC_FP64], e).max() <= 1e-10
+    for prec in forms[2:]:
+        rd = _rel_diff(out[prec], e)
+        assert np.sqrt(np.mean(rd ** 2)) <= 1e-4, (prec, np.sqrt(np.mean(rd ** 2)))
+    ot, od, _ = _oracle_build(d, 32, bounds)
+    ax, ay, _ = O.compute_fmm_force(od["x"], od["y"], od["m"], ot, O.m2l_lists(ot), O.p2p_lists(ot), p, eps,
+                                    want_phi=True)
+    rd = _rel_diff(e, (ax, ay))
+    assert np.median(rd) <= 1e-6 and rd.max() <= 1e-9, (np.median(rd), rd.max())
+
+
 def _forms_run(d, p, eps, bounds, algo=fmm.ALGO_REFERENCE, n_max=32):
     out = {}
     for prec in (fmm.PREC_FP64, fmm.PREC_MIXED_FP32, fmm.PREC_MIXED_BF16X3):
