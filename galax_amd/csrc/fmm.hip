@@ -532,6 +532,95 @@ __global__ void __launch_bounds__(256) k_m2l_exact(const uint32_t* __restrict__ 
   }
 }
 
+// CORRECTED variant, p <= 7: one wave per target, one lane per pair. Each lane
+// builds its pair's derivative table in registers (the same recurrence as
+// derivs_recurrence, fully unrolled) and contracts it with its source's
+// multipole into S local coefficients; the wave then sums the lanes' locals
+// (xor butterfly, a fixed order) and adds them over batches of 64 pairs. The
+// table and the locals stay in registers, so a pair costs about 2(2p+1)^2 +
+// 2 S^2 FP64 operations with no LDS traffic and no serial steps.
+template <int P>
+__global__ void __launch_bounds__(64) k_m2l_exact_lp(const uint32_t* __restrict__ off, const uint32_t* __restrict__ dat,
+                                                     const double* __restrict__ comx, const double* __restrict__ comy,
+                                                     double eps2, const double* __restrict__ mp,
+                                                     double* __restrict__ loc) {
+  constexpr int S = num_moments(P);
+  constexpr int Q = 2 * P;                       // highest derivative order
+  constexpr int NG = (Q + 1) * (Q + 2) / 2;      // D(n, l) at n (n + 1) / 2 + l
+  const uint32_t t = blockIdx.x;
+  const int lane = threadIdx.x;
+  const uint32_t e0 = off[t], e1 = off[t + 1];
+  const double tx = comx[t], ty = comy[t];
+  double Lt[S];
+#pragma unroll
+  for (int k = 0; k < S; ++k) Lt[k] = 0.0;
+  for (uint32_t eb = e0; eb < e1; eb += 64) {  // uniform
+    const uint32_t e = eb + (uint32_t)lane;
+    double L[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) L[k] = 0.0;
+    if (e < e1) {
+      const uint32_t sidx = dat[e];
+      const double x = tx - comx[sidx], y = ty - comy[sidx];
+      const double r2 = x * x + y * y + eps2;
+      const double ir2 = 1.0 / r2;
+      double D[NG];
+      D[0] = 1.0 / sqrt(r2);
+#pragma unroll
+      for (int n = 1; n <= Q; ++n) {
+#pragma unroll
+        for (int k = 0; k <= n; ++k) {
+          const int l = n - k;
+          double v;
+          if (k >= 1) {
+            v = -(2.0 * k - 1.0) * x * D[(n - 1) * n / 2 + l];
+            if (k >= 2) v -= (double)(k - 1) * (k - 1) * D[(n - 2) * (n - 1) / 2 + l];
+            if (l >= 1) v -= 2.0 * l * y * D[(n - 1) * n / 2 + (l - 1)];
+            if (l >= 2) v -= (double)l * (l - 1) * D[(n - 2) * (n - 1) / 2 + (l - 2)];
+          } else {
+            v = -(2.0 * l - 1.0) * y * D[(n - 1) * n / 2 + (l - 1)];
+            if (l >= 2) v -= (double)(l - 1) * (l - 1) * D[(n - 2) * (n - 1) / 2 + (l - 2)];
+          }
+          D[n * (n + 1) / 2 + l] = v * ir2;
+        }
+      }
+      const double* ms = mp + (size_t)sidx * S;
+#pragma unroll
+      for (int i = 0; i <= P; ++i)
+#pragma unroll
+        for (int j = 0; j <= P - i; ++j) {
+          double mv = ms[moment_index(i, j)];
+          if (!isfinite(mv)) mv = 0.0;
+          // w * G = sign/(I!J!) * (-D); the 1/(I!J!) is applied once at the end
+          const double smv = ((i + j) % 2 == 0) ? -mv : mv;
+#pragma unroll
+          for (int I = 0; I <= P; ++I)
+#pragma unroll
+            for (int J = 0; J <= P - I; ++J) {
+              const int n = i + I + j + J;
+              L[moment_index(I, J)] += smv * D[n * (n + 1) / 2 + (j + J)];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      double v = L[k];
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+      Lt[k] += v;
+    }
+  }
+  // every lane holds the sums; lane k writes coefficient k
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    if (lane == (k & 63)) {
+      int I, J;
+      coef_ij<P>(k, I, J);
+      loc[(size_t)t * S + k] = Lt[k] * (1.0 / (c_tabs.fact[I] * c_tabs.fact[J]));
+    }
+  }
+}
+
 // --------------------------------------------------------------- L2L ----
 
 // exact: L_child[oi,oj] += sum_{i>=oi, j>=oj, i+j<=p} L_par[i,j] C(i,oi) C(j,oj) dx^(i-oi) dy^(j-oj)
@@ -1086,8 +1175,15 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   if (h->timing) GX_HIP(hipEventRecord(ev[3], s));
   if (h->timing) GX_HIP(hipEventRecord(ev[4], s));
   if (corrected) {
-    k_m2l_exact<P><<<nblk(nn, 4), 256, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(),
-                                               (uint32_t)nn, comx, comy, eps * eps, mp, loc);
+    // lane per pair up to p = 7 (the table and locals fit in registers);
+    // above, the wave-per-pair kernel
+    if constexpr (P <= 7) {
+      k_m2l_exact_lp<P><<<(unsigned)nn, 64, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(), comx,
+                                                    comy, eps * eps, mp, loc);
+    } else {
+      k_m2l_exact<P><<<nblk(nn, 4), 256, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(),
+                                                 (uint32_t)nn, comx, comy, eps * eps, mp, loc);
+    }
     GX_HIP(hipGetLastError());
   } else if (live_hi > 0 && h->cfg.precision != GALAX_PREC_FP64_EXACT) {
     GX_CHECK(m2l_mfma_run(h, mp, loc));
