@@ -532,7 +532,7 @@ __global__ void __launch_bounds__(256) k_m2l_exact(const uint32_t* __restrict__ 
   }
 }
 
-// CORRECTED variant, p <= 7: one wave per target, one lane per pair. Each lane
+// CORRECTED variant, p <= 8: one wave per target, one lane per pair. Each lane
 // builds its pair's derivative table in registers (the same recurrence as
 // derivs_recurrence, fully unrolled) and contracts it with its source's
 // multipole into S local coefficients; the wave then sums the lanes' locals
@@ -1175,9 +1175,9 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   if (h->timing) GX_HIP(hipEventRecord(ev[3], s));
   if (h->timing) GX_HIP(hipEventRecord(ev[4], s));
   if (corrected) {
-    // lane per pair up to p = 7 (the table and locals fit in registers);
+    // lane per pair up to p = 8 (at p = 8 some of the table spills, still faster);
     // above, the wave-per-pair kernel
-    if constexpr (P <= 7) {
+    if constexpr (P <= 8) {
       k_m2l_exact_lp<P><<<(unsigned)nn, 64, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(), comx,
                                                     comy, eps * eps, mp, loc);
     } else {
