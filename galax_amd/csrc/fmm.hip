@@ -621,6 +621,101 @@ __global__ void __launch_bounds__(64) k_m2l_exact_lp(const uint32_t* __restrict_
   }
 }
 
+// The same in FP32 for the mixed forms (p <= 8). Each pair is rescaled by
+// 2^a, a power of two near its distance: x' = x 2^-a, so the table is
+// D' = D 2^{a(n+1)} (the recurrence is homogeneous), the multipole enters as
+// M' = M 2^{-a(i+j)} and the pair's locals leave as L' 2^{-a(I+J+1)}, all
+// exact scalings; every FP32 quantity is then of moderate size. The lanes'
+// locals are summed in FP64.
+template <int P>
+__global__ void __launch_bounds__(64) k_m2l_exact_lp32(const uint32_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ dat,
+                                                       const double* __restrict__ comx,
+                                                       const double* __restrict__ comy, double eps2,
+                                                       const double* __restrict__ mp, double* __restrict__ loc) {
+  constexpr int S = num_moments(P);
+  constexpr int Q = 2 * P;
+  constexpr int NG = (Q + 1) * (Q + 2) / 2;
+  const uint32_t t = blockIdx.x;
+  const int lane = threadIdx.x;
+  const uint32_t e0 = off[t], e1 = off[t + 1];
+  const double tx = comx[t], ty = comy[t];
+  double Lt[S];
+#pragma unroll
+  for (int k = 0; k < S; ++k) Lt[k] = 0.0;
+  for (uint32_t eb = e0; eb < e1; eb += 64) {  // uniform
+    const uint32_t e = eb + (uint32_t)lane;
+    float L[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) L[k] = 0.f;
+    int a = 0;
+    if (e < e1) {
+      const uint32_t sidx = dat[e];
+      const double x = tx - comx[sidx], y = ty - comy[sidx];
+      const double r2 = x * x + y * y + eps2;
+      // 2^a within a factor 2 of the distance; a zero or non-finite distance
+      // keeps a = 0 and gives non-finite locals, as the FP64 kernel does
+      if (r2 > 0.0 && isfinite(r2)) a = ilogb(r2) / 2;
+      const float xs = (float)ldexp(x, -a), ys = (float)ldexp(y, -a);
+      const double r2s = ldexp(r2, -2 * a);
+      const float ir2 = (float)(1.0 / r2s);
+      float D[NG];
+      D[0] = (float)(1.0 / sqrt(r2s));
+#pragma unroll
+      for (int n = 1; n <= Q; ++n) {
+#pragma unroll
+        for (int k = 0; k <= n; ++k) {
+          const int l = n - k;
+          float v;
+          if (k >= 1) {
+            v = -(2.f * k - 1.f) * xs * D[(n - 1) * n / 2 + l];
+            if (k >= 2) v -= (float)((k - 1) * (k - 1)) * D[(n - 2) * (n - 1) / 2 + l];
+            if (l >= 1) v -= 2.f * l * ys * D[(n - 1) * n / 2 + (l - 1)];
+            if (l >= 2) v -= (float)(l * (l - 1)) * D[(n - 2) * (n - 1) / 2 + (l - 2)];
+          } else {
+            v = -(2.f * l - 1.f) * ys * D[(n - 1) * n / 2 + (l - 1)];
+            if (l >= 2) v -= (float)((l - 1) * (l - 1)) * D[(n - 2) * (n - 1) / 2 + (l - 2)];
+          }
+          D[n * (n + 1) / 2 + l] = v * ir2;
+        }
+      }
+      const double* ms = mp + (size_t)sidx * S;
+#pragma unroll
+      for (int i = 0; i <= P; ++i)
+#pragma unroll
+        for (int j = 0; j <= P - i; ++j) {
+          double mv = ms[moment_index(i, j)];
+          if (!isfinite(mv)) mv = 0.0;
+          const float smv = (float)ldexp(((i + j) % 2 == 0) ? -mv : mv, -a * (i + j));
+#pragma unroll
+          for (int I = 0; I <= P; ++I)
+#pragma unroll
+            for (int J = 0; J <= P - I; ++J) {
+              const int n = i + I + j + J;
+              L[moment_index(I, J)] = fmaf(smv, D[n * (n + 1) / 2 + (j + J)], L[moment_index(I, J)]);
+            }
+        }
+    }
+#pragma unroll
+    for (int I = 0; I <= P; ++I)
+#pragma unroll
+      for (int J = 0; J <= P - I; ++J) {
+        double v = ldexp((double)L[moment_index(I, J)], -a * (I + J + 1));
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+        Lt[moment_index(I, J)] += v;
+      }
+  }
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    if (lane == (k & 63)) {
+      int I, J;
+      coef_ij<P>(k, I, J);
+      loc[(size_t)t * S + k] = Lt[k] * (1.0 / (c_tabs.fact[I] * c_tabs.fact[J]));
+    }
+  }
+}
+
 // --------------------------------------------------------------- L2L ----
 
 // exact: L_child[oi,oj] += sum_{i>=oi, j>=oj, i+j<=p} L_par[i,j] C(i,oi) C(j,oj) dx^(i-oi) dy^(j-oj)
@@ -1178,8 +1273,13 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
     // lane per pair up to p = 8 (at p = 8 some of the table spills, still faster);
     // above, the wave-per-pair kernel
     if constexpr (P <= 8) {
-      k_m2l_exact_lp<P><<<(unsigned)nn, 64, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(), comx,
-                                                    comy, eps * eps, mp, loc);
+      const bool mixed = h->cfg.precision == GALAX_PREC_MIXED_FP32 || h->cfg.precision == GALAX_PREC_MIXED_BF16X3;
+      if (mixed)
+        k_m2l_exact_lp32<P><<<(unsigned)nn, 64, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(),
+                                                        comx, comy, eps * eps, mp, loc);
+      else
+        k_m2l_exact_lp<P><<<(unsigned)nn, 64, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(), comx,
+                                                      comy, eps * eps, mp, loc);
     } else {
       k_m2l_exact<P><<<nblk(nn, 4), 256, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(),
                                                  (uint32_t)nn, comx, comy, eps * eps, mp, loc);
