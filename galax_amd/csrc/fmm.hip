@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "common.h"
 
@@ -533,27 +534,107 @@ __global__ void __launch_bounds__(256) k_m2l_exact(const uint32_t* __restrict__ 
 }
 
 // CORRECTED variant, p <= 8: one wave per target, one lane per pair. Each lane
-// builds its pair's derivative table in registers (the same recurrence as
-// derivs_recurrence, fully unrolled) and contracts it with its source's
-// multipole into S local coefficients; the wave then sums the lanes' locals
-// (xor butterfly, a fixed order) and adds them over batches of 64 pairs. The
-// table and the locals stay in registers, so a pair costs about 2(2p+1)^2 +
-// 2 S^2 FP64 operations with no LDS traffic and no serial steps.
+// runs its pair's derivative recurrence order by order (the recurrence of
+// derivs_recurrence, fully unrolled) and scatters every new entry
+// D(k, l) straight into the S local coefficients it feeds,
+//   L(I, J) += s(i, j) M(i, j) D(i + I, j + J),  i = k - I, j = l - J,
+// so only three orders of the table are live and the moments, the locals and
+// that window stay in registers. The wave then sums the lanes' locals with a
+// halving exchange (lane c ends with coefficient c, a fixed order) and adds
+// them over batches of 64 pairs. A pair costs about 4(2p+1)^2 + 2 S^2
+// operations with no LDS traffic and no serial steps.
+// Calls f(std::integral_constant<int, i>) for i = B .. E-1: loop indices that
+// are constants to the front end, so `if constexpr` prunes the dead terms.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+template <int P, typename T>
+__device__ __forceinline__ void m2l_pair_scatter(T x, T y, T ir2, T d0, const T (&smv)[num_moments(P)],
+                                                 T (&L)[num_moments(P)]) {
+  constexpr int Q = 2 * P;
+  T D[3][Q + 1];  // order n lives in D[n % 3], entry l = number of y derivatives
+  static_for<0, Q + 1>([&](auto nc) {
+    constexpr int n = decltype(nc)::value;
+    static_for<0, n + 1>([&](auto lc) {
+      constexpr int l = decltype(lc)::value;
+      constexpr int k = n - l;
+      T v;
+      if constexpr (n == 0) {
+        v = d0;
+      } else {
+        const T* D1 = D[(n + 2) % 3];
+        const T* D2 = D[(n + 1) % 3];
+        if constexpr (k >= 1) {
+          v = -(T)(2 * k - 1) * x * D1[l];
+          if constexpr (k >= 2) v -= (T)((k - 1) * (k - 1)) * D2[l];
+          if constexpr (l >= 1) v -= (T)(2 * l) * y * D1[l - 1];
+          if constexpr (l >= 2) v -= (T)(l * (l - 1)) * D2[l - 2];
+        } else {
+          v = -(T)(2 * l - 1) * y * D1[l - 1];
+          if constexpr (l >= 2) v -= (T)((l - 1) * (l - 1)) * D2[l - 2];
+        }
+        v *= ir2;
+      }
+      D[n % 3][l] = v;
+      // the locals this entry feeds: I <= k, J <= l, I + J <= P, i + j <= P
+      static_for<0, (k < P ? k : P) + 1>([&](auto Ic) {
+        constexpr int I = decltype(Ic)::value;
+        constexpr int jlo = (n - P - I > 0 ? n - P - I : 0);  // J >= n - P - I
+        constexpr int jhi = (l < P - I ? l : P - I);
+        static_for<jlo, jhi + 1>([&](auto Jc) {
+          constexpr int J = decltype(Jc)::value;
+          L[moment_index(I, J)] = fma(smv[moment_index(k - I, l - J)], v, L[moment_index(I, J)]);
+        });
+      });
+    });
+  });
+}
+
+// Sums v[0..S) over the wave; lane c returns the sum of coefficient c (0 for
+// c >= S). Each of the six steps halves the coefficients a lane carries: the
+// lane keeps the half its lane bit selects and adds its partner's copy of it.
+template <int S>
+__device__ __forceinline__ double wave_sum_scatter(const double (&v)[S], int lane) {
+  static_assert(S <= 64, "at most one coefficient per lane");
+  double w[32];
+  {
+    const bool hi = lane & 32;
+#pragma unroll
+    for (int c = 0; c < 32; ++c) {
+      const double lo_v = c < S ? v[c] : 0.0, hi_v = c + 32 < S ? v[c + 32] : 0.0;
+      w[c] = (hi ? hi_v : lo_v) + __shfl_xor(hi ? lo_v : hi_v, 32, 64);
+    }
+  }
+#pragma unroll
+  for (int st = 0; st < 5; ++st) {  // a counted loop, so that it unrolls
+    const int h = 16 >> st;
+    const bool hi = lane & h;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      if (c >= h) continue;
+      const double lo_v = w[c], hi_v = w[c + h];
+      w[c] = (hi ? hi_v : lo_v) + __shfl_xor(hi ? lo_v : hi_v, h, 64);
+    }
+  }
+  return w[0];
+}
+
 template <int P>
 __global__ void __launch_bounds__(64) k_m2l_exact_lp(const uint32_t* __restrict__ off, const uint32_t* __restrict__ dat,
                                                      const double* __restrict__ comx, const double* __restrict__ comy,
                                                      double eps2, const double* __restrict__ mp,
                                                      double* __restrict__ loc) {
   constexpr int S = num_moments(P);
-  constexpr int Q = 2 * P;                       // highest derivative order
-  constexpr int NG = (Q + 1) * (Q + 2) / 2;      // D(n, l) at n (n + 1) / 2 + l
   const uint32_t t = blockIdx.x;
   const int lane = threadIdx.x;
   const uint32_t e0 = off[t], e1 = off[t + 1];
   const double tx = comx[t], ty = comy[t];
-  double Lt[S];
-#pragma unroll
-  for (int k = 0; k < S; ++k) Lt[k] = 0.0;
+  double acc = 0.0;  // coefficient `lane`
   for (uint32_t eb = e0; eb < e1; eb += 64) {  // uniform
     const uint32_t e = eb + (uint32_t)lane;
     double L[S];
@@ -563,61 +644,25 @@ __global__ void __launch_bounds__(64) k_m2l_exact_lp(const uint32_t* __restrict_
       const uint32_t sidx = dat[e];
       const double x = tx - comx[sidx], y = ty - comy[sidx];
       const double r2 = x * x + y * y + eps2;
-      const double ir2 = 1.0 / r2;
-      double D[NG];
-      D[0] = 1.0 / sqrt(r2);
-#pragma unroll
-      for (int n = 1; n <= Q; ++n) {
-#pragma unroll
-        for (int k = 0; k <= n; ++k) {
-          const int l = n - k;
-          double v;
-          if (k >= 1) {
-            v = -(2.0 * k - 1.0) * x * D[(n - 1) * n / 2 + l];
-            if (k >= 2) v -= (double)(k - 1) * (k - 1) * D[(n - 2) * (n - 1) / 2 + l];
-            if (l >= 1) v -= 2.0 * l * y * D[(n - 1) * n / 2 + (l - 1)];
-            if (l >= 2) v -= (double)l * (l - 1) * D[(n - 2) * (n - 1) / 2 + (l - 2)];
-          } else {
-            v = -(2.0 * l - 1.0) * y * D[(n - 1) * n / 2 + (l - 1)];
-            if (l >= 2) v -= (double)(l - 1) * (l - 1) * D[(n - 2) * (n - 1) / 2 + (l - 2)];
-          }
-          D[n * (n + 1) / 2 + l] = v * ir2;
-        }
-      }
       const double* ms = mp + (size_t)sidx * S;
+      double smv[S];
 #pragma unroll
-      for (int i = 0; i <= P; ++i)
+      for (int q = 0; q <= P; ++q)
 #pragma unroll
-        for (int j = 0; j <= P - i; ++j) {
-          double mv = ms[moment_index(i, j)];
+        for (int j = 0; j <= q; ++j) {
+          double mv = ms[moment_index(q - j, j)];
           if (!isfinite(mv)) mv = 0.0;
           // w * G = sign/(I!J!) * (-D); the 1/(I!J!) is applied once at the end
-          const double smv = ((i + j) % 2 == 0) ? -mv : mv;
-#pragma unroll
-          for (int I = 0; I <= P; ++I)
-#pragma unroll
-            for (int J = 0; J <= P - I; ++J) {
-              const int n = i + I + j + J;
-              L[moment_index(I, J)] += smv * D[n * (n + 1) / 2 + (j + J)];
-            }
+          smv[moment_index(q - j, j)] = (q % 2 == 0) ? -mv : mv;
         }
+      m2l_pair_scatter<P, double>(x, y, 1.0 / r2, 1.0 / sqrt(r2), smv, L);
     }
-#pragma unroll
-    for (int k = 0; k < S; ++k) {
-      double v = L[k];
-#pragma unroll
-      for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-      Lt[k] += v;
-    }
+    acc += wave_sum_scatter<S>(L, lane);
   }
-  // every lane holds the sums; lane k writes coefficient k
-#pragma unroll
-  for (int k = 0; k < S; ++k) {
-    if (lane == (k & 63)) {
-      int I, J;
-      coef_ij<P>(k, I, J);
-      loc[(size_t)t * S + k] = Lt[k] * (1.0 / (c_tabs.fact[I] * c_tabs.fact[J]));
-    }
+  if (lane < S) {
+    int I, J;
+    coef_ij<P>(lane, I, J);
+    loc[(size_t)t * S + lane] = acc * (1.0 / (c_tabs.fact[I] * c_tabs.fact[J]));
   }
 }
 
@@ -634,15 +679,11 @@ __global__ void __launch_bounds__(64) k_m2l_exact_lp32(const uint32_t* __restric
                                                        const double* __restrict__ comy, double eps2,
                                                        const double* __restrict__ mp, double* __restrict__ loc) {
   constexpr int S = num_moments(P);
-  constexpr int Q = 2 * P;
-  constexpr int NG = (Q + 1) * (Q + 2) / 2;
   const uint32_t t = blockIdx.x;
   const int lane = threadIdx.x;
   const uint32_t e0 = off[t], e1 = off[t + 1];
   const double tx = comx[t], ty = comy[t];
-  double Lt[S];
-#pragma unroll
-  for (int k = 0; k < S; ++k) Lt[k] = 0.0;
+  double acc = 0.0;  // coefficient `lane`
   for (uint32_t eb = e0; eb < e1; eb += 64) {  // uniform
     const uint32_t e = eb + (uint32_t)lane;
     float L[S];
@@ -656,63 +697,31 @@ __global__ void __launch_bounds__(64) k_m2l_exact_lp32(const uint32_t* __restric
       // 2^a within a factor 2 of the distance; a zero or non-finite distance
       // keeps a = 0 and gives non-finite locals, as the FP64 kernel does
       if (r2 > 0.0 && isfinite(r2)) a = ilogb(r2) / 2;
-      const float xs = (float)ldexp(x, -a), ys = (float)ldexp(y, -a);
       const double r2s = ldexp(r2, -2 * a);
-      const float ir2 = (float)(1.0 / r2s);
-      float D[NG];
-      D[0] = (float)(1.0 / sqrt(r2s));
-#pragma unroll
-      for (int n = 1; n <= Q; ++n) {
-#pragma unroll
-        for (int k = 0; k <= n; ++k) {
-          const int l = n - k;
-          float v;
-          if (k >= 1) {
-            v = -(2.f * k - 1.f) * xs * D[(n - 1) * n / 2 + l];
-            if (k >= 2) v -= (float)((k - 1) * (k - 1)) * D[(n - 2) * (n - 1) / 2 + l];
-            if (l >= 1) v -= 2.f * l * ys * D[(n - 1) * n / 2 + (l - 1)];
-            if (l >= 2) v -= (float)(l * (l - 1)) * D[(n - 2) * (n - 1) / 2 + (l - 2)];
-          } else {
-            v = -(2.f * l - 1.f) * ys * D[(n - 1) * n / 2 + (l - 1)];
-            if (l >= 2) v -= (float)((l - 1) * (l - 1)) * D[(n - 2) * (n - 1) / 2 + (l - 2)];
-          }
-          D[n * (n + 1) / 2 + l] = v * ir2;
-        }
-      }
       const double* ms = mp + (size_t)sidx * S;
+      float smv[S];
 #pragma unroll
-      for (int i = 0; i <= P; ++i)
+      for (int q = 0; q <= P; ++q)
 #pragma unroll
-        for (int j = 0; j <= P - i; ++j) {
-          double mv = ms[moment_index(i, j)];
+        for (int j = 0; j <= q; ++j) {
+          double mv = ms[moment_index(q - j, j)];
           if (!isfinite(mv)) mv = 0.0;
-          const float smv = (float)ldexp(((i + j) % 2 == 0) ? -mv : mv, -a * (i + j));
-#pragma unroll
-          for (int I = 0; I <= P; ++I)
-#pragma unroll
-            for (int J = 0; J <= P - I; ++J) {
-              const int n = i + I + j + J;
-              L[moment_index(I, J)] = fmaf(smv, D[n * (n + 1) / 2 + (j + J)], L[moment_index(I, J)]);
-            }
+          smv[moment_index(q - j, j)] = (float)ldexp((q % 2 == 0) ? -mv : mv, -a * q);
         }
+      m2l_pair_scatter<P, float>((float)ldexp(x, -a), (float)ldexp(y, -a), (float)(1.0 / r2s),
+                                 (float)(1.0 / sqrt(r2s)), smv, L);
     }
+    double Ld[S];
 #pragma unroll
-    for (int I = 0; I <= P; ++I)
+    for (int q = 0; q <= P; ++q)
 #pragma unroll
-      for (int J = 0; J <= P - I; ++J) {
-        double v = ldexp((double)L[moment_index(I, J)], -a * (I + J + 1));
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-        Lt[moment_index(I, J)] += v;
-      }
+      for (int j = 0; j <= q; ++j) Ld[moment_index(q - j, j)] = ldexp((double)L[moment_index(q - j, j)], -a * (q + 1));
+    acc += wave_sum_scatter<S>(Ld, lane);
   }
-#pragma unroll
-  for (int k = 0; k < S; ++k) {
-    if (lane == (k & 63)) {
-      int I, J;
-      coef_ij<P>(k, I, J);
-      loc[(size_t)t * S + k] = Lt[k] * (1.0 / (c_tabs.fact[I] * c_tabs.fact[J]));
-    }
+  if (lane < S) {
+    int I, J;
+    coef_ij<P>(lane, I, J);
+    loc[(size_t)t * S + lane] = acc * (1.0 / (c_tabs.fact[I] * c_tabs.fact[J]));
   }
 }
 
