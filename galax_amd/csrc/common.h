@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -242,5 +243,94 @@ int kdk_device(galax_fmm* h, double dt, uint64_t n_steps, uint64_t every, galax_
 // Host restatement of the kernel-derivative numerator polynomials
 // N_{k,l}[a][b] (galax-core/src/lib.rs:1059-1130); they depend on eps and p only.
 std::vector<double> kernel_polys(double eps, int p);
+
+// ------------------------------------------- corrected M2L, lane per pair ----
+// One lane runs one pair's derivative recurrence order by order (fully
+// unrolled) and scatters every new entry D(k, l) straight into the
+// coefficients it feeds: out(I, J) += in(k - I, l - J) D(k, l) over
+// I + J <= p and (k - I) + (l - J) <= p. The M2L (in = signed multipole,
+// out = locals) and its adjoint (in = local cotangent, out = multipole
+// cotangent) are both this contraction. Only three orders of the table are
+// live, so the table, `in` and `out` stay in registers.
+// Calls f(std::integral_constant<int, i>) for i = B .. E-1: loop indices that
+// are constants to the front end, so `if constexpr` prunes the dead terms.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+template <int P, typename T>
+__device__ __forceinline__ void m2l_pair_scatter(T x, T y, T ir2, T d0, const T (&smv)[num_moments(P)],
+                                                 T (&L)[num_moments(P)]) {
+  constexpr int Q = 2 * P;
+  T D[3][Q + 1];  // order n lives in D[n % 3], entry l = number of y derivatives
+  static_for<0, Q + 1>([&](auto nc) {
+    constexpr int n = decltype(nc)::value;
+    static_for<0, n + 1>([&](auto lc) {
+      constexpr int l = decltype(lc)::value;
+      constexpr int k = n - l;
+      T v;
+      if constexpr (n == 0) {
+        v = d0;
+      } else {
+        const T* D1 = D[(n + 2) % 3];
+        const T* D2 = D[(n + 1) % 3];
+        if constexpr (k >= 1) {
+          v = -(T)(2 * k - 1) * x * D1[l];
+          if constexpr (k >= 2) v -= (T)((k - 1) * (k - 1)) * D2[l];
+          if constexpr (l >= 1) v -= (T)(2 * l) * y * D1[l - 1];
+          if constexpr (l >= 2) v -= (T)(l * (l - 1)) * D2[l - 2];
+        } else {
+          v = -(T)(2 * l - 1) * y * D1[l - 1];
+          if constexpr (l >= 2) v -= (T)((l - 1) * (l - 1)) * D2[l - 2];
+        }
+        v *= ir2;
+      }
+      D[n % 3][l] = v;
+      // the locals this entry feeds: I <= k, J <= l, I + J <= P, i + j <= P
+      static_for<0, (k < P ? k : P) + 1>([&](auto Ic) {
+        constexpr int I = decltype(Ic)::value;
+        constexpr int jlo = (n - P - I > 0 ? n - P - I : 0);  // J >= n - P - I
+        constexpr int jhi = (l < P - I ? l : P - I);
+        static_for<jlo, jhi + 1>([&](auto Jc) {
+          constexpr int J = decltype(Jc)::value;
+          L[moment_index(I, J)] = fma(smv[moment_index(k - I, l - J)], v, L[moment_index(I, J)]);
+        });
+      });
+    });
+  });
+}
+
+// Sums v[0..S) over the wave; lane c returns the sum of coefficient c (0 for
+// c >= S). Each of the six steps halves the coefficients a lane carries: the
+// lane keeps the half its lane bit selects and adds its partner's copy of it.
+template <int S>
+__device__ __forceinline__ double wave_sum_scatter(const double (&v)[S], int lane) {
+  static_assert(S <= 64, "at most one coefficient per lane");
+  double w[32];
+  {
+    const bool hi = lane & 32;
+#pragma unroll
+    for (int c = 0; c < 32; ++c) {
+      const double lo_v = c < S ? v[c] : 0.0, hi_v = c + 32 < S ? v[c + 32] : 0.0;
+      w[c] = (hi ? hi_v : lo_v) + __shfl_xor(hi ? lo_v : hi_v, 32, 64);
+    }
+  }
+#pragma unroll
+  for (int st = 0; st < 5; ++st) {  // a counted loop, so that it unrolls
+    const int h = 16 >> st;
+    const bool hi = lane & h;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      if (c >= h) continue;
+      const double lo_v = w[c], hi_v = w[c + h];
+      w[c] = (hi ? hi_v : lo_v) + __shfl_xor(hi ? lo_v : hi_v, h, 64);
+    }
+  }
+  return w[0];
+}
 
 }  // namespace gx

@@ -535,95 +535,14 @@ __global__ void __launch_bounds__(256) k_m2l_exact(const uint32_t* __restrict__ 
 
 // CORRECTED variant, p <= 8: one wave per target, one lane per pair. Each lane
 // runs its pair's derivative recurrence order by order (the recurrence of
-// derivs_recurrence, fully unrolled) and scatters every new entry
-// D(k, l) straight into the S local coefficients it feeds,
+// derivs_recurrence) and scatters every new entry D(k, l) straight into the S
+// local coefficients it feeds (m2l_pair_scatter),
 //   L(I, J) += s(i, j) M(i, j) D(i + I, j + J),  i = k - I, j = l - J,
 // so only three orders of the table are live and the moments, the locals and
 // that window stay in registers. The wave then sums the lanes' locals with a
 // halving exchange (lane c ends with coefficient c, a fixed order) and adds
 // them over batches of 64 pairs. A pair costs about 4(2p+1)^2 + 2 S^2
 // operations with no LDS traffic and no serial steps.
-// Calls f(std::integral_constant<int, i>) for i = B .. E-1: loop indices that
-// are constants to the front end, so `if constexpr` prunes the dead terms.
-template <int B, int E, typename F>
-__device__ __forceinline__ void static_for(F&& f) {
-  if constexpr (B < E) {
-    f(std::integral_constant<int, B>{});
-    static_for<B + 1, E>(f);
-  }
-}
-
-template <int P, typename T>
-__device__ __forceinline__ void m2l_pair_scatter(T x, T y, T ir2, T d0, const T (&smv)[num_moments(P)],
-                                                 T (&L)[num_moments(P)]) {
-  constexpr int Q = 2 * P;
-  T D[3][Q + 1];  // order n lives in D[n % 3], entry l = number of y derivatives
-  static_for<0, Q + 1>([&](auto nc) {
-    constexpr int n = decltype(nc)::value;
-    static_for<0, n + 1>([&](auto lc) {
-      constexpr int l = decltype(lc)::value;
-      constexpr int k = n - l;
-      T v;
-      if constexpr (n == 0) {
-        v = d0;
-      } else {
-        const T* D1 = D[(n + 2) % 3];
-        const T* D2 = D[(n + 1) % 3];
-        if constexpr (k >= 1) {
-          v = -(T)(2 * k - 1) * x * D1[l];
-          if constexpr (k >= 2) v -= (T)((k - 1) * (k - 1)) * D2[l];
-          if constexpr (l >= 1) v -= (T)(2 * l) * y * D1[l - 1];
-          if constexpr (l >= 2) v -= (T)(l * (l - 1)) * D2[l - 2];
-        } else {
-          v = -(T)(2 * l - 1) * y * D1[l - 1];
-          if constexpr (l >= 2) v -= (T)((l - 1) * (l - 1)) * D2[l - 2];
-        }
-        v *= ir2;
-      }
-      D[n % 3][l] = v;
-      // the locals this entry feeds: I <= k, J <= l, I + J <= P, i + j <= P
-      static_for<0, (k < P ? k : P) + 1>([&](auto Ic) {
-        constexpr int I = decltype(Ic)::value;
-        constexpr int jlo = (n - P - I > 0 ? n - P - I : 0);  // J >= n - P - I
-        constexpr int jhi = (l < P - I ? l : P - I);
-        static_for<jlo, jhi + 1>([&](auto Jc) {
-          constexpr int J = decltype(Jc)::value;
-          L[moment_index(I, J)] = fma(smv[moment_index(k - I, l - J)], v, L[moment_index(I, J)]);
-        });
-      });
-    });
-  });
-}
-
-// Sums v[0..S) over the wave; lane c returns the sum of coefficient c (0 for
-// c >= S). Each of the six steps halves the coefficients a lane carries: the
-// lane keeps the half its lane bit selects and adds its partner's copy of it.
-template <int S>
-__device__ __forceinline__ double wave_sum_scatter(const double (&v)[S], int lane) {
-  static_assert(S <= 64, "at most one coefficient per lane");
-  double w[32];
-  {
-    const bool hi = lane & 32;
-#pragma unroll
-    for (int c = 0; c < 32; ++c) {
-      const double lo_v = c < S ? v[c] : 0.0, hi_v = c + 32 < S ? v[c + 32] : 0.0;
-      w[c] = (hi ? hi_v : lo_v) + __shfl_xor(hi ? lo_v : hi_v, 32, 64);
-    }
-  }
-#pragma unroll
-  for (int st = 0; st < 5; ++st) {  // a counted loop, so that it unrolls
-    const int h = 16 >> st;
-    const bool hi = lane & h;
-#pragma unroll
-    for (int c = 0; c < 16; ++c) {
-      if (c >= h) continue;
-      const double lo_v = w[c], hi_v = w[c + h];
-      w[c] = (hi ? hi_v : lo_v) + __shfl_xor(hi ? lo_v : hi_v, h, 64);
-    }
-  }
-  return w[0];
-}
-
 template <int P>
 __global__ void __launch_bounds__(64) k_m2l_exact_lp(const uint32_t* __restrict__ off, const uint32_t* __restrict__ dat,
                                                      const double* __restrict__ comx, const double* __restrict__ comy,

@@ -454,10 +454,10 @@ __global__ void __launch_bounds__(256) k_vjp_m2l_exact(const uint32_t* __restric
   }
 }
 
-// Adjoint of the corrected M2L, p <= 6: one wave per source s, one lane per
-// partner t (the lists are symmetric), as k_m2l_exact_lp: the lane builds the
-// pair's derivative table in registers and contracts it with lbar_t scaled by
-// 1/(I!J!); the wave sums its lanes in a fixed order.
+// Adjoint of the corrected M2L, p <= 8: one wave per source s, one lane per
+// partner t (the lists are symmetric), as k_m2l_exact_lp: the lane runs the
+// pair's derivative recurrence and scatters it against lbar_t scaled by
+// 1/(I!J!) (m2l_pair_scatter); the wave sums its lanes in a fixed order.
 template <int P>
 __global__ void __launch_bounds__(64) k_vjp_m2l_exact_lp(const uint32_t* __restrict__ off,
                                                          const uint32_t* __restrict__ dat,
@@ -465,15 +465,11 @@ __global__ void __launch_bounds__(64) k_vjp_m2l_exact_lp(const uint32_t* __restr
                                                          const double* __restrict__ comy, double eps2,
                                                          const double* __restrict__ lbar, double* __restrict__ mbar) {
   constexpr int S = num_moments(P);
-  constexpr int Q = 2 * P;
-  constexpr int NG = (Q + 1) * (Q + 2) / 2;  // D(n, l) at n (n + 1) / 2 + l
   const uint32_t s = blockIdx.x;
   const int lane = threadIdx.x;
   const uint32_t r0 = off[s], r1 = off[s + 1];
   const double sx = comx[s], sy = comy[s];
-  double At[S];
-#pragma unroll
-  for (int k = 0; k < S; ++k) At[k] = 0.0;
+  double acc = 0.0;  // coefficient `lane`
   for (uint32_t rb = r0; rb < r1; rb += 64) {  // uniform
     const uint32_t r = rb + (uint32_t)lane;
     double A[S];
@@ -482,58 +478,23 @@ __global__ void __launch_bounds__(64) k_vjp_m2l_exact_lp(const uint32_t* __restr
     if (r < r1) {
       const uint32_t t = dat[r];
       const double x = comx[t] - sx, y = comy[t] - sy;
-      const double r2 = x * x + y * y + eps2, ir2 = 1.0 / r2;
-      double D[NG];
-      D[0] = 1.0 / sqrt(r2);
-#pragma unroll
-      for (int n = 1; n <= Q; ++n) {
-#pragma unroll
-        for (int k = 0; k <= n; ++k) {
-          const int l = n - k;
-          double v;
-          if (k >= 1) {
-            v = -(2.0 * k - 1.0) * x * D[(n - 1) * n / 2 + l];
-            if (k >= 2) v -= (double)(k - 1) * (k - 1) * D[(n - 2) * (n - 1) / 2 + l];
-            if (l >= 1) v -= 2.0 * l * y * D[(n - 1) * n / 2 + (l - 1)];
-            if (l >= 2) v -= (double)l * (l - 1) * D[(n - 2) * (n - 1) / 2 + (l - 2)];
-          } else {
-            v = -(2.0 * l - 1.0) * y * D[(n - 1) * n / 2 + (l - 1)];
-            if (l >= 2) v -= (double)(l - 1) * (l - 1) * D[(n - 2) * (n - 1) / 2 + (l - 2)];
-          }
-          D[n * (n + 1) / 2 + l] = v * ir2;
-        }
-      }
+      const double r2 = x * x + y * y + eps2;
       const double* Lb = lbar + (size_t)t * S;
+      double lv[S];
 #pragma unroll
-      for (int I = 0; I <= P; ++I)
+      for (int q = 0; q <= P; ++q)
 #pragma unroll
-        for (int J = 0; J <= P - I; ++J) {
-          const double lv = Lb[moment_index(I, J)] / (dfact(I) * dfact(J));
-#pragma unroll
-          for (int i = 0; i <= P; ++i)
-#pragma unroll
-            for (int j = 0; j <= P - i; ++j) {
-              const int n = i + I + j + J;
-              A[moment_index(i, j)] += D[n * (n + 1) / 2 + (j + J)] * lv;
-            }
-        }
+        for (int J = 0; J <= q; ++J) lv[moment_index(q - J, J)] = Lb[moment_index(q - J, J)] / (dfact(q - J) * dfact(J));
+      // A(i, j) += D(i + I, j + J) lv(I, J)
+      m2l_pair_scatter<P, double>(x, y, 1.0 / r2, 1.0 / sqrt(r2), lv, A);
     }
-#pragma unroll
-    for (int k = 0; k < S; ++k) {
-      double v = A[k];
-#pragma unroll
-      for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-      At[k] += v;
-    }
+    acc += wave_sum_scatter<S>(A, lane);
   }
-#pragma unroll
-  for (int k = 0; k < S; ++k) {
-    if (lane == (k & 63)) {
-      int i, j;
-      cij<P>(k, i, j);
-      const double sg = ((i + j) & 1) ? 1.0 : -1.0;  // (-1)^{i+j} * (G = -D)
-      mbar[(size_t)s * S + k] = sg * At[k];
-    }
+  if (lane < S) {
+    int i, j;
+    cij<P>(lane, i, j);
+    const double sg = ((i + j) & 1) ? 1.0 : -1.0;  // (-1)^{i+j} * (G = -D)
+    mbar[(size_t)s * S + lane] = sg * acc;
   }
 }
 
@@ -590,7 +551,7 @@ static int vjp_impl(galax_fmm* h, const double* x, const double* y, const double
   }
   // M2L adjoint
   if (corrected) {
-    if constexpr (P <= 6) {  // above, the lane's table and sums no longer fit in registers
+    if constexpr (P <= 8) {  // above, the lane's window and sums no longer fit in registers
       k_vjp_m2l_exact_lp<P><<<(unsigned)nn, 64, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(),
                                                         comx, comy, eps2, lbar, mbar);
     } else {
