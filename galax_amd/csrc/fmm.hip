@@ -689,6 +689,111 @@ __global__ void k_l2l(const uint32_t* __restrict__ par, const double* __restrict
   loc[(size_t)c * S + k] += l2l_sum<P>(loc + (size_t)p * S, &c_tabs.binom[0][0], MAX_P + 1, oi, oj, dx, dy);
 }
 
+// CORRECTED variant, p <= 8: the all-level M2M and L2L, one thread per node.
+// The thread keeps the source expansion and the offset's powers in
+// registers and applies the shift with its binomial and factorial constants
+// known at compile time: no LDS, no barriers, no table loads, about 2 S^2 / 4
+// operations per shift. The sums are the same as k_m2m's and l2l_sum's, in
+// another order and with multiplications by 1/a! instead of divisions by a!.
+constexpr double ct_fact(int a) { return a <= 1 ? 1.0 : a * ct_fact(a - 1); }
+constexpr double ct_binom(int n, int k) { return ct_fact(n) / (ct_fact(k) * ct_fact(n - k)); }
+
+// parents in [lo, hi): M_p(oi, oj) = sum over children c and i <= oi, j <= oj
+// of M_c(i, j) dx^(oi-i) dy^(oj-j) / ((oi-i)! (oj-j)!), (dx, dy) = com_c - com_p
+template <int P>
+__global__ void __launch_bounds__(256) k_m2m_nodes(const uint32_t* __restrict__ cf, const uint8_t* __restrict__ cc,
+                                                   const double* __restrict__ comx, const double* __restrict__ comy,
+                                                   uint32_t lo, uint32_t hi, double* __restrict__ mp) {
+  constexpr int S = num_moments(P);
+  const uint32_t p = lo + blockIdx.x * 256u + threadIdx.x;
+  if (p >= hi) return;
+  const uint32_t c0 = cf[p], nc = cc[p];
+  const double px = comx[p], py = comy[p];
+  double acc[S];
+#pragma unroll
+  for (int k = 0; k < S; ++k) acc[k] = 0.0;
+  for (uint32_t ci = 0; ci < nc; ++ci) {
+    const uint32_t c = c0 + ci;
+    const double dx = comx[c] - px, dy = comy[c] - py;
+    double tx[P + 1], ty[P + 1];  // dx^a / a!, dy^b / b!
+    tx[0] = ty[0] = 1.0;
+#pragma unroll
+    for (int a = 1; a <= P; ++a) {
+      tx[a] = tx[a - 1] * dx;
+      ty[a] = ty[a - 1] * dy;
+    }
+    static_for<2, P + 1>([&](auto ac) {
+      constexpr int a = decltype(ac)::value;
+      tx[a] *= 1.0 / ct_fact(a);
+      ty[a] *= 1.0 / ct_fact(a);
+    });
+    const double* mc = mp + (size_t)c * S;
+    double m[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) m[k] = mc[k];
+    static_for<0, P + 1>([&](auto nc_) {
+      constexpr int n = decltype(nc_)::value;
+      static_for<0, n + 1>([&](auto jc) {
+        constexpr int oj = decltype(jc)::value, oi = n - oj;
+        double sum = acc[moment_index(oi, oj)];
+        static_for<0, oi + 1>([&](auto ic) {
+          constexpr int i = decltype(ic)::value;
+          static_for<0, oj + 1>([&](auto jjc) {
+            constexpr int j = decltype(jjc)::value;
+            sum = fma(m[moment_index(i, j)], tx[oi - i] * ty[oj - j], sum);
+          });
+        });
+        acc[moment_index(oi, oj)] = sum;
+      });
+    });
+  }
+  double* out = mp + (size_t)p * S;
+#pragma unroll
+  for (int k = 0; k < S; ++k) out[k] = isfinite(acc[k]) ? acc[k] : 0.0;  // non-finite -> 0 (galax-core/src/lib.rs:934-936)
+}
+
+// children in [lo, hi): L_c(oi, oj) += sum over i >= oi, j >= oj, i + j <= p
+// of L_p(i, j) C(i, oi) C(j, oj) dx^(i-oi) dy^(j-oj), (dx, dy) = com_c - com_p
+template <int P>
+__global__ void __launch_bounds__(256) k_l2l_nodes(const uint32_t* __restrict__ par, const double* __restrict__ comx,
+                                                   const double* __restrict__ comy, uint32_t lo, uint32_t hi,
+                                                   double* __restrict__ loc) {
+  constexpr int S = num_moments(P);
+  const uint32_t c = lo + blockIdx.x * 256u + threadIdx.x;
+  if (c >= hi) return;
+  const uint32_t p = par[c];
+  if (p == NONE) return;
+  const double dx = comx[c] - comx[p], dy = comy[c] - comy[p];
+  double ex[P + 1], ey[P + 1];
+  ex[0] = ey[0] = 1.0;
+#pragma unroll
+  for (int a = 1; a <= P; ++a) {
+    ex[a] = ex[a - 1] * dx;
+    ey[a] = ey[a - 1] * dy;
+  }
+  const double* lp = loc + (size_t)p * S;
+  double l[S];
+#pragma unroll
+  for (int k = 0; k < S; ++k) l[k] = lp[k];
+  double* lc = loc + (size_t)c * S;
+  static_for<0, P + 1>([&](auto nc_) {
+    constexpr int n = decltype(nc_)::value;
+    static_for<0, n + 1>([&](auto jc) {
+      constexpr int oj = decltype(jc)::value, oi = n - oj;
+      double sum = 0.0;
+      static_for<oi, P + 1>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        static_for<oj, P - i + 1>([&](auto jjc) {
+          constexpr int j = decltype(jjc)::value;
+          constexpr double bc = ct_binom(i, oi) * ct_binom(j, oj);
+          sum = fma(l[moment_index(i, j)] * bc, ex[i - oi] * ey[j - oj], sum);
+        });
+      });
+      lc[moment_index(oi, oj)] += sum;
+    });
+  });
+}
+
 // ------------------------------------------------------ L2P + P2P ----
 
 // exact far field of one body (galax-core/src/lib.rs:1321-1345). ROWS: the
@@ -1190,6 +1295,14 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   {
     for (uint32_t l = 1; l < top; ++l) {
       uint32_t lo = t.level_offset[l], hi = t.level_offset[l + 1];
+      if constexpr (P <= 8) {
+        if (corrected) {
+          k_m2m_nodes<P><<<nblk(hi - lo), 256, 0, s>>>(t.child_first.as<uint32_t>(), t.child_count.as<uint8_t>(),
+                                                        comx, comy, lo, hi, mp);
+          GX_HIP(hipGetLastError());
+          continue;
+        }
+      }
       k_m2m<P><<<nblk(hi - lo, num_moments(P) <= 32 ? 8 : 4), 256, 0, s>>>(
           t.child_first.as<uint32_t>(), t.child_count.as<uint8_t>(), comx, comy, lo, hi, mp);
       GX_HIP(hipGetLastError());
@@ -1230,7 +1343,10 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   if (corrected) {
     for (uint32_t l = t.n_levels - 1; l >= l2l_stop; --l) {
       uint32_t lo = t.level_offset[l - 1], hi = t.level_offset[l];
-      k_l2l<P><<<nblk((size_t)(hi - lo) * S), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, lo, hi, loc);
+      if constexpr (P <= 8)
+        k_l2l_nodes<P><<<nblk(hi - lo), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, lo, hi, loc);
+      else
+        k_l2l<P><<<nblk((size_t)(hi - lo) * S), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, lo, hi, loc);
       GX_HIP(hipGetLastError());
     }
   } else if (t.n_levels >= 2 && !fuse_l2l) {

@@ -216,7 +216,7 @@ def test_upload_tree_matches_device_build():
 
 @pytest.mark.parametrize("p,bound", [(4, 5e-4), (6, 2e-5), (8, 2e-6)])
 def test_corrected_variant_matches_oracle_and_direct_sum(p, bound):
-    """p = 4 and 6 run the lane-per-pair M2L, p = 8 the wave-per-pair one."""
+    """p = 4, 6 and 8 run the lane-per-pair M2L and the thread-per-node M2M and L2L."""
     d = ic.plummer(20000, seed=203)
     bounds = (-50.0, 50.0, -50.0, 50.0)
     eps, theta = 0.05, 0.3
