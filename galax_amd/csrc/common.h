@@ -254,6 +254,10 @@ std::vector<double> kernel_polys(double eps, int p);
 // live, so the table, `in` and `out` stay in registers.
 // Calls f(std::integral_constant<int, i>) for i = B .. E-1: loop indices that
 // are constants to the front end, so `if constexpr` prunes the dead terms.
+// a! and C(n, k) as compile-time constants (exact in double for n <= 18)
+constexpr double ct_fact(int a) { return a <= 1 ? 1.0 : a * ct_fact(a - 1); }
+constexpr double ct_binom(int n, int k) { return ct_fact(n) / (ct_fact(k) * ct_fact(n - k)); }
+
 template <int B, int E, typename F>
 __device__ __forceinline__ void static_for(F&& f) {
   if constexpr (B < E) {

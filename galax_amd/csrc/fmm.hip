@@ -695,8 +695,6 @@ __global__ void k_l2l(const uint32_t* __restrict__ par, const double* __restrict
 // known at compile time: no LDS, no barriers, no table loads, about 2 S^2 / 4
 // operations per shift. The sums are the same as k_m2m's and l2l_sum's, in
 // another order and with multiplications by 1/a! instead of divisions by a!.
-constexpr double ct_fact(int a) { return a <= 1 ? 1.0 : a * ct_fact(a - 1); }
-constexpr double ct_binom(int n, int k) { return ct_fact(n) / (ct_fact(k) * ct_fact(n - k)); }
 
 // parents in [lo, hi): M_p(oi, oj) = sum over children c and i <= oi, j <= oj
 // of M_c(i, j) dx^(oi-i) dy^(oj-j) / ((oi-i)! (oj-j)!), (dx, dy) = com_c - com_p

@@ -382,6 +382,104 @@ __global__ void k_vjp_m2m(const uint32_t* __restrict__ par, const double* __rest
   mbar[(size_t)c * S + k] += acc;
 }
 
+// CORRECTED variant, p <= 8: the same two adjoints with one thread per node,
+// the cotangents and the offset's powers in registers and the shift
+// constants known at compile time (as k_m2m_nodes / k_l2l_nodes).
+template <int P>
+__global__ void __launch_bounds__(256) k_vjp_l2l_nodes(const uint32_t* __restrict__ cf,
+                                                       const uint8_t* __restrict__ cc,
+                                                       const double* __restrict__ comx,
+                                                       const double* __restrict__ comy, uint32_t lo, uint32_t hi,
+                                                       double* __restrict__ lbar) {
+  constexpr int S = num_moments(P);
+  const uint32_t p = lo + blockIdx.x * 256u + threadIdx.x;
+  if (p >= hi) return;
+  const uint32_t c0 = cf[p], nc = cc[p];
+  const double px = comx[p], py = comy[p];
+  double acc[S];
+#pragma unroll
+  for (int k = 0; k < S; ++k) acc[k] = 0.0;
+  for (uint32_t ci = 0; ci < nc; ++ci) {
+    const uint32_t c = c0 + ci;
+    const double dx = comx[c] - px, dy = comy[c] - py;
+    double ex[P + 1], ey[P + 1];
+    ex[0] = ey[0] = 1.0;
+#pragma unroll
+    for (int a = 1; a <= P; ++a) {
+      ex[a] = ex[a - 1] * dx;
+      ey[a] = ey[a - 1] * dy;
+    }
+    const double* lc = lbar + (size_t)c * S;
+    double l[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) l[k] = lc[k];
+    static_for<0, P + 1>([&](auto nc_) {
+      constexpr int n = decltype(nc_)::value;
+      static_for<0, n + 1>([&](auto jc) {
+        constexpr int j = decltype(jc)::value, i = n - j;
+        double sum = acc[moment_index(i, j)];
+        static_for<0, i + 1>([&](auto oic) {
+          constexpr int oi = decltype(oic)::value;
+          static_for<0, j + 1>([&](auto ojc) {
+            constexpr int oj = decltype(ojc)::value;
+            constexpr double bc = ct_binom(i, oi) * ct_binom(j, oj);
+            sum = fma(l[moment_index(oi, oj)] * bc, ex[i - oi] * ey[j - oj], sum);
+          });
+        });
+        acc[moment_index(i, j)] = sum;
+      });
+    });
+  }
+  double* out = lbar + (size_t)p * S;
+#pragma unroll
+  for (int k = 0; k < S; ++k) out[k] += acc[k];
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) k_vjp_m2m_nodes(const uint32_t* __restrict__ par,
+                                                       const double* __restrict__ comx,
+                                                       const double* __restrict__ comy, uint32_t lo, uint32_t hi,
+                                                       double* __restrict__ mbar) {
+  constexpr int S = num_moments(P);
+  const uint32_t c = lo + blockIdx.x * 256u + threadIdx.x;
+  if (c >= hi) return;
+  const uint32_t p = par[c];
+  if (p == NONE) return;
+  const double dx = comx[c] - comx[p], dy = comy[c] - comy[p];
+  double tx[P + 1], ty[P + 1];  // dx^a / a!, dy^b / b!
+  tx[0] = ty[0] = 1.0;
+#pragma unroll
+  for (int a = 1; a <= P; ++a) {
+    tx[a] = tx[a - 1] * dx;
+    ty[a] = ty[a - 1] * dy;
+  }
+  static_for<2, P + 1>([&](auto ac) {
+    constexpr int a = decltype(ac)::value;
+    tx[a] *= 1.0 / ct_fact(a);
+    ty[a] *= 1.0 / ct_fact(a);
+  });
+  const double* mp_ = mbar + (size_t)p * S;
+  double mv[S];
+#pragma unroll
+  for (int k = 0; k < S; ++k) mv[k] = mp_[k];
+  double* mc = mbar + (size_t)c * S;
+  static_for<0, P + 1>([&](auto nc_) {
+    constexpr int n = decltype(nc_)::value;
+    static_for<0, n + 1>([&](auto jc) {
+      constexpr int j = decltype(jc)::value, i = n - j;
+      double sum = 0.0;
+      static_for<i, P + 1>([&](auto Ic) {
+        constexpr int I = decltype(Ic)::value;
+        static_for<j, P - I + 1>([&](auto Jc) {
+          constexpr int J = decltype(Jc)::value;
+          sum = fma(mv[moment_index(I, J)], tx[I - i] * ty[J - j], sum);
+        });
+      });
+      mc[moment_index(i, j)] += sum;
+    });
+  });
+}
+
 // ------------------------------------------------------ M2L adjoint ----
 // Mbar_s[i,j] += sum_pairs sum_{I,J} (-1)^{i+j}/(I!J!) G[i+I, j+J] Lbar_t[I,J]
 
@@ -545,6 +643,14 @@ static int vjp_impl(galax_fmm* h, const double* x, const double* y, const double
   const uint32_t top = corrected ? t.n_levels : std::min<uint32_t>(t.n_levels, 2);
   for (uint32_t l = 1; l < top; ++l) {
     uint32_t lo = t.level_offset[l], hi = t.level_offset[l + 1];
+    if constexpr (P <= 8) {
+      if (corrected) {
+        k_vjp_l2l_nodes<P><<<nblk(hi - lo), 256, 0, s>>>(t.child_first.as<uint32_t>(), t.child_count.as<uint8_t>(),
+                                                          comx, comy, lo, hi, lbar);
+        GX_HIP(hipGetLastError());
+        continue;
+      }
+    }
     k_vjp_l2l<P><<<nblk((size_t)(hi - lo) * S), 256, 0, s>>>(t.child_first.as<uint32_t>(),
                                                              t.child_count.as<uint8_t>(), comx, comy, lo, hi, lbar);
     GX_HIP(hipGetLastError());
@@ -567,6 +673,13 @@ static int vjp_impl(galax_fmm* h, const double* x, const double* y, const double
   // M2M adjoint, top-down
   for (uint32_t l = top; l-- > 1;) {
     uint32_t lo = t.level_offset[l - 1], hi = t.level_offset[l];
+    if constexpr (P <= 8) {
+      if (corrected) {
+        k_vjp_m2m_nodes<P><<<nblk(hi - lo), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, lo, hi, mbar);
+        GX_HIP(hipGetLastError());
+        continue;
+      }
+    }
     k_vjp_m2m<P><<<nblk((size_t)(hi - lo) * S), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, lo, hi, mbar);
     GX_HIP(hipGetLastError());
   }
