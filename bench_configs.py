@@ -11,7 +11,8 @@
       with the tree rebuilt every 10 steps.
   c3  disk4m: p = 2, 4, 6, 8 x {FP64, MIXED_FP32, MIXED_BF16X3}: RMS / max
       relative force error vs the FP64 direct sum on 10,000 samples, forces/s;
-      and the corrected variant at p = 4, 6, 8 (FP64, MIXED_FP32).
+      and the corrected variant at p = 4, 6, 8 (theta = 0.3) and p = 8
+      (theta = 0.5), FP64 and MIXED_FP32.
   c4  galaxy262k, both algorithms: VJP with the FP32 and FP64 forwards;
       central differences (h = 1e-6, FP64) on 64 random coordinates; a
       50-step leapfrog (dt = 1e-4) with an MSE loss, checkpointed every 10
@@ -231,13 +232,14 @@ def c3(args, fmm, torch, ic):
             if r is not ref:
                 del r
         del ref
-    # the corrected variant (theta = 0.3): the FMM's own truncation error per p
+    # the corrected variant: the FMM's own truncation error per p and opening
+    # angle (p = 8 at theta = 0.5 is the cheapest point within 1e-5 here)
     out["corrected_sweep"] = []
-    for p in (4, 6, 8):
+    for p, theta in ((4, 0.3), (6, 0.3), (8, 0.3), (8, 0.5)):
         ref = None
         for name, prec in (("fp64", fmm.PREC_FP64), ("mixed_fp32", fmm.PREC_MIXED_FP32)):
-            r = _eval_form(fmm, torch, d, spec, p, prec, steps=3, warmup=1, algo=fmm.ALGO_CORRECTED, theta=0.3)
-            row = {"p": p, "form": name, "theta": 0.3, "ms_per_step": r["ms"], "bodies_per_s": n / (r["ms"] * 1e-3),
+            r = _eval_form(fmm, torch, d, spec, p, prec, steps=3, warmup=1, algo=fmm.ALGO_CORRECTED, theta=theta)
+            row = {"p": p, "form": name, "theta": theta, "ms_per_step": r["ms"], "bodies_per_s": n / (r["ms"] * 1e-3),
                    "errors_vs_direct": _errors(r["ax"], r["ay"], ex, ey, si.astype(np.int64)), "passes_ms": r["passes"]}
             if ref is None:
                 ref = r
@@ -245,7 +247,8 @@ def c3(args, fmm, torch, ic):
                 d64 = np.hypot(r["ax"] - ref["ax"], r["ay"] - ref["ay"]) / np.maximum(np.hypot(ref["ax"], ref["ay"]), 1e-300)
                 row["rms_vs_fp64"] = float(np.sqrt(np.mean(d64 ** 2)))
                 row["speedup_vs_fp64"] = ref["ms"] / r["ms"]
-            _log(f"[c3] corrected p={p} {name}: {r['ms']:.3f} ms, rms vs direct {row['errors_vs_direct']['rms_vec']:.3e}")
+            _log(f"[c3] corrected p={p} theta={theta} {name}: {r['ms']:.3f} ms, "
+                 f"rms vs direct {row['errors_vs_direct']['rms_vec']:.3e}")
             out["corrected_sweep"].append(row)
             if r is not ref:
                 del r

@@ -66,7 +66,8 @@ enum {
 enum {
   GALAX_PREC_FP64 = 0,          /* everything in FP64; M2L on FP64 MFMA (grouped by cache key) */
   GALAX_PREC_MIXED_FP32 = 1,    /* M2L products on FP32 MFMA, P2P in FP32 on leaf-relative coordinates;
-                                   expansions, geometry and sums in FP64 */
+                                   expansions, geometry and sums in FP64 (GALAX_ALGO_CORRECTED, p <= 8:
+                                   its per-pair M2L in FP32 on pairs rescaled by powers of two) */
   GALAX_PREC_MIXED_BF16X3 = 2,  /* as MIXED_FP32 with M2L as a 3-term split BF16 MFMA product */
   GALAX_PREC_FP64_EXACT = 3     /* FP64 with galax-core's M2L summation order (per pair, on the VALU):
                                    expansions bit-identical to the reference's arithmetic */
