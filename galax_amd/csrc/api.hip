@@ -34,45 +34,155 @@ __global__ void k_radii(const double* __restrict__ x, const double* __restrict__
   if (lane == 0) R[nd] = sqrt(r2);
 }
 
-namespace {
-struct DttCtx {
+// The dual tree traversal, breadth-first on the device. Every visited pair
+// (a, b) is kept in one buffer (a in the high 32 bits). A round turns each
+// pair of the previous round that is neither accepted nor a leaf pair into
+// its child pairs: both children lists of a self pair (i <= j), else the
+// children of the larger-radius non-leaf node. The pairs visited, hence the
+// lists, are those of the recursive traversal whatever the order of visits.
+// One pass then emits the accepted pairs (both directions) and the leaf
+// pairs as sort keys; sorted, they give each node's ascending M2L list and
+// each leaf's P2P list, the leaf itself first (the key 0 of its row).
+struct DttTree {
   const double *comx, *comy, *R;
   const uint32_t* cf;
   const uint8_t* cc;
   double theta;
-  std::vector<std::vector<uint32_t>> m2l, p2p;
 };
 
-void dtt(DttCtx& c, uint32_t a, uint32_t b) {
-  const bool la = c.cc[a] == 0, lb = c.cc[b] == 0;
-  if (a == b) {
-    if (la) {
-      c.p2p[a].insert(c.p2p[a].begin(), a);
-      return;
-    }
-    for (uint32_t i = 0; i < c.cc[a]; ++i)
-      for (uint32_t j = i; j < c.cc[a]; ++j) dtt(c, c.cf[a] + i, c.cf[a] + j);
-    return;
-  }
-  double dx = c.comx[a] - c.comx[b], dy = c.comy[a] - c.comy[b];
-  double d = std::sqrt(dx * dx + dy * dy);
-  if (c.R[a] + c.R[b] < c.theta * d) {
-    c.m2l[a].push_back(b);
-    c.m2l[b].push_back(a);
-    return;
-  }
-  if (la && lb) {
-    c.p2p[a].push_back(b);
-    c.p2p[b].push_back(a);
-    return;
-  }
-  bool split_a = lb || (!la && c.R[a] >= c.R[b]);
-  if (split_a) {
-    for (uint32_t i = 0; i < c.cc[a]; ++i) dtt(c, c.cf[a] + i, b);
-  } else {
-    for (uint32_t i = 0; i < c.cc[b]; ++i) dtt(c, a, c.cf[b] + i);
+enum : uint32_t { DTT_SPLIT_A, DTT_SPLIT_B, DTT_SELF, DTT_LEAF_SELF, DTT_M2L, DTT_P2P };
+
+__device__ __forceinline__ uint32_t dtt_kind(const DttTree& t, uint32_t a, uint32_t b) {
+#pragma clang fp contract(off)
+  const bool la = t.cc[a] == 0, lb = t.cc[b] == 0;
+  if (a == b) return la ? DTT_LEAF_SELF : DTT_SELF;
+  const double dx = t.comx[a] - t.comx[b], dy = t.comy[a] - t.comy[b];
+  const double d = sqrt(dx * dx + dy * dy);
+  if (t.R[a] + t.R[b] < t.theta * d) return DTT_M2L;
+  if (la && lb) return DTT_P2P;
+  return (lb || (!la && t.R[a] >= t.R[b])) ? DTT_SPLIT_A : DTT_SPLIT_B;
+}
+
+__device__ __forceinline__ uint32_t dtt_children(const DttTree& t, uint32_t kind, uint32_t a, uint32_t b) {
+  if (kind == DTT_SELF) return (uint32_t)t.cc[a] * (t.cc[a] + 1u) / 2u;
+  if (kind == DTT_SPLIT_A) return t.cc[a];
+  if (kind == DTT_SPLIT_B) return t.cc[b];
+  return 0;
+}
+
+__global__ void k_dtt_count(DttTree t, const uint64_t* __restrict__ pairs, uint32_t lo, uint32_t hi,
+                            uint32_t* __restrict__ cnt) {
+  const uint32_t i = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= hi) return;
+  const uint64_t pr = pairs[i];
+  const uint32_t a = (uint32_t)(pr >> 32), b = (uint32_t)pr;
+  cnt[i - lo] = dtt_children(t, dtt_kind(t, a, b), a, b);
+}
+
+// child pairs of [lo, hi) at hi + pos[i - lo]
+__global__ void k_dtt_expand(DttTree t, uint64_t* pairs, uint32_t lo, uint32_t hi, const uint32_t* __restrict__ pos) {
+  const uint32_t i = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= hi) return;
+  const uint64_t pr = pairs[i];
+  const uint32_t a = (uint32_t)(pr >> 32), b = (uint32_t)pr;
+  const uint32_t kind = dtt_kind(t, a, b);
+  uint64_t* out = pairs + hi + pos[i - lo];
+  if (kind == DTT_SELF) {
+    const uint32_t c0 = t.cf[a], nc = t.cc[a];
+    for (uint32_t ci = 0; ci < nc; ++ci)
+      for (uint32_t cj = ci; cj < nc; ++cj) *out++ = ((uint64_t)(c0 + ci) << 32) | (c0 + cj);
+  } else if (kind == DTT_SPLIT_A) {
+    for (uint32_t ci = 0; ci < t.cc[a]; ++ci) *out++ = ((uint64_t)(t.cf[a] + ci) << 32) | b;
+  } else if (kind == DTT_SPLIT_B) {
+    for (uint32_t cj = 0; cj < t.cc[b]; ++cj) *out++ = ((uint64_t)a << 32) | (t.cf[b] + cj);
   }
 }
+
+// per visited pair: 2 M2L keys if accepted, 2 P2P keys if a pair of distinct leaves
+__global__ void k_dtt_emit_count(DttTree t, const uint64_t* __restrict__ pairs, uint32_t n, uint32_t* __restrict__ mcnt,
+                                 uint32_t* __restrict__ pcnt) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t pr = pairs[i];
+  const uint32_t kind = dtt_kind(t, (uint32_t)(pr >> 32), (uint32_t)pr);
+  mcnt[i] = kind == DTT_M2L ? 2u : 0u;
+  pcnt[i] = kind == DTT_P2P ? 2u : 0u;
+}
+
+// M2L key (a << wm) | b; P2P key (a << wp) | (b + 1), the leaf itself being 0
+__global__ void k_dtt_emit(DttTree t, const uint64_t* __restrict__ pairs, uint32_t n, const uint32_t* __restrict__ mpos,
+                           const uint32_t* __restrict__ ppos, int wm, int wp, uint64_t* __restrict__ mkeys,
+                           uint64_t* __restrict__ pkeys) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t pr = pairs[i];
+  const uint32_t a = (uint32_t)(pr >> 32), b = (uint32_t)pr;
+  const uint32_t kind = dtt_kind(t, a, b);
+  if (kind == DTT_M2L) {
+    mkeys[mpos[i]] = ((uint64_t)a << wm) | b;
+    mkeys[mpos[i] + 1] = ((uint64_t)b << wm) | a;
+  } else if (kind == DTT_P2P) {
+    pkeys[ppos[i]] = ((uint64_t)a << wp) | (b + 1u);
+    pkeys[ppos[i] + 1] = ((uint64_t)b << wp) | (a + 1u);
+  }
+}
+
+__global__ void k_dtt_self_keys(uint32_t nl, int wp, uint64_t* __restrict__ pkeys) {
+  const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a < nl) pkeys[a] = (uint64_t)a << wp;
+}
+
+// sorted keys -> CSR entries and per-row counts (cnt zeroed, rows + 1 entries)
+__global__ void k_dtt_split(const uint64_t* __restrict__ keys, uint32_t n, int w, bool self_first,
+                            uint32_t* __restrict__ dat, uint32_t* __restrict__ cnt) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t k = keys[i];
+  const uint32_t row = (uint32_t)(k >> w), col = (uint32_t)(k & ((1ull << w) - 1));
+  dat[i] = self_first ? (col == 0 ? row : col - 1u) : col;
+  atomicAdd(&cnt[row], 1u);
+}
+
+namespace {
+
+int key_bits(uint64_t n) {  // bits for the values 0 .. n
+  int w = 1;
+  while (w < 32 && (1ull << w) <= n) ++w;
+  return w;
+}
+
+int grow_pairs(DBuf& b, size_t keep, size_t need, hipStream_t s) {
+  if (need * 8 <= b.bytes) return 0;
+  DBuf nb;
+  GX_CHECK(dbuf_reserve(nb, std::max(need, 2 * b.bytes / 8) * 8, s));
+  if (keep) GX_HIP(hipMemcpyAsync(nb.p, b.p, keep * 8, hipMemcpyDeviceToDevice, s));
+  GX_HIP(hipStreamSynchronize(s));
+  dbuf_free(b);
+  b = nb;
+  return 0;
+}
+
+// sorts n keys of `bits` bits in ws.sort_ka and splits them into dat / off (rows + 1)
+int keys_to_csr(galax_fmm* h, size_t n, int w, int bits, bool self_first, uint32_t rows, DBuf& off, DBuf& dat) {
+  hipStream_t s = h->stream;
+  Workspace& ws = h->ws;
+  GX_CHECK(dbuf_reserve(ws.sort_kb, n * 8 + 8, s));
+  GX_CHECK(dbuf_reserve(ws.sort_va, n * 4 + 4, s));
+  GX_CHECK(dbuf_reserve(ws.sort_vb, n * 4 + 4, s));
+  GX_CHECK(radix_sort_bits(h, ws.sort_ka.as<uint64_t>(), ws.sort_va.as<uint32_t>(), ws.sort_kb.as<uint64_t>(),
+                           ws.sort_vb.as<uint32_t>(), n, bits));
+  GX_CHECK(dbuf_reserve(off, ((size_t)rows + 1) * 4, s));
+  GX_CHECK(dbuf_reserve(dat, n * 4 + 4, s));
+  GX_HIP(hipMemsetAsync(off.p, 0, ((size_t)rows + 1) * 4, s));
+  if (n) {
+    k_dtt_split<<<nb256(n), 256, 0, s>>>(ws.sort_ka.as<uint64_t>(), (uint32_t)n, w, self_first, dat.as<uint32_t>(),
+                                         off.as<uint32_t>());
+    GX_HIP(hipGetLastError());
+  }
+  GX_CHECK(scan_u32(h, off.as<uint32_t>(), off.as<uint32_t>(), (size_t)rows + 1, false, false, nullptr));
+  return 0;
+}
+
 }  // namespace
 
 // Dual tree traversal over the installed tree with the acceptance test
@@ -80,56 +190,73 @@ void dtt(DttCtx& c, uint32_t a, uint32_t b) {
 int build_lists_corrected(galax_fmm* h, const double* d_x, const double* d_y) {
   DevTree& t = h->tree;
   hipStream_t s = h->stream;
-  size_t nn = t.n_nodes;
+  const size_t nn = t.n_nodes, nl = t.n_leaves;
   h->corrected_ready = false;
   if (nn == 0) {
     h->corrected_ready = true;
     return 0;
   }
-  GX_CHECK(dbuf_reserve(h->ws.a, nn * 8, s));
+  Workspace& ws = h->ws;
+  GX_CHECK(dbuf_reserve(ws.a, nn * 8, s));
   k_radii<<<(unsigned)((nn + 3) / 4), 256, 0, s>>>(d_x, d_y, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(),
                                                    t.com_x.as<double>(), t.com_y.as<double>(), (uint32_t)nn,
-                                                   h->ws.a.as<double>());
+                                                   ws.a.as<double>());
   GX_HIP(hipGetLastError());
-  std::vector<double> comx(nn), comy(nn), R(nn);
-  std::vector<uint32_t> cf(nn);
-  std::vector<uint8_t> cc(nn);
-  GX_HIP(hipMemcpyAsync(comx.data(), t.com_x.p, nn * 8, hipMemcpyDeviceToHost, s));
-  GX_HIP(hipMemcpyAsync(comy.data(), t.com_y.p, nn * 8, hipMemcpyDeviceToHost, s));
-  GX_HIP(hipMemcpyAsync(R.data(), h->ws.a.p, nn * 8, hipMemcpyDeviceToHost, s));
-  GX_HIP(hipMemcpyAsync(cf.data(), t.child_first.p, nn * 4, hipMemcpyDeviceToHost, s));
-  GX_HIP(hipMemcpyAsync(cc.data(), t.child_count.p, nn, hipMemcpyDeviceToHost, s));
+  const DttTree dt{t.com_x.as<double>(), t.com_y.as<double>(), ws.a.as<double>(), t.child_first.as<uint32_t>(),
+                   t.child_count.as<uint8_t>(), h->cfg.theta};
+  // rounds: pairs [lo, hi) expand into [hi, hi + total)
+  DBuf& P = h->c_dtt_pairs;
+  GX_CHECK(grow_pairs(P, 0, std::max<size_t>(4 * nn, 1024), s));
+  const uint64_t root_pair = ((uint64_t)(nn - 1) << 32) | (uint64_t)(nn - 1);
+  GX_HIP(hipMemcpyAsync(P.p, &root_pair, 8, hipMemcpyHostToDevice, s));
   GX_HIP(hipStreamSynchronize(s));
-  DttCtx c{comx.data(), comy.data(), R.data(), cf.data(), cc.data(), h->cfg.theta, {}, {}};
-  c.m2l.assign(nn, {});
-  c.p2p.assign(nn, {});
-  uint32_t root = (uint32_t)nn - 1;
-  dtt(c, root, root);
-  std::vector<uint32_t> moff(nn + 1, 0), goff(t.n_leaves + 1, 0), mdat, gdat;
-  for (size_t i = 0; i < nn; ++i) {
-    std::sort(c.m2l[i].begin(), c.m2l[i].end());
-    moff[i + 1] = moff[i] + (uint32_t)c.m2l[i].size();
+  size_t lo = 0, hi = 1;
+  while (lo < hi) {
+    const size_t n = hi - lo;
+    GX_CHECK(dbuf_reserve(ws.b, (n + 1) * 4, s));
+    uint32_t* cnt = ws.b.as<uint32_t>();
+    k_dtt_count<<<nb256(n), 256, 0, s>>>(dt, P.as<uint64_t>(), (uint32_t)lo, (uint32_t)hi, cnt);
+    GX_HIP(hipGetLastError());
+    uint32_t total = 0;
+    GX_CHECK(scan_u32(h, cnt, cnt, n, false, false, &total));
+    if (hi + total >= (1ull << 32)) return set_error(GALAX_ERR_UNSUPPORTED, "dual tree traversal above 2^32 pairs");
+    GX_CHECK(grow_pairs(P, hi, hi + total, s));
+    if (total) {
+      k_dtt_expand<<<nb256(n), 256, 0, s>>>(dt, P.as<uint64_t>(), (uint32_t)lo, (uint32_t)hi, cnt);
+      GX_HIP(hipGetLastError());
+    }
+    lo = hi;
+    hi += total;
   }
-  mdat.reserve(moff[nn]);
-  for (size_t i = 0; i < nn; ++i) mdat.insert(mdat.end(), c.m2l[i].begin(), c.m2l[i].end());
-  for (size_t i = 0; i < t.n_leaves; ++i) {
-    auto& v = c.p2p[i];
-    if (!v.empty()) std::sort(v.begin() + 1, v.end());
-    goff[i + 1] = goff[i] + (uint32_t)v.size();
-  }
-  gdat.reserve(goff[t.n_leaves]);
-  for (size_t i = 0; i < t.n_leaves; ++i) gdat.insert(gdat.end(), c.p2p[i].begin(), c.p2p[i].end());
-  h->c_m2l_total = mdat.size();
-  h->c_gat_total = gdat.size();
-  GX_CHECK(dbuf_reserve(h->c_m2l_off, moff.size() * 4, s));
-  GX_CHECK(dbuf_reserve(h->c_m2l_dat, mdat.size() * 4 + 4, s));
-  GX_CHECK(dbuf_reserve(h->c_gat_off, goff.size() * 4, s));
-  GX_CHECK(dbuf_reserve(h->c_gat_dat, gdat.size() * 4 + 4, s));
-  GX_HIP(hipMemcpyAsync(h->c_m2l_off.p, moff.data(), moff.size() * 4, hipMemcpyHostToDevice, s));
-  if (!mdat.empty()) GX_HIP(hipMemcpyAsync(h->c_m2l_dat.p, mdat.data(), mdat.size() * 4, hipMemcpyHostToDevice, s));
-  GX_HIP(hipMemcpyAsync(h->c_gat_off.p, goff.data(), goff.size() * 4, hipMemcpyHostToDevice, s));
-  if (!gdat.empty()) GX_HIP(hipMemcpyAsync(h->c_gat_dat.p, gdat.data(), gdat.size() * 4, hipMemcpyHostToDevice, s));
+  const size_t nv = hi;
+  // M2L and P2P keys of the visited pairs
+  GX_CHECK(dbuf_reserve(ws.c, (nv + 1) * 4, s));
+  GX_CHECK(dbuf_reserve(ws.d, (nv + 1) * 4, s));
+  uint32_t *mpos = ws.c.as<uint32_t>(), *ppos = ws.d.as<uint32_t>();
+  k_dtt_emit_count<<<nb256(nv), 256, 0, s>>>(dt, P.as<uint64_t>(), (uint32_t)nv, mpos, ppos);
+  GX_HIP(hipGetLastError());
+  uint32_t mtot = 0, ptot = 0;
+  GX_CHECK(scan_u32(h, mpos, mpos, nv, false, false, &mtot));
+  GX_CHECK(scan_u32(h, ppos, ppos, nv, false, false, &ptot));
+  const int wm = key_bits(nn), wp = key_bits(nl + 1);
+  // M2L: keys in ws.sort_ka; P2P: keys in ws.e (self keys first), sorted after the M2L lists
+  GX_CHECK(dbuf_reserve(ws.sort_ka, ((size_t)mtot + 1) * 8, s));
+  GX_CHECK(dbuf_reserve(ws.e, ((size_t)ptot + nl + 1) * 8, s));
+  uint64_t* pkeys = ws.e.as<uint64_t>();
+  k_dtt_self_keys<<<nb256(nl), 256, 0, s>>>((uint32_t)nl, wp, pkeys);
+  GX_HIP(hipGetLastError());
+  k_dtt_emit<<<nb256(nv), 256, 0, s>>>(dt, P.as<uint64_t>(), (uint32_t)nv, mpos, ppos, wm, wp,
+                                       ws.sort_ka.as<uint64_t>(), pkeys + nl);
+  GX_HIP(hipGetLastError());
+  GX_CHECK(keys_to_csr(h, mtot, wm, 2 * wm, false, (uint32_t)nn, h->c_m2l_off, h->c_m2l_dat));
+  const size_t pn = (size_t)ptot + nl;
+  GX_CHECK(dbuf_reserve(ws.sort_ka, pn * 8 + 8, s));
+  GX_HIP(hipMemcpyAsync(ws.sort_ka.p, pkeys, pn * 8, hipMemcpyDeviceToDevice, s));
+  GX_CHECK(keys_to_csr(h, pn, wp, 2 * wp, true, (uint32_t)nl, h->c_gat_off, h->c_gat_dat));
   GX_HIP(hipStreamSynchronize(s));
+  h->c_m2l_total = mtot;
+  h->c_gat_total = pn;
+  h->c_dtt_visited = nv;
   GX_CHECK(count_interactions(h, h->c_gat_off.as<uint32_t>(), h->c_gat_dat.as<uint32_t>(), &h->c_p2p_interactions));
   GX_CHECK(build_p2p_items(h, h->c_gat_off.as<uint32_t>(), (uint32_t)t.n_leaves, h->c_items));
   h->corrected_ready = true;
@@ -322,7 +449,7 @@ void galax_fmm_destroy(galax_fmm_t* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   DBuf* bufs[] = {&h->x, &h->y, &h->m, &h->vx, &h->vy, &h->ax, &h->ay, &h->phi, &h->mp, &h->loc, &h->mp32,
                   &h->loc32, &h->gtab, &h->key_of_pair, &h->perm, &h->hash, &h->tmp_u32, &h->flag,
-                  &h->c_m2l_off, &h->c_m2l_dat, &h->c_gat_off, &h->c_gat_dat, &h->c_src_pairs,
+                  &h->c_m2l_off, &h->c_m2l_dat, &h->c_gat_off, &h->c_gat_dat, &h->c_dtt_pairs,
                   &h->key_dxy, &h->mx_t64, &h->mx_that, &h->mx_sig, &h->mx_zero,
                   &h->vj_lbar, &h->vj_mbar, &h->vj_leaf_of, &h->vj_fwd, &h->vj_out,
                   &h->mx_fwd.src, &h->mx_fwd.tile_off, &h->mx_fwd.step_key, &h->mx_fwd.step_first,
@@ -479,6 +606,30 @@ int galax_fmm_tree_info(galax_fmm_t* h, size_t* n_nodes, size_t* n_leaves, uint3
   if (n_levels) *n_levels = h->tree.n_levels;
   if (m2l_total) *m2l_total = h->tree.m2l_total;
   if (p2p_total) *p2p_total = h->tree.p2p_total;
+  return 0;
+}
+
+int galax_fmm_export_corrected_lists(galax_fmm_t* h, galax_lists_view_t* m2l, galax_lists_view_t* p2p) {
+  if (!h) return set_error(GALAX_ERR_INVALID, "NULL handle");
+  if (h->cfg.algo != GALAX_ALGO_CORRECTED || !h->corrected_ready)
+    return set_error(GALAX_ERR_INVALID, "no corrected-variant lists: evaluate with GALAX_ALGO_CORRECTED first");
+  DevTree& t = h->tree;
+  hipStream_t s = h->stream;
+  const size_t nn = t.n_nodes;
+  // rows: nn for the M2L lists, the leaves [0, n_leaves) for the P2P lists
+  auto ex = [&](galax_lists_view_t* lv, DBuf& off, DBuf& dat, size_t rows, size_t total) -> int {
+    if (!lv) return 0;
+    if (lv->n_nodes != nn) return set_error(GALAX_ERR_INVALID, "list view holds %zu nodes, tree has %zu", lv->n_nodes, nn);
+    if (!lv->offsets) return set_error(GALAX_ERR_INVALID, "NULL offsets");
+    std::vector<uint32_t> o(rows + 1, 0);
+    if (nn) GX_HIP(hipMemcpyAsync(o.data(), off.p, (rows + 1) * 4, hipMemcpyDeviceToHost, s));
+    if (lv->data && total) GX_HIP(hipMemcpyAsync(lv->data, dat.p, total * 4, hipMemcpyDeviceToHost, s));
+    GX_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i <= nn; ++i) lv->offsets[i] = i <= rows ? o[i] : o[rows];
+    return 0;
+  };
+  GX_CHECK(ex(m2l, h->c_m2l_off, h->c_m2l_dat, nn, h->c_m2l_total));
+  GX_CHECK(ex(p2p, h->c_gat_off, h->c_gat_dat, t.n_leaves, h->c_gat_total));
   return 0;
 }
 

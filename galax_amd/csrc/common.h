@@ -134,9 +134,9 @@ struct galax_fmm {
   size_t n_live_targets = 0, n_live_pairs = 0;
   bool keys_valid = false;
   gx::Workspace ws;
-  // corrected-variant lists (built on the host from the device tree)
-  gx::DBuf c_m2l_off, c_m2l_dat, c_gat_off, c_gat_dat, c_src_pairs;
-  size_t c_m2l_total = 0, c_gat_total = 0;
+  // corrected-variant lists (dual tree traversal on the device)
+  gx::DBuf c_m2l_off, c_m2l_dat, c_gat_off, c_gat_dat, c_dtt_pairs;  // c_dtt_pairs: every visited pair
+  size_t c_m2l_total = 0, c_gat_total = 0, c_dtt_visited = 0;
   uint64_t c_p2p_interactions = 0;
   gx::P2PItems c_items;
   gx::DBuf p2p_part;  // partial near fields of targets split over several items

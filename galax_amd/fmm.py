@@ -101,6 +101,7 @@ def lib():
         "galax_fmm_tree_info": [vp, vp, vp, vp, vp, vp],
         "galax_fmm_set_softening": [vp, d],
         "galax_fmm_export_tree": [vp, vp, vp, vp],
+        "galax_fmm_export_corrected_lists": [vp, vp, vp],
         "galax_fmm_evaluate": [vp, C.POINTER(_Bodies)],
         "galax_fmm_evaluate_device": [vp] * 7,
         "galax_fmm_last_timings": [vp, C.POINTER(_Timings)],
@@ -296,6 +297,24 @@ class GpuContext:
         lm, lp = _ListsView(nn, _ptr(mo), _ptr(md)), _ListsView(nn, _ptr(po), _ptr(pd))
         _check(lib().galax_fmm_export_tree(self.h, C.byref(tv), C.byref(lm), C.byref(lp)))
         return t, (mo, md[:info["m2l_total"]]), (po, pd[:info["p2p_total"]])
+
+    def export_corrected_lists(self):
+        """The corrected variant's lists, built on the device at the first
+        evaluation: (m2l (offsets, data), p2p (offsets, data)), one row per
+        node (P2P rows: the leaf itself first, internal nodes empty)."""
+        nn = self.tree_info()["n_nodes"]
+        out = []
+        for which in (0, 1):
+            off = np.zeros(nn + 1, np.uint64)
+            lv = _ListsView(nn, _ptr(off), None)
+            views = (C.byref(lv), None) if which == 0 else (None, C.byref(lv))
+            _check(lib().galax_fmm_export_corrected_lists(self.h, *views))
+            dat = np.zeros(max(int(off[-1]), 1), np.uint32)
+            lv = _ListsView(nn, _ptr(off), _ptr(dat))
+            views = (C.byref(lv), None) if which == 0 else (None, C.byref(lv))
+            _check(lib().galax_fmm_export_corrected_lists(self.h, *views))
+            out.append((off, dat[:int(off[-1])]))
+        return out[0], out[1]
 
     # -- evaluation ---------------------------------------------------------
     def evaluate(self, b: BodiesSoA):

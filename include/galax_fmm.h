@@ -181,6 +181,15 @@ GALAX_API int galax_fmm_tree_info(galax_fmm_t* h, size_t* n_nodes, size_t* n_lea
 GALAX_API int galax_fmm_export_tree(galax_fmm_t* h, galax_tree_view_t* tree, galax_lists_view_t* m2l,
                           galax_lists_view_t* p2p);
 
+/* No reference counterpart (GALAX_ALGO_CORRECTED has none): copies the
+ * corrected variant's lists out for checks. They are built on the device by a
+ * dual tree traversal at the first evaluation after a tree build. M2L: each
+ * node's accepted partners, ascending; P2P: each leaf itself, then its
+ * unaccepted leaf partners, ascending (internal nodes empty). Views hold
+ * n_nodes rows; a view with data == NULL receives its offsets only (the
+ * total is offsets[n_nodes]). Either view may be NULL. */
+GALAX_API int galax_fmm_export_corrected_lists(galax_fmm_t* h, galax_lists_view_t* m2l, galax_lists_view_t* p2p);
+
 /* ~compute_fmm_force (galax-core/src/lib.rs:1466-1499) and
  * GpuContext::step + read_accelerations (galax-gpu/src/lib.rs:464-566):
  * reads x, y, m in the tree's body order, sanitizes them like :1475-1482

@@ -214,6 +214,41 @@ def test_upload_tree_matches_device_build():
     assert np.median(_rel_diff((b.ax, b.ay), (ax, ay))) <= 1e-6
 
 
+CORRECTED_LIST_CASES = [
+    ("plummer_t03", lambda: ic.plummer(20000, seed=203), 32, (-50.0, 50.0, -50.0, 50.0), 0.3),
+    ("uniform_t05", lambda: ic.uniform(6000, seed=101), 32, (0.0, 100.0, 0.0, 100.0), 0.5),
+    ("snap_demo_t04", lambda: _snap("galx_demo_0010.bin"), 32, (-50.0, 50.0, -50.0, 50.0), 0.4),
+    ("tiny_leaves_t05", lambda: ic.uniform(700, seed=105), 1, (0.0, 100.0, 0.0, 100.0), 0.5),
+    ("galaxy_t03", lambda: ic.galaxy(4096, seed=104), 16, (-50.0, 50.0, -50.0, 50.0), 0.3),
+]
+
+
+@pytest.mark.parametrize("name,gen,n_max,bounds,theta", CORRECTED_LIST_CASES, ids=[c[0] for c in CORRECTED_LIST_CASES])
+def test_corrected_lists_bit_exact(name, gen, n_max, bounds, theta):
+    """The device dual tree traversal gives the oracle's corrected-variant
+    lists entry for entry (M2L ascending; P2P the leaf first, then ascending)."""
+    d = gen()
+    ctx, b, _ = _gpu_build(d, n_max=n_max, bounds=bounds, algo=fmm.ALGO_CORRECTED, theta=theta)
+    ctx.evaluate(b)  # the first evaluation builds the lists
+    ot, od, _ = _oracle_build(d, n_max, bounds)
+    m2l, p2p = O.corrected_lists(ot, od["x"], od["y"], theta)
+    gm2l, gp2p = ctx.export_corrected_lists()
+    _assert_lists_equal(gm2l, m2l)
+    _assert_lists_equal(gp2p, p2p)
+
+
+def test_corrected_lists_bit_exact_full_size_plummer1m():
+    d = ic.plummer(1 << 20)
+    bounds, theta = (-50.0, 50.0, -50.0, 50.0), 0.4
+    ctx, b, _ = _gpu_build(d, n_max=32, bounds=bounds, algo=fmm.ALGO_CORRECTED, theta=theta)
+    ctx.evaluate(b)
+    ot, od, _ = _oracle_build(d, 32, bounds)
+    m2l, p2p = O.corrected_lists(ot, od["x"], od["y"], theta)
+    gm2l, gp2p = ctx.export_corrected_lists()
+    _assert_lists_equal(gm2l, m2l)
+    _assert_lists_equal(gp2p, p2p)
+
+
 @pytest.mark.parametrize("p,bound", [(4, 5e-4), (6, 2e-5), (8, 2e-6)])
 def test_corrected_variant_matches_oracle_and_direct_sum(p, bound):
     """p = 4, 6 and 8 run the lane-per-pair M2L and the thread-per-node M2M and L2L."""
