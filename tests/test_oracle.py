@@ -534,6 +534,49 @@ def test_corrected_variant_is_accurate():
     assert e_ref > 100 * e[-1], (e_ref, e)
 
 
+@pytest.mark.parametrize("n,n_max,theta,seed", [(1500, 16, 0.5, 31), (900, 4, 0.3, 32), (400, 1, 0.7, 33)])
+def test_corrected_lists_partition_every_pair(n, n_max, theta, seed):
+    """The dual tree traversal's lists, by brute force over all leaf pairs:
+    each pair of distinct leaves interacts exactly once, either as a P2P pair
+    or through exactly one accepted M2L pair of their ancestors-or-selves;
+    accepted pairs pass R_A + R_B < theta d and P2P pairs fail it; both lists
+    are symmetric, ascending, and each leaf's P2P list starts with itself."""
+    b = ic.plummer(n, seed=seed)
+    t = O.tree_build(b, n_max, (-50, 50, -50, 50))
+    d = t.export()
+    nn = len(t)
+    R = O.node_radii(t, b["x"], b["y"])
+    m2l, p2p = O.corrected_lists(t, b["x"], b["y"], theta)
+    ml, pl = m2l.as_lists(), p2p.as_lists()
+    leaves = [i for i in range(nn) if d["is_leaf"][i]]
+
+    def acc(a, c):
+        dx, dy = d["com_x"][a] - d["com_x"][c], d["com_y"][a] - d["com_y"][c]
+        return R[a] + R[c] < theta * np.sqrt(dx * dx + dy * dy)
+
+    anc = {}
+    for lf in leaves:
+        chain, v = [], lf
+        while v >= 0:
+            chain.append(v)
+            v = int(d["parent"][v])
+        anc[lf] = chain
+    mset = {(a, c) for a in range(nn) for c in ml[a]}
+    pset = {(a, c) for a in range(nn) for c in pl[a]}
+    for a in range(nn):
+        assert ml[a] == sorted(ml[a])
+        assert all(acc(a, c) and (c, a) in mset for c in ml[a])
+        if d["is_leaf"][a]:
+            assert pl[a][0] == a and pl[a][1:] == sorted(pl[a][1:])
+            assert all(not acc(a, c) and (c, a) in pset and d["is_leaf"][c] for c in pl[a][1:])
+        else:
+            assert pl[a] == []
+    for i, la in enumerate(leaves):
+        for lb in leaves[i + 1:]:
+            via_m2l = sum((A, B) in mset for A in anc[la] for B in anc[lb])
+            assert via_m2l + ((la, lb) in pset) == 1, (la, lb, via_m2l)
+
+
 def test_threaded_oracle_matches_sequential_uncached():
     b = ic.plummer(3000, seed=6)
     t = O.tree_build(b, 32, (-50, 50, -50, 50))
