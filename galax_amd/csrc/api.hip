@@ -633,6 +633,20 @@ int galax_fmm_export_corrected_lists(galax_fmm_t* h, galax_lists_view_t* m2l, ga
   return 0;
 }
 
+int galax_fmm_export_expansions(galax_fmm_t* h, double* multipoles, double* locals, size_t n_nodes) {
+  if (!h) return set_error(GALAX_ERR_INVALID, "NULL handle");
+  const size_t nn = h->tree.n_nodes, S = (size_t)(h->cfg.p + 1) * (h->cfg.p + 2) / 2;
+  if (n_nodes != nn) return set_error(GALAX_ERR_INVALID, "arrays hold %zu nodes, tree has %zu", n_nodes, nn);
+  if (nn == 0) return 0;
+  if (!h->mp.p || h->mp.bytes < nn * S * 8 || !h->loc.p || h->loc.bytes < nn * S * 8)
+    return set_error(GALAX_ERR_INVALID, "no expansions yet: evaluate first");
+  hipStream_t s = h->stream;
+  if (multipoles) GX_HIP(hipMemcpyAsync(multipoles, h->mp.p, nn * S * 8, hipMemcpyDeviceToHost, s));
+  if (locals) GX_HIP(hipMemcpyAsync(locals, h->loc.p, nn * S * 8, hipMemcpyDeviceToHost, s));
+  GX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
 int galax_fmm_export_tree(galax_fmm_t* h, galax_tree_view_t* tv, galax_lists_view_t* m2l, galax_lists_view_t* p2p) {
   if (!h) return set_error(GALAX_ERR_INVALID, "NULL handle");
   DevTree& t = h->tree;

@@ -102,6 +102,7 @@ def lib():
         "galax_fmm_set_softening": [vp, d],
         "galax_fmm_export_tree": [vp, vp, vp, vp],
         "galax_fmm_export_corrected_lists": [vp, vp, vp],
+        "galax_fmm_export_expansions": [vp, vp, vp, C.c_size_t],
         "galax_fmm_evaluate": [vp, C.POINTER(_Bodies)],
         "galax_fmm_evaluate_device": [vp] * 7,
         "galax_fmm_last_timings": [vp, C.POINTER(_Timings)],
@@ -297,6 +298,14 @@ class GpuContext:
         lm, lp = _ListsView(nn, _ptr(mo), _ptr(md)), _ListsView(nn, _ptr(po), _ptr(pd))
         _check(lib().galax_fmm_export_tree(self.h, C.byref(tv), C.byref(lm), C.byref(lp)))
         return t, (mo, md[:info["m2l_total"]]), (po, pd[:info["p2p_total"]])
+
+    def export_expansions(self):
+        """(multipoles, locals) of the last evaluation, each (n_nodes, S)."""
+        nn = self.tree_info()["n_nodes"]
+        S = (self.cfg.p + 1) * (self.cfg.p + 2) // 2
+        mp, loc = np.zeros((nn, S)), np.zeros((nn, S))
+        _check(lib().galax_fmm_export_expansions(self.h, _ptr(mp), _ptr(loc), nn))
+        return mp, loc
 
     def export_corrected_lists(self):
         """The corrected variant's lists, built on the device at the first

@@ -181,6 +181,16 @@ GALAX_API int galax_fmm_tree_info(galax_fmm_t* h, size_t* n_nodes, size_t* n_lea
 GALAX_API int galax_fmm_export_tree(galax_fmm_t* h, galax_tree_view_t* tree, galax_lists_view_t* m2l,
                           galax_lists_view_t* p2p);
 
+/* No reference counterpart: copies the multipole and local coefficients of
+ * the last evaluation out for checks, n_nodes x S doubles each (node-major,
+ * galax-core's coefficient order, S = (p+1)(p+2)/2). Only the nodes an
+ * evaluation uses hold values: with GALAX_ALGO_REFERENCE the multipoles of
+ * levels 0-1 and the M2L locals of levels 0-1 (the leaves' share of their
+ * parents' locals is added inside the force kernel, not stored); with
+ * GALAX_ALGO_CORRECTED every node, the locals after L2L except at the leaves.
+ * Either pointer may be NULL. */
+GALAX_API int galax_fmm_export_expansions(galax_fmm_t* h, double* multipoles, double* locals, size_t n_nodes);
+
 /* No reference counterpart (GALAX_ALGO_CORRECTED has none): copies the
  * corrected variant's lists out for checks. They are built on the device by a
  * dual tree traversal at the first evaluation after a tree build. M2L: each

@@ -153,6 +153,44 @@ def test_forces_match_oracle_reference(p, prec):
     np.testing.assert_allclose(b.phi, phi, rtol=1e-9, atol=1e-12 * np.abs(phi).max())
 
 
+def _levels(ex):
+    """Node levels of an exported tree (ids are level-ordered, children first)."""
+    lv = np.zeros(len(ex["is_leaf"]), np.int64)
+    for i in range(len(lv)):
+        if not ex["is_leaf"][i]:
+            lv[i] = 1 + lv[ex["children"][i][0]]
+    return lv
+
+
+@pytest.mark.parametrize("p", [2, 4, 6, 8])
+def test_expansions_bit_exact(p):
+    """P2M + M2M (levels 0-1) bit for bit in both FP64 forms; the M2L locals of
+    levels 0-1 bit for bit in FP64_EXACT (galax-core's per-pair order) and to
+    rounding in FP64 (matrix cores, grouped by key)."""
+    d = ic.plummer(12000, seed=206)
+    bounds, eps = (-50.0, 50.0, -50.0, 50.0), 0.02
+    ot, od, _ = _oracle_build(d, 32, bounds)
+    ex = ot.export()
+    live = _levels(ex) <= 1
+    S = (p + 1) * (p + 2) // 2
+    mp_o = O.p2m(od["x"], od["y"], od["m"], ot, p)
+    O.m2m(mp_o, ot, p)
+    loc_o = np.zeros_like(mp_o)
+    O.m2l(loc_o, mp_o, O.m2l_lists(ot), ot, p, eps)
+    mp_o, loc_o = mp_o.reshape(-1, S)[live], loc_o.reshape(-1, S)[live]
+    for prec in (fmm.PREC_FP64_EXACT, fmm.PREC_FP64):
+        ctx, b, _ = _gpu_build(d, p=p, eps=eps, bounds=bounds, precision=prec)
+        ctx.evaluate(b)
+        mp_g, loc_g = ctx.export_expansions()
+        mp_g, loc_g = mp_g[live], loc_g[live]
+        assert np.array_equal(mp_g.view(np.uint64), mp_o.view(np.uint64)), (prec, np.argwhere(mp_g != mp_o)[:5])
+        if prec == fmm.PREC_FP64_EXACT:
+            assert np.array_equal(loc_g.view(np.uint64), loc_o.view(np.uint64)), np.argwhere(loc_g != loc_o)[:5]
+        else:
+            scale = np.abs(loc_o).max(axis=1, keepdims=True)
+            assert np.max(np.abs(loc_g - loc_o) / np.maximum(scale, 1e-300)) < 1e-12
+
+
 @pytest.mark.parametrize("name", ["galx_demo_0010.bin", "galx_viz_demo_0010.bin"])
 def test_forces_on_reference_snapshots(name):
     """Forces on post-step states of the reference's own runs (GALX fixtures)."""
