@@ -355,7 +355,7 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
 
 struct TileArgs {
   const uint32_t *item_tile, *item_s0, *item_s1, *item_slot;
-  uint32_t nitems, per_xcd, n_rows;
+  uint32_t nitems, n_rows;
   const uint32_t *step_key, *step_ids;
   const double* t64;
   const float* that;
@@ -381,8 +381,9 @@ __global__ void __launch_bounds__(256) k_m2l_tiles(TileArgs a) {
   constexpr int S = LN::S, RB = LN::RB, K = LN::K, KL = LN::KL, KS = K / 4;
   static_assert(!TR || MODE == MX_F64, "the transposed (adjoint) product is FP64");
   __shared__ double red[4][16][17];
-  // XCD-contiguous item ranges: block b runs on XCD b % 8
-  const uint32_t item = (blockIdx.x % 8) * a.per_xcd + blockIdx.x / 8;
+  // one block per item in item order: the hardware deals blocks to the XCDs
+  // in turn, which spreads the costly tiles of dense regions over all of them
+  const uint32_t item = blockIdx.x;
   if (item >= a.nitems) return;  // uniform over the block
   const uint32_t tile = a.item_tile[item], s0 = a.item_s0[item], s1 = a.item_s1[item], slot = a.item_slot[item];
   // w is wave-uniform: readfirstlane lets the step records and operator addresses live in scalar registers
@@ -827,7 +828,6 @@ static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
   a.item_s1 = T.item_s1.as<uint32_t>();
   a.item_slot = T.item_slot.as<uint32_t>();
   a.nitems = (uint32_t)T.nitems;
-  a.per_xcd = (uint32_t)((T.nitems + 7) / 8);
   a.n_rows = (uint32_t)T.n_rows;
   a.step_key = T.step_key.as<uint32_t>();
   a.step_ids = T.step_ids.as<uint32_t>();
@@ -856,7 +856,7 @@ static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
   a.out = out;
   a.part = T.part.as<double>();
   if (!TR && h->timing) GX_HIP(hipEventRecord(h->ev[8], s));
-  k_m2l_tiles<P, MODE, TR><<<8 * a.per_xcd, 256, 0, s>>>(a);
+  k_m2l_tiles<P, MODE, TR><<<a.nitems, 256, 0, s>>>(a);
   GX_HIP(hipGetLastError());
   if (!TR && h->timing) GX_HIP(hipEventRecord(h->ev[9], s));
   if (T.nmulti) {
