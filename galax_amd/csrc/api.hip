@@ -258,7 +258,8 @@ int build_lists_corrected(galax_fmm* h, const double* d_x, const double* d_y) {
   h->c_gat_total = pn;
   h->c_dtt_visited = nv;
   GX_CHECK(count_interactions(h, h->c_gat_off.as<uint32_t>(), h->c_gat_dat.as<uint32_t>(), &h->c_p2p_interactions));
-  GX_CHECK(build_p2p_items(h, h->c_gat_off.as<uint32_t>(), (uint32_t)t.n_leaves, h->c_items));
+  GX_CHECK(build_p2p_items(h, h->c_gat_off.as<uint32_t>(), h->c_gat_dat.as<uint32_t>(), (uint32_t)t.n_leaves,
+                           h->c_items));
   h->corrected_ready = true;
   return 0;
 }

@@ -175,7 +175,7 @@ int radix_sort_bits(galax_fmm* h, uint64_t* ka, uint32_t* va, uint64_t* kb, uint
 // P2P work items: each target leaf's gather list in chunks of at most
 // P2P_ITEM_LEAVES source leaves (balances leaves with thousands of sources).
 constexpr uint32_t P2P_ITEM_LEAVES = 32;
-int build_p2p_items(galax_fmm* h, const uint32_t* goff, uint32_t n_leaves, P2PItems& it);
+int build_p2p_items(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, uint32_t n_leaves, P2PItems& it);
 // Sums the partial near-field slots of targets split over several items into
 // (o0, o1, o2) per body (o2 may be null); three doubles per body and slot.
 hipError_t p2p_combine(const P2PItems& it, const uint32_t* goff, const uint32_t* bs, const uint32_t* be,

@@ -871,7 +871,34 @@ __global__ void k_items_fill(const uint32_t* __restrict__ goff, uint32_t nl, con
   if (n > 1) multi[moff[t]] = t;
 }
 
-int build_p2p_items(galax_fmm* h, const uint32_t* goff, uint32_t nl, P2PItems& it) {
+// Work estimate of an item: target bodies in 32-body passes times source bodies.
+__global__ void k_items_cost(const uint32_t* __restrict__ tgt, const uint32_t* __restrict__ g0,
+                             const uint32_t* __restrict__ g1, const uint32_t* __restrict__ gdat,
+                             const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be, uint32_t ni,
+                             uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ni) return;
+  const uint32_t t = tgt[i];
+  uint64_t ns = 0;
+  for (uint32_t g = g0[i]; g < g1[i]; ++g) ns += be[gdat[g]] - bs[gdat[g]];
+  const uint64_t cost = (uint64_t)((be[t] - bs[t] + 31u) / 32u) * ns;
+  key[i] = 0xFFFFFFFFull - (cost < 0xFFFFFFFFull ? cost : 0xFFFFFFFFull);  // ascending key = descending cost
+  val[i] = i;
+}
+
+__global__ void k_items_permute(const uint32_t* __restrict__ ord, uint32_t ni, const uint32_t* __restrict__ a0,
+                                const uint32_t* __restrict__ a1, const uint32_t* __restrict__ a2,
+                                const uint32_t* __restrict__ a3, uint32_t* __restrict__ b) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ni) return;
+  const uint32_t i = ord[j];
+  b[j] = a0[i];
+  b[ni + j] = a1[i];
+  b[2 * ni + j] = a2[i];
+  b[3 * ni + j] = a3[i];
+}
+
+int build_p2p_items(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, uint32_t nl, P2PItems& it) {
   hipStream_t s = h->stream;
   GX_CHECK(dbuf_reserve(it.part_off, (size_t)(nl + 1) * 4, s));
   GX_CHECK(dbuf_reserve(h->ws.f, (size_t)(nl + 1) * 8, s));
@@ -897,6 +924,34 @@ int build_p2p_items(galax_fmm* h, const uint32_t* goff, uint32_t nl, P2PItems& i
   k_items_fill<<<nblk(nl), 256, 0, s>>>(goff, nl, ioff, poff, moff, it.tgt.as<uint32_t>(), it.g0.as<uint32_t>(),
                                         it.g1.as<uint32_t>(), it.slot.as<uint32_t>(), it.multi.as<uint32_t>());
   GX_HIP(hipGetLastError());
+  // items in decreasing work (stable): the costly items of dense regions start
+  // first instead of forming the last round of blocks. Slots, and so the
+  // combine's summation order, do not depend on the order items run in.
+  if (ni > 1) {
+    Workspace& ws = h->ws;
+    GX_CHECK(dbuf_reserve(ws.sort_ka, (size_t)ni * 8, s));
+    GX_CHECK(dbuf_reserve(ws.sort_kb, (size_t)ni * 8, s));
+    GX_CHECK(dbuf_reserve(ws.sort_va, (size_t)ni * 4, s));
+    GX_CHECK(dbuf_reserve(ws.sort_vb, (size_t)ni * 4, s));
+    k_items_cost<<<nblk(ni), 256, 0, s>>>(it.tgt.as<uint32_t>(), it.g0.as<uint32_t>(), it.g1.as<uint32_t>(), gdat,
+                                          h->tree.body_start.as<uint32_t>(), h->tree.body_end.as<uint32_t>(), ni,
+                                          ws.sort_ka.as<uint64_t>(), ws.sort_va.as<uint32_t>());
+    GX_HIP(hipGetLastError());
+    GX_CHECK(radix_sort_bits(h, ws.sort_ka.as<uint64_t>(), ws.sort_va.as<uint32_t>(), ws.sort_kb.as<uint64_t>(),
+                             ws.sort_vb.as<uint32_t>(), ni, 32));
+    DBuf tmp;
+    GX_CHECK(dbuf_reserve(tmp, (size_t)ni * 16, s));
+    uint32_t* b = tmp.as<uint32_t>();
+    k_items_permute<<<nblk(ni), 256, 0, s>>>(ws.sort_va.as<uint32_t>(), ni, it.tgt.as<uint32_t>(),
+                                             it.g0.as<uint32_t>(), it.g1.as<uint32_t>(), it.slot.as<uint32_t>(), b);
+    GX_HIP(hipGetLastError());
+    GX_HIP(hipMemcpyAsync(it.tgt.p, b, (size_t)ni * 4, hipMemcpyDeviceToDevice, s));
+    GX_HIP(hipMemcpyAsync(it.g0.p, b + ni, (size_t)ni * 4, hipMemcpyDeviceToDevice, s));
+    GX_HIP(hipMemcpyAsync(it.g1.p, b + 2 * (size_t)ni, (size_t)ni * 4, hipMemcpyDeviceToDevice, s));
+    GX_HIP(hipMemcpyAsync(it.slot.p, b + 3 * (size_t)ni, (size_t)ni * 4, hipMemcpyDeviceToDevice, s));
+    GX_HIP(hipStreamSynchronize(s));
+    dbuf_free(tmp);
+  }
   GX_HIP(hipStreamSynchronize(s));
   return 0;
 }
@@ -932,7 +987,7 @@ static int build_gather(galax_fmm* h) {
   GX_HIP(hipGetLastError());
   GX_CHECK(scan_u32(h, goff, goff, nl + 1, false, false, nullptr));
   GX_CHECK(count_interactions(h, goff, t.gat_dat.as<uint32_t>(), &t.p2p_interactions));
-  GX_CHECK(build_p2p_items(h, goff, nl, t.items));
+  GX_CHECK(build_p2p_items(h, goff, t.gat_dat.as<uint32_t>(), nl, t.items));
   return 0;
 }
 
