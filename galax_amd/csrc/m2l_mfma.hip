@@ -744,6 +744,24 @@ static int build_tiles(galax_fmm* h, const uint32_t* row_of_pair, const uint32_t
     }
     if (nit > 1) nslots += nit;
   }
+  // items in decreasing step count (stable): the longest start first instead
+  // of forming the last round of blocks; each writes its own slot, so the
+  // combine's order does not depend on the order items run in
+  {
+    std::vector<uint32_t> ord(it_tile.size());
+    for (uint32_t q = 0; q < ord.size(); ++q) ord[q] = q;
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](uint32_t u, uint32_t v) { return it_s1[u] - it_s0[u] > it_s1[v] - it_s0[v]; });
+    auto perm = [&](std::vector<uint32_t>& a) {
+      std::vector<uint32_t> b(a.size());
+      for (size_t q = 0; q < a.size(); ++q) b[q] = a[ord[q]];
+      a.swap(b);
+    };
+    perm(it_tile);
+    perm(it_s0);
+    perm(it_s1);
+    perm(it_slot);
+  }
   T.nitems = it_tile.size();
   T.nmulti = m_tile.size();
   T.nslots = nslots;
