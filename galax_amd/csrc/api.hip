@@ -143,6 +143,16 @@ __global__ void k_dtt_split(const uint64_t* __restrict__ keys, uint32_t n, int w
   atomicAdd(&cnt[row], 1u);
 }
 
+// Targets in decreasing M2L list length (stable), for the lane-per-pair M2L
+// kernels: the long lists start first instead of forming the last blocks.
+__global__ void k_order_keys(const uint32_t* __restrict__ off, uint32_t n, uint64_t* __restrict__ key,
+                             uint32_t* __restrict__ val) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  key[i] = 0xFFFFFFFFull - (off[i + 1] - off[i]);
+  val[i] = i;
+}
+
 namespace {
 
 int key_bits(uint64_t n) {  // bits for the values 0 .. n
@@ -254,6 +264,17 @@ int build_lists_corrected(galax_fmm* h, const double* d_x, const double* d_y) {
   GX_HIP(hipMemcpyAsync(ws.sort_ka.p, pkeys, pn * 8, hipMemcpyDeviceToDevice, s));
   GX_CHECK(keys_to_csr(h, pn, wp, 2 * wp, true, (uint32_t)nl, h->c_gat_off, h->c_gat_dat));
   GX_HIP(hipStreamSynchronize(s));
+  {  // the M2L targets by decreasing list length
+    GX_CHECK(dbuf_reserve(ws.sort_ka, nn * 8, s));
+    GX_CHECK(dbuf_reserve(ws.sort_kb, nn * 8, s));
+    GX_CHECK(dbuf_reserve(ws.sort_vb, nn * 4, s));
+    GX_CHECK(dbuf_reserve(h->c_m2l_order, nn * 4, s));
+    k_order_keys<<<nb256(nn), 256, 0, s>>>(h->c_m2l_off.as<uint32_t>(), (uint32_t)nn, ws.sort_ka.as<uint64_t>(),
+                                           h->c_m2l_order.as<uint32_t>());
+    GX_HIP(hipGetLastError());
+    GX_CHECK(radix_sort_bits(h, ws.sort_ka.as<uint64_t>(), h->c_m2l_order.as<uint32_t>(), ws.sort_kb.as<uint64_t>(),
+                             ws.sort_vb.as<uint32_t>(), nn, 32));
+  }
   h->c_m2l_total = mtot;
   h->c_gat_total = pn;
   h->c_dtt_visited = nv;
@@ -450,7 +471,7 @@ void galax_fmm_destroy(galax_fmm_t* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   DBuf* bufs[] = {&h->x, &h->y, &h->m, &h->vx, &h->vy, &h->ax, &h->ay, &h->phi, &h->mp, &h->loc, &h->mp32,
                   &h->loc32, &h->gtab, &h->key_of_pair, &h->perm, &h->hash, &h->tmp_u32, &h->flag,
-                  &h->c_m2l_off, &h->c_m2l_dat, &h->c_gat_off, &h->c_gat_dat, &h->c_dtt_pairs,
+                  &h->c_m2l_off, &h->c_m2l_dat, &h->c_gat_off, &h->c_gat_dat, &h->c_dtt_pairs, &h->c_m2l_order,
                   &h->key_dxy, &h->mx_t64, &h->mx_that, &h->mx_sig, &h->mx_zero,
                   &h->vj_lbar, &h->vj_mbar, &h->vj_leaf_of, &h->vj_fwd, &h->vj_out,
                   &h->mx_fwd.src, &h->mx_fwd.tile_off, &h->mx_fwd.step_key, &h->mx_fwd.step_first,

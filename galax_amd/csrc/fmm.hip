@@ -544,12 +544,12 @@ __global__ void __launch_bounds__(256) k_m2l_exact(const uint32_t* __restrict__ 
 // them over batches of 64 pairs. A pair costs about 4(2p+1)^2 + 2 S^2
 // operations with no LDS traffic and no serial steps.
 template <int P>
-__global__ void __launch_bounds__(64) k_m2l_exact_lp(const uint32_t* __restrict__ off, const uint32_t* __restrict__ dat,
+__global__ void __launch_bounds__(64) k_m2l_exact_lp(const uint32_t* __restrict__ off, const uint32_t* __restrict__ dat, const uint32_t* __restrict__ order,
                                                      const double* __restrict__ comx, const double* __restrict__ comy,
                                                      double eps2, const double* __restrict__ mp,
                                                      double* __restrict__ loc) {
   constexpr int S = num_moments(P);
-  const uint32_t t = blockIdx.x;
+  const uint32_t t = order[blockIdx.x];
   const int lane = threadIdx.x;
   const uint32_t e0 = off[t], e1 = off[t + 1];
   const double tx = comx[t], ty = comy[t];
@@ -593,12 +593,12 @@ __global__ void __launch_bounds__(64) k_m2l_exact_lp(const uint32_t* __restrict_
 // locals are summed in FP64.
 template <int P>
 __global__ void __launch_bounds__(64) k_m2l_exact_lp32(const uint32_t* __restrict__ off,
-                                                       const uint32_t* __restrict__ dat,
+                                                       const uint32_t* __restrict__ dat, const uint32_t* __restrict__ order,
                                                        const double* __restrict__ comx,
                                                        const double* __restrict__ comy, double eps2,
                                                        const double* __restrict__ mp, double* __restrict__ loc) {
   constexpr int S = num_moments(P);
-  const uint32_t t = blockIdx.x;
+  const uint32_t t = order[blockIdx.x];
   const int lane = threadIdx.x;
   const uint32_t e0 = off[t], e1 = off[t + 1];
   const double tx = comx[t], ty = comy[t];
@@ -1315,10 +1315,11 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
       const bool mixed = h->cfg.precision == GALAX_PREC_MIXED_FP32 || h->cfg.precision == GALAX_PREC_MIXED_BF16X3;
       if (mixed)
         k_m2l_exact_lp32<P><<<(unsigned)nn, 64, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(),
-                                                        comx, comy, eps * eps, mp, loc);
+                                                        h->c_m2l_order.as<uint32_t>(), comx, comy, eps * eps, mp,
+                                                        loc);
       else
-        k_m2l_exact_lp<P><<<(unsigned)nn, 64, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(), comx,
-                                                      comy, eps * eps, mp, loc);
+        k_m2l_exact_lp<P><<<(unsigned)nn, 64, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(),
+                                                      h->c_m2l_order.as<uint32_t>(), comx, comy, eps * eps, mp, loc);
     } else {
       k_m2l_exact<P><<<nblk(nn, 4), 256, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(),
                                                  (uint32_t)nn, comx, comy, eps * eps, mp, loc);

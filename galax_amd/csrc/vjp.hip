@@ -558,12 +558,12 @@ __global__ void __launch_bounds__(256) k_vjp_m2l_exact(const uint32_t* __restric
 // 1/(I!J!) (m2l_pair_scatter); the wave sums its lanes in a fixed order.
 template <int P>
 __global__ void __launch_bounds__(64) k_vjp_m2l_exact_lp(const uint32_t* __restrict__ off,
-                                                         const uint32_t* __restrict__ dat,
+                                                         const uint32_t* __restrict__ dat, const uint32_t* __restrict__ order,
                                                          const double* __restrict__ comx,
                                                          const double* __restrict__ comy, double eps2,
                                                          const double* __restrict__ lbar, double* __restrict__ mbar) {
   constexpr int S = num_moments(P);
-  const uint32_t s = blockIdx.x;
+  const uint32_t s = order[blockIdx.x];
   const int lane = threadIdx.x;
   const uint32_t r0 = off[s], r1 = off[s + 1];
   const double sx = comx[s], sy = comy[s];
@@ -659,7 +659,7 @@ static int vjp_impl(galax_fmm* h, const double* x, const double* y, const double
   if (corrected) {
     if constexpr (P <= 8) {  // above, the lane's window and sums no longer fit in registers
       k_vjp_m2l_exact_lp<P><<<(unsigned)nn, 64, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(),
-                                                        comx, comy, eps2, lbar, mbar);
+                                                        h->c_m2l_order.as<uint32_t>(), comx, comy, eps2, lbar, mbar);
     } else {
       k_vjp_m2l_exact<P><<<nblk(nn, 4), 256, 0, s>>>(h->c_m2l_off.as<uint32_t>(), h->c_m2l_dat.as<uint32_t>(),
                                                      (uint32_t)nn, comx, comy, eps2, lbar, mbar);
