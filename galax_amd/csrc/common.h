@@ -74,6 +74,7 @@ struct P2PItems {
 struct M2LTiles {
   DBuf src, tile_off, step_key, step_first, step_cnt, step_ids;
   DBuf lr_first, lr_len, virt, hubs;  // long runs: source range; pre-summed rows; runs taking a block
+  DBuf lr_order;                      // long runs by decreasing length (the pre-sum kernel's order)
   DBuf item_tile, item_s0, item_s1, item_slot;  // per work item
   DBuf multi_tile, multi_slot, multi_cnt;       // per tile with several items
   DBuf part;                                    // partial results, 16 x S doubles per slot

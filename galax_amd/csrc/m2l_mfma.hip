@@ -229,7 +229,8 @@ __global__ void k_mx_step_ids(const uint32_t* __restrict__ step_grp, const uint3
 // adjoint): one block per run, the four waves take sources w, w + 4, ... and
 // are added in wave order.
 template <int P>
-__global__ void __launch_bounds__(256) k_m2l_run_sums_w(const uint32_t* __restrict__ lr_first,
+__global__ void __launch_bounds__(256) k_m2l_run_sums_w(const uint32_t* __restrict__ lr_order,
+                                                        const uint32_t* __restrict__ lr_first,
                                                         const uint32_t* __restrict__ lr_len, uint32_t nlr,
                                                         const uint32_t* __restrict__ src,
                                                         const double* __restrict__ in, double* __restrict__ virt) {
@@ -237,10 +238,10 @@ __global__ void __launch_bounds__(256) k_m2l_run_sums_w(const uint32_t* __restri
   constexpr int S = num_moments(P);
   constexpr int RPW = S <= 32 ? 2 : 1, LW = 64 / RPW;
   const int lane = threadIdx.x & 63;
-  const uint32_t r = (blockIdx.x * 4 + (threadIdx.x >> 6)) * RPW + lane / LW;
+  const uint32_t q = (blockIdx.x * 4 + (threadIdx.x >> 6)) * RPW + lane / LW;
   const int kl = lane % LW;
-  if (r >= nlr) return;
-  const uint32_t f = lr_first[r], len = lr_len[r];
+  if (q >= nlr) return;
+  const uint32_t r = lr_order[q], f = lr_first[r], len = lr_len[r];
   if (len > MX_WAVE_RUN) return;
   for (int k = kl; k < S; k += LW) {
     double acc = 0.0;
@@ -710,6 +711,15 @@ static int build_tiles(galax_fmm* h, const uint32_t* row_of_pair, const uint32_t
       GX_CHECK(dbuf_reserve(T.hubs, (size_t)std::max<uint32_t>(nh, 1) * 4, s));
       k_mx_hub_list<<<nblk(nlr), 256, 0, s>>>(T.lr_len.as<uint32_t>(), hf, nlr, T.hubs.as<uint32_t>());
       GX_HIP(hipGetLastError());
+      // the pre-sum kernel's order: decreasing length (stable), so that the two
+      // runs of a wave are alike and the longest start first
+      std::vector<uint32_t> len(nlr), ord(nlr);
+      GX_HIP(hipMemcpyAsync(len.data(), T.lr_len.p, (size_t)nlr * 4, hipMemcpyDeviceToHost, s));
+      GX_HIP(hipStreamSynchronize(s));
+      for (uint32_t q = 0; q < nlr; ++q) ord[q] = q;
+      std::stable_sort(ord.begin(), ord.end(), [&](uint32_t u, uint32_t v) { return len[u] > len[v]; });
+      GX_CHECK(upload_u32(T.lr_order, ord, s));
+      GX_HIP(hipStreamSynchronize(s));
     }
     GX_CHECK(dbuf_reserve(T.step_ids, (size_t)ns * MX_GROUP * 16 * 4, s));
     k_mx_step_ids<<<nblk((size_t)ns * 16), 256, 0, s>>>(T.step_first.as<uint32_t>(), gfirst, lrid,
@@ -852,7 +862,8 @@ static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
   a.n_in = (uint32_t)h->tree.n_nodes;
   if (T.nlong) {
     GX_CHECK(dbuf_reserve(T.virt, T.nlong * S * 8, s));
-    k_m2l_run_sums_w<P><<<nblk(T.nlong, S <= 32 ? 8 : 4), 256, 0, s>>>(T.lr_first.as<uint32_t>(), T.lr_len.as<uint32_t>(),
+    k_m2l_run_sums_w<P><<<nblk(T.nlong, S <= 32 ? 8 : 4), 256, 0, s>>>(T.lr_order.as<uint32_t>(),
+                                                                       T.lr_first.as<uint32_t>(), T.lr_len.as<uint32_t>(),
                                                           (uint32_t)T.nlong, T.src.as<uint32_t>(), in,
                                                           T.virt.as<double>());
     if (T.nhub)
