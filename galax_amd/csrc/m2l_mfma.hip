@@ -711,13 +711,15 @@ static int build_tiles(galax_fmm* h, const uint32_t* row_of_pair, const uint32_t
       GX_CHECK(dbuf_reserve(T.hubs, (size_t)std::max<uint32_t>(nh, 1) * 4, s));
       k_mx_hub_list<<<nblk(nlr), 256, 0, s>>>(T.lr_len.as<uint32_t>(), hf, nlr, T.hubs.as<uint32_t>());
       GX_HIP(hipGetLastError());
-      // the pre-sum kernel's order: decreasing length (stable), so that the two
-      // runs of a wave are alike and the longest start first
+      // the pre-sum kernel's order: decreasing power-of-two length class
+      // (stable), so that the two runs of a wave are alike, the longest start
+      // first and neighbouring runs of one class stay together
       std::vector<uint32_t> len(nlr), ord(nlr);
       GX_HIP(hipMemcpyAsync(len.data(), T.lr_len.p, (size_t)nlr * 4, hipMemcpyDeviceToHost, s));
       GX_HIP(hipStreamSynchronize(s));
       for (uint32_t q = 0; q < nlr; ++q) ord[q] = q;
-      std::stable_sort(ord.begin(), ord.end(), [&](uint32_t u, uint32_t v) { return len[u] > len[v]; });
+      auto cls = [](uint32_t n) { return n ? 32 - __builtin_clz(n) : 0; };
+      std::stable_sort(ord.begin(), ord.end(), [&](uint32_t u, uint32_t v) { return cls(len[u]) > cls(len[v]); });
       GX_CHECK(upload_u32(T.lr_order, ord, s));
       GX_HIP(hipStreamSynchronize(s));
     }
@@ -754,14 +756,17 @@ static int build_tiles(galax_fmm* h, const uint32_t* row_of_pair, const uint32_t
     }
     if (nit > 1) nslots += nit;
   }
-  // items in decreasing step count (stable): the longest start first instead
-  // of forming the last round of blocks; each writes its own slot, so the
+  // items by decreasing power-of-two step class (stable): the longest start
+  // first instead of forming the last round of blocks, and neighbouring tiles
+  // of one class stay together; each item writes its own slot, so the
   // combine's order does not depend on the order items run in
   {
+    auto cls = [](uint32_t n) { return n ? 32 - __builtin_clz(n) : 0; };
     std::vector<uint32_t> ord(it_tile.size());
     for (uint32_t q = 0; q < ord.size(); ++q) ord[q] = q;
-    std::stable_sort(ord.begin(), ord.end(),
-                     [&](uint32_t u, uint32_t v) { return it_s1[u] - it_s0[u] > it_s1[v] - it_s0[v]; });
+    std::stable_sort(ord.begin(), ord.end(), [&](uint32_t u, uint32_t v) {
+      return cls(it_s1[u] - it_s0[u]) > cls(it_s1[v] - it_s0[v]);
+    });
     auto perm = [&](std::vector<uint32_t>& a) {
       std::vector<uint32_t> b(a.size());
       for (size_t q = 0; q < a.size(); ++q) b[q] = a[ord[q]];

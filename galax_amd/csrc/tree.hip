@@ -871,7 +871,8 @@ __global__ void k_items_fill(const uint32_t* __restrict__ goff, uint32_t nl, con
   if (n > 1) multi[moff[t]] = t;
 }
 
-// Work estimate of an item: target bodies in 32-body passes times source bodies.
+// Work class of an item: target bodies in 32-body passes times source bodies,
+// as a power of two.
 __global__ void k_items_cost(const uint32_t* __restrict__ tgt, const uint32_t* __restrict__ g0,
                              const uint32_t* __restrict__ g1, const uint32_t* __restrict__ gdat,
                              const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be, uint32_t ni,
@@ -882,7 +883,10 @@ __global__ void k_items_cost(const uint32_t* __restrict__ tgt, const uint32_t* _
   uint64_t ns = 0;
   for (uint32_t g = g0[i]; g < g1[i]; ++g) ns += be[gdat[g]] - bs[gdat[g]];
   const uint64_t cost = (uint64_t)((be[t] - bs[t] + 31u) / 32u) * ns;
-  key[i] = 0xFFFFFFFFull - (cost < 0xFFFFFFFFull ? cost : 0xFFFFFFFFull);  // ascending key = descending cost
+  // power-of-two cost classes, costliest first; the stable sort keeps the
+  // spatial order within a class (neighbouring items share sources in L2)
+  const uint32_t cls = cost ? 64u - (uint32_t)__clzll((unsigned long long)cost) : 0u;
+  key[i] = 64u - cls;
   val[i] = i;
 }
 
@@ -924,9 +928,10 @@ int build_p2p_items(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, ui
   k_items_fill<<<nblk(nl), 256, 0, s>>>(goff, nl, ioff, poff, moff, it.tgt.as<uint32_t>(), it.g0.as<uint32_t>(),
                                         it.g1.as<uint32_t>(), it.slot.as<uint32_t>(), it.multi.as<uint32_t>());
   GX_HIP(hipGetLastError());
-  // items in decreasing work (stable): the costly items of dense regions start
-  // first instead of forming the last round of blocks. Slots, and so the
-  // combine's summation order, do not depend on the order items run in.
+  // items by decreasing power-of-two work class (stable): the costly items of
+  // dense regions start first instead of forming the last round of blocks.
+  // Slots, and so the combine's summation order, do not depend on the order
+  // items run in.
   if (ni > 1) {
     Workspace& ws = h->ws;
     GX_CHECK(dbuf_reserve(ws.sort_ka, (size_t)ni * 8, s));
@@ -938,7 +943,7 @@ int build_p2p_items(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, ui
                                           ws.sort_ka.as<uint64_t>(), ws.sort_va.as<uint32_t>());
     GX_HIP(hipGetLastError());
     GX_CHECK(radix_sort_bits(h, ws.sort_ka.as<uint64_t>(), ws.sort_va.as<uint32_t>(), ws.sort_kb.as<uint64_t>(),
-                             ws.sort_vb.as<uint32_t>(), ni, 32));
+                             ws.sort_vb.as<uint32_t>(), ni, 7));
     DBuf tmp;
     GX_CHECK(dbuf_reserve(tmp, (size_t)ni * 16, s));
     uint32_t* b = tmp.as<uint32_t>();
