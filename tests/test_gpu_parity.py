@@ -366,6 +366,48 @@ def test_small_n(n):
     assert np.isfinite(b.ax).all()
 
 
+@pytest.mark.parametrize("n", [1, 2, 31, 33, 200])
+def test_small_n_corrected(n):
+    """The corrected variant on trees of one or a few leaves: the device
+    traversal's lists equal the oracle's and the forces are finite; with one
+    leaf they are the direct sum."""
+    d = ic.uniform(n, seed=208)
+    bounds = (0.0, 100.0, 0.0, 100.0)
+    ctx, b, _ = _gpu_build(d, p=4, n_max=8, bounds=bounds, algo=fmm.ALGO_CORRECTED, theta=0.5)
+    ctx.evaluate(b)
+    assert np.isfinite(b.ax).all() and np.isfinite(b.ay).all()
+    ot, od, _ = _oracle_build(d, 8, bounds)
+    m2l, p2p = O.corrected_lists(ot, od["x"], od["y"], 0.5)
+    gm2l, gp2p = ctx.export_corrected_lists()
+    _assert_lists_equal(gm2l, m2l)
+    _assert_lists_equal(gp2p, p2p)
+    if n <= 8:
+        ex, ey, _ = fmm.compute_n2_force(b.x, b.y, b.mass, 0.1)
+        np.testing.assert_allclose(b.ax, ex, rtol=1e-12, atol=1e-15)
+
+
+def test_coincident_bodies_corrected():
+    """Bodies stacked on a few points: zero radii, and zero distances between
+    leaves on one point (never accepted). The lists equal the oracle's and the
+    forces the oracle's corrected evaluation."""
+    pts = np.array([[10.0, 10.0], [10.0, 10.0], [60.0, 35.0], [60.0, 35.0], [80.0, 90.0]])
+    rep = np.repeat(pts, 40, axis=0)
+    n = len(rep)
+    d = {"x": rep[:, 0].copy(), "y": rep[:, 1].copy(), "vx": np.zeros(n), "vy": np.zeros(n),
+         "m": np.linspace(1.0, 2.0, n), "ax": np.zeros(n), "ay": np.zeros(n)}
+    bounds = (0.0, 100.0, 0.0, 100.0)
+    ctx, b, _ = _gpu_build(d, p=4, n_max=16, bounds=bounds, algo=fmm.ALGO_CORRECTED, theta=0.5)
+    ctx.evaluate(b)
+    ot, od, _ = _oracle_build(d, 16, bounds)
+    m2l, p2p = O.corrected_lists(ot, od["x"], od["y"], 0.5)
+    gm2l, gp2p = ctx.export_corrected_lists()
+    _assert_lists_equal(gm2l, m2l)
+    _assert_lists_equal(gp2p, p2p)
+    ax, ay, _ = O.compute_fmm_force(od["x"], od["y"], od["m"], ot, m2l, p2p, 4, 0.1, algo=1)
+    assert np.all(np.isfinite(b.ax)) and np.all(np.isfinite(b.ay))
+    assert np.max(_rel_diff((b.ax, b.ay), (ax, ay))) <= 1e-9
+
+
 def test_kdk_matches_oracle_simulate():
     d = ic.plummer(4000, seed=207)
     bounds = (-50.0, 50.0, -50.0, 50.0)
