@@ -223,6 +223,10 @@ __global__ void __launch_bounds__(256) k_m2m(const uint32_t* __restrict__ cf, co
       const double* mc = smc[w][sub];
       const double* ex = sdx[w][sub];
       const double* ey = sdy[w][sub];
+      // a child at the parent's COM (a single child usually is): every term
+      // but i = oi, j = oj is +-0, which leaves the running sum at +0, so the
+      // sum is 0 + M_c(oi, oj) exactly as the full loop gives it
+      const bool at_com = Q == 1 || (ex[1] == 0.0 && ey[1] == 0.0);
 #pragma unroll
       for (int r = 0; r < (S + LW - 1) / LW; ++r) {
         const int k = sl + r * LW;
@@ -230,10 +234,14 @@ __global__ void __launch_bounds__(256) k_m2m(const uint32_t* __restrict__ cf, co
           int oi, oj;
           coef_ij<P>(k, oi, oj);
           double sum = 0.0;
-          for (int i = 0; i <= oi; ++i)
-            for (int j = 0; j <= oj; ++j)
-              sum += div_exact(mc[moment_index(i, j)] * ex[oi - i] * ey[oj - j], sd[oi - i][oj - j],
-                               sr[oi - i][oj - j]);
+          if (at_com) {
+            sum = 0.0 + mc[k];
+          } else {
+            for (int i = 0; i <= oi; ++i)
+              for (int j = 0; j <= oj; ++j)
+                sum += div_exact(mc[moment_index(i, j)] * ex[oi - i] * ey[oj - j], sd[oi - i][oj - j],
+                                 sr[oi - i][oj - j]);
+          }
           acc[r] += sum;
         }
       }
