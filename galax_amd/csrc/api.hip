@@ -278,6 +278,8 @@ int build_lists_corrected(galax_fmm* h, const double* d_x, const double* d_y) {
   h->c_m2l_total = mtot;
   h->c_gat_total = pn;
   h->c_dtt_visited = nv;
+  GX_HIP(hipStreamSynchronize(s));
+  dbuf_free(P);  // the visited pairs (8 bytes each) are needed only while the lists are built
   GX_CHECK(count_interactions(h, h->c_gat_off.as<uint32_t>(), h->c_gat_dat.as<uint32_t>(), &h->c_p2p_interactions));
   GX_CHECK(build_p2p_items(h, h->c_gat_off.as<uint32_t>(), h->c_gat_dat.as<uint32_t>(), (uint32_t)t.n_leaves,
                            h->c_items));

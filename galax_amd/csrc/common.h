@@ -136,7 +136,7 @@ struct galax_fmm {
   bool keys_valid = false;
   gx::Workspace ws;
   // corrected-variant lists (dual tree traversal on the device)
-  gx::DBuf c_m2l_off, c_m2l_dat, c_gat_off, c_gat_dat, c_dtt_pairs;  // c_dtt_pairs: every visited pair
+  gx::DBuf c_m2l_off, c_m2l_dat, c_gat_off, c_gat_dat, c_dtt_pairs;  // c_dtt_pairs: visited pairs while building
   gx::DBuf c_m2l_order;  // M2L targets (sources, for the adjoint) by decreasing list length
   size_t c_m2l_total = 0, c_gat_total = 0, c_dtt_visited = 0;
   uint64_t c_p2p_interactions = 0;
