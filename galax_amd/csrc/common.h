@@ -75,6 +75,8 @@ struct M2LTiles {
   DBuf src, tile_off, step_key, step_first, step_cnt, step_ids;
   DBuf lr_first, lr_len, virt, hubs;  // long runs: source range; pre-summed rows; runs taking a block
   DBuf lr_order;                      // long runs by decreasing length (the pre-sum kernel's order)
+  DBuf hub_first, hub_len, hub_coff, hub_part;  // hub runs in chunks: rows and partial sums per chunk
+  size_t nhub_chunks = 0;
   DBuf item_tile, item_s0, item_s1, item_slot;  // per work item
   DBuf multi_tile, multi_slot, multi_cnt;       // per tile with several items
   DBuf part;                                    // partial results, 16 x S doubles per slot

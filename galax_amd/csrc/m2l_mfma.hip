@@ -93,7 +93,8 @@ struct Lanes {
 
 constexpr uint32_t MX_ITEM_STEPS = 32;  // steps per work item
 constexpr uint32_t MX_GROUP = 4;        // longest run summed inside the product kernel; longer runs are pre-summed
-constexpr uint32_t MX_WAVE_RUN = 256;   // longest run pre-summed by one wave; longer runs take a block
+constexpr uint32_t MX_WAVE_RUN = 256;   // longest run pre-summed by one wave; longer runs (hubs) go in chunks
+constexpr uint32_t MX_HUB_CHUNK = 128;  // rows per chunk of a hub run
 
 // row of each live pair: its target (CSR row) or its source
 __global__ void k_mx_pair_rows(const uint32_t* __restrict__ off, uint32_t nt, uint32_t* __restrict__ tgt) {
@@ -254,31 +255,44 @@ __global__ void __launch_bounds__(256) k_m2l_run_sums_w(const uint32_t* __restri
   }
 }
 
+// Hub runs in chunks of MX_HUB_CHUNK rows: a half-wave (wave) per chunk sums
+// its rows in list order into a partial row; then one thread per (hub,
+// coefficient) adds the hub's partials in chunk order (a fixed order).
 template <int P>
-__global__ void __launch_bounds__(256) k_m2l_run_sums(const uint32_t* __restrict__ hubs,
-                                                      const uint32_t* __restrict__ lr_first,
-                                                      const uint32_t* __restrict__ lr_len,
-                                                      const uint32_t* __restrict__ src, const double* __restrict__ in,
-                                                      double* __restrict__ virt) {
+__global__ void __launch_bounds__(256) k_m2l_hub_chunks(const uint32_t* __restrict__ ch_first,
+                                                        const uint32_t* __restrict__ ch_len, uint32_t nch,
+                                                        const uint32_t* __restrict__ src,
+                                                        const double* __restrict__ in, double* __restrict__ part) {
   constexpr int S = num_moments(P);
-  __shared__ double red[4][64];
-  const uint32_t r = hubs[blockIdx.x], f = lr_first[r], len = lr_len[r];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int k0 = 0; k0 < S; k0 += 64) {
-    const int k = k0 + lane;
+  constexpr int RPW = S <= 32 ? 2 : 1, LW = 64 / RPW;
+  const int lane = threadIdx.x & 63;
+  const uint32_t q = (blockIdx.x * 4 + (threadIdx.x >> 6)) * RPW + lane / LW;
+  const int kl = lane % LW;
+  if (q >= nch) return;
+  const uint32_t f = ch_first[q], len = ch_len[q];
+  for (int k = kl; k < S; k += LW) {
     double acc = 0.0;
-    if (k < S) {
 #pragma unroll 4
-      for (uint32_t i = (uint32_t)w; i < len; i += 4) {
-        const double v = in[(size_t)src[f + i] * S + k];
-        acc += isfinite(v) ? v : 0.0;
-      }
+    for (uint32_t i = 0; i < len; ++i) {
+      const double v = in[(size_t)src[f + i] * S + k];
+      acc += isfinite(v) ? v : 0.0;
     }
-    red[w][lane] = acc;
-    __syncthreads();
-    if (w == 0 && k < S) virt[(size_t)r * S + k] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
-    __syncthreads();
+    part[(size_t)q * S + k] = acc;
   }
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) k_m2l_hub_sum(const uint32_t* __restrict__ hubs,
+                                                     const uint32_t* __restrict__ coff, uint32_t nhub,
+                                                     const double* __restrict__ part, double* __restrict__ virt) {
+  constexpr int S = num_moments(P);
+  const size_t tt = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tt >= (size_t)nhub * S) return;
+  const uint32_t hb = (uint32_t)(tt / S);
+  const int k = (int)(tt % S);
+  double acc = 0.0;
+  for (uint32_t c = coff[hb]; c < coff[hb + 1]; ++c) acc += part[(size_t)c * S + k];
+  virt[(size_t)hubs[hb] * S + k] = acc;
 }
 
 // FP64 operator of each key: (16 RB)^2 row-major, zero padded
@@ -721,6 +735,28 @@ static int build_tiles(galax_fmm* h, const uint32_t* row_of_pair, const uint32_t
       auto cls = [](uint32_t n) { return n ? 32 - __builtin_clz(n) : 0; };
       std::stable_sort(ord.begin(), ord.end(), [&](uint32_t u, uint32_t v) { return cls(len[u]) > cls(len[v]); });
       GX_CHECK(upload_u32(T.lr_order, ord, s));
+      // hub runs in chunks of MX_HUB_CHUNK rows
+      T.nhub_chunks = 0;
+      if (nh) {
+        std::vector<uint32_t> hubs(nh), first(nlr), cf, cl, coff(nh + 1, 0);
+        GX_HIP(hipMemcpyAsync(hubs.data(), T.hubs.p, (size_t)nh * 4, hipMemcpyDeviceToHost, s));
+        GX_HIP(hipMemcpyAsync(first.data(), T.lr_first.p, (size_t)nlr * 4, hipMemcpyDeviceToHost, s));
+        GX_HIP(hipStreamSynchronize(s));
+        for (uint32_t hb = 0; hb < nh; ++hb) {
+          const uint32_t r = hubs[hb];
+          coff[hb] = (uint32_t)cf.size();
+          for (uint32_t i = 0; i < len[r]; i += MX_HUB_CHUNK) {
+            cf.push_back(first[r] + i);
+            cl.push_back(std::min(MX_HUB_CHUNK, len[r] - i));
+          }
+        }
+        coff[nh] = (uint32_t)cf.size();
+        T.nhub_chunks = cf.size();
+        GX_CHECK(upload_u32(T.hub_first, cf, s));
+        GX_CHECK(upload_u32(T.hub_len, cl, s));
+        GX_CHECK(upload_u32(T.hub_coff, coff, s));
+        GX_CHECK(dbuf_reserve(T.hub_part, T.nhub_chunks * num_moments(MAX_P) * 8, s));
+      }
       GX_HIP(hipStreamSynchronize(s));
     }
     GX_CHECK(dbuf_reserve(T.step_ids, (size_t)ns * MX_GROUP * 16 * 4, s));
@@ -871,10 +907,15 @@ static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
                                                                        T.lr_first.as<uint32_t>(), T.lr_len.as<uint32_t>(),
                                                           (uint32_t)T.nlong, T.src.as<uint32_t>(), in,
                                                           T.virt.as<double>());
-    if (T.nhub)
-      k_m2l_run_sums<P><<<(unsigned)T.nhub, 256, 0, s>>>(T.hubs.as<uint32_t>(), T.lr_first.as<uint32_t>(),
-                                                         T.lr_len.as<uint32_t>(), T.src.as<uint32_t>(), in,
+    if (T.nhub) {
+      k_m2l_hub_chunks<P><<<nblk(T.nhub_chunks, S <= 32 ? 8 : 4), 256, 0, s>>>(
+          T.hub_first.as<uint32_t>(), T.hub_len.as<uint32_t>(), (uint32_t)T.nhub_chunks, T.src.as<uint32_t>(), in,
+          T.hub_part.as<double>());
+      GX_HIP(hipGetLastError());
+      k_m2l_hub_sum<P><<<nblk(T.nhub * S), 256, 0, s>>>(T.hubs.as<uint32_t>(), T.hub_coff.as<uint32_t>(),
+                                                         (uint32_t)T.nhub, T.hub_part.as<double>(),
                                                          T.virt.as<double>());
+    }
     GX_HIP(hipGetLastError());
   }
   a.virt = T.virt.as<double>();
