@@ -503,7 +503,8 @@ def _forms_run(d, p, eps, bounds, algo=fmm.ALGO_REFERENCE, n_max=32):
 @pytest.mark.parametrize("p", [4, 6, 8])
 def test_mixed_forms_match_fp64_reference_algo(p):
     """FP32-MFMA M2L + FP32 P2P, and the BF16x3 M2L, against the FP64 path of the
-    same algorithm: RMS relative force difference <= 1e-5 (BASELINE metric)."""
+    same algorithm: RMS relative force difference <= 1e-5 (the mixed forms'
+    accuracy target)."""
     d = ic.plummer(50_000, seed=301)
     res = _forms_run(d, p, 0.02, (-50.0, 50.0, -50.0, 50.0))
     ref = res[fmm.PREC_FP64]
