@@ -258,6 +258,21 @@ std::vector<double> kernel_polys(double eps, int p);
 // live, so the table, `in` and `out` stay in registers.
 // Calls f(std::integral_constant<int, i>) for i = B .. E-1: loop indices that
 // are constants to the front end, so `if constexpr` prunes the dead terms.
+// Workgroups are dealt to the 8 XCDs in turn (block b to XCD b % 8), and each
+// XCD has its own L2. This maps block b to a work item so that runs of
+// XCD_RUN consecutive items (neighbouring targets, which read mostly the same
+// sources) go to one XCD while the runs rotate over the XCDs, which keeps the
+// balance of the round-robin deal. A bijection of [0, n) (the last partial
+// group keeps the identity).
+constexpr uint32_t XCD_RUN = 16;
+__device__ __forceinline__ uint32_t xcd_runs(uint32_t b, uint32_t n) {
+  constexpr uint32_t G = 8 * XCD_RUN;
+  const uint32_t g = b / G;
+  if ((g + 1) * G > n) return b;
+  const uint32_t r = b % G;
+  return g * G + (r % 8) * XCD_RUN + r / 8;
+}
+
 // a! and C(n, k) as compile-time constants (exact in double for n <= 18)
 constexpr double ct_fact(int a) { return a <= 1 ? 1.0 : a * ct_fact(a - 1); }
 constexpr double ct_binom(int n, int k) { return ct_fact(n) / (ct_fact(k) * ct_fact(n - k)); }

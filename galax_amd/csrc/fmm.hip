@@ -904,7 +904,7 @@ __global__ void __launch_bounds__(64) k_l2p_p2p(const double* __restrict__ x, co
   __shared__ double sloc[S], spar[S];
   static_assert(P2P_ITEM_LEAVES <= 64, "one lane per source leaf of an item");
   __shared__ V4 src[P2P_CHUNK + 1];
-  const uint32_t item = blockIdx.x;
+  const uint32_t item = xcd_runs(blockIdx.x, gridDim.x);
   const uint32_t leaf = itgt[item], g0 = ig0[item], g1 = ig1[item], slot = islot[item];
   const bool first = g0 == goff[leaf];
   const int lane = threadIdx.x;

@@ -239,7 +239,7 @@ __global__ void __launch_bounds__(256) k_m2l_run_sums_w(const uint32_t* __restri
   constexpr int S = num_moments(P);
   constexpr int RPW = S <= 32 ? 2 : 1, LW = 64 / RPW;
   const int lane = threadIdx.x & 63;
-  const uint32_t q = (blockIdx.x * 4 + (threadIdx.x >> 6)) * RPW + lane / LW;
+  const uint32_t q = (xcd_runs(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6)) * RPW + lane / LW;
   const int kl = lane % LW;
   if (q >= nlr) return;
   const uint32_t r = lr_order[q], f = lr_first[r], len = lr_len[r];
@@ -396,9 +396,8 @@ __global__ void __launch_bounds__(256) k_m2l_tiles(TileArgs a) {
   constexpr int S = LN::S, RB = LN::RB, K = LN::K, KL = LN::KL, KS = K / 4;
   static_assert(!TR || MODE == MX_F64, "the transposed (adjoint) product is FP64");
   __shared__ double red[4][16][17];
-  // one block per item in item order: the hardware deals blocks to the XCDs
-  // in turn, which spreads the costly tiles of dense regions over all of them
-  const uint32_t item = blockIdx.x;
+  // one block per item; runs of consecutive items share an XCD (xcd_runs)
+  const uint32_t item = xcd_runs(blockIdx.x, gridDim.x);
   if (item >= a.nitems) return;  // uniform over the block
   const uint32_t tile = a.item_tile[item], s0 = a.item_s0[item], s1 = a.item_s1[item], slot = a.item_slot[item];
   // w is wave-uniform: readfirstlane lets the step records and operator addresses live in scalar registers

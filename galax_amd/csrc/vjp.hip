@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(64) k_vjp_near(const double* __restrict__ x, c
   static_assert(P2P_ITEM_LEAVES <= 64, "one lane per source leaf of an item");
   __shared__ V4 sa[VJP_CHUNK + 1];
   __shared__ T sb[VJP_CHUNK + 1];
-  const uint32_t item = blockIdx.x;
+  const uint32_t item = xcd_runs(blockIdx.x, gridDim.x);
   const uint32_t leaf = itgt[item], g0 = ig0[item], g1 = ig1[item], slot = islot[item];
   const int lane = threadIdx.x, tl = lane & 31, half = lane >> 5;
   const uint32_t ts = bs[leaf], te = be[leaf];
