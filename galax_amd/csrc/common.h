@@ -264,7 +264,7 @@ std::vector<double> kernel_polys(double eps, int p);
 // sources) go to one XCD while the runs rotate over the XCDs, which keeps the
 // balance of the round-robin deal. A bijection of [0, n) (the last partial
 // group keeps the identity).
-constexpr uint32_t XCD_RUN = 16;
+constexpr uint32_t XCD_RUN = 32;
 __device__ __forceinline__ uint32_t xcd_runs(uint32_t b, uint32_t n) {
   constexpr uint32_t G = 8 * XCD_RUN;
   const uint32_t g = b / G;
