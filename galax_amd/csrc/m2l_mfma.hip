@@ -488,10 +488,24 @@ __global__ void __launch_bounds__(256) k_m2l_tiles(TileArgs a) {
       for (int e = 0; e < KL; ++e) v[q][e] = 0.0;
       if (iq != NONE) {
         const double* row = iq < a.n_in ? a.in + (size_t)iq * S : a.virt + (size_t)(iq - a.n_in) * S;
+        if constexpr (S % 2 == 0) {
+          // even rows on 16-byte aligned bases (checked by the host): each
+          // even k starts an aligned pair, and k < S holds for both or neither
 #pragma unroll
-        for (int e = 0; e < KL; ++e) {
-          const int k = LN::k_of(e, kq);
-          if (k < S) v[q][e] = row[k];
+          for (int e = 0; e < KL; e += 2) {
+            const int k = LN::k_of(e, kq);
+            if (k < S) {
+              const double2 w = *reinterpret_cast<const double2*>(row + k);
+              v[q][e] = w.x;
+              v[q][e + 1] = w.y;
+            }
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < KL; ++e) {
+            const int k = LN::k_of(e, kq);
+            if (k < S) v[q][e] = row[k];
+          }
         }
       }
     }
@@ -927,6 +941,8 @@ static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
   a.that = h->mx_that.as<float>();
   a.sig_exp = h->mx_sig.as<int>();
   a.in = in;
+  if (((uintptr_t)a.in | (uintptr_t)a.virt) & 15)
+    return set_error(GALAX_ERR_INVALID, "M2L source rows need 16-byte aligned bases");
   a.out = out;
   a.part = T.part.as<double>();
   if (!TR && h->timing) GX_HIP(hipEventRecord(h->ev[8], s));
