@@ -18,8 +18,8 @@
 // block whose four waves take the steps round-robin and are summed in wave
 // order through LDS; a tile with several items writes partials that a combine
 // pass sums in item order. Results therefore do not depend on scheduling.
-// Blocks are mapped so that each XCD (and its L2) takes a contiguous range of
-// items, i.e. of neighbouring tiles that share sources and keys.
+// Blocks are mapped so that each XCD (and its L2) takes runs of XCD_RUN
+// consecutive items, i.e. of neighbouring tiles that share sources and keys.
 //
 // The adjoint (VJP) uses the same machinery with rows = sources and the
 // transposed FP64 operator: mbar_s += T_K^T (sum of lbar_t over its group).
@@ -224,26 +224,46 @@ __global__ void k_mx_step_ids(const uint32_t* __restrict__ step_grp, const uint3
   }
 }
 
+// Lanes per run of the pre-sums: a lane per coefficient pair when S is even
+// (each row then starts on a 16-byte boundary), else a lane per coefficient.
+inline __host__ __device__ constexpr int sums_lanes(int s) {
+  return s % 2 == 0 ? (s / 2 <= 16 ? 16 : s / 2 <= 32 ? 32 : 64) : (s <= 32 ? 32 : 64);
+}
+
 // Pre-sums of the long runs, every evaluation, in a fixed order. Runs of at
-// most MX_WAVE_RUN pairs: one wave per run, lane per coefficient, sources in
-// list order. Longer runs (sources listed by very many rows, as in the
-// adjoint): one block per run, the four waves take sources w, w + 4, ... and
-// are added in wave order.
+// most MX_WAVE_RUN pairs: sums_lanes(S) lanes per run, sources in list order.
+// Longer runs (hubs: sources listed by very many rows, as in the adjoint) are
+// summed in chunks by k_m2l_hub_chunks and k_m2l_hub_sum.
 template <int P>
 __global__ void __launch_bounds__(256) k_m2l_run_sums_w(const uint32_t* __restrict__ lr_order,
                                                         const uint32_t* __restrict__ lr_first,
                                                         const uint32_t* __restrict__ lr_len, uint32_t nlr,
                                                         const uint32_t* __restrict__ src,
                                                         const double* __restrict__ in, double* __restrict__ virt) {
-  // one wave per run, or one half-wave per run when S <= 32
+  // LW lanes per run (sums_lanes), RPW runs per wave
   constexpr int S = num_moments(P);
-  constexpr int RPW = S <= 32 ? 2 : 1, LW = 64 / RPW;
+  constexpr int LW = sums_lanes(S), RPW = 64 / LW;
   const int lane = threadIdx.x & 63;
   const uint32_t q = (xcd_runs(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6)) * RPW + lane / LW;
   const int kl = lane % LW;
   if (q >= nlr) return;
   const uint32_t r = lr_order[q], f = lr_first[r], len = lr_len[r];
   if (len > MX_WAVE_RUN) return;
+  if constexpr (S % 2 == 0) {
+    // a lane per coefficient pair: each coefficient is still summed alone,
+    // in list order
+    for (int k2 = kl; k2 < S / 2; k2 += LW) {
+      double ax = 0.0, ay = 0.0;
+#pragma unroll 4
+      for (uint32_t i = 0; i < len; ++i) {
+        const double2 v = *reinterpret_cast<const double2*>(in + (size_t)src[f + i] * S + 2 * k2);
+        ax += isfinite(v.x) ? v.x : 0.0;
+        ay += isfinite(v.y) ? v.y : 0.0;
+      }
+      *reinterpret_cast<double2*>(virt + (size_t)r * S + 2 * k2) = make_double2(ax, ay);
+    }
+    return;
+  }
   for (int k = kl; k < S; k += LW) {
     double acc = 0.0;
 #pragma unroll 4
@@ -916,7 +936,8 @@ static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
   a.n_in = (uint32_t)h->tree.n_nodes;
   if (T.nlong) {
     GX_CHECK(dbuf_reserve(T.virt, T.nlong * S * 8, s));
-    k_m2l_run_sums_w<P><<<nblk(T.nlong, S <= 32 ? 8 : 4), 256, 0, s>>>(T.lr_order.as<uint32_t>(),
+    if ((uintptr_t)in & 15) return set_error(GALAX_ERR_INVALID, "M2L source rows need 16-byte aligned bases");
+    k_m2l_run_sums_w<P><<<nblk(T.nlong, 4 * (64 / sums_lanes(S))), 256, 0, s>>>(T.lr_order.as<uint32_t>(),
                                                                        T.lr_first.as<uint32_t>(), T.lr_len.as<uint32_t>(),
                                                           (uint32_t)T.nlong, T.src.as<uint32_t>(), in,
                                                           T.virt.as<double>());
