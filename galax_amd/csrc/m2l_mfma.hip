@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(256) k_m2l_run_sums_w(const uint32_t* __restri
   }
 }
 
-// Hub runs in chunks of MX_HUB_CHUNK rows: a half-wave (wave) per chunk sums
+// Hub runs in chunks of MX_HUB_CHUNK rows: sums_lanes(S) lanes per chunk sum
 // its rows in list order into a partial row; then one thread per (hub,
 // coefficient) adds the hub's partials in chunk order (a fixed order).
 template <int P>
@@ -284,12 +284,25 @@ __global__ void __launch_bounds__(256) k_m2l_hub_chunks(const uint32_t* __restri
                                                         const uint32_t* __restrict__ src,
                                                         const double* __restrict__ in, double* __restrict__ part) {
   constexpr int S = num_moments(P);
-  constexpr int RPW = S <= 32 ? 2 : 1, LW = 64 / RPW;
+  constexpr int LW = sums_lanes(S), RPW = 64 / LW;
   const int lane = threadIdx.x & 63;
   const uint32_t q = (blockIdx.x * 4 + (threadIdx.x >> 6)) * RPW + lane / LW;
   const int kl = lane % LW;
   if (q >= nch) return;
   const uint32_t f = ch_first[q], len = ch_len[q];
+  if constexpr (S % 2 == 0) {
+    for (int k2 = kl; k2 < S / 2; k2 += LW) {
+      double ax = 0.0, ay = 0.0;
+#pragma unroll 4
+      for (uint32_t i = 0; i < len; ++i) {
+        const double2 v = *reinterpret_cast<const double2*>(in + (size_t)src[f + i] * S + 2 * k2);
+        ax += isfinite(v.x) ? v.x : 0.0;
+        ay += isfinite(v.y) ? v.y : 0.0;
+      }
+      *reinterpret_cast<double2*>(part + (size_t)q * S + 2 * k2) = make_double2(ax, ay);
+    }
+    return;
+  }
   for (int k = kl; k < S; k += LW) {
     double acc = 0.0;
 #pragma unroll 4
@@ -942,7 +955,7 @@ static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
                                                           (uint32_t)T.nlong, T.src.as<uint32_t>(), in,
                                                           T.virt.as<double>());
     if (T.nhub) {
-      k_m2l_hub_chunks<P><<<nblk(T.nhub_chunks, S <= 32 ? 8 : 4), 256, 0, s>>>(
+      k_m2l_hub_chunks<P><<<nblk(T.nhub_chunks, 4 * (64 / sums_lanes(S))), 256, 0, s>>>(
           T.hub_first.as<uint32_t>(), T.hub_len.as<uint32_t>(), (uint32_t)T.nhub_chunks, T.src.as<uint32_t>(), in,
           T.hub_part.as<double>());
       GX_HIP(hipGetLastError());
