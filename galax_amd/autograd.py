@@ -76,11 +76,16 @@ def _kdk_segment(gctx, m, dt, k, x, y, vx, vy, ax, ay):
 def leapfrog(x, y, vx, vy, m, gctx: fmm.GpuContext, dt: float, n_steps: int, checkpoint_every: int = 10):
     """n_steps KDK steps (tree static), differentiable in x, y, vx, vy, m.
     Segments of `checkpoint_every` steps keep only their inputs; their
-    intermediate states are recomputed during backward."""
+    intermediate states are recomputed during backward. checkpoint_every = 0
+    turns checkpointing off (one segment, every activation kept)."""
+    if checkpoint_every < 0:
+        raise ValueError(f"checkpoint_every must be >= 0, got {checkpoint_every}")
+    if n_steps < 0:
+        raise ValueError(f"n_steps must be >= 0, got {n_steps}")
     ax, ay = fmm_forces(x, y, m, gctx)
     done = 0
     while done < n_steps:
-        k = min(checkpoint_every, n_steps - done)
+        k = min(checkpoint_every, n_steps - done) if checkpoint_every > 0 else n_steps - done
         if checkpoint_every > 0 and torch.is_grad_enabled():
             x, y, vx, vy, ax, ay = tcp.checkpoint(_kdk_segment, gctx, m, dt, k, x, y, vx, vy, ax, ay,
                                                   use_reentrant=False)

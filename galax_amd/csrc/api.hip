@@ -227,9 +227,12 @@ int build_lists_corrected(galax_fmm* h, const double* d_x, const double* d_y) {
     uint32_t* cnt = ws.b.as<uint32_t>();
     k_dtt_count<<<nb256(n), 256, 0, s>>>(dt, P.as<uint64_t>(), (uint32_t)lo, (uint32_t)hi, cnt);
     GX_HIP(hipGetLastError());
+    // pair ids and the scan are 32-bit: check the exact 64-bit total first
+    uint64_t total64 = 0;
+    GX_CHECK(sum_u32(h, cnt, n, &total64));
+    if (hi + total64 >= (1ull << 32)) return set_error(GALAX_ERR_UNSUPPORTED, "dual tree traversal above 2^32 pairs");
     uint32_t total = 0;
     GX_CHECK(scan_u32(h, cnt, cnt, n, false, false, &total));
-    if (hi + total >= (1ull << 32)) return set_error(GALAX_ERR_UNSUPPORTED, "dual tree traversal above 2^32 pairs");
     GX_CHECK(grow_pairs(P, hi, hi + total, s));
     if (total) {
       k_dtt_expand<<<nb256(n), 256, 0, s>>>(dt, P.as<uint64_t>(), (uint32_t)lo, (uint32_t)hi, cnt);
@@ -245,6 +248,11 @@ int build_lists_corrected(galax_fmm* h, const double* d_x, const double* d_y) {
   uint32_t *mpos = ws.c.as<uint32_t>(), *ppos = ws.d.as<uint32_t>();
   k_dtt_emit_count<<<nb256(nv), 256, 0, s>>>(dt, P.as<uint64_t>(), (uint32_t)nv, mpos, ppos);
   GX_HIP(hipGetLastError());
+  uint64_t m64 = 0, p64 = 0;
+  GX_CHECK(sum_u32(h, mpos, nv, &m64));
+  GX_CHECK(sum_u32(h, ppos, nv, &p64));
+  if (m64 >= 0xFFFFFFF0ull || p64 + nl >= 0xFFFFFFF0ull)
+    return set_error(GALAX_ERR_UNSUPPORTED, "corrected lists above 2^32 entries");
   uint32_t mtot = 0, ptot = 0;
   GX_CHECK(scan_u32(h, mpos, mpos, nv, false, false, &mtot));
   GX_CHECK(scan_u32(h, ppos, ppos, nv, false, false, &ptot));
@@ -508,6 +516,7 @@ void galax_fmm_destroy(galax_fmm_t* h) {
 }
 
 static int build_common(galax_fmm_t* h, size_t n) {
+  h->has_tree = false;  // a build that fails part-way leaves no usable tree
   for (hipEvent_t& e : h->bev)
     if (!e) GX_HIP(hipEventCreate(&e));
   h->build_timed = true;
@@ -607,8 +616,9 @@ int galax_fmm_upload_tree(galax_fmm_t* h, const galax_tree_view_t* tree, const g
                           const galax_lists_view_t* p2p) {
   if (!h || !tree || !m2l || !p2p) return set_error(GALAX_ERR_INVALID, "NULL argument");
   h->build_timed = false;  // an uploaded tree has no device build stages
+  // install_tree_host checks everything on the host before it changes any
+  // state; once it starts uploading, the previous tree is gone
   GX_CHECK(install_tree_host(h, tree, m2l, p2p));
-  if (h->tree.n_bodies > h->max_n) return set_error(GALAX_ERR_INVALID, "tree holds more bodies than max_n");
   return after_tree_installed(h);
 }
 
@@ -748,6 +758,56 @@ int galax_fmm_evaluate(galax_fmm_t* h, galax_bodies_t* b) {
     GX_CHECK(down(s, b->m, h->m, n));
     GX_HIP(hipStreamSynchronize(s));
   }
+  return 0;
+}
+
+int galax_fmm_crosscheck(galax_fmm_t* h, galax_bodies_t* b, double threshold, double* max_rel, double* mean_rel,
+                         size_t* count) {
+  if (!h || !b) return set_error(GALAX_ERR_INVALID, "NULL argument");
+  if (std::isnan(threshold)) return set_error(GALAX_ERR_INVALID, "threshold is NaN");
+  size_t n = h->tree.n_bodies;
+  if (b->n != n) return set_error(GALAX_ERR_INVALID, "bodies.n = %zu but the tree holds %zu bodies", b->n, n);
+  if (n && (!b->x || !b->y || !b->m || !b->ax || !b->ay))
+    return set_error(GALAX_ERR_INVALID, "x, y, m, ax, ay are required");
+  hipStream_t s = h->stream;
+  GX_CHECK(up(s, h->x, b->x, n));
+  GX_CHECK(up(s, h->y, b->y, n));
+  GX_CHECK(up(s, h->m, b->m, n));
+  GX_CHECK(prepare_eval(h, h->x.as<double>(), h->y.as<double>()));
+  // the context's form ...
+  GX_CHECK(evaluate(h, h->x.as<double>(), h->y.as<double>(), h->m.as<double>(), h->ax.as<double>(),
+                    h->ay.as<double>(), nullptr));
+  std::vector<double> gax(n), gay(n);
+  GX_CHECK(down(s, gax.data(), h->ax, n));
+  GX_CHECK(down(s, gay.data(), h->ay, n));
+  GX_HIP(hipStreamSynchronize(s));
+  // ... then galax-core's arithmetic on the same tree (FP64_EXACT uses no
+  // matrix-core operators, so the context's cached ones stay valid)
+  const uint32_t prec = h->cfg.precision;
+  h->cfg.precision = GALAX_PREC_FP64_EXACT;
+  const int rc = evaluate(h, h->x.as<double>(), h->y.as<double>(), h->m.as<double>(), h->ax.as<double>(),
+                          h->ay.as<double>(), nullptr);
+  h->cfg.precision = prec;
+  GX_CHECK(rc);
+  GX_CHECK(down(s, b->ax, h->ax, n));
+  GX_CHECK(down(s, b->ay, h->ay, n));
+  GX_HIP(hipStreamSynchronize(s));
+  // GpuContext::crosscheck's metric (galax-gpu/src/lib.rs:646-658)
+  double mx = 0.0, sum = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    const double cm = std::sqrt(b->ax[i] * b->ax[i] + b->ay[i] * b->ay[i]);
+    const double gm = std::sqrt(gax[i] * gax[i] + gay[i] * gay[i]);
+    const double den = std::fmax(std::fmax(cm, gm), 1e-30);
+    const double dx = gax[i] - b->ax[i], dy = gay[i] - b->ay[i];
+    const double rel = std::sqrt(dx * dx + dy * dy) / den;
+    mx = std::fmax(mx, rel);
+    sum += rel;
+  }
+  if (max_rel) *max_rel = mx;
+  if (mean_rel) *mean_rel = n ? sum / (double)n : 0.0;
+  if (count) *count = n;
+  if (mx > threshold)
+    return set_error(GALAX_ERR_THRESHOLD, "crosscheck max relative error %.2e exceeds threshold %.2e", mx, threshold);
   return 0;
 }
 

@@ -175,6 +175,8 @@ int install_tree_host(galax_fmm* h, const galax_tree_view_t* t, const galax_list
 int build_lists_corrected(galax_fmm* h, const double* d_x, const double* d_y);
 int ensure_node_capacity(galax_fmm* h, size_t nodes);
 int count_interactions(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, uint64_t* out);
+// exact 64-bit sum of n uint32 device values (uses flag[14..15])
+int sum_u32(galax_fmm* h, const uint32_t* d_in, size_t n, uint64_t* total);
 int radix_sort_bits(galax_fmm* h, uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb, size_t n, int bits);
 // P2P work items: each target leaf's gather list in chunks of at most
 // P2P_ITEM_LEAVES source leaves (balances leaves with thousands of sources).

@@ -589,7 +589,7 @@ int build_tree_constrained(galax_fmm* h, size_t n) {
     double* u = h->ws.h.as<double>();
     size_t nn = t.n_nodes;
     NodeArrays a = node_arrays(h);
-    k_constrained_check<<<nblk(nl), 128, 0, s>>>(a, (uint32_t)nl, (uint32_t)(nn - 1), u, u + nn, u + 2 * nn,
+    k_constrained_check<<<nblk(nl, 128), 128, 0, s>>>(a, (uint32_t)nl, (uint32_t)(nn - 1), u, u + nn, u + 2 * nn,
                                                   u + 3 * nn, viol, err);
     GX_HIP(hipGetLastError());
     std::vector<uint32_t> hv(nl + 1);
@@ -1050,6 +1050,11 @@ int install_tree_host(galax_fmm* h, const galax_tree_view_t* tv, const galax_lis
   hipStream_t s = h->stream;
   size_t nn = tv->n_nodes;
   if (m2l->n_nodes != nn || p2p->n_nodes != nn) return set_error(GALAX_ERR_INVALID, "list sizes != tree size");
+  if (nn >= 0xFFFFFFF0ull) return set_error(GALAX_ERR_INVALID, "too many nodes");
+  if (nn && (!tv->center_x || !tv->center_y || !tv->half_width || !tv->mass || !tv->com_x || !tv->com_y ||
+             !tv->body_start || !tv->body_end || !tv->children || !tv->parent || !tv->is_leaf))
+    return set_error(GALAX_ERR_INVALID, "NULL tree array");
+  if (!m2l->offsets || !p2p->offsets) return set_error(GALAX_ERR_INVALID, "NULL list offsets");
   std::vector<uint32_t> level(nn, 0), cf(nn, NONE), par(nn, NONE), bs(nn), be(nn);
   std::vector<uint8_t> cc(nn, 0), lv8(nn);
   size_t nl = 0, nb = 0;
@@ -1074,9 +1079,17 @@ int install_tree_host(galax_fmm* h, const galax_tree_view_t* tv, const galax_lis
     level[i] = mx;
     if (mx > 255) return set_error(GALAX_ERR_INVALID, "tree deeper than 255 levels");
     lv8[i] = (uint8_t)mx;
-    par[i] = tv->parent[i] < 0 ? NONE : (uint32_t)tv->parent[i];
+    const int32_t pi = tv->parent[i];
+    if (pi >= 0 && ((size_t)pi <= i || (size_t)pi >= nn))
+      return set_error(GALAX_ERR_INVALID, "parent %d of node %zu is out of range", pi, i);
+    par[i] = pi < 0 ? NONE : (uint32_t)pi;
+    if (tv->body_start[i] > tv->body_end[i] || tv->body_end[i] > h->max_n)
+      return set_error(GALAX_ERR_INVALID, "node %zu: body range [%llu, %llu) not within [0, max_n = %zu]", i,
+                       (unsigned long long)tv->body_start[i], (unsigned long long)tv->body_end[i], h->max_n);
     bs[i] = (uint32_t)tv->body_start[i];
     be[i] = (uint32_t)tv->body_end[i];
+    if (k > 0 && (bs[i] != bs[cf[i]] || be[i] != be[cf[i] + k - 1]))
+      return set_error(GALAX_ERR_INVALID, "node %zu: body range differs from its children's", i);
     if (k == 0) {
       ++nl;
       nb = std::max<size_t>(nb, be[i]);
@@ -1089,6 +1102,28 @@ int install_tree_host(galax_fmm* h, const galax_tree_view_t* tv, const galax_lis
     if (level[i] < level[i - 1]) return set_error(GALAX_ERR_INVALID, "node ids are not level-ordered");
   for (size_t i = 0; i < nl; ++i)
     if (level[i] != 0) return set_error(GALAX_ERR_INVALID, "leaves must come first");
+  for (size_t i = 0; i < nn; ++i)
+    for (uint32_t c = cf[i]; cc[i] && c < cf[i] + cc[i]; ++c)
+      if (par[c] != i) return set_error(GALAX_ERR_INVALID, "child %u of node %zu has parent %d", c, i, tv->parent[c]);
+  // lists: offsets from 0, non-decreasing, u32-sized; entries name nodes (M2L)
+  // or leaves (P2P), which the device kernels index without further checks
+  auto check_lists = [&](const galax_lists_view_t* lv, size_t bound, const char* what) -> int {
+    if (lv->offsets[0] != 0) return set_error(GALAX_ERR_INVALID, "%s offsets[0] != 0", what);
+    for (size_t i = 0; i < nn; ++i)
+      if (lv->offsets[i + 1] < lv->offsets[i])
+        return set_error(GALAX_ERR_INVALID, "%s offsets decrease at node %zu", what, i);
+    const uint64_t total = lv->offsets[nn];
+    if (total >= 0xFFFFFFF0ull) return set_error(GALAX_ERR_INVALID, "%s lists too long", what);
+    if (total && !lv->data) return set_error(GALAX_ERR_INVALID, "NULL %s list data", what);
+    for (uint64_t k = 0; k < total; ++k)
+      if (lv->data[k] >= bound)
+        return set_error(GALAX_ERR_INVALID, "%s list entry %llu = %u is not below %zu", what, (unsigned long long)k,
+                         lv->data[k], bound);
+    return 0;
+  };
+  GX_CHECK(check_lists(m2l, nn, "M2L"));
+  GX_CHECK(check_lists(p2p, nl, "P2P"));
+  h->has_tree = false;  // from here on the previous tree is overwritten
   GX_CHECK(ensure_node_capacity(h, nn));
   auto up = [&](DBuf& d, const void* src, size_t bytes) -> int {
     GX_HIP(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, s));

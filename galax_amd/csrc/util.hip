@@ -173,6 +173,42 @@ int scan_u32(galax_fmm* h, const uint32_t* d_in, uint32_t* d_out, size_t n, bool
   return 0;
 }
 
+// 64-bit sum of n uint32 values: a block sum per 2048 values, then one
+// 64-bit atomic add per block (the order of the adds does not change an
+// integer sum).
+__global__ void k_sum_u32_u64(const uint32_t* __restrict__ in, size_t n, unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long part[SCAN_THREADS / 64];
+  const size_t base = (size_t)blockIdx.x * SCAN_CHUNK;
+  unsigned long long v = 0;
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const size_t i = base + (size_t)k * SCAN_THREADS + threadIdx.x;
+    if (i < n) v += in[i];
+  }
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_down(v, d, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < SCAN_THREADS / 64; ++w) t += part[w];
+    atomicAdd(out, t);
+  }
+}
+
+int sum_u32(galax_fmm* h, const uint32_t* d_in, size_t n, uint64_t* total) {
+  *total = 0;
+  if (n == 0) return 0;
+  GX_CHECK(dbuf_reserve(h->flag, 64, h->stream));
+  unsigned long long* d = reinterpret_cast<unsigned long long*>(h->flag.as<uint32_t>() + 14);
+  GX_HIP(hipMemsetAsync(d, 0, 8, h->stream));
+  k_sum_u32_u64<<<(unsigned)((n + SCAN_CHUNK - 1) / SCAN_CHUNK), SCAN_THREADS, 0, h->stream>>>(d_in, n, d);
+  GX_HIP(hipGetLastError());
+  unsigned long long v = 0;
+  GX_HIP(hipMemcpyAsync(&v, d, 8, hipMemcpyDeviceToHost, h->stream));
+  GX_HIP(hipStreamSynchronize(h->stream));
+  *total = v;
+  return 0;
+}
+
 int build_mark(galax_fmm* h, int i) {
   if (h->build_timed) GX_HIP(hipEventRecord(h->bev[i], h->stream));
   return 0;

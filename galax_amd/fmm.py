@@ -105,6 +105,7 @@ def lib():
         "galax_fmm_export_expansions": [vp, vp, vp, C.c_size_t],
         "galax_fmm_evaluate": [vp, C.POINTER(_Bodies)],
         "galax_fmm_evaluate_device": [vp] * 7,
+        "galax_fmm_crosscheck": [vp, C.POINTER(_Bodies), d, vp, vp, vp],
         "galax_fmm_last_timings": [vp, C.POINTER(_Timings)],
         "galax_fmm_last_build_timings": [vp, C.POINTER(_BuildTimings)],
         "galax_fmm_set_timing": [vp, C.c_int],
@@ -225,6 +226,14 @@ class GpuContext:
         _check(lib().galax_fmm_create(C.byref(c), cfg.max_n, C.byref(h)))
         self.h = h
         self._acc = None
+        self._n = None  # bodies of the installed tree
+        self._device = None  # torch device index of the handle (the current one at creation)
+        try:
+            import torch
+            if torch.cuda.is_available():
+                self._device = torch.cuda.current_device()
+        except ImportError:
+            pass
 
     def close(self):
         if getattr(self, "h", None):
@@ -244,7 +253,9 @@ class GpuContext:
         Tree::build; returns perm with new[i] = old[perm[i]]."""
         perm = np.zeros(len(b), np.uint64)
         cb = b._c()
+        self._n = None
         _check(lib().galax_fmm_build_tree(self.h, C.byref(cb), _ptr(perm)))
+        self._n = len(b)
         return perm
 
     def upload_tree_data(self, tree: dict, m2l_csr, p2p_csr):
@@ -267,7 +278,10 @@ class GpuContext:
             dat = np.ascontiguousarray(dat if len(dat) else np.zeros(1), np.uint32)
             keep[f"o{len(lv)}"], keep[f"d{len(lv)}"] = off, dat
             lv.append(_ListsView(nn, _ptr(off), _ptr(dat)))
+        self._n = None
         _check(lib().galax_fmm_upload_tree(self.h, C.byref(tv), C.byref(lv[0]), C.byref(lv[1])))
+        leaf = np.asarray(tree["is_leaf"]) != 0
+        self._n = int(np.asarray(tree["body_end"])[leaf].max()) if leaf.any() else 0
 
     def set_softening(self, eps: float):
         _check(lib().galax_fmm_set_softening(self.h, eps))
@@ -347,15 +361,55 @@ class GpuContext:
         ax[:] = self._acc[0]
         ay[:] = self._acc[1]
 
+    def _dev_ptrs(self, n, **arrays):
+        """Device pointers of torch tensors checked against what the library
+        reads and writes: float64, contiguous, on the handle's device, n
+        elements (None passes through). Raw integers are taken as device
+        pointers to n doubles, unchecked. Torch's pending work on the
+        tensors' device is finished first: the library runs on its own stream."""
+        out, dev = [], None
+        for name, t in arrays.items():
+            if t is None:
+                out.append(None)
+                continue
+            if not hasattr(t, "data_ptr"):
+                out.append(int(t))
+                continue
+            import torch
+            if t.dtype != torch.float64:
+                raise TypeError(f"{name}: expected float64, got {t.dtype}")
+            if not t.is_cuda:
+                raise ValueError(f"{name}: expected a device (CUDA/HIP) tensor")
+            if self._device is not None and t.device.index != self._device:
+                raise ValueError(f"{name}: tensor on device {t.device.index}, context on {self._device}")
+            if not t.is_contiguous():
+                raise ValueError(f"{name}: tensor is not contiguous")
+            if t.numel() != n:
+                raise ValueError(f"{name}: {t.numel()} elements, expected {n}")
+            dev = t.device
+            out.append(t.data_ptr())
+        if dev is not None:
+            import torch
+            torch.cuda.current_stream(dev).synchronize()
+        return out
+
+    def _tree_n(self):
+        if self._n is None:
+            raise GalaxError(-4, "no tree: call build_tree, build_tree_device or upload_tree_data")
+        return self._n
+
     def evaluate_device(self, x, y, m, ax, ay, phi=None):
-        """Device-resident arrays (torch CUDA tensors or raw pointers)."""
-        p = lambda t: None if t is None else (t.data_ptr() if hasattr(t, "data_ptr") else int(t))
-        _check(lib().galax_fmm_evaluate_device(self.h, p(x), p(y), p(m), p(ax), p(ay), p(phi)))
+        """Device-resident arrays (torch CUDA tensors, checked, or raw pointers)
+        of the installed tree's bodies."""
+        ptrs = self._dev_ptrs(self._tree_n(), x=x, y=y, m=m, ax=ax, ay=ay, phi=phi)
+        _check(lib().galax_fmm_evaluate_device(self.h, *ptrs))
 
     def build_tree_device(self, x, y, m, vx=None, vy=None, ax=None, ay=None):
-        p = lambda t: None if t is None else (t.data_ptr() if hasattr(t, "data_ptr") else int(t))
         n = int(x.numel()) if hasattr(x, "numel") else len(x)
-        _check(lib().galax_fmm_build_tree_device(self.h, p(x), p(y), p(vx), p(vy), p(m), p(ax), p(ay), n))
+        ptrs = self._dev_ptrs(n, x=x, y=y, vx=vx, vy=vy, m=m, ax=ax, ay=ay)
+        self._n = None
+        _check(lib().galax_fmm_build_tree_device(self.h, *ptrs, n))
+        self._n = n
 
     def timings(self) -> Timings:
         t = _Timings()
@@ -394,9 +448,9 @@ class GpuContext:
         return xb, yb, mb
 
     def vjp_device(self, x, y, m, ax_bar, ay_bar, x_bar, y_bar, m_bar):
-        p = lambda t: None if t is None else (t.data_ptr() if hasattr(t, "data_ptr") else int(t))
-        _check(lib().galax_fmm_vjp_device(self.h, p(x), p(y), p(m), p(ax_bar), p(ay_bar), p(x_bar), p(y_bar),
-                                          p(m_bar)))
+        ptrs = self._dev_ptrs(self._tree_n(), x=x, y=y, m=m, ax_bar=ax_bar, ay_bar=ay_bar, x_bar=x_bar, y_bar=y_bar,
+                              m_bar=m_bar)
+        _check(lib().galax_fmm_vjp_device(self.h, *ptrs))
 
     def validate(self, b: BodiesSoA, samples: np.ndarray):
         """validate_fmm (galax-core/src/lib.rs:1503-1564) with given samples:
@@ -407,6 +461,22 @@ class GpuContext:
         _check(lib().galax_fmm_validate(self.h, C.byref(cb), _ptr(s), len(s), C.byref(mx), C.byref(mn),
                                         C.byref(rv)))
         return mx.value, mn.value, rv.value
+
+    def crosscheck(self, b: BodiesSoA, threshold: float = 0.5):
+        """GpuContext::crosscheck (galax-gpu/src/lib.rs:623-667): the context's
+        form against FP64_EXACT (galax-core's arithmetic) on the same tree.
+        Returns (max_rel, mean_rel, n); leaves the FP64_EXACT accelerations in
+        b.ax, b.ay; raises GalaxError(ThresholdExceeded) where the reference
+        panics (max_rel > threshold)."""
+        mx, mn, cnt = C.c_double(), C.c_double(), C.c_size_t()
+        cb = b._c()
+        _check(lib().galax_fmm_crosscheck(self.h, C.byref(cb), threshold, C.byref(mx), C.byref(mn), C.byref(cnt)))
+        return mx.value, mn.value, cnt.value
+
+    async def read_accelerations_async(self, ax: np.ndarray, ay: np.ndarray):
+        """GpuContext::read_accelerations_async (galax-gpu/src/lib.rs:571-621):
+        as on the reference's native target, the blocking read_accelerations."""
+        self.read_accelerations(ax, ay)
 
     def n2_debug(self, b: BodiesSoA):
         """GpuContext::n2_debug (galax-gpu/src/lib.rs:670-714), in FP64."""

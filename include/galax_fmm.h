@@ -113,8 +113,10 @@ typedef struct {
 } galax_lists_view_t;
 
 /* Energy / momentum diagnostics (galax-integrate/src/lib.rs:192-199). potential
- * is the true -1/2 sum m_i phi_i; potential_heuristic is galax-core's
- * compute_fmm_potential_energy (lib.rs:1671-1683). */
+ * is 1/2 sum_i m_i phi_i with phi_i = -sum_{j != i} m_j / sqrt(r_ij^2 + eps^2)
+ * (direct_potential's sign, galax-core/src/lib.rs:1627-1635) from the FMM
+ * potentials; potential_heuristic is galax-core's compute_fmm_potential_energy
+ * (lib.rs:1671-1683). */
 typedef struct {
   uint64_t step;
   double kinetic, potential, potential_heuristic, momentum_x, momentum_y, angular_momentum;
@@ -206,6 +208,18 @@ GALAX_API int galax_fmm_export_corrected_lists(galax_fmm_t* h, galax_lists_view_
  * (writing the sanitized values back to the caller's arrays when any changed),
  * writes ax, ay and, when non-NULL, phi. Synchronous. */
 GALAX_API int galax_fmm_evaluate(galax_fmm_t* h, galax_bodies_t* bodies);
+
+/* ~GpuContext::crosscheck (galax-gpu/src/lib.rs:623-667): evaluates the
+ * bodies in the context's precision form, then in GALAX_PREC_FP64_EXACT
+ * (galax-core's arithmetic: expansions bit-identical to compute_fmm_force's)
+ * on the same tree, and compares them per body as the reference does:
+ * |a_form - a_exact| / max(|a_exact|, |a_form|, 1e-30). Writes the
+ * FP64_EXACT accelerations to ax, ay (the reference leaves the CPU result in
+ * bodies.ax/ay) and max, mean and count to the optional outputs. Returns
+ * GALAX_ERR_THRESHOLD (outputs written) where the reference panics: when the
+ * maximum exceeds threshold. */
+GALAX_API int galax_fmm_crosscheck(galax_fmm_t* h, galax_bodies_t* bodies, double threshold, double* max_rel,
+                                   double* mean_rel, size_t* count);
 
 /* As galax_fmm_evaluate on device-resident arrays of the installed tree's n
  * bodies, enqueued on the handle's stream and synchronised before return.

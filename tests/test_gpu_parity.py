@@ -131,6 +131,39 @@ def test_constrained_tree_bit_exact():
     _assert_lists_equal(gp2p, O.p2p_lists(ot))
 
 
+def _split_cases():
+    from constrained_cases import SPLIT_CASES
+    return SPLIT_CASES
+
+
+@pytest.mark.parametrize("name,gen,n_max", _split_cases(), ids=[c[0] for c in _split_cases()])
+def test_constrained_tree_splits_bit_exact(name, gen, n_max):
+    """build_constrained on inputs where it really splits leaves (bodies at the
+    1e-15 scale; chain2048_cap splits on all 10 checks): the device's
+    neighbour search and split loop give the oracle's tree and lists bit for
+    bit, with more leaves than plain n_max chunks (galax-core/src/lib.rs:382-423,
+    541-588)."""
+    from constrained_cases import BOUNDS
+
+    d = gen()
+    n = len(d["x"])
+    print(f"constrained split case {name}: n={n} n_max={n_max}", flush=True)
+    ctx, b, perm = _gpu_build(d, n_max=n_max, bounds=BOUNDS, tree_mode=fmm.TREE_CONSTRAINED)
+    assert np.array_equal(perm, np.arange(n))  # build_constrained does not sort
+    ot, od, _ = _oracle_build(d, n_max, BOUNDS, constrained=True)
+    gt, gm2l, gp2p = ctx.export_tree()
+    _assert_tree_equal(gt, ot.export())
+    assert int(np.asarray(gt["is_leaf"]).sum()) > -(-n // n_max)
+    _assert_lists_equal(gm2l, O.m2l_lists(ot))
+    _assert_lists_equal(gp2p, O.p2p_lists(ot))
+    if n_max <= 32:
+        # forces on the split tree (FP64, p = 4, eps = 0.1): near the oracle's
+        ctx.evaluate(b)
+        ax, ay, _ = O.compute_fmm_force(od["x"], od["y"], od["m"], ot, O.m2l_lists(ot), O.p2p_lists(ot), 4, 0.1)
+        rd = np.hypot(b.ax - ax, b.ay - ay) / np.maximum(np.hypot(ax, ay), 1e-300)
+        assert float(np.median(rd)) <= 1e-9 and float(rd.max()) <= 1e-6
+
+
 def _rel_diff(a, b):
     den = np.maximum(np.hypot(b[0], b[1]), 1e-300)
     return np.hypot(a[0] - b[0], a[1] - b[1]) / den
@@ -427,6 +460,126 @@ def test_kdk_matches_oracle_simulate():
     dg = O.diagnostics(od["x"], od["y"], od["vx"], od["vy"], od["m"], od["ax"], od["ay"])
     assert diags[-1]["kinetic"] == pytest.approx(dg["kinetic"], rel=1e-6)
     assert diags[-1]["potential_heuristic"] == pytest.approx(dg["potential"], rel=1e-6)
+    # momentum and angular momentum (lib.rs:1647-1669): sums of terms of both
+    # signs, so compared on the scale of the sum of their magnitudes
+    mv = float(np.sum(od["m"] * np.hypot(od["vx"], od["vy"])))
+    lz = float(np.sum(od["m"] * np.hypot(od["x"], od["y"]) * np.hypot(od["vx"], od["vy"])))
+    assert abs(diags[-1]["momentum_x"] - dg["momentum_x"]) <= 1e-9 * mv
+    assert abs(diags[-1]["momentum_y"] - dg["momentum_y"]) <= 1e-9 * mv
+    assert abs(diags[-1]["angular_momentum"] - dg["angular_momentum"]) <= 1e-9 * lz
+    # the potential energy 1/2 sum m_i phi_i from the FMM potentials: the
+    # oracle's potentials of the final state (same tree, same algorithm)
+    _, _, ophi = O.compute_fmm_force(od["x"], od["y"], od["m"], ot, m2l, p2p, p, eps, want_phi=True)
+    assert diags[-1]["potential"] == pytest.approx(0.5 * float(np.sum(od["m"] * ophi)), rel=1e-9)
+
+
+def test_kdk_energy_from_potentials_matches_direct_sum():
+    """Device-resident KDK with the corrected variant (p = 8, theta = 0.3):
+    every diagnostic against a host computation on the returned state, the
+    potential energy 1/2 sum m_i phi_i against the FP64 direct sum's
+    potentials (direct_potential's sign, lib.rs:1627-1635; no self term), and
+    the total energy conserved to 1e-4 over 20 steps."""
+    d = ic.plummer(20_000, seed=208)
+    bounds = (-50.0, 50.0, -50.0, 50.0)
+    eps = 0.05
+    ctx, b, _ = _gpu_build(d, p=8, eps=eps, bounds=bounds, algo=fmm.ALGO_CORRECTED, theta=0.3)
+    b.phi = np.zeros(len(b))
+    ctx.evaluate(b)
+    dt = 2e-3
+    diags = ctx.simulate(b, dt, 20, energy_every=10)
+    assert len(diags) == 2
+    last = diags[-1]
+    ex, ey, ephi = fmm.compute_n2_force(b.x, b.y, b.mass, eps, want_phi=True)
+    pe = 0.5 * float(np.sum(b.mass * ephi))
+    assert last["potential"] == pytest.approx(pe, rel=1e-6)
+    assert last["kinetic"] == pytest.approx(0.5 * float(np.sum(b.mass * (b.vx ** 2 + b.vy ** 2))), rel=1e-12)
+    mv = float(np.sum(b.mass * np.hypot(b.vx, b.vy)))
+    assert abs(last["momentum_x"] - float(np.sum(b.mass * b.vx))) <= 1e-12 * mv
+    assert abs(last["momentum_y"] - float(np.sum(b.mass * b.vy))) <= 1e-12 * mv
+    lz = float(np.sum(b.mass * (b.x * b.vy - b.y * b.vx)))
+    assert abs(last["angular_momentum"] - lz) <= 1e-12 * float(np.sum(b.mass * np.hypot(b.x, b.y) * np.hypot(b.vx, b.vy)))
+    e0 = diags[0]["kinetic"] + diags[0]["potential"]
+    e1 = last["kinetic"] + last["potential"]
+    assert abs(e1 - e0) <= 1e-4 * abs(e0)
+
+
+def _upload_inputs(d, n_max, bounds):
+    ot, od, _ = _oracle_build(d, n_max, bounds)
+    return ot.export(), O.m2l_lists(ot).csr(), O.p2p_lists(ot).csr(), od
+
+
+def test_upload_tree_rejects_bad_input_before_any_change():
+    """upload_tree_data (galax-gpu/src/lib.rs:392-462) with a corrupted tree or
+    lists is rejected on the host with InvalidArgument, before anything is
+    uploaded: the tree installed before stays in use and gives the same forces."""
+    d = ic.plummer(3000, seed=209)
+    bounds = (-50.0, 50.0, -50.0, 50.0)
+    tree, m2l, p2p, od = _upload_inputs(d, 32, bounds)
+    n = len(od["x"])
+    ctx = _ctx(n, p=4, eps=0.05, bounds=bounds)
+    ctx.upload_tree_data(tree, m2l, p2p)
+    b = _bodies(od)
+    ctx.evaluate(b)
+    ax0, ay0 = b.ax.copy(), b.ay.copy()
+    nn = len(tree["mass"])
+    nl = int(tree["is_leaf"].sum())
+
+    def bad_lists(csr, k, v):
+        off, dat = csr[0].copy(), csr[1].copy()
+        dat[k] = v
+        return off, dat
+
+    cases = {
+        "p2p entry >= n_leaves": (tree, m2l, bad_lists(p2p, 1, nl)),
+        "m2l entry >= n_nodes": (tree, bad_lists(m2l, 0, nn), p2p),
+        "decreasing offsets": (tree, (np.where(np.arange(nn + 1) == 1, m2l[0][2] + 1, m2l[0]).astype(np.uint64),
+                                      m2l[1]), p2p),
+        "body_end > max_n": (dict(tree, body_end=np.where(np.arange(nn) == nn - 1, n + 1, tree["body_end"])), m2l,
+                             p2p),
+        "parent out of range": (dict(tree, parent=np.where(np.arange(nn) == 0, nn + 5, tree["parent"]).astype(np.int32)),
+                                m2l, p2p),
+        "range differs from children": (dict(tree, body_start=np.where(np.arange(nn) == nn - 1, 1,
+                                                                        tree["body_start"])), m2l, p2p),
+    }
+    for name, (t, ml, pl) in cases.items():
+        try:
+            ctx.upload_tree_data(t, ml, pl)
+        except fmm.GalaxError as e:
+            assert e.code == -4, (name, e)
+        else:
+            raise AssertionError(f"upload with {name} was accepted")
+    ctx._n = n  # the wrapper forgets the body count of a failed upload; the library keeps the tree
+    b2 = _bodies(od)
+    ctx.evaluate(b2)
+    assert np.array_equal(b2.ax, ax0) and np.array_equal(b2.ay, ay0)
+
+
+def test_crosscheck_against_fp64_exact():
+    """GpuContext::crosscheck (galax-gpu/src/lib.rs:623-667): the mixed form
+    against galax-core's arithmetic (FP64_EXACT) on the same tree. The metric
+    is the reference's; bodies.ax/ay end up holding the FP64_EXACT forces;
+    a threshold below the error gives ThresholdExceeded (the reference panics)."""
+    d = ic.plummer(20_000, seed=210)
+    bounds = (-50.0, 50.0, -50.0, 50.0)
+    ctx, b, _ = _gpu_build(d, p=6, eps=0.05, bounds=bounds, precision=fmm.PREC_MIXED_FP32)
+    ctx.evaluate(b)
+    am, bm = b.ax.copy(), b.ay.copy()
+    mx, mean, cnt = ctx.crosscheck(b, 0.5)
+    assert cnt == len(b)
+    ctx_e, be, _ = _gpu_build(d, p=6, eps=0.05, bounds=bounds, precision=fmm.PREC_FP64_EXACT)
+    ctx_e.evaluate(be)
+    assert np.array_equal(b.ax, be.ax) and np.array_equal(b.ay, be.ay)
+    den = np.maximum(np.maximum(np.hypot(be.ax, be.ay), np.hypot(am, bm)), 1e-30)
+    rel = np.hypot(am - be.ax, bm - be.ay) / den
+    assert mx == pytest.approx(float(rel.max()), rel=1e-12)
+    assert mean == pytest.approx(float(rel.mean()), rel=1e-9)
+    assert 0.0 < mx < 1e-3
+    with pytest.raises(fmm.GalaxError) as ei:
+        ctx.crosscheck(b, mx / 2)
+    assert ei.value.code == -7
+    # the context's own form is unchanged afterwards
+    ctx.evaluate(b)
+    assert np.array_equal(b.ax, am) and np.array_equal(b.ay, bm)
 
 
 def test_errors_are_codes_not_crashes():

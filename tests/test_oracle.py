@@ -603,6 +603,121 @@ def test_neighbors_geometric_no_self_and_adjacent():  # lib.rs:2689
                 assert ratio <= 8.0 and _touch(d, i, k)
 
 
+def _node_table(centres, hws):
+    """A tree of free-standing leaf nodes (no links): neighbors_geometric scans
+    every node, so this is all it reads."""
+    nn = len(hws)
+    d = {"center_x": np.array([c[0] for c in centres], float), "center_y": np.array([c[1] for c in centres], float),
+         "half_width": np.array(hws, float), "mass": np.ones(nn), "com_x": np.zeros(nn), "com_y": np.zeros(nn),
+         "body_start": np.zeros(nn, np.uint64), "body_end": np.zeros(nn, np.uint64),
+         "children": -np.ones((nn, 4), np.int32), "parent": -np.ones(nn, np.int32), "is_leaf": np.ones(nn, np.uint8)}
+    return O.Tree.from_arrays(d, 1, (0.0, 1.0, 0.0, 1.0)), d
+
+
+# Query node 0 at (0,0), hw 1; test points E, NE, N, NW, W, SW, S, SE at +-2.
+NEIGHBOUR_TABLES = [
+    # Nodes 1 and 2 both contain E = (2,0) at squared distance 0.5 and touch
+    # node 0: the tie goes to the lower id. Node 3 contains E at distance 0
+    # but its size ratio is 9 > 8. No node contains another test point.
+    ("tie_low_id_wins", [(0, 0), (1.5, -0.5), (1.5, 0.5), (2, 0)], [1, 1, 1, 9], [1, -1, -1, -1, -1, -1, -1, -1]),
+    ("tie_low_id_wins_swapped", [(0, 0), (1.5, 0.5), (1.5, -0.5), (2, 0)], [1, 1, 1, 9],
+     [1, -1, -1, -1, -1, -1, -1, -1]),
+    # Ratio exactly 8 is allowed: node 4 (hw 8, centre (2,0)) contains every
+    # test point and is nearest to E; for the others it is the only candidate.
+    ("ratio_8_allowed", [(0, 0), (1.5, -0.5), (1.5, 0.5), (2, 0), (2, 0)], [1, 1, 1, 9, 8], [4] * 8),
+    # Containment slack: |2 - (1 - 4e-15)| exceeds 1 + 1e-15 (node 1 out),
+    # |2 - (1 - 5e-16)| does not (node 2 in); neither reaches the other points.
+    ("containment_slack", [(0, 0), (1 - 4e-15, 0), (1 - 5e-16, 0)], [1, 1, 1], [2, -1, -1, -1, -1, -1, -1, -1]),
+    # A node touching node 0's box exactly from outside contains E but is not
+    # cells_adjacent (|3 - 0| > 1 + 1 + 1e-15): None everywhere.
+    ("outside_not_adjacent", [(0, 0), (3, 0)], [1, 1], [-1] * 8),
+]
+
+
+@pytest.mark.parametrize("name,centres,hws,want", NEIGHBOUR_TABLES, ids=[t[0] for t in NEIGHBOUR_TABLES])
+def test_neighbors_geometric_hand_tables(name, centres, hws, want):
+    """lib.rs:382-423 on node tables whose answers are worked out by hand (the
+    comments above): nearest containing adjacent node, lowest id on ties,
+    ratio <= 8, the 1e-15 slack, and None."""
+    from constrained_cases import brute_neighbors
+
+    t, d = _node_table(centres, hws)
+    assert list(O.neighbors_geometric(t, 0)) == want
+    assert brute_neighbors(d["center_x"], d["center_y"], d["half_width"], 0) == want
+
+
+def _neighbour_trees():
+    from constrained_cases import BOUNDS, chain_case, hand_case, random_case
+
+    yield "uniform", O.tree_build(rng_bodies(300, 31, 0, 10), 8, (0, 10, 0, 10))
+    yield "plummer", O.tree_build(ic.plummer(600, seed=32), 16, (-50, 50, -50, 50))
+    yield "snapshot_order", O.build_constrained(ic.reference_test_bodies(300), 16, (0, 100, 0, 100))
+    yield "hand", O.build_constrained(hand_case(), 4, BOUNDS)
+    yield "random37", O.build_constrained(random_case(37, 16, 8), 8, BOUNDS)
+    yield "chain256", O.build_constrained(chain_case(256), 256, BOUNDS)
+
+
+def test_neighbors_geometric_brute_force_all_nodes():
+    """Every node of trees at the ordinary scale and at the 1e-15 scale: the
+    oracle equals the brute-force specification (vectorised candidate sets,
+    lexsort on (distance, id))."""
+    from constrained_cases import brute_neighbors
+
+    for name, t in _neighbour_trees():
+        d = t.export()
+        cx, cy, hw = d["center_x"], d["center_y"], d["half_width"]
+        found = 0
+        for i in range(len(t)):
+            got = list(O.neighbors_geometric(t, i))
+            assert got == brute_neighbors(cx, cy, hw, i), (name, i)
+            found += sum(k >= 0 for k in got)
+        assert found > 0, name
+
+
+def test_build_constrained_hand_case():
+    """Worked by hand, n_max = 4. First build: leaf A = [0,4) spans x in
+    [0, 4e-15] (centre 2e-15, hw 2e-15 + 1e-15 = 3e-15); leaf B = [4,8) is 4
+    coincident bodies at 6.5e-15 (hw 1e-15). A's east test point (8e-15, 0)
+    is 1.5e-15 <= 2e-15 from B's centre, B touches A (|4.5e-15| <= 5e-15,
+    ratio 3), and B is nearer than their parent (centre 3.25e-15), so B is
+    A's east neighbour and 3e-15 > 2.1e-15: split at (0+4)/2 = 2. Second
+    build: [0,2) hw ~1.67e-15, [2,6) hw ~2.92e-15, [6,8) hw 1e-15. The only
+    containing neighbours now are [0,2) (west of [2,6), ratio 1.75 < 2.1) and
+    [2,6) (west of [6,8), smaller leaf): no split, done."""
+    from constrained_cases import BOUNDS, hand_case, rederive_constrained
+
+    t = O.build_constrained(hand_case(), 4, BOUNDS)
+    d = t.export()
+    leaf = d["is_leaf"] == 1
+    assert sorted(zip(d["body_start"][leaf].tolist(), d["body_end"][leaf].tolist())) == [(0, 2), (2, 6), (6, 8)]
+    _, st = rederive_constrained(hand_case(), 4)
+    assert st == {"iterations": 2, "splits": 1, "small": 0}
+
+
+def test_build_constrained_matches_rederivation():
+    """The oracle's build_constrained equals the numpy re-derivation (its own
+    Morton keys, leaf and parent geometry, brute-force neighbours) bit for bit
+    on every splitting case, including one that reaches the 10-check cap and
+    cases where a violating leaf of <= 2 bodies is left unsplit."""
+    from constrained_cases import BOUNDS, SPLIT_CASES, rederive_constrained
+
+    seen_small = False
+    for name, gen, n_max in SPLIT_CASES:
+        want, st = rederive_constrained(gen(), n_max)
+        got = O.build_constrained(gen(), n_max, BOUNDS).export()
+        for k in ("center_x", "center_y", "half_width"):
+            assert np.array_equal(got[k].view(np.uint64), want[k].view(np.uint64)), (name, k)
+        for k in ("body_start", "body_end", "is_leaf"):
+            assert np.array_equal(got[k].astype(np.int64), want[k].astype(np.int64)), (name, k)
+        n = len(gen()["x"])
+        assert int(got["is_leaf"].sum()) > -(-n // n_max), name  # more leaves than plain n_max chunks
+        assert st["splits"] > 0, name
+        seen_small |= st["small"] > 0
+        if name == "chain2048_cap":
+            assert st["iterations"] == 10
+    assert seen_small
+
+
 def test_build_constrained_ratio():  # lib.rs:2716
     def max_ratio(t):
         d = t.export()
