@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libgalax_fmm.so")
 OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["util.hip", "tree.hip", "fmm.hip", "m2l_mfma.hip", "vjp.hip", "api.hip", "snapshot.hip"]
+SOURCES = ["util.hip", "tree.hip", "fmm.hip", "m2l_mfma.hip", "vjp.hip", "api.hip", "direct.hip", "snapshot.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-fvisibility=hidden"]

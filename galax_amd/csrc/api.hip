@@ -894,49 +894,6 @@ int galax_fmm_vjp_device(galax_fmm_t* h, double* d_x, double* d_y, double* d_m, 
   return gx::vjp(h, d_x, d_y, d_m, d_axb, d_ayb, d_xb, d_yb, d_mb);
 }
 
-int galax_direct_sum(const double* x, const double* y, const double* m, size_t n, double softening,
-                     const uint64_t* targets, size_t n_targets, double* ax, double* ay, double* phi) {
-  if (n && (!x || !y || !m)) return set_error(GALAX_ERR_INVALID, "x, y, m are required");
-  size_t nt = targets ? n_targets : n;
-  if (nt && (!ax || !ay)) return set_error(GALAX_ERR_INVALID, "ax, ay are required");
-  if (targets)
-    for (size_t k = 0; k < nt; ++k)
-      if (targets[k] >= n) return set_error(GALAX_ERR_INVALID, "target index %llu >= n", (unsigned long long)targets[k]);
-  hipStream_t s;
-  GX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  double *dx = nullptr, *dy = nullptr, *dm = nullptr, *dax = nullptr, *day = nullptr, *dphi = nullptr;
-  uint64_t* dt = nullptr;
-  int rc = 0;
-  auto fail = [&](hipError_t e) {
-    if (e != hipSuccess && rc == 0) rc = set_error(GALAX_ERR_DEVICE, "direct sum: %s", hipGetErrorString(e));
-  };
-  size_t nb = std::max<size_t>(n, 1) * 8, nbt = std::max<size_t>(nt, 1) * 8;
-  fail(hipMalloc(&dx, nb));
-  fail(hipMalloc(&dy, nb));
-  fail(hipMalloc(&dm, nb));
-  fail(hipMalloc(&dax, nbt));
-  fail(hipMalloc(&day, nbt));
-  fail(hipMalloc(&dphi, nbt));
-  if (targets) fail(hipMalloc(&dt, nbt));
-  if (rc == 0 && n) {
-    fail(hipMemcpyAsync(dx, x, n * 8, hipMemcpyHostToDevice, s));
-    fail(hipMemcpyAsync(dy, y, n * 8, hipMemcpyHostToDevice, s));
-    fail(hipMemcpyAsync(dm, m, n * 8, hipMemcpyHostToDevice, s));
-    if (targets) fail(hipMemcpyAsync(dt, targets, nt * 8, hipMemcpyHostToDevice, s));
-  }
-  if (rc == 0) rc = direct_sum_device(s, dx, dy, dm, n, softening, dt, nt, dax, day, dphi);
-  if (rc == 0 && nt) {
-    fail(hipMemcpyAsync(ax, dax, nt * 8, hipMemcpyDeviceToHost, s));
-    fail(hipMemcpyAsync(ay, day, nt * 8, hipMemcpyDeviceToHost, s));
-    if (phi) fail(hipMemcpyAsync(phi, dphi, nt * 8, hipMemcpyDeviceToHost, s));
-  }
-  fail(hipStreamSynchronize(s));
-  for (void* p : {(void*)dx, (void*)dy, (void*)dm, (void*)dax, (void*)day, (void*)dphi, (void*)dt})
-    if (p) (void)hipFree(p);
-  (void)hipStreamDestroy(s);
-  return rc;
-}
-
 int galax_fmm_validate(galax_fmm_t* h, galax_bodies_t* b, const uint64_t* samples, size_t ns, double* max_rel,
                        double* mean_rel, double* rms_vec) {
   if (!h || !b || (!samples && ns)) return set_error(GALAX_ERR_INVALID, "NULL argument");

@@ -242,8 +242,6 @@ int m2l_mfma_run(galax_fmm* h, const double* mp, double* loc);
 // mbar_s = sum over pairs (t, s) of T_K^T lbar_t, for sources s < n_live_targets.
 int m2l_adjoint_setup(galax_fmm* h);
 int m2l_adjoint_run(galax_fmm* h, const double* lbar, double* mbar);
-int direct_sum_device(hipStream_t s, const double* x, const double* y, const double* m, size_t n, double eps,
-                      const uint64_t* targets, size_t nt, double* ax, double* ay, double* phi);
 int kdk_device(galax_fmm* h, double dt, uint64_t n_steps, uint64_t every, galax_diag_t* diag);
 
 // Host restatement of the kernel-derivative numerator polynomials

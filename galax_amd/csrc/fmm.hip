@@ -1084,53 +1084,6 @@ hipError_t p2p_combine(const P2PItems& it, const uint32_t* goff, const uint32_t*
 
 // ------------------------------------------------------ direct sum ----
 
-__global__ void k_direct(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ m,
-                         size_t n, double eps2, const uint64_t* __restrict__ targets, size_t nt,
-                         double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ phi) {
-  __shared__ double sx[256], sy[256], sm[256];
-  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool valid = k < nt;
-  size_t i = valid ? (targets ? (size_t)targets[k] : k) : 0;
-  double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0;
-  double accx = 0.0, accy = 0.0, accp = 0.0;
-  for (size_t b = 0; b < n; b += 256) {
-    __syncthreads();
-    size_t j = b + threadIdx.x;
-    if (j < n) {
-      sx[threadIdx.x] = x[j];
-      sy[threadIdx.x] = y[j];
-      sm[threadIdx.x] = m[j];
-    }
-    __syncthreads();
-    size_t cnt = n - b < 256 ? n - b : 256;
-    if (valid) {
-      for (size_t jj = 0; jj < cnt; ++jj) {
-        if (b + jj == i) continue;
-        double dx = sx[jj] - xi, dy = sy[jj] - yi;
-        double r2 = dx * dx + dy * dy + eps2;
-        double inv = 1.0 / sqrt(r2);
-        double f = sm[jj] * (inv * inv * inv);
-        accx += f * dx;
-        accy += f * dy;
-        accp -= sm[jj] * inv;
-      }
-    }
-  }
-  if (valid) {
-    ax[k] = accx;
-    ay[k] = accy;
-    if (phi) phi[k] = accp;
-  }
-}
-
-int direct_sum_device(hipStream_t s, const double* x, const double* y, const double* m, size_t n, double eps,
-                      const uint64_t* targets, size_t nt, double* ax, double* ay, double* phi) {
-  if (nt == 0) return 0;
-  k_direct<<<nblk(nt), 256, 0, s>>>(x, y, m, n, eps * eps, targets, nt, ax, ay, phi);
-  GX_HIP(hipGetLastError());
-  return 0;
-}
-
 // ------------------------------------------------------ orchestration ----
 
 #define GX_P_CASES(F) \
