@@ -10,10 +10,14 @@
 // function of (n, n_targets) only, so every entry point gives the same sums
 // in the same order: bit-identical results.
 //
-// galax_direct_sum takes its scratch (device copies of the inputs, partials)
-// from hipMalloc and frees it before it returns.
+// Entry points:
+//   galax_direct_sum          scratch (device copies of the inputs, partials)
+//                             from hipMalloc, freed before return;
+//   galax_direct_sum_noalloc  the same kernels on a file-scope device array of
+//                             DIRECT_SCRATCH_BYTES; allocates nothing.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "common.h"
 
@@ -143,6 +147,39 @@ size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
 }  // namespace
 
+// ------------------------------------------------------ no-alloc scratch ----
+
+// The allocation-free entry point's device memory: staging for the inputs and
+// outputs plus the partial sums. One copy per GPU, held until the process ends.
+constexpr size_t DIRECT_SCRATCH_BYTES = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[DIRECT_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy at its own address). Host code uses
+// this address, never the array's name, for kernel arguments and copies.
+int scratch_address(void** out) {
+  int dev = 0;
+  GX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return set_error(GALAX_ERR_UNSUPPORTED, "device %d beyond %d", dev, MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(GALAX_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                       hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
 }  // namespace gx
 
 extern "C" {
@@ -188,6 +225,57 @@ int galax_direct_sum(const double* x, const double* y, const double* m, size_t n
     if (p) (void)hipFree(p);
   (void)hipStreamDestroy(s);
   return rc;
+}
+
+size_t galax_direct_sum_noalloc_bytes(size_t n, size_t n_targets) {
+  using namespace gx;
+  const DirectPlan plan = direct_plan(n, n_targets);
+  return 3 * align256(n * 8) + 4 * align256(n_targets * 8) + align256(part_bytes(plan, n_targets));
+}
+
+int galax_direct_sum_noalloc(const double* x, const double* y, const double* m, size_t n, double softening,
+                             const uint64_t* targets, size_t n_targets, double* ax, double* ay, double* phi) {
+  using namespace gx;
+  const size_t nt = targets ? n_targets : n;
+  GX_CHECK(check_args(x, y, m, n, targets, nt, ax, ay));
+  const size_t need = galax_direct_sum_noalloc_bytes(n, nt);
+  if (need > DIRECT_SCRATCH_BYTES)
+    return set_error(GALAX_ERR_INVALID,
+                     "direct sum of %zu bodies at %zu targets needs %zu bytes of scratch, above the %zu-byte maximum "
+                     "of galax_direct_sum_noalloc",
+                     n, nt, need, DIRECT_SCRATCH_BYTES);
+  void* base = nullptr;
+  GX_CHECK(scratch_address(&base));
+  unsigned char* p = static_cast<unsigned char*>(base);
+  auto take = [&](size_t bytes) {
+    unsigned char* q = p;
+    p += align256(bytes);
+    return q;
+  };
+  double* dx = reinterpret_cast<double*>(take(n * 8));
+  double* dy = reinterpret_cast<double*>(take(n * 8));
+  double* dm = reinterpret_cast<double*>(take(n * 8));
+  uint64_t* dt = reinterpret_cast<uint64_t*>(take(nt * 8));
+  double* dax = reinterpret_cast<double*>(take(nt * 8));
+  double* day = reinterpret_cast<double*>(take(nt * 8));
+  double* dphi = reinterpret_cast<double*>(take(nt * 8));
+  const DirectPlan plan = direct_plan(n, nt);
+  double* dpart = reinterpret_cast<double*>(take(part_bytes(plan, nt)));
+  hipStream_t s = hipStreamPerThread;  // no stream is created
+  if (n) {
+    GX_HIP(hipMemcpyAsync(dx, x, n * 8, hipMemcpyHostToDevice, s));
+    GX_HIP(hipMemcpyAsync(dy, y, n * 8, hipMemcpyHostToDevice, s));
+    GX_HIP(hipMemcpyAsync(dm, m, n * 8, hipMemcpyHostToDevice, s));
+    if (targets && nt) GX_HIP(hipMemcpyAsync(dt, targets, nt * 8, hipMemcpyHostToDevice, s));
+  }
+  GX_CHECK(direct_launch(s, plan, dx, dy, dm, n, softening, targets ? dt : nullptr, nt, dpart, dax, day, dphi));
+  if (nt) {
+    GX_HIP(hipMemcpyAsync(ax, dax, nt * 8, hipMemcpyDeviceToHost, s));
+    GX_HIP(hipMemcpyAsync(ay, day, nt * 8, hipMemcpyDeviceToHost, s));
+    if (phi) GX_HIP(hipMemcpyAsync(phi, dphi, nt * 8, hipMemcpyDeviceToHost, s));
+  }
+  GX_HIP(hipStreamSynchronize(s));
+  return 0;
 }
 
 }  // extern "C"

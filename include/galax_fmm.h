@@ -273,6 +273,21 @@ GALAX_API int galax_fmm_vjp_device(galax_fmm_t* h, double* d_x, double* d_y, dou
 GALAX_API int galax_direct_sum(const double* x, const double* y, const double* m, size_t n, double softening,
                      const uint64_t* targets, size_t n_targets, double* ax, double* ay, double* phi);
 
+/* No counterpart in the reference; for callers that cannot allocate device
+ * memory (serving loops, harnesses that forbid hipMalloc): galax_direct_sum
+ * without any allocation. Its device memory (staging for the inputs and
+ * outputs, partial sums) is a fixed array of 64 MiB inside the library, one
+ * per GPU, held until the process ends; galax_direct_sum_noalloc_bytes(n,
+ * n_targets) is what a call needs, and a call needing more is rejected with
+ * GALAX_ERR_INVALID (e.g. n = 2^20 with 10,000 targets fits; all n targets
+ * fit up to ~600,000 bodies). Runs on the calling thread's per-thread
+ * default stream of the current device and returns synchronised. Results
+ * are bit-identical to galax_direct_sum's. All calls on one GPU share the
+ * array: calls that overlap in time on one GPU (from several threads) race. */
+GALAX_API int galax_direct_sum_noalloc(const double* x, const double* y, const double* m, size_t n, double softening,
+                                       const uint64_t* targets, size_t n_targets, double* ax, double* ay, double* phi);
+GALAX_API size_t galax_direct_sum_noalloc_bytes(size_t n, size_t n_targets);
+
 /* ~validate_fmm (galax-core/src/lib.rs:1503-1564) with caller-chosen sample
  * indices: evaluates the FMM, computes the direct sum on the device for the
  * samples and returns (max, mean) of | |a_fmm| - |a_exact| | / max(|a_exact|, 1e-20)
