@@ -2,6 +2,7 @@
 // host-side orchestration around the device passes.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -202,6 +203,7 @@ int build_lists_corrected(galax_fmm* h, const double* d_x, const double* d_y) {
   hipStream_t s = h->stream;
   const size_t nn = t.n_nodes, nl = t.n_leaves;
   h->corrected_ready = false;
+  h->c_sym.valid = false;
   if (nn == 0) {
     h->corrected_ready = true;
     return 0;
@@ -467,6 +469,10 @@ int galax_fmm_create(const galax_fmm_config_t* cfg, size_t max_n, galax_fmm_t** 
     return set_error(GALAX_ERR_INIT, "hipStreamCreate failed");
   }
   h->max_n = max_n;
+  {  // measurement switch: the FP32 forms' near field on the gather kernel
+    const char* g = std::getenv("GALAX_NEAR_GATHER");
+    h->near_gather = g && g[0] == '1';
+  }
   int rc = reserve_bodies(h, max_n);
   if (rc) {
     galax_fmm_destroy(h);
@@ -501,6 +507,8 @@ void galax_fmm_destroy(galax_fmm_t* h) {
     DBuf* ib[] = {&it->tgt, &it->g0, &it->g1, &it->slot, &it->part_off, &it->multi};
     for (DBuf* b : ib) dbuf_free(*b);
   }
+  sym_p2p_free(t.sym);
+  sym_p2p_free(h->c_sym);
   dbuf_free(h->p2p_part);
   Workspace& w = h->ws;
   DBuf* wb[] = {&w.a, &w.b, &w.c, &w.d, &w.e, &w.f, &w.g, &w.h, &w.sort_ka, &w.sort_kb, &w.sort_va, &w.sort_vb,

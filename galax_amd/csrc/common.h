@@ -63,6 +63,38 @@ struct P2PItems {
   size_t n_items = 0, n_multi = 0, n_part = 0;
 };
 
+// Symmetric near field (the FP32 forms, leaves of at most SYM_LEAF bodies).
+// Each unordered leaf pair {t, b} of the (symmetric) gather lists is
+// evaluated once. The reference fixes the pair set (b in t's list, t < b,
+// galax-core/src/lib.rs:1415-1437), not which side computes a pair; here the
+// leaf with the longer gather list owns it (ties: the lower id), so the few
+// leaves listed by thousands of others sum their pairs in registers. A work
+// item covers one owner leaf t and at most SYM_ITEM_PARTNERS of its owned
+// partners own_dat[u0 .. u1): forces on t go to the item's own slot, forces
+// on a partner's bodies to the pair's slot u (the position in own_dat). Each
+// leaf's results are then its own slots (item order) and the slots of the
+// pairs it does not own (ascending owner), listed in refs (own slots tagged
+// with SYM_OWN_TAG). They are summed in chunks of SYM_CHUNK refs, in parallel,
+// and leaves with several chunks add their chunk sums in chunk order: a
+// fixed order, so the result does not depend on scheduling.
+constexpr uint32_t SYM_LEAF = 32;
+constexpr uint32_t SYM_ITEM_PARTNERS = 16;
+constexpr uint32_t SYM_CHUNK = 32;
+constexpr uint32_t SYM_OWN_TAG = 0x80000000u;
+struct SymP2P {
+  bool valid = false;
+  size_t n_items = 0, n_pairs = 0, n_leaves = 0, n_chunks = 0, n_heavy = 0, n_cpart = 0;
+  DBuf own_off, own_dat;                   // per owner leaf: owned partners (n_leaves + 1 / n_pairs)
+  DBuf itgt, iu0, iu1, iown;               // per item, in execution order
+  DBuf refs;                               // per leaf (ref_off): own slots then foreign pair slots
+  DBuf ch_leaf, ch_r0, ch_r1, ch_dst;      // per chunk of refs; ch_dst NONE = the leaf's final sum
+  DBuf heavy_leaf, heavy_p0, heavy_n;      // leaves with several chunks: first chunk sum, count
+  DBuf rel;                                // float4 per body: x - com, y - com, m, 0 (leaf COM)
+  DBuf own2, ownp, pair2, pairp, cpart2, cpartp;  // partial sums
+};
+int build_sym_p2p(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, uint32_t n_leaves, SymP2P& sp);
+void sym_p2p_free(SymP2P& sp);
+
 // M2L work in tiles of 16 rows (targets, or sources for the adjoint). The live
 // pairs are grouped by (row, key) runs; a tile's steps are the distinct keys of
 // its groups. step_ids lists, per step, each of its 16 columns' sources
@@ -103,6 +135,7 @@ struct DevTree {
   size_t m2l_total = 0, p2p_total = 0, gat_total = 0;
   uint64_t p2p_interactions = 0;  // directed body-body pairs of the gather lists
   P2PItems items;
+  SymP2P sym;  // symmetric near field over the gather lists (built on first use)
   size_t capacity = 0;  // node capacity of the node arrays
 };
 
@@ -143,6 +176,8 @@ struct galax_fmm {
   size_t c_m2l_total = 0, c_gat_total = 0, c_dtt_visited = 0;
   uint64_t c_p2p_interactions = 0;
   gx::P2PItems c_items;
+  gx::SymP2P c_sym;   // symmetric near field over the corrected lists
+  bool near_gather = false;  // GALAX_NEAR_GATHER=1 at create: the FP32 forms use the gather kernel
   gx::DBuf p2p_part;  // partial near fields of targets split over several items
   bool corrected_ready = false;
   galax_timings_t timings{};

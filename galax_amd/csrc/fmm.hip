@@ -90,7 +90,8 @@ __global__ void __launch_bounds__(256) k_p2m(double* __restrict__ x, double* __r
                                              const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be,
                                              const double* __restrict__ comx, const double* __restrict__ comy,
                                              uint32_t n_leaves, double* __restrict__ mp,
-                                             uint32_t* __restrict__ changed, uint32_t* __restrict__ next) {
+                                             uint32_t* __restrict__ changed, uint32_t* __restrict__ next,
+                                             float4* __restrict__ rel) {
 #pragma clang fp contract(off)
   constexpr int S = num_moments(P);
   constexpr int LPW = S <= 32 ? 2 : 1;  // leaves per wave
@@ -140,6 +141,8 @@ __global__ void __launch_bounds__(256) k_p2m(double* __restrict__ x, double* __r
           atomicAdd(changed, 1u);
         }
         const double dx = xb - cx, dy = yb - cy;
+        // the symmetric FP32 near field's sources: leaf-relative coordinates
+        if (rel && k0 == 0) rel[b] = make_float4((float)dx, (float)dy, (float)mb, 0.f);
         double px = 1.0, py = 1.0;
         spx[w][base + sl][0] = 1.0;
         spy[w][base + sl][0] = 1.0;
@@ -1082,6 +1085,278 @@ hipError_t p2p_combine(const P2PItems& it, const uint32_t* goff, const uint32_t*
   return hipGetLastError();
 }
 
+// ------------------------------------------------ symmetric near field ----
+
+// v from lane (q + K) mod 16 or (q - K) mod 16 of the caller's 16-lane row
+// (a DPP row rotation; the compiler folds it into the instruction that reads
+// v). Which of the two does not matter: rotation 16 - K undoes rotation K.
+template <int K>
+__device__ __forceinline__ float row_rot(float v) {
+  if constexpr ((K & 15) == 0)
+    return v;
+  else
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x120 + (K & 15), 0xF, 0xF, true));
+}
+
+// One wave per work item: leaf t (target side, "i") with its owned partners
+// own_dat[u0 .. u1) ("j"), in FP32 on coordinates relative to t's COM (each
+// partner's leaf-relative coordinates plus the FP64 COM difference, rounded
+// once). A 32 x 32 leaf pair is four 16 x 16 blocks, one per 16-lane row r:
+// row r holds i-bodies 16 (r & 1) + q and j-bodies 16 (r >> 1) + q (q = lane
+// within the row). In step K the lane pairs its j with the i of the row lane
+// rotated by K, so the i operands arrive through DPP and no LDS is used.
+// Per unordered pair: one rsq, the force on j (summed in the lane) and on i
+// (summed per rotation K in acc_i[K], over every partner of the item; the 16
+// sums are rotated back once at the end). Forces on a partner's bodies go to
+// the pair's slot; forces on t's bodies, plus on the first item its self
+// block (j-side sums only, every ordered pair of t once, the diagonal masked)
+// and its far field in FP64 (as in k_l2p_p2p), go to the item's own slot.
+// Padding bodies sit at 3e19 with zero mass: r^2 overflows to inf, 1/r = 0.
+template <int P, bool PHI>
+__global__ void __launch_bounds__(64) k_p2p_sym(
+    const float4* __restrict__ rel, const double* __restrict__ x, const double* __restrict__ y,
+    const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be, const double* __restrict__ comx,
+    const double* __restrict__ comy, const double* __restrict__ loc, const uint32_t* __restrict__ par, int fuse_l2l,
+    const uint32_t* __restrict__ own_off, const uint32_t* __restrict__ own_dat, const uint32_t* __restrict__ itgt,
+    const uint32_t* __restrict__ iu0, const uint32_t* __restrict__ iu1, const uint32_t* __restrict__ iown,
+    float eps2, double2* __restrict__ own2, double* __restrict__ ownp, float2* __restrict__ pair2,
+    float* __restrict__ pairp) {
+  constexpr int S = num_moments(P);
+  constexpr float FAR = 3e19f;
+  __shared__ double sloc[S], spar[S];
+  const uint32_t item = xcd_runs(blockIdx.x, gridDim.x);
+  const uint32_t t = itgt[item], u0 = iu0[item], u1 = iu1[item], own = iown[item];
+  const bool first = u0 == own_off[t];
+  const int lane = threadIdx.x, r = lane >> 4, q = lane & 15;
+  const uint32_t ts = bs[t], na = be[t] - ts;
+  const double ox = comx[t], oy = comy[t];
+  const uint32_t ia = 16u * (uint32_t)(r & 1) + (uint32_t)q;   // i-layout
+  const uint32_t jb = 16u * (uint32_t)(r >> 1) + (uint32_t)q;  // j-layout
+  float xi = FAR, yi = FAR, mi = 0.f;
+  if (ia < na) {
+    const float4 v = rel[ts + ia];
+    xi = v.x;
+    yi = v.y;
+    mi = v.z;
+  }
+  float aix[16], aiy[16], aip[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) aix[k] = aiy[k] = aip[k] = 0.f;
+  // self block: every ordered pair (i, j) of t, sums on the j side
+  float sx = 0.f, sy = 0.f, sp = 0.f;
+  if (first) {
+    float xj = FAR, yj = FAR;
+    if (jb < na) {
+      const float4 v = rel[ts + jb];
+      xj = v.x;
+      yj = v.y;
+    }
+    const bool diag = (r & 1) == (r >> 1);  // rows 0 and 3: i == j at K = 0
+    static_for<0, 16>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      const float dx = row_rot<K>(xi) - xj, dy = row_rot<K>(yi) - yj;
+      const float r2 = fmaf(dx, dx, fmaf(dy, dy, eps2));
+      float rs = Near<float>::rsq(r2);
+      if constexpr (K == 0) rs = diag ? 0.f : rs;
+      const float fmi = ((rs * rs) * rs) * row_rot<K>(mi);
+      sx = fmaf(fmi, dx, sx);
+      sy = fmaf(fmi, dy, sy);
+      if (PHI) sp = fmaf(-row_rot<K>(mi), rs, sp);
+    });
+  }
+  // partner data one step ahead: its loads are in flight during the
+  // previous partner's arithmetic
+  auto fetch = [&](uint32_t u, uint32_t& nb, float4& v, double& cx, double& cy) {
+    const uint32_t b = own_dat[u];
+    const uint32_t b0 = bs[b];
+    nb = be[b] - b0;
+    cx = comx[b];
+    cy = comy[b];
+    v = make_float4(FAR, FAR, 0.f, 0.f);
+    if (jb < nb) v = rel[b0 + jb];
+  };
+  uint32_t nb_n = 0;
+  float4 v_n = make_float4(FAR, FAR, 0.f, 0.f);
+  double cx_n = 0.0, cy_n = 0.0;
+  if (u0 < u1) fetch(u0, nb_n, v_n, cx_n, cy_n);
+  for (uint32_t u = u0; u < u1; ++u) {
+    const uint32_t nb = nb_n;
+    const float4 v = v_n;
+    const double cxb = cx_n, cyb = cy_n;
+    if (u + 1 < u1) fetch(u + 1, nb_n, v_n, cx_n, cy_n);
+    float xj = FAR, yj = FAR, mj = 0.f;
+    if (jb < nb) {
+      xj = v.x + (float)(cxb - ox);
+      yj = v.y + (float)(cyb - oy);
+      mj = v.z;
+    }
+    float ajx = 0.f, ajy = 0.f, ajp = 0.f;
+    static_for<0, 16>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      const float dx = row_rot<K>(xi) - xj, dy = row_rot<K>(yi) - yj;  // x_i - x_j
+      const float r2 = fmaf(dx, dx, fmaf(dy, dy, eps2));
+      const float rs = Near<float>::rsq(r2);
+      const float r3 = (rs * rs) * rs;
+      const float fmi = r3 * row_rot<K>(mi), fmj = r3 * mj;
+      ajx = fmaf(fmi, dx, ajx);  // on j, towards i
+      ajy = fmaf(fmi, dy, ajy);
+      aix[K] = fmaf(-fmj, dx, aix[K]);  // on i, towards j
+      aiy[K] = fmaf(-fmj, dy, aiy[K]);
+      if (PHI) {  // (padding: r^2 = inf and 1/r = 0, so no term with r^2 here)
+        ajp = fmaf(-row_rot<K>(mi), rs, ajp);
+        aip[K] = fmaf(-mj, rs, aip[K]);
+      }
+    });
+    // rows (0, 1) and (2, 3) hold the same j-bodies
+    ajx += __shfl_xor(ajx, 16, 64);
+    ajy += __shfl_xor(ajy, 16, 64);
+    if (PHI) ajp += __shfl_xor(ajp, 16, 64);
+    if ((r & 1) == 0 && jb < nb) {
+      const size_t o = (size_t)u * SYM_LEAF + jb;
+      pair2[o] = make_float2(ajx, ajy);
+      if (PHI) pairp[o] = ajp;
+    }
+  }
+  // the i side: acc_i[K] of lane q belongs to the i of its rotated lane
+  float tix = 0.f, tiy = 0.f, tip = 0.f;
+  static_for<0, 16>([&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    tix += row_rot<16 - K>(aix[K]);
+    tiy += row_rot<16 - K>(aiy[K]);
+    if (PHI) tip += row_rot<16 - K>(aip[K]);
+  });
+  tix += __shfl_xor(tix, 32, 64);  // rows r and r ^ 2 hold the same i-bodies
+  tiy += __shfl_xor(tiy, 32, 64);
+  if (PHI) tip += __shfl_xor(tip, 32, 64);
+  // lanes 0-31 now hold body e = lane on the i side
+  double fx = 0.0, fy = 0.0, fp = 0.0;
+  if (first) {  // uniform over the block
+    sx += __shfl_xor(sx, 16, 64);
+    sy += __shfl_xor(sy, 16, 64);
+    if (PHI) sp += __shfl_xor(sp, 16, 64);
+    const int src = lane < 16 ? lane : lane + 16;  // body 16 + q sits in row 2
+    sx = __shfl(sx, src, 64);
+    sy = __shfl(sy, src, 64);
+    if (PHI) sp = __shfl(sp, src, 64);
+    const uint32_t pa = fuse_l2l ? par[t] : NONE;
+    for (int k = lane; k < S; k += 64) {
+      sloc[k] = loc[(size_t)t * S + k];
+      if (pa != NONE) spar[k] = loc[(size_t)pa * S + k];
+    }
+    __syncthreads();
+    const int half = lane >> 5;
+    const uint32_t e = (uint32_t)(lane & 31);
+    if ((half == 0 || pa != NONE) && e < na) {
+      const double fcx = half ? comx[pa] : ox, fcy = half ? comy[pa] : oy;
+      l2p_body<P, true>(half ? spar : sloc, x[ts + e] - fcx, y[ts + e] - fcy, &fx, &fy, &fp);
+    }
+    if (pa != NONE) {
+      fx += __shfl_xor(fx, 32, 64);
+      fy += __shfl_xor(fy, 32, 64);
+      if (PHI) fp += __shfl_xor(fp, 32, 64);
+    }
+  }
+  if (lane < 32 && (uint32_t)lane < na) {
+    const size_t o = (size_t)own * SYM_LEAF + (size_t)lane;
+    own2[o] = make_double2((0.0 - fx) + (double)(tix + sx), (0.0 - fy) + (double)(tiy + sy));
+    if (PHI) ownp[o] = fp + (double)(tip + sp);
+  }
+}
+
+// A chunk of a leaf's refs (32 lanes, one per body; 8 chunks per block),
+// summed in ref order in FP64: into the leaf's output when it is the leaf's
+// only chunk, else into a chunk sum.
+template <bool PHI>
+__global__ void __launch_bounds__(256) k_sym_chunk_sum(uint32_t nch, const uint32_t* __restrict__ ch_leaf,
+                                                       const uint32_t* __restrict__ ch_r0,
+                                                       const uint32_t* __restrict__ ch_r1,
+                                                       const uint32_t* __restrict__ ch_dst,
+                                                       const uint32_t* __restrict__ refs,
+                                                       const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be,
+                                                       const double2* __restrict__ own2,
+                                                       const double* __restrict__ ownp,
+                                                       const float2* __restrict__ pair2,
+                                                       const float* __restrict__ pairp, double2* __restrict__ cpart2,
+                                                       double* __restrict__ cpartp, double* __restrict__ ax,
+                                                       double* __restrict__ ay, double* __restrict__ phi) {
+  __shared__ uint32_t sref[8][SYM_CHUNK];
+  const uint32_t cl = threadIdx.x >> 5;
+  const uint32_t c = blockIdx.x * 8 + cl;
+  const uint32_t e = threadIdx.x & 31;
+  const bool live = c < nch;
+  uint32_t t = 0, b0 = 0, r0 = 0, nr = 0;
+  bool body = false;
+  if (live) {
+    t = ch_leaf[c];
+    b0 = bs[t];
+    body = e < be[t] - b0;
+    r0 = ch_r0[c];
+    nr = ch_r1[c] - r0;
+  }
+  // the chunk's refs (at most SYM_CHUNK), loaded one per lane; its own-slot
+  // refs come first (a leaf lists its items before its other pairs)
+  const uint32_t myref = (live && e < nr) ? refs[r0 + e] : 0u;
+  sref[cl][e] = myref;
+  const unsigned long long own_lanes = __ballot((myref & SYM_OWN_TAG) != 0u);
+  const uint32_t nown = (uint32_t)__popcll((own_lanes >> (threadIdx.x & 32)) & 0xFFFFFFFFull);
+  __syncthreads();
+  if (!live || !body) return;
+  double sx = 0.0, sy = 0.0, sp = 0.0;
+#pragma unroll 2
+  for (uint32_t k = 0; k < nown; ++k) {
+    const size_t o = (size_t)(sref[cl][k] & ~SYM_OWN_TAG) * SYM_LEAF + e;
+    const double2 v = own2[o];
+    sx += v.x;
+    sy += v.y;
+    if (PHI) sp += ownp[o];
+  }
+#pragma unroll 8
+  for (uint32_t k = nown; k < nr; ++k) {
+    const size_t o = (size_t)sref[cl][k] * SYM_LEAF + e;
+    const float2 v = pair2[o];
+    sx += (double)v.x;
+    sy += (double)v.y;
+    if (PHI) sp += (double)pairp[o];
+  }
+  const uint32_t dst = ch_dst[c];
+  if (dst == NONE) {
+    ax[b0 + e] = sx;
+    ay[b0 + e] = sy;
+    if (PHI) phi[b0 + e] = sp;
+  } else {
+    cpart2[(size_t)dst * SYM_LEAF + e] = make_double2(sx, sy);
+    if (PHI) cpartp[(size_t)dst * SYM_LEAF + e] = sp;
+  }
+}
+
+// Leaves with several chunks: the chunk sums added in chunk order.
+template <bool PHI>
+__global__ void __launch_bounds__(256) k_sym_heavy_sum(uint32_t nheavy, const uint32_t* __restrict__ heavy_leaf,
+                                                       const uint32_t* __restrict__ heavy_p0,
+                                                       const uint32_t* __restrict__ heavy_n,
+                                                       const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be,
+                                                       const double2* __restrict__ cpart2,
+                                                       const double* __restrict__ cpartp, double* __restrict__ ax,
+                                                       double* __restrict__ ay, double* __restrict__ phi) {
+  const uint32_t k = blockIdx.x * 8 + (threadIdx.x >> 5);
+  const uint32_t e = threadIdx.x & 31;
+  if (k >= nheavy) return;
+  const uint32_t t = heavy_leaf[k];
+  const uint32_t b0 = bs[t];
+  if (e >= be[t] - b0) return;
+  double sx = 0.0, sy = 0.0, sp = 0.0;
+  const uint32_t p0 = heavy_p0[k], n = heavy_n[k];
+  for (uint32_t c = 0; c < n; ++c) {
+    const double2 v = cpart2[(size_t)(p0 + c) * SYM_LEAF + e];
+    sx += v.x;
+    sy += v.y;
+    if (PHI) sp += cpartp[(size_t)(p0 + c) * SYM_LEAF + e];
+  }
+  ax[b0 + e] = sx;
+  ay[b0 + e] = sy;
+  if (PHI) phi[b0 + e] = sp;
+}
+
 // ------------------------------------------------------ direct sum ----
 
 // ------------------------------------------------------ orchestration ----
@@ -1241,13 +1516,31 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   const uint32_t top = corrected ? t.n_levels : std::min<uint32_t>(t.n_levels, 2);
   const uint32_t live_hi = t.level_offset[top];
   hipEvent_t* ev = h->ev;
+  const uint32_t* goff = corrected ? h->c_gat_off.as<uint32_t>() : t.gat_off.as<uint32_t>();
+  const uint32_t* gdat = corrected ? h->c_gat_dat.as<uint32_t>() : t.gat_dat.as<uint32_t>();
+  const bool fp32_near =
+      h->cfg.precision == GALAX_PREC_MIXED_FP32 || h->cfg.precision == GALAX_PREC_MIXED_BF16X3;
+  // the FP32 forms with leaves of at most SYM_LEAF bodies evaluate each
+  // unordered leaf pair once (k_p2p_sym); P2M then also writes the bodies'
+  // leaf-relative FP32 coordinates
+  SymP2P* sym = nullptr;
+  if (near && fp32_near && h->cfg.n_max <= SYM_LEAF && !h->near_gather && nl > 0) {
+    SymP2P& sp = corrected ? h->c_sym : t.sym;
+    if (!sp.valid) GX_CHECK(build_sym_p2p(h, goff, gdat, (uint32_t)nl, sp));
+    if (d_phi && !sp.pairp.p) {
+      GX_CHECK(dbuf_reserve(sp.ownp, std::max<size_t>(sp.n_items, 1) * SYM_LEAF * 8, s));
+      GX_CHECK(dbuf_reserve(sp.pairp, std::max<size_t>(sp.n_pairs, 1) * SYM_LEAF * 4, s));
+      GX_CHECK(dbuf_reserve(sp.cpartp, std::max<size_t>(sp.n_cpart, 1) * SYM_LEAF * 8, s));
+    }
+    sym = &sp;
+  }
   if (h->timing) GX_HIP(hipEventRecord(ev[1], s));
   {
     uint32_t* changed = h->flag.as<uint32_t>() + 4 + h->san_cur;
     uint32_t* next = h->flag.as<uint32_t>() + 4 + (h->san_cur ^ 1u);
     k_p2m<P><<<nblk(nl, num_moments(P) <= 32 ? 8 : 4), 256, 0, s>>>(
         d_x, d_y, d_m, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), comx, comy, (uint32_t)nl, mp, changed,
-        next);
+        next, sym ? sym->rel.as<float4>() : nullptr);
     GX_HIP(hipGetLastError());
   }
   if (h->timing) GX_HIP(hipEventRecord(ev[2], s));
@@ -1314,13 +1607,9 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
     GX_HIP(hipGetLastError());
   }
   if (h->timing) GX_HIP(hipEventRecord(ev[6], s));
-  const uint32_t* goff = corrected ? h->c_gat_off.as<uint32_t>() : t.gat_off.as<uint32_t>();
-  const uint32_t* gdat = corrected ? h->c_gat_dat.as<uint32_t>() : t.gat_dat.as<uint32_t>();
   const P2PItems& it = corrected ? h->c_items : t.items;
   const uint32_t nmax = h->cfg.n_max;
-  GX_CHECK(dbuf_reserve(h->p2p_part, std::max<size_t>(it.n_part, 1) * nmax * 3 * 8, s));
-  const bool fp32_near =
-      h->cfg.precision == GALAX_PREC_MIXED_FP32 || h->cfg.precision == GALAX_PREC_MIXED_BF16X3;
+  if (!sym) GX_CHECK(dbuf_reserve(h->p2p_part, std::max<size_t>(it.n_part, 1) * nmax * 3 * 8, s));
 #define GX_LAUNCH_P2P(T, PH)                                                                                 \
   k_l2p_p2p<P, T, PH><<<(unsigned)it.n_items, 64, 0, s>>>(                                                  \
       d_x, d_y, d_m, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), comx, comy, loc, goff, gdat,     \
@@ -1329,6 +1618,37 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   if (h->timing) GX_HIP(hipEventRecord(ev[10], s));
   if (!near) {  // expansions only (the VJP's forward): no L2P / P2P
     if (h->timing) GX_HIP(hipEventRecord(ev[11], s));
+    if (h->timing) GX_HIP(hipEventRecord(ev[7], s));
+    return 0;
+  }
+  if (sym) {
+#define GX_LAUNCH_SYM(PH)                                                                                       \
+  k_p2p_sym<P, PH><<<(unsigned)sym->n_items, 64, 0, s>>>(                                                     \
+      sym->rel.as<float4>(), d_x, d_y, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), comx, comy, loc, \
+      t.parent.as<uint32_t>(), fuse_l2l ? 1 : 0, sym->own_off.as<uint32_t>(), sym->own_dat.as<uint32_t>(),      \
+      sym->itgt.as<uint32_t>(), sym->iu0.as<uint32_t>(), sym->iu1.as<uint32_t>(), sym->iown.as<uint32_t>(),      \
+      (float)(eps * eps), sym->own2.as<double2>(), sym->ownp.as<double>(), sym->pair2.as<float2>(),            \
+      sym->pairp.as<float>());                                                                                  \
+  GX_HIP(hipGetLastError());                                                                                    \
+  if (h->timing) GX_HIP(hipEventRecord(ev[11], s));                                                             \
+  k_sym_chunk_sum<PH><<<(unsigned)((sym->n_chunks + 7) / 8), 256, 0, s>>>(                                     \
+      (uint32_t)sym->n_chunks, sym->ch_leaf.as<uint32_t>(), sym->ch_r0.as<uint32_t>(), sym->ch_r1.as<uint32_t>(), \
+      sym->ch_dst.as<uint32_t>(), sym->refs.as<uint32_t>(), t.body_start.as<uint32_t>(),                        \
+      t.body_end.as<uint32_t>(), sym->own2.as<double2>(), sym->ownp.as<double>(), sym->pair2.as<float2>(),     \
+      sym->pairp.as<float>(), sym->cpart2.as<double2>(), sym->cpartp.as<double>(), d_ax, d_ay, d_phi);         \
+  GX_HIP(hipGetLastError());                                                                                    \
+  if (sym->n_heavy)                                                                                             \
+    k_sym_heavy_sum<PH><<<(unsigned)((sym->n_heavy + 7) / 8), 256, 0, s>>>(                                    \
+        (uint32_t)sym->n_heavy, sym->heavy_leaf.as<uint32_t>(), sym->heavy_p0.as<uint32_t>(),                   \
+        sym->heavy_n.as<uint32_t>(), t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(),                    \
+        sym->cpart2.as<double2>(), sym->cpartp.as<double>(), d_ax, d_ay, d_phi)
+    if (d_phi) {
+      GX_LAUNCH_SYM(true);
+    } else {
+      GX_LAUNCH_SYM(false);
+    }
+#undef GX_LAUNCH_SYM
+    GX_HIP(hipGetLastError());
     if (h->timing) GX_HIP(hipEventRecord(ev[7], s));
     return 0;
   }

@@ -961,8 +961,247 @@ int build_p2p_items(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, ui
   return 0;
 }
 
+// ------------------------------------------------ symmetric near field ----
+
+// Does leaf t own the pair {t, b}? The one with the longer gather list; on a
+// tie the lower id.
+__device__ __forceinline__ bool sym_owns(const uint32_t* __restrict__ goff, uint32_t t, uint32_t b) {
+  const uint32_t gt = goff[t + 1] - goff[t], gb = goff[b + 1] - goff[b];
+  return gt > gb || (gt == gb && t < b);
+}
+
+// Per leaf: owned partners and the items they need.
+__global__ void k_sym_count(const uint32_t* __restrict__ goff, const uint32_t* __restrict__ gdat, uint32_t nl,
+                            uint32_t* __restrict__ nown, uint32_t* __restrict__ nit) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > nl) return;
+  uint32_t c = 0;
+  if (t < nl)
+    for (uint32_t g = goff[t]; g < goff[t + 1]; ++g) {
+      const uint32_t b = gdat[g];
+      c += (b != t && sym_owns(goff, t, b)) ? 1u : 0u;
+    }
+  nown[t] = c;
+  nit[t] = t < nl ? max(1u, (c + SYM_ITEM_PARTNERS - 1) / SYM_ITEM_PARTNERS) : 0u;
+}
+
+// Owned partners in gather-list order; each pair's key (partner, owner)
+// sorts the non-owned pairs of every leaf by owner.
+__global__ void k_sym_own_fill(const uint32_t* __restrict__ goff, const uint32_t* __restrict__ gdat, uint32_t nl,
+                               const uint32_t* __restrict__ own_off, uint32_t* __restrict__ own_dat, int w,
+                               uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nl) return;
+  uint32_t u = own_off[t];
+  for (uint32_t g = goff[t]; g < goff[t + 1]; ++g) {
+    const uint32_t b = gdat[g];
+    if (b != t && sym_owns(goff, t, b)) {
+      own_dat[u] = b;
+      key[u] = ((uint64_t)b << w) | t;
+      val[u] = u;
+      ++u;
+    }
+  }
+}
+
+__global__ void k_sym_low_split(const uint64_t* __restrict__ key, uint32_t np, int w, uint32_t* __restrict__ cnt) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  atomicAdd(&cnt[(uint32_t)(key[i] >> w)], 1u);
+}
+
+__global__ void k_sym_items(const uint32_t* __restrict__ own_off, const uint32_t* __restrict__ item_off, uint32_t nl,
+                            uint32_t* __restrict__ itgt, uint32_t* __restrict__ iu0, uint32_t* __restrict__ iu1,
+                            uint32_t* __restrict__ iown, uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nl) return;
+  const uint32_t e = own_off[t + 1];
+  for (uint32_t i = item_off[t], u = own_off[t]; i < item_off[t + 1]; ++i, u += SYM_ITEM_PARTNERS) {
+    itgt[i] = t;
+    iu0[i] = u;
+    iu1[i] = min(u + SYM_ITEM_PARTNERS, e);
+    iown[i] = i;
+    // costliest first (partners, plus the self block of the first item); the
+    // stable sort keeps the spatial order within a class
+    const uint32_t cost = (iu1[i] - u) + (u == own_off[t] ? 1u : 0u);
+    key[i] = 64u - (cost ? 32u - (uint32_t)__clz(cost) : 0u);
+    val[i] = i;
+  }
+}
+
+// Per leaf: the number of refs (its items, then its non-owned pairs) and of
+// chunks; heavy leaves (several chunks) need chunk sums.
+__global__ void k_sym_ref_count(const uint32_t* __restrict__ item_off, const uint32_t* __restrict__ low_off,
+                                uint32_t nl, uint32_t* __restrict__ nref, uint32_t* __restrict__ nch,
+                                uint32_t* __restrict__ npart, uint32_t* __restrict__ heavy) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > nl) return;
+  if (t == nl) {
+    nref[t] = nch[t] = npart[t] = heavy[t] = 0;
+    return;
+  }
+  const uint32_t r = (item_off[t + 1] - item_off[t]) + (low_off[t + 1] - low_off[t]);
+  const uint32_t c = (r + SYM_CHUNK - 1) / SYM_CHUNK;
+  nref[t] = r;
+  nch[t] = c;
+  npart[t] = c > 1 ? c : 0u;
+  heavy[t] = c > 1 ? 1u : 0u;
+}
+
+__global__ void k_sym_ref_fill(const uint32_t* __restrict__ item_off, const uint32_t* __restrict__ low_off,
+                               const uint32_t* __restrict__ low_slot, uint32_t nl, const uint32_t* __restrict__ ref_off,
+                               const uint32_t* __restrict__ ch_off, const uint32_t* __restrict__ part_off,
+                               const uint32_t* __restrict__ heavy_off, uint32_t* __restrict__ refs,
+                               uint32_t* __restrict__ ch_leaf, uint32_t* __restrict__ ch_r0,
+                               uint32_t* __restrict__ ch_r1, uint32_t* __restrict__ ch_dst,
+                               uint32_t* __restrict__ heavy_leaf, uint32_t* __restrict__ heavy_p0,
+                               uint32_t* __restrict__ heavy_n) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nl) return;
+  uint32_t r = ref_off[t];
+  for (uint32_t i = item_off[t]; i < item_off[t + 1]; ++i) refs[r++] = SYM_OWN_TAG | i;
+  for (uint32_t l = low_off[t]; l < low_off[t + 1]; ++l) refs[r++] = low_slot[l];
+  const uint32_t r0 = ref_off[t], r1 = ref_off[t + 1];
+  const uint32_t nc = ch_off[t + 1] - ch_off[t];
+  for (uint32_t c = 0; c < nc; ++c) {
+    const uint32_t ci = ch_off[t] + c;
+    ch_leaf[ci] = t;
+    ch_r0[ci] = r0 + c * SYM_CHUNK;
+    ch_r1[ci] = min(r0 + (c + 1) * SYM_CHUNK, r1);
+    ch_dst[ci] = nc > 1 ? part_off[t] + c : NONE;
+  }
+  if (nc > 1) {
+    const uint32_t hi = heavy_off[t];
+    heavy_leaf[hi] = t;
+    heavy_p0[hi] = part_off[t];
+    heavy_n[hi] = nc;
+  }
+}
+
+void sym_p2p_free(SymP2P& sp) {
+  DBuf* b[] = {&sp.own_off, &sp.own_dat, &sp.itgt,       &sp.iu0,      &sp.iu1,     &sp.iown,  &sp.refs,
+               &sp.ch_leaf, &sp.ch_r0,   &sp.ch_r1,      &sp.ch_dst,   &sp.heavy_leaf, &sp.heavy_p0, &sp.heavy_n,
+               &sp.rel,     &sp.own2,    &sp.ownp,       &sp.pair2,    &sp.pairp,   &sp.cpart2, &sp.cpartp};
+  for (DBuf* d : b) dbuf_free(*d);
+  sp.valid = false;
+  sp.n_items = sp.n_pairs = sp.n_leaves = sp.n_chunks = sp.n_heavy = sp.n_cpart = 0;
+}
+
+int build_sym_p2p(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, uint32_t nl, SymP2P& sp) {
+  hipStream_t s = h->stream;
+  Workspace& ws = h->ws;
+  sp.valid = false;
+  const size_t L1 = (size_t)nl + 1;
+  // per-leaf counters in workspace a: nown/own_off, nit/item_off, low_off,
+  // nref/ref_off, nch/ch_off, npart/part_off, heavy/heavy_off
+  GX_CHECK(dbuf_reserve(ws.a, L1 * 7 * 4, s));
+  uint32_t* own_off = ws.a.as<uint32_t>();
+  uint32_t* item_off = own_off + L1;
+  uint32_t* low_off = item_off + L1;
+  uint32_t* ref_off = low_off + L1;
+  uint32_t* ch_off = ref_off + L1;
+  uint32_t* part_off = ch_off + L1;
+  uint32_t* heavy_off = part_off + L1;
+  k_sym_count<<<nblk(L1), 256, 0, s>>>(goff, gdat, nl, own_off, item_off);
+  GX_HIP(hipGetLastError());
+  uint64_t np64 = 0, ni64 = 0;
+  GX_CHECK(sum_u32(h, own_off, L1, &np64));
+  GX_CHECK(sum_u32(h, item_off, L1, &ni64));
+  if ((np64 + 1) * SYM_LEAF >= 0xFFFFFFF0ull || (ni64 + 1) * SYM_LEAF >= 0xFFFFFFF0ull)
+    return set_error(GALAX_ERR_UNSUPPORTED, "symmetric near field: too many leaf pairs");
+  uint32_t np = 0, ni = 0;
+  GX_CHECK(scan_u32(h, own_off, own_off, L1, false, false, &np));
+  GX_CHECK(scan_u32(h, item_off, item_off, L1, false, false, &ni));
+  GX_CHECK(dbuf_reserve(sp.own_off, L1 * 4, s));
+  GX_HIP(hipMemcpyAsync(sp.own_off.p, own_off, L1 * 4, hipMemcpyDeviceToDevice, s));
+  const size_t nsort = std::max<size_t>(std::max<size_t>(ni, np), 1);
+  GX_CHECK(dbuf_reserve(ws.sort_ka, nsort * 8, s));
+  GX_CHECK(dbuf_reserve(ws.sort_kb, nsort * 8, s));
+  GX_CHECK(dbuf_reserve(ws.sort_va, nsort * 4, s));
+  GX_CHECK(dbuf_reserve(ws.sort_vb, nsort * 4, s));
+  // owned partners; the non-owned pairs of each leaf in ascending owner
+  GX_CHECK(dbuf_reserve(sp.own_dat, (size_t)np * 4 + 4, s));
+  GX_HIP(hipMemsetAsync(low_off, 0, L1 * 4, s));
+  const int w = bits_for(nl);
+  k_sym_own_fill<<<nblk(nl), 256, 0, s>>>(goff, gdat, nl, own_off, sp.own_dat.as<uint32_t>(), w,
+                                          ws.sort_ka.as<uint64_t>(), ws.sort_va.as<uint32_t>());
+  GX_HIP(hipGetLastError());
+  DBuf low_slot;
+  GX_CHECK(dbuf_reserve(low_slot, (size_t)np * 4 + 4, s));
+  if (np) {
+    GX_CHECK(radix_sort_bits(h, ws.sort_ka.as<uint64_t>(), ws.sort_va.as<uint32_t>(), ws.sort_kb.as<uint64_t>(),
+                             ws.sort_vb.as<uint32_t>(), np, 2 * w));
+    k_sym_low_split<<<nblk(np), 256, 0, s>>>(ws.sort_ka.as<uint64_t>(), np, w, low_off);
+    GX_HIP(hipGetLastError());
+    GX_HIP(hipMemcpyAsync(low_slot.p, ws.sort_va.p, (size_t)np * 4, hipMemcpyDeviceToDevice, s));
+  }
+  GX_CHECK(scan_u32(h, low_off, low_off, L1, false, false, nullptr));
+  // items, ordered by decreasing work class
+  GX_CHECK(dbuf_reserve(sp.itgt, (size_t)ni * 4 + 4, s));
+  GX_CHECK(dbuf_reserve(sp.iu0, (size_t)ni * 4 + 4, s));
+  GX_CHECK(dbuf_reserve(sp.iu1, (size_t)ni * 4 + 4, s));
+  GX_CHECK(dbuf_reserve(sp.iown, (size_t)ni * 4 + 4, s));
+  k_sym_items<<<nblk(nl), 256, 0, s>>>(own_off, item_off, nl, sp.itgt.as<uint32_t>(), sp.iu0.as<uint32_t>(),
+                                       sp.iu1.as<uint32_t>(), sp.iown.as<uint32_t>(), ws.sort_ka.as<uint64_t>(),
+                                       ws.sort_va.as<uint32_t>());
+  GX_HIP(hipGetLastError());
+  if (ni > 1) {
+    GX_CHECK(radix_sort_bits(h, ws.sort_ka.as<uint64_t>(), ws.sort_va.as<uint32_t>(), ws.sort_kb.as<uint64_t>(),
+                             ws.sort_vb.as<uint32_t>(), ni, 7));
+    DBuf tmp;
+    GX_CHECK(dbuf_reserve(tmp, (size_t)ni * 16, s));
+    uint32_t* b = tmp.as<uint32_t>();
+    k_items_permute<<<nblk(ni), 256, 0, s>>>(ws.sort_va.as<uint32_t>(), ni, sp.itgt.as<uint32_t>(),
+                                             sp.iu0.as<uint32_t>(), sp.iu1.as<uint32_t>(), sp.iown.as<uint32_t>(), b);
+    GX_HIP(hipGetLastError());
+    GX_HIP(hipMemcpyAsync(sp.itgt.p, b, (size_t)ni * 4, hipMemcpyDeviceToDevice, s));
+    GX_HIP(hipMemcpyAsync(sp.iu0.p, b + ni, (size_t)ni * 4, hipMemcpyDeviceToDevice, s));
+    GX_HIP(hipMemcpyAsync(sp.iu1.p, b + 2 * (size_t)ni, (size_t)ni * 4, hipMemcpyDeviceToDevice, s));
+    GX_HIP(hipMemcpyAsync(sp.iown.p, b + 3 * (size_t)ni, (size_t)ni * 4, hipMemcpyDeviceToDevice, s));
+    GX_HIP(hipStreamSynchronize(s));
+    dbuf_free(tmp);
+  }
+  // refs and chunks
+  k_sym_ref_count<<<nblk(L1), 256, 0, s>>>(item_off, low_off, nl, ref_off, ch_off, part_off, heavy_off);
+  GX_HIP(hipGetLastError());
+  uint32_t nref = 0, nch = 0, ncp = 0, nheavy = 0;
+  GX_CHECK(scan_u32(h, ref_off, ref_off, L1, false, false, &nref));
+  GX_CHECK(scan_u32(h, ch_off, ch_off, L1, false, false, &nch));
+  GX_CHECK(scan_u32(h, part_off, part_off, L1, false, false, &ncp));
+  GX_CHECK(scan_u32(h, heavy_off, heavy_off, L1, false, false, &nheavy));
+  GX_CHECK(dbuf_reserve(sp.refs, (size_t)nref * 4 + 4, s));
+  GX_CHECK(dbuf_reserve(sp.ch_leaf, (size_t)nch * 4 + 4, s));
+  GX_CHECK(dbuf_reserve(sp.ch_r0, (size_t)nch * 4 + 4, s));
+  GX_CHECK(dbuf_reserve(sp.ch_r1, (size_t)nch * 4 + 4, s));
+  GX_CHECK(dbuf_reserve(sp.ch_dst, (size_t)nch * 4 + 4, s));
+  GX_CHECK(dbuf_reserve(sp.heavy_leaf, (size_t)nheavy * 4 + 4, s));
+  GX_CHECK(dbuf_reserve(sp.heavy_p0, (size_t)nheavy * 4 + 4, s));
+  GX_CHECK(dbuf_reserve(sp.heavy_n, (size_t)nheavy * 4 + 4, s));
+  k_sym_ref_fill<<<nblk(nl), 256, 0, s>>>(item_off, low_off, low_slot.as<uint32_t>(), nl, ref_off, ch_off, part_off,
+                                          heavy_off, sp.refs.as<uint32_t>(), sp.ch_leaf.as<uint32_t>(),
+                                          sp.ch_r0.as<uint32_t>(), sp.ch_r1.as<uint32_t>(), sp.ch_dst.as<uint32_t>(),
+                                          sp.heavy_leaf.as<uint32_t>(), sp.heavy_p0.as<uint32_t>(),
+                                          sp.heavy_n.as<uint32_t>());
+  GX_HIP(hipGetLastError());
+  GX_CHECK(dbuf_reserve(sp.rel, std::max<size_t>(h->tree.n_bodies, 1) * 16, s));
+  GX_CHECK(dbuf_reserve(sp.own2, std::max<size_t>(ni, 1) * SYM_LEAF * 16, s));
+  GX_CHECK(dbuf_reserve(sp.pair2, std::max<size_t>(np, 1) * SYM_LEAF * 8, s));
+  GX_CHECK(dbuf_reserve(sp.cpart2, std::max<size_t>(ncp, 1) * SYM_LEAF * 16, s));
+  GX_HIP(hipStreamSynchronize(s));
+  dbuf_free(low_slot);
+  sp.n_items = ni;
+  sp.n_pairs = np;
+  sp.n_leaves = nl;
+  sp.n_chunks = nch;
+  sp.n_heavy = nheavy;
+  sp.n_cpart = ncp;
+  sp.valid = true;
+  return 0;
+}
+
 static int build_gather(galax_fmm* h) {
   DevTree& t = h->tree;
+  t.sym.valid = false;  // built from the gather lists at its first use
   hipStream_t s = h->stream;
   uint32_t nl = (uint32_t)t.n_leaves;
   GX_CHECK(dbuf_reserve(h->ws.e, (nl + 1) * sizeof(uint32_t), s));
