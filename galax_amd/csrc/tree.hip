@@ -617,106 +617,209 @@ int build_tree_constrained(galax_fmm* h, size_t n) {
 
 // ------------------------------------------------------------ lists ----
 
-__device__ __forceinline__ bool touch_child(const NodeArrays& a, uint32_t child, uint32_t node) {
-  // galax-core/src/lib.rs:1000-1004
-  double cdx = fabs(a.cx[child] - a.cx[node]), cdy = fabs(a.cy[child] - a.cy[node]);
-  double hs = a.hw[node] + a.hw[child];
-  return cdx <= hs + 1e-15 && cdy <= hs + 1e-15;
-}
+// ------------------------------------------- list candidates, pruned ----
+//
+// InteractionLists::build (galax-core/src/lib.rs:972-1016) tests, for a node
+// c with parent P, every node pn of level(c) + 1 in ascending id, and appends
+// each child q of a pn that touches P when q does not touch c, until 64
+// entries; build_p2p_lists (:1371-1389) tests every leaf against a leaf.
+// Scanned directly that is O(sum of n_l n_(l+1)): at 2^20 Plummer bodies
+// ~4.6e9 candidates, most producing nothing (from height 7 up every child
+// touches every node: the lists are empty, yet each level is scanned).
+//
+// Here every level L has a segment tree over its nodes in id order (a heap of
+// 2 P2 entries, P2 = 2^ceil(log2 n_L)). A segment stores
+//   the union of its nodes' boxes [cx - hw, cx + hw] (x and y), and
+//   over all children q of its nodes: max (q.c - q.hw) and min (q.c + q.hw).
+// A node touches P only if its box meets P's box widened by 1e-15; a child
+// q fails to touch c only if q.c - q.hw > c.c + c.hw + 1e-15 or
+// q.c + q.hw < c.c - c.hw - 1e-15 (x or y). A query walks the tree depth
+// first in ascending id, skips segments where either condition cannot hold
+// (with a slack of 1e-13 of the magnitudes, far above the rounding of the
+// reference's own comparisons), applies the reference's exact tests to single
+// nodes and stops at 64 entries: the same entries in the same order, visiting
+// only segments that can contribute.
+struct SegLevel {
+  uint32_t base, p2, lo, n;  // heap entries [base, base + 2 p2); node ids [lo, lo + n)
+};
+struct SegTree {
+  double *x0, *x1, *y0, *y1, *ax, *bx, *ay, *by;
+};
 
-// wave-wide inclusive prefix sum of one u32 per lane
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t u = __shfl_up(v, d, 64);
-    if (lane >= d) v += u;
+__global__ void k_seg_leaves(NodeArrays a, const SegLevel* __restrict__ lvls, uint32_t nlv, SegTree st) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  // one thread per heap leaf slot of every level: levels in order, slots [0, p2)
+  uint32_t l = 0, off = 0;
+  while (l < nlv && i >= off + lvls[l].p2) off += lvls[l++].p2;
+  if (l >= nlv) return;
+  const SegLevel L = lvls[l];
+  const uint32_t k = i - off;
+  const size_t e = (size_t)L.base + L.p2 + k;
+  const double inf = __builtin_huge_val();
+  if (k >= L.n) {  // padding: meets nothing, no child
+    st.x0[e] = st.y0[e] = inf;
+    st.x1[e] = st.y1[e] = -inf;
+    st.ax[e] = st.ay[e] = -inf;
+    st.bx[e] = st.by[e] = inf;
+    return;
   }
-  return v;
+  const uint32_t nd = L.lo + k;
+  const double cx = a.cx[nd], cy = a.cy[nd], hw = a.hw[nd];
+  st.x0[e] = cx - hw;
+  st.x1[e] = cx + hw;
+  st.y0[e] = cy - hw;
+  st.y1[e] = cy + hw;
+  double ax = -inf, bx = inf, ay = -inf, by = inf;
+  const uint32_t cf = a.cf[nd];
+  if (cf != NONE)
+    for (uint32_t q = cf; q < cf + a.cc[nd]; ++q) {
+      const double qx = a.cx[q], qy = a.cy[q], qh = a.hw[q];
+      ax = fmax(ax, qx - qh);
+      bx = fmin(bx, qx + qh);
+      ay = fmax(ay, qy - qh);
+      by = fmin(by, qy + qh);
+    }
+  st.ax[e] = ax;
+  st.bx[e] = bx;
+  st.ay[e] = ay;
+  st.by[e] = by;
 }
 
-// M2L lists, as InteractionLists::build (galax-core/src/lib.rs:972-1016): for
-// node c with parent P, scan the nodes pn of P's level in ascending id, skip
-// pn == P and those whose box does not touch P's (the reference's test, with
-// its `>` form), and append every child q of pn whose box does not touch c,
-// until 64 entries. One wave per parent P fills the rows of all of P's
-// children: its lanes test 64 consecutive candidates pn at a time and a
-// wave prefix sum places each lane's appends after those of lower ids, so
-// every row holds exactly the reference's first 64 entries in its order.
-// Rows are 64 slots wide, like the reference's [u32; 64], and compacted
-// afterwards. The level is scanned directly: parent half-widths double every
-// round, so at the upper levels nearly every box touches every other and
-// pruning by box gains nothing.
-__global__ void __launch_bounds__(256) k_m2l_rows(NodeArrays a, uint32_t plo, uint32_t phi,
-                                                  uint32_t* __restrict__ rows, uint32_t* __restrict__ lens) {
-  const uint32_t P = plo + blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (P >= phi) return;  // uniform over the wave
-  const uint32_t cf = a.cf[P];
-  const uint32_t cc = a.cc[P];
+// internal entries at height h (1 = parents of the leaf slots) of every level
+// whose tree is that tall
+__global__ void k_seg_up(const SegLevel* __restrict__ lvls, uint32_t nlv, uint32_t h, SegTree st) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t l = 0; l < nlv; ++l) {
+    const uint32_t cnt = lvls[l].p2 >> h;
+    if (i < cnt) {
+      const size_t e = (size_t)lvls[l].base + cnt + i;
+      const size_t a = (size_t)lvls[l].base + 2 * (cnt + i), b = a + 1;
+      st.x0[e] = fmin(st.x0[a], st.x0[b]);
+      st.x1[e] = fmax(st.x1[a], st.x1[b]);
+      st.y0[e] = fmin(st.y0[a], st.y0[b]);
+      st.y1[e] = fmax(st.y1[a], st.y1[b]);
+      st.ax[e] = fmax(st.ax[a], st.ax[b]);
+      st.bx[e] = fmin(st.bx[a], st.bx[b]);
+      st.ay[e] = fmax(st.ay[a], st.ay[b]);
+      st.by[e] = fmin(st.by[a], st.by[b]);
+      return;
+    }
+    i -= cnt;
+  }
+}
+
+constexpr double SEG_SLACK = 1e-13;
+
+// Can the segment hold a node whose box meets [x0, x1] x [y0, y1]?
+__device__ __forceinline__ bool seg_meets(const SegTree& st, size_t e, double x0, double x1, double y0, double y1,
+                                          double mag) {
+  const double sx0 = st.x0[e], sx1 = st.x1[e], sy0 = st.y0[e], sy1 = st.y1[e];
+  if (!(sx0 <= sx1)) return false;  // empty
+  const double sl = SEG_SLACK * (mag + fmax(fmax(fabs(sx0), fabs(sx1)), fmax(fabs(sy0), fabs(sy1))));
+  return sx1 >= x0 - sl && sx0 <= x1 + sl && sy1 >= y0 - sl && sy0 <= y1 + sl;
+}
+
+// Can the segment hold a child that does not touch c (centre cx, cy, half
+// width ch)?
+__device__ __forceinline__ bool seg_far_child(const SegTree& st, size_t e, double cx, double cy, double ch,
+                                              double mag) {
+  const double ax = st.ax[e], bx = st.bx[e], ay = st.ay[e], by = st.by[e];
+  if (ax == -__builtin_huge_val()) return false;  // no child in the segment
+  const double sl = SEG_SLACK * (mag + fmax(fmax(fabs(ax), fabs(bx)), fmax(fabs(ay), fabs(by))));
+  const double r = ch + 1e-15;
+  return ax > cx + r - sl || bx < cx - r + sl || ay > cy + r - sl || by < cy - r + sl;
+}
+
+constexpr int SEG_STACK = 40;
+
+// M2L rows: one thread per node c below the root.
+__global__ void __launch_bounds__(256) k_m2l_rows_seg(NodeArrays a, uint32_t nn, const SegLevel* __restrict__ lvls,
+                                                      uint32_t nlv, SegTree st, uint32_t* __restrict__ rows,
+                                                      uint32_t* __restrict__ lens) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nn) return;
+  const uint32_t P = a.par[c];
+  const uint32_t lc = a.lv[c];
+  if (P == NONE || lc + 1 >= nlv) {
+    lens[c] = 0;
+    return;
+  }
+  const SegLevel L = lvls[lc + 1];
+  const double ccx = a.cx[c], ccy = a.cy[c], chw = a.hw[c];
   const double pcx = a.cx[P], pcy = a.cy[P], phw = a.hw[P];
-  uint32_t cnt[4] = {0, 0, 0, 0};
-  for (uint32_t base = plo; base < phi; base += 64) {
-    const uint32_t pn = base + (uint32_t)lane;
-    uint32_t qf = 0, qc = 0;
-    if (pn < phi && pn != P) {
+  const double magp = 1.0 + fabs(pcx) + fabs(pcy) + phw;
+  const double magc = 1.0 + fabs(ccx) + fabs(ccy) + chw;
+  const double px0 = pcx - phw - 1e-15, px1 = pcx + phw + 1e-15, py0 = pcy - phw - 1e-15, py1 = pcy + phw + 1e-15;
+  uint32_t* row = rows + (size_t)c * LIST_CAP;
+  uint32_t len = 0;
+  uint32_t stack[SEG_STACK];
+  int sp = 0;
+  stack[sp++] = 1;
+  while (sp > 0 && len < LIST_CAP) {
+    const uint32_t s = stack[--sp];
+    if (s >= L.p2) {  // a single node
+      const uint32_t k = s - L.p2;
+      if (k >= L.n) continue;
+      const uint32_t pn = L.lo + k;
+      if (pn == P) continue;
+      // galax-core/src/lib.rs:991-997
       const double dx = fabs(a.cx[pn] - pcx), dy = fabs(a.cy[pn] - pcy);
       const double h_sum = phw + a.hw[pn];
-      if (!(dx > h_sum + 1e-15 || dy > h_sum + 1e-15)) {
-        qf = a.cf[pn];
-        qc = a.cc[pn];
+      if (dx > h_sum + 1e-15 || dy > h_sum + 1e-15) continue;
+      const uint32_t qf = a.cf[pn], qc = a.cc[pn];
+      for (uint32_t q = qf; qf != NONE && q < qf + qc && len < LIST_CAP; ++q) {
+        // :999-1006
+        const double cdx = fabs(a.cx[q] - ccx), cdy = fabs(a.cy[q] - ccy);
+        const double hs = chw + a.hw[q];
+        if (cdx <= hs + 1e-15 && cdy <= hs + 1e-15) continue;
+        row[len++] = q;
       }
+      continue;
     }
-    bool done = true;
-    for (uint32_t k = 0; k < cc; ++k) {
-      uint32_t m = 0, c = 0;  // this lane's appendable children of pn for child k of P
-      for (uint32_t j = 0; j < qc; ++j)
-        if (!touch_child(a, qf + j, cf + k)) {
-          m |= 1u << j;
-          ++c;
-        }
-      const uint32_t incl = wave_incl_scan(c, lane);
-      const uint32_t total = __shfl(incl, 63, 64);
-      uint32_t pos = cnt[k] + incl - c;
-      for (uint32_t j = 0; j < qc; ++j)
-        if ((m >> j) & 1u) {
-          if (pos < LIST_CAP) rows[(size_t)(cf + k) * LIST_CAP + pos] = qf + j;
-          ++pos;
-        }
-      cnt[k] = min(LIST_CAP, cnt[k] + total);
-      done = done && cnt[k] == LIST_CAP;
+    const size_t e = (size_t)L.base + s;
+    if (!seg_meets(st, e, px0, px1, py0, py1, magp) || !seg_far_child(st, e, ccx, ccy, chw, magc)) continue;
+    if (sp + 2 <= SEG_STACK) {
+      stack[sp++] = 2 * s + 1;  // right half after the left one
+      stack[sp++] = 2 * s;
     }
-    if (done) break;  // every row is full (uniform: cnt is the same in all lanes)
   }
-  if (lane == 0)
-    for (uint32_t k = 0; k < cc; ++k) lens[cf + k] = cnt[k];
+  lens[c] = len;
 }
 
-// P2P lists, as build_p2p_lists (galax-core/src/lib.rs:1371-1389): the leaf
-// itself, then every other leaf in ascending id whose box touches it, until 64.
-// One wave per leaf; 64 candidates per iteration, placed by a prefix sum.
-__global__ void __launch_bounds__(256) k_p2p_rows(NodeArrays a, uint32_t n_leaves, uint32_t* __restrict__ rows,
-                                                  uint32_t* __restrict__ lens) {
-  const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (i >= n_leaves) return;  // uniform over the wave
-  const double cx = a.cx[i], cy = a.cy[i], hw = a.hw[i];
-  uint32_t* row = rows + (size_t)i * LIST_CAP;
-  if (lane == 0) row[0] = i;
-  uint32_t cnt = 1;
-  for (uint32_t base = 0; base < n_leaves && cnt < LIST_CAP; base += 64) {
-    const uint32_t b = base + (uint32_t)lane;
-    uint32_t hit = 0;
-    if (b < n_leaves && b != i) {
+// P2P rows: one thread per leaf; the leaf itself first, then every other leaf
+// whose box touches it, ascending (galax-core/src/lib.rs:1371-1389).
+__global__ void __launch_bounds__(256) k_p2p_rows_seg(NodeArrays a, uint32_t n_leaves, const SegLevel* __restrict__ lvls,
+                                                      SegTree st, uint32_t* __restrict__ rows,
+                                                      uint32_t* __restrict__ lens) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_leaves) return;
+  const SegLevel L = lvls[0];
+  const double cx = a.cx[c], cy = a.cy[c], hw = a.hw[c];
+  const double mag = 1.0 + fabs(cx) + fabs(cy) + hw;
+  const double x0 = cx - hw - 1e-15, x1 = cx + hw + 1e-15, y0 = cy - hw - 1e-15, y1 = cy + hw + 1e-15;
+  uint32_t* row = rows + (size_t)c * LIST_CAP;
+  row[0] = c;
+  uint32_t len = 1;
+  uint32_t stack[SEG_STACK];
+  int sp = 0;
+  stack[sp++] = 1;
+  while (sp > 0 && len < LIST_CAP) {
+    const uint32_t s = stack[--sp];
+    if (s >= L.p2) {
+      const uint32_t b = L.lo + (s - L.p2);
+      if (s - L.p2 >= L.n || b == c) continue;
       const double dx = fabs(cx - a.cx[b]), dy = fabs(cy - a.cy[b]);
       const double hs = hw + a.hw[b];
-      hit = (dx <= hs + 1e-15 && dy <= hs + 1e-15) ? 1u : 0u;
+      if (dx <= hs + 1e-15 && dy <= hs + 1e-15) row[len++] = b;
+      continue;
     }
-    const uint32_t incl = wave_incl_scan(hit, lane);
-    const uint32_t pos = cnt + incl - hit;
-    if (hit && pos < LIST_CAP) row[pos] = b;
-    cnt = min(LIST_CAP, cnt + __shfl(incl, 63, 64));
+    if (!seg_meets(st, (size_t)L.base + s, x0, x1, y0, y1, mag)) continue;
+    if (sp + 2 <= SEG_STACK) {
+      stack[sp++] = 2 * s + 1;
+      stack[sp++] = 2 * s;
+    }
   }
-  if (lane == 0) lens[i] = cnt;
+  lens[c] = len;
 }
 
 __global__ void k_rows_compact(const uint32_t* __restrict__ rows, const uint32_t* __restrict__ off, uint32_t nn,
@@ -1248,13 +1351,39 @@ int build_lists_reference(galax_fmm* h) {
   GX_CHECK(dbuf_reserve(h->ws.c, (nn + 1) * sizeof(uint32_t), s));
   uint32_t* rows = h->ws.d.as<uint32_t>();
   uint32_t* lens = h->ws.c.as<uint32_t>();
-  // M2L lists: parents level by level
-  GX_HIP(hipMemsetAsync(lens, 0, (nn + 1) * sizeof(uint32_t), s));
-  for (uint32_t l = 1; l < t.n_levels; ++l) {
-    uint32_t lo = t.level_offset[l], hi = t.level_offset[l + 1];
-    k_m2l_rows<<<nblk(hi - lo, 4), 256, 0, s>>>(a, lo, hi, rows, lens);
+  // segment trees over every level's nodes (candidates of both list builds)
+  std::vector<SegLevel> lv(t.n_levels);
+  size_t tot = 0, slots = 0;
+  uint32_t maxh = 0;
+  for (uint32_t l = 0; l < t.n_levels; ++l) {
+    const uint32_t lo = t.level_offset[l], n = t.level_offset[l + 1] - lo;
+    uint32_t p2 = 1, hgt = 0;
+    while (p2 < n) {
+      p2 <<= 1;
+      ++hgt;
+    }
+    lv[l] = SegLevel{(uint32_t)tot, p2, lo, n};
+    tot += 2 * (size_t)p2;
+    slots += p2;
+    maxh = std::max(maxh, hgt);
+  }
+  GX_CHECK(dbuf_reserve(h->ws.g, tot * 8 * sizeof(double) + t.n_levels * sizeof(SegLevel), s));
+  double* sb = h->ws.g.as<double>();
+  SegTree st{sb, sb + tot, sb + 2 * tot, sb + 3 * tot, sb + 4 * tot, sb + 5 * tot, sb + 6 * tot, sb + 7 * tot};
+  SegLevel* dlv = reinterpret_cast<SegLevel*>(sb + 8 * tot);
+  GX_HIP(hipMemcpyAsync(dlv, lv.data(), t.n_levels * sizeof(SegLevel), hipMemcpyHostToDevice, s));
+  k_seg_leaves<<<nblk(slots), 256, 0, s>>>(a, dlv, t.n_levels, st);
+  GX_HIP(hipGetLastError());
+  for (uint32_t hh = 1; hh <= maxh; ++hh) {
+    size_t cnt = 0;
+    for (const SegLevel& L : lv) cnt += L.p2 >> hh;
+    k_seg_up<<<nblk(cnt), 256, 0, s>>>(dlv, t.n_levels, hh, st);
     GX_HIP(hipGetLastError());
   }
+  // M2L lists
+  k_m2l_rows_seg<<<nblk(nn), 256, 0, s>>>(a, (uint32_t)nn, dlv, t.n_levels, st, rows, lens);
+  GX_HIP(hipGetLastError());
+  GX_HIP(hipMemsetAsync(lens + nn, 0, sizeof(uint32_t), s));
   GX_CHECK(dbuf_reserve(t.m2l_off, (nn + 1) * sizeof(uint32_t), s));
   uint32_t mt = 0;
   GX_CHECK(scan_u32(h, lens, t.m2l_off.as<uint32_t>(), nn + 1, false, false, &mt));
@@ -1265,7 +1394,7 @@ int build_lists_reference(galax_fmm* h) {
   GX_HIP(hipGetLastError());
   // P2P lists (leaves only; other rows empty)
   GX_HIP(hipMemsetAsync(lens, 0, (nn + 1) * sizeof(uint32_t), s));
-  k_p2p_rows<<<nblk(t.n_leaves, 4), 256, 0, s>>>(a, (uint32_t)t.n_leaves, rows, lens);
+  k_p2p_rows_seg<<<nblk(t.n_leaves), 256, 0, s>>>(a, (uint32_t)t.n_leaves, dlv, st, rows, lens);
   GX_HIP(hipGetLastError());
   GX_CHECK(dbuf_reserve(t.p2p_off, (nn + 1) * sizeof(uint32_t), s));
   uint32_t pt = 0;
