@@ -259,9 +259,8 @@ __global__ void k_round_mark(const uint32_t* __restrict__ flag, const uint32_t* 
   if (flag[j]) cf[ce + pidx[j]] = cs + j;
 }
 
-__global__ void k_round_parents(NodeArrays a, uint32_t ce, uint32_t np, uint8_t level, int constrained_style) {
-  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= np) return;
+__device__ __forceinline__ void round_parent(NodeArrays a, uint32_t ce, uint32_t np, uint32_t q, uint8_t level,
+                                             int constrained_style) {
   uint32_t p = ce + q;
   uint32_t first = a.cf[p];
   uint32_t last = (q + 1 < np) ? a.cf[p + 1] : ce;
@@ -303,6 +302,116 @@ __global__ void k_round_parents(NodeArrays a, uint32_t ce, uint32_t np, uint8_t 
   a.cc[p] = (uint8_t)(last - first);
   a.par[p] = NONE;
   a.lv[p] = level;
+}
+
+__global__ void k_round_parents(NodeArrays a, uint32_t ce, uint32_t np, uint8_t level, int constrained_style) {
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  round_parent(a, ce, np, q, level, constrained_style);
+}
+
+// Block-wide exclusive scan (sum or max) of one value per thread of a
+// 1024-thread block; *tot gets the reduction. Ends with a barrier.
+template <bool MAX>
+__device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* wsum, uint32_t* tot) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t u = __shfl_up(inc, d, 64);
+    if (lane >= d) inc = MAX ? (u > inc ? u : inc) : inc + u;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (int k = 0; k < 16; ++k) {
+      const uint32_t x = wsum[k];
+      wsum[k] = acc;
+      acc = MAX ? (x > acc ? x : acc) : acc + x;
+    }
+    wsum[16] = acc;
+  }
+  __syncthreads();
+  const uint32_t pre = wsum[w];
+  const uint32_t ex_in_wave = MAX ? __shfl_up(inc, 1, 64) : inc - v;  // exclusive within the wave
+  uint32_t r;
+  if (MAX)
+    r = lane == 0 ? pre : (ex_in_wave > pre ? ex_in_wave : pre);
+  else
+    r = pre + ex_in_wave;
+  *tot = wsum[16];
+  __syncthreads();
+  return r;
+}
+
+// Every remaining parent round of at most FUSED_ROUND nodes in one
+// 1024-thread workgroup: per round the steps of k_round_heads, the inclusive
+// max scan, k_round_chunks, the exclusive sum scan, k_round_mark and
+// round_parent (the same arithmetic), with barriers between them, so the
+// ~20 rounds cost one launch instead of ~6 kernels and a host read each.
+// A node's group key is the Morton key of its first body (a parent shares
+// its first child's, galax-core/src/lib.rs:278-282), kept per node in nkey so
+// the head test reads one array in order; run starts and chunk heads are
+// bytes in LDS. Stops early (state left in st) when a round is larger than
+// FUSED_ROUND or its parents would not fit in `cap` nodes; the host goes on
+// from there. st: [cs, ce, shift, level, rounds done]; lvl_end[r] = ce after
+// round r.
+constexpr uint32_t FUSED_ROUND = 32768;
+__global__ void __launch_bounds__(1024) k_rounds_fused(NodeArrays a, const uint64_t* __restrict__ keys,
+                                                       int constrained_style, uint32_t cap, uint64_t* __restrict__ nkey,
+                                                       uint32_t* __restrict__ st, uint32_t* __restrict__ lvl_end,
+                                                       uint32_t max_rounds) {
+  __shared__ uint32_t wsum[17];
+  __shared__ uint8_t flg[FUSED_ROUND];
+  uint32_t cs = st[0], ce = st[1], shift = st[2], lvl = st[3], r = 0;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t j = cs + tid; j < ce; j += 1024) nkey[j] = keys[a.bs[j]];
+  __syncthreads();
+  while (ce - cs > 1 && ce - cs <= FUSED_ROUND && r < max_rounds && lvl < 250) {
+    const uint32_t c = ce - cs;
+    const uint32_t sh = shift & 63;  // release-mode Rust masks the shift amount
+    for (uint32_t j = tid; j < c; j += 1024)
+      flg[j] = (j == 0 || (nkey[cs + j] >> sh) != (nkey[cs + j - 1] >> sh)) ? 1 : 0;
+    __syncthreads();
+    const uint32_t K = (c + 1023) / 1024, lo = min(tid * K, c), hi = min(lo + K, c);
+    uint32_t m = 0;  // last run start in this thread's chunk (0 = none: j = 0 always starts one)
+    for (uint32_t j = lo; j < hi; ++j)
+      if (flg[j]) m = j;
+    uint32_t tot;
+    uint32_t run = block_scan_excl<true>(m, wsum, &tot);
+    uint32_t nf = 0;
+    for (uint32_t j = lo; j < hi; ++j) {
+      if (flg[j]) run = j;
+      const uint8_t f = ((j - run) % 4u == 0u) ? 1 : 0;  // a new parent every 4 nodes of a run
+      flg[j] = f;
+      nf += f;
+    }
+    uint32_t np = 0;
+    uint32_t pidx = block_scan_excl<false>(nf, wsum, &np);
+    if (ce + np > cap) break;  // uniform: the host grows the arrays and goes on
+    for (uint32_t j = lo; j < hi; ++j)
+      if (flg[j]) a.cf[ce + pidx++] = cs + j;
+    __syncthreads();
+    ++lvl;
+    for (uint32_t q = tid; q < np; q += 1024) {
+      round_parent(a, ce, np, q, (uint8_t)lvl, constrained_style);
+      nkey[ce + q] = nkey[a.cf[ce + q]];
+    }
+    __syncthreads();
+    cs = ce;
+    ce += np;
+    shift += 2;
+    if (tid == 0) lvl_end[r] = ce;
+    ++r;
+  }
+  if (tid == 0) {
+    st[0] = cs;
+    st[1] = ce;
+    st[2] = shift;
+    st[3] = lvl;
+    st[4] = r;
+  }
 }
 
 template <class T>
@@ -361,6 +470,35 @@ static int build_from_leaf_starts(galax_fmm* h, size_t n, size_t n_leaves, bool 
   GX_CHECK(dbuf_reserve(h->ws.g, (n_leaves + 1) * sizeof(uint32_t), s));
   while (ce - cs > 1) {
     uint32_t c = (uint32_t)(ce - cs);
+    if (c <= FUSED_ROUND) {
+      // the remaining rounds in one launch; the arrays sized for 32 more
+      // rounds of c nodes (the kernel stops early if they would not fit)
+      const uint32_t max_rounds = 64;
+      GX_CHECK(ensure_node_capacity(h, ce + (size_t)c * 32 + 64));
+      a = node_arrays(h);
+      GX_CHECK(dbuf_reserve(h->ws.h, (8 + max_rounds) * sizeof(uint32_t) + t.capacity * sizeof(uint64_t), s));
+      uint32_t* dst = h->ws.h.as<uint32_t>();
+      uint64_t* nkey = reinterpret_cast<uint64_t*>(dst + 8 + max_rounds);
+      uint32_t hst[8 + 64] = {(uint32_t)cs, (uint32_t)ce, shift, lvl, 0, 0, 0, 0};
+      GX_HIP(hipMemcpyAsync(dst, hst, 8 * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+      k_rounds_fused<<<1, 1024, 0, s>>>(a, t.keys.as<uint64_t>(), constrained_style ? 1 : 0, (uint32_t)t.capacity,
+                                         nkey, dst, dst + 8, max_rounds);
+      GX_HIP(hipGetLastError());
+      GX_HIP(hipMemcpyAsync(hst, dst, (8 + max_rounds) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      GX_HIP(hipStreamSynchronize(s));
+      for (uint32_t k = 0; k < hst[4]; ++k) t.level_offset.push_back(hst[8 + k]);
+      cs = hst[0];
+      ce = hst[1];
+      shift = hst[2];
+      lvl = hst[3];
+      if (lvl > 250) return set_error(GALAX_ERR_INVALID, "tree deeper than 250 levels");
+      if (ce - cs <= 1) break;
+      c = (uint32_t)(ce - cs);
+      if (c <= FUSED_ROUND && hst[4] == 0) {
+        GX_CHECK(ensure_node_capacity(h, ce + (size_t)c + 64));  // room for one more round, then fused again
+        a = node_arrays(h);
+      }
+    }
     ++lvl;
     if (lvl > 250) return set_error(GALAX_ERR_INVALID, "tree deeper than 250 levels");
     uint32_t* head = h->ws.e.as<uint32_t>();
@@ -911,15 +1049,35 @@ __global__ void k_pair_split(const uint64_t* __restrict__ keys, uint32_t n, int 
 }
 
 // Directed body-body interactions of the gather lists (self pairs excluded).
-__global__ void k_count_interactions(uint32_t n_leaves, const uint32_t* __restrict__ goff,
-                                     const uint32_t* __restrict__ gdat, const uint32_t* __restrict__ bs,
-                                     const uint32_t* __restrict__ be, unsigned long long* __restrict__ total) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_leaves) return;
-  unsigned long long src = 0;
-  for (uint32_t g = goff[t]; g < goff[t + 1]; ++g) src += be[gdat[g]] - bs[gdat[g]];
-  unsigned long long nt = be[t] - bs[t];
-  atomicAdd(total, nt * src - nt);
+// One thread per gather entry (t, s): n_t n_s directed pairs, less the
+// body itself when s = t; its leaf t found by binary search of the offsets.
+// (A thread per leaf left the longest lists, thousands of entries, to one
+// thread.) Block sums, then one 64-bit atomic add per block.
+__global__ void __launch_bounds__(256) k_count_interactions(uint32_t n_leaves, const uint32_t* __restrict__ goff,
+                                                            const uint32_t* __restrict__ gdat,
+                                                            const uint32_t* __restrict__ bs,
+                                                            const uint32_t* __restrict__ be,
+                                                            unsigned long long* __restrict__ total) {
+  __shared__ unsigned long long part[4];
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long v = 0;
+  if (g < goff[n_leaves]) {
+    uint32_t lo = 0, hi = n_leaves;  // the leaf t with goff[t] <= g < goff[t + 1]
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (goff[mid] <= g)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const uint32_t t = lo, src = gdat[g];
+    const unsigned long long nt = be[t] - bs[t];
+    v = nt * (unsigned long long)(be[src] - bs[src]) - (src == t ? nt : 0ull);
+  }
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_down(v, d, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(total, part[0] + part[1] + part[2] + part[3]);
 }
 
 int count_interactions(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, uint64_t* out) {
@@ -928,8 +1086,12 @@ int count_interactions(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat,
   GX_CHECK(dbuf_reserve(h->flag, 64, s));
   unsigned long long* d = reinterpret_cast<unsigned long long*>(h->flag.as<uint32_t>() + 12);
   GX_HIP(hipMemsetAsync(d, 0, 8, s));
-  k_count_interactions<<<nblk(t.n_leaves), 256, 0, s>>>((uint32_t)t.n_leaves, goff, gdat,
-                                                        t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), d);
+  uint32_t ng = 0;  // gather entries
+  GX_HIP(hipMemcpyAsync(&ng, goff + t.n_leaves, 4, hipMemcpyDeviceToHost, s));
+  GX_HIP(hipStreamSynchronize(s));
+  if (ng)
+    k_count_interactions<<<nblk(ng), 256, 0, s>>>((uint32_t)t.n_leaves, goff, gdat, t.body_start.as<uint32_t>(),
+                                                  t.body_end.as<uint32_t>(), d);
   GX_HIP(hipGetLastError());
   unsigned long long v = 0;
   GX_HIP(hipMemcpyAsync(&v, d, 8, hipMemcpyDeviceToHost, s));
