@@ -1149,6 +1149,14 @@ ORC_API void orc_lists_export_csr(const orc_lists* l, uint64_t* offsets, uint32_
   }
 }
 
+// CSR import (lists built elsewhere, e.g. exported from the device build)
+ORC_API orc_lists* orc_lists_import_csr(size_t nn, const uint64_t* offsets, const uint32_t* data) {
+  orc_lists* h = new orc_lists;
+  h->l.data.resize(nn);
+  for (size_t i = 0; i < nn; ++i) h->l.data[i].assign(data + offsets[i], data + offsets[i + 1]);
+  return h;
+}
+
 ORC_API void orc_lists_free(orc_lists* l) { delete l; }
 
 ORC_API void orc_kernel_derivs(double dx, double dy, double eps, int p, double* out) {

@@ -68,6 +68,8 @@ def lib():
     L.orc_lists_total.argtypes = [_vp]
     L.orc_lists_export_csr.argtypes = [_vp, _vp, _vp]
     L.orc_lists_free.argtypes = [_vp]
+    L.orc_lists_import_csr.restype = _vp
+    L.orc_lists_import_csr.argtypes = [_sz, _vp, _vp]
     L.orc_kernel_derivs.argtypes = [_d, _d, _d, C.c_int, _vp]
     L.orc_p2m.argtypes = [_vp, _vp, _vp, _vp, C.c_int, _vp]
     L.orc_m2m.argtypes = [_vp, _vp, C.c_int]
@@ -194,6 +196,12 @@ class Lists:
         data = np.zeros(max(tot, 1), np.uint32)
         lib().orc_lists_export_csr(self.h, _p(off), _p(data))
         return off, data[:tot]
+
+    @classmethod
+    def from_csr(cls, offsets, data):
+        off = np.ascontiguousarray(offsets, np.uint64)
+        dat = np.ascontiguousarray(data if len(data) else np.zeros(1), np.uint32)
+        return cls(lib().orc_lists_import_csr(len(off) - 1, _p(off), _p(dat)), len(off) - 1)
 
     def as_lists(self):
         off, data = self.csr()

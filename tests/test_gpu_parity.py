@@ -722,29 +722,29 @@ def test_mixed_forms_match_fp64_plummer1m_p6():
 def test_corrected_plummer1m_within_1e5_of_direct_sum():
     """The full 2^20-body Plummer workload as the bench runs it (galax-cli's
     softening of the sorted bodies) with the corrected variant at p = 8,
-    theta = 0.4, FP64 and MIXED_FP32, against the FP64 direct sum on sampled
-    targets (size-independent: the direct sum is exact for each of them).
-    The RMS of the relative error is dominated by the few bodies whose force
-    nearly cancels (|a| ~ 1e-3 of the median), so it depends on the sample:
-    on the bench's sample (SplitMix64(4242), 10,000 draws) it is within 1e-5;
-    on another (seed 77) the RMS |da| / RMS |a|, the median and the 99th
-    percentile are checked."""
+    theta = 0.35, FP64 and MIXED_FP32, against the FP64 direct sum on sampled
+    targets (size-independent: the direct sum is exact for each of them). The
+    RMS of the relative error is dominated by the few bodies whose force
+    nearly cancels, so it varies between samples: it must stay within 1e-5 on
+    each of three independent 10,000-draw samples (measured 1.1e-6 .. 5.2e-6;
+    at theta = 0.4 one of them gives 1.6e-5). The median, the 99th
+    percentile and RMS |da| / RMS |a| are checked as well."""
     d = ic.plummer(1 << 20)
     bounds = (-50.0, 50.0, -50.0, 50.0)
     out = {}
     for prec in (fmm.PREC_FP64, fmm.PREC_MIXED_FP32):
-        ctx, b, _ = _gpu_build(d, p=8, bounds=bounds, algo=fmm.ALGO_CORRECTED, theta=0.4, precision=prec)
+        ctx, b, _ = _gpu_build(d, p=8, bounds=bounds, algo=fmm.ALGO_CORRECTED, theta=0.35, precision=prec)
         eps = fmm.default_softening(b.x, b.y)  # galax-cli/src/main.rs:182-190
         ctx.set_softening(eps)
         ctx.evaluate(b)
         out[prec] = (b.ax.copy(), b.ay.copy())
-    for seed, k in ((4242, 10_000), (77, 4_000)):
-        si = np.unique((ic.SplitMix64(seed).next_u64(k) % np.uint64(len(b))).astype(np.int64))
+    for seed in (4242, 77, 2026):
+        si = np.unique((ic.SplitMix64(seed).next_u64(10_000) % np.uint64(len(b))).astype(np.int64))
         ex, ey, _ = fmm.compute_n2_force(b.x, b.y, b.mass, eps, targets=si.astype(np.uint64))
         for prec, (ax, ay) in out.items():
             rd = _rel_diff((ax[si], ay[si]), (ex, ey))
-            if seed == 4242:
-                assert np.sqrt(np.mean(rd ** 2)) <= 1e-5, (prec, np.sqrt(np.mean(rd ** 2)))
+            print(f"corrected p=8 theta=0.35 prec={prec} seed={seed}: rms {np.sqrt(np.mean(rd ** 2)):.3e}", flush=True)
+            assert np.sqrt(np.mean(rd ** 2)) <= 1e-5, (prec, seed, np.sqrt(np.mean(rd ** 2)))
             dabs = np.hypot(ax[si] - ex, ay[si] - ey)
             norm = np.sqrt(np.mean(dabs ** 2)) / np.sqrt(np.mean(ex ** 2 + ey ** 2))
             assert norm <= 1e-5 and np.median(rd) <= 1e-6 and np.percentile(rd, 99) <= 1e-4, (
