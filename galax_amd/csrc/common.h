@@ -53,6 +53,13 @@ struct DBuf {
 };
 int dbuf_reserve(DBuf& b, size_t bytes, hipStream_t s);
 void dbuf_free(DBuf& b);
+// a temporary buffer freed when the scope ends (early error returns included)
+struct ScopedDBuf : DBuf {
+  ScopedDBuf() = default;
+  ScopedDBuf(const ScopedDBuf&) = delete;
+  ScopedDBuf& operator=(const ScopedDBuf&) = delete;
+  ~ScopedDBuf() { dbuf_free(*this); }
+};
 
 // P2P work items over gather lists: item i covers target leaf tgt[i] and the
 // gather entries [g0[i], g1[i]). A target with several items writes partial
@@ -162,8 +169,8 @@ struct galax_fmm {
   // M2L on the matrix cores (m2l_mfma.hip): forward tiles over targets,
   // adjoint tiles over sources, and the per-key operators
   gx::M2LTiles mx_fwd, mx_adj;
-  gx::DBuf mx_t64, mx_that, mx_sig, mx_zero;  // mx_zero: one zero source row (missing ids read it)
-  bool mx_valid = false, mx_adj_valid = false, mx_t64_valid = false;
+  gx::DBuf mx_t64, mx_t64T, mx_that, mx_sig, mx_zero;  // mx_zero: one zero source row (missing ids read it)
+  bool mx_valid = false, mx_adj_valid = false;
   // adjoint (VJP) buffers
   gx::DBuf vj_lbar, vj_mbar, vj_leaf_of, vj_fwd, vj_out;
   size_t n_keys = 0;

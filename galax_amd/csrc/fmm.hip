@@ -344,27 +344,20 @@ __global__ void k_pair_targets(const uint32_t* __restrict__ off, uint32_t n_targ
   for (uint32_t e = off[t]; e < off[t + 1]; ++e) tgt_of_pair[e] = t;
 }
 
-// One thread per pair. The representative of a key is its smallest pair index
-// (first touch in list order). A slot already holding the key and an index
-// <= e needs no atomic: slots only ever take a key once and indices only
-// decrease, so a plain read that shows both is final for this pair.
-__global__ void k_keys_insert(const uint32_t* __restrict__ dat, const uint32_t* __restrict__ tgt_of_pair,
-                              uint32_t n_pairs, const double* __restrict__ comx, const double* __restrict__ comy,
-                              unsigned long long* __restrict__ hkeys, uint32_t* __restrict__ hrep, uint64_t mask,
-                              uint32_t* __restrict__ err) {
-  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n_pairs) return;
-  const uint32_t t = tgt_of_pair[e], s = dat[e];
-  uint64_t key;
-  if (!pack_key(comx[t] - comx[s], comy[t] - comy[s], &key)) {
-    atomicOr(err, 1u);
-    return;
-  }
+// Global table: insert the key (if new) and lower its representative to e.
+// The representative of a key is its smallest pair index (first touch in
+// list order). A slot already holding the key and an index <= e needs no
+// atomic: slots only ever take a key once and indices only decrease, so a
+// read that shows both is final. The probes are ordinary cached loads; a
+// stale read is safe (an empty slot sends the pair to the CAS, which returns
+// the key a concurrent insert left there; a stale index is only larger).
+__device__ __forceinline__ void keys_global_insert(unsigned long long* hkeys, uint32_t* hrep, uint64_t mask,
+                                                   uint64_t key, uint32_t e) {
   uint64_t slot = mix64(key) & mask;
   for (;;) {
-    const unsigned long long cur = __atomic_load_n(&hkeys[slot], __ATOMIC_RELAXED);
+    const unsigned long long cur = hkeys[slot];
     if (cur == key) {
-      if (__atomic_load_n(&hrep[slot], __ATOMIC_RELAXED) > e) atomicMin(&hrep[slot], e);
+      if (hrep[slot] > e) atomicMin(&hrep[slot], e);
       return;
     }
     if (cur == EMPTY_KEY) {
@@ -376,6 +369,54 @@ __global__ void k_keys_insert(const uint32_t* __restrict__ dat, const uint32_t* 
     }
     slot = (slot + 1) & mask;
   }
+}
+
+// A few thousand keys are shared by millions of pairs, and device-scope
+// atomics on the same few lines serialize in memory (an XCD's L2 keeps its
+// stale copy of a line that another XCD's atomic changed, so the read-first
+// filter above does not help across XCDs). Each block therefore first
+// reduces its KI_PAIRS consecutive pairs to their distinct keys with the
+// smallest index each, in an LDS hash, and only those go to the global table.
+// A pair whose key finds no LDS slot within KI_PROBES probes goes directly.
+constexpr uint32_t KI_PAIRS = 32768, KI_SLOTS = 4096, KI_PROBES = 64;
+
+__global__ void __launch_bounds__(1024) k_keys_insert(const uint32_t* __restrict__ dat,
+                                                      const uint32_t* __restrict__ tgt_of_pair, uint32_t n_pairs,
+                                                      const double* __restrict__ comx, const double* __restrict__ comy,
+                                                      unsigned long long* hkeys, uint32_t* hrep, uint64_t mask,
+                                                      uint32_t* __restrict__ err) {
+  __shared__ unsigned long long lk[KI_SLOTS];
+  __shared__ uint32_t le[KI_SLOTS];
+  for (uint32_t i = threadIdx.x; i < KI_SLOTS; i += blockDim.x) {
+    lk[i] = EMPTY_KEY;
+    le[i] = 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  const uint32_t e0 = blockIdx.x * KI_PAIRS, e1 = min(e0 + KI_PAIRS, n_pairs);
+  for (uint32_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    const uint32_t t = tgt_of_pair[e], s = dat[e];
+    uint64_t key;
+    if (!pack_key(comx[t] - comx[s], comy[t] - comy[s], &key)) {
+      atomicOr(err, 1u);
+      continue;
+    }
+    uint32_t ls = (uint32_t)mix64(key) & (KI_SLOTS - 1);
+    bool done = false;
+    for (uint32_t q = 0; q < KI_PROBES && !done; ++q) {
+      unsigned long long cur = lk[ls];
+      if (cur == EMPTY_KEY) cur = atomicCAS(&lk[ls], (unsigned long long)EMPTY_KEY, (unsigned long long)key);
+      if (cur == EMPTY_KEY || cur == key) {
+        atomicMin(&le[ls], e);
+        done = true;
+      } else {
+        ls = (ls + 1) & (KI_SLOTS - 1);
+      }
+    }
+    if (!done) keys_global_insert(hkeys, hrep, mask, key, e);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < KI_SLOTS; i += blockDim.x)
+    if (lk[i] != EMPTY_KEY) keys_global_insert(hkeys, hrep, mask, lk[i], le[i]);
 }
 
 __global__ void k_keys_flag(const unsigned long long* __restrict__ hkeys, uint64_t nslots, uint32_t* __restrict__ f) {
@@ -1459,8 +1500,8 @@ static int m2l_keys_setup(galax_fmm* h, uint32_t n_targets) {
   const double* comx = t.com_x.as<double>();
   const double* comy = t.com_y.as<double>();
   k_pair_targets<<<nblk(n_targets), 256, 0, s>>>(t.m2l_off.as<uint32_t>(), n_targets, h->tmp_u32.as<uint32_t>());
-  k_keys_insert<<<nblk(n_pairs), 256, 0, s>>>(t.m2l_dat.as<uint32_t>(), h->tmp_u32.as<uint32_t>(), n_pairs, comx, comy,
-                                              hkeys, hrep, mask, err);
+  k_keys_insert<<<(unsigned)((n_pairs + KI_PAIRS - 1) / KI_PAIRS), 1024, 0, s>>>(
+      t.m2l_dat.as<uint32_t>(), h->tmp_u32.as<uint32_t>(), n_pairs, comx, comy, hkeys, hrep, mask, err);
   GX_HIP(hipGetLastError());
   uint32_t herr = 0;
   GX_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s));
@@ -1688,7 +1729,7 @@ int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, 
   if (!corrected && !h->keys_valid) {
     uint32_t top = std::min<uint32_t>(t.n_levels, 2);
     GX_CHECK(m2l_keys_setup(h, t.level_offset[top]));
-    h->mx_valid = h->mx_adj_valid = h->mx_t64_valid = false;
+    h->mx_valid = h->mx_adj_valid = false;
   }
   if (!corrected && h->cfg.precision != GALAX_PREC_FP64_EXACT && !h->mx_valid) GX_CHECK(m2l_mfma_setup(h));
   GX_CHECK(dbuf_reserve(h->flag, 64, s));

@@ -75,18 +75,21 @@ __device__ __forceinline__ void coef_ij2(int k, int& i, int& j) {
   i = n - j;
 }
 
-// Contraction index carried by slot e (0 <= e < KL) of lane quarter kq.
-// 16x16x4 forms: lane quarter kq holds the KS = K/4 consecutive indices
-// kq*KS + e, one per instruction. 16x16x32 bf16: instruction c takes 8
-// consecutive indices c*32 + 8*kq + (0..7) from each lane.
+// Contraction index carried by slot e (0 <= e < KL) of lane quarter kq:
+//     k = 8 (e / 2) + 2 kq + (e % 2).
+// The 16x16x4 forms contract the KL = K/4 slots one per instruction; the
+// 16x16x32 bf16 form takes slots 8c .. 8c + 7 of each lane in instruction c.
+// Slots 2m and 2m + 1 of the four lane quarters cover k = 8m .. 8m + 7, so a
+// 16-byte load of a source row's pair (k, k + 1) per lane reads 64
+// contiguous bytes of each row per instruction.
 template <int P, int MODE>
 struct Lanes {
   static constexpr int S = num_moments(P), RB = row_blocks(P), K = 16 * RB;
   static constexpr int KC = (K + 31) / 32;
   static constexpr int KL = MODE == MX_BF16X3 ? KC * 8 : K / 4;
-  static __device__ __forceinline__ int k_of(int e, int kq) {
-    return MODE == MX_BF16X3 ? (e >> 3) * 32 + 8 * kq + (e & 7) : kq * (K / 4) + e;
-  }
+  static __host__ __device__ __forceinline__ int k_of(int e, int kq) { return 8 * (e >> 1) + 2 * kq + (e & 1); }
+  // operator values per key, in fragment order (below)
+  static constexpr int PER_KEY = RB * KL * 64;
 };
 
 // ------------------------------------------------------------- setup ----
@@ -328,18 +331,39 @@ __global__ void __launch_bounds__(256) k_m2l_hub_sum(const uint32_t* __restrict_
   virt[(size_t)hubs[hb] * S + k] = acc;
 }
 
-// FP64 operator of each key: (16 RB)^2 row-major, zero padded
-template <int P>
+// Operators in the product kernel's fragment order. Lane l = 16 kq + c
+// feeds A[16 rb + c][k_of(e, kq)] as slot e of row block rb; a key's values
+// are stored as [rb][e / V][lane][e % V] with V = 16 / sizeof(value), so each
+// 16-byte load of a wave reads 1 KB of contiguous memory. Zero padded.
+template <int P, int MODE>
+__host__ __device__ __forceinline__ void frag_pos(int t, int& r, int& k) {
+  typedef Lanes<P, MODE> LN;
+  constexpr int V = MODE == MX_F64 ? 2 : 4;
+  const int v = t % V, lane = (t / V) % 64, eq = (t / (V * 64)) % (LN::KL / V), rb = t / (V * 64 * (LN::KL / V));
+  r = 16 * rb + (lane & 15);
+  k = LN::k_of(eq * V + v, lane >> 4);
+}
+
+// FP64 operator of each key in fragment order; TR: the transposed operator
+// (A = T_K^T, the adjoint)
+template <int P, bool TR>
 __global__ void k_mx_operators64(const double* __restrict__ gtab, uint32_t nk, double* __restrict__ t64) {
-  constexpr int S = num_moments(P), K = 16 * row_blocks(P), ST = 2 * P + 1, GS = ST * ST;
+  typedef Lanes<P, MX_F64> LN;
+  constexpr int S = num_moments(P), ST = 2 * P + 1, GS = ST * ST;
   uint32_t key = blockIdx.x;
   if (key >= nk) return;
   const double* G = gtab + (size_t)key * GS;
   double fact[P + 1];
   fact[0] = 1.0;
   for (int q = 1; q <= P; ++q) fact[q] = fact[q - 1] * q;
-  for (int t = threadIdx.x; t < K * K; t += blockDim.x) {
-    int r = t / K, c = t % K;
+  for (int t = threadIdx.x; t < LN::PER_KEY; t += blockDim.x) {
+    int r, c;
+    frag_pos<P, MX_F64>(t, r, c);
+    if (TR) {
+      const int q = r;
+      r = c;
+      c = q;
+    }
     double v = 0.0;
     if (r < S && c < S) {
       int I, J, i, j;
@@ -349,16 +373,17 @@ __global__ void k_mx_operators64(const double* __restrict__ gtab, uint32_t nk, d
       v = w * G[(i + I + j + J) * ST + (j + J)];
       if (!isfinite(v)) v = 0.0;
     }
-    t64[(size_t)key * K * K + t] = v;
+    t64[(size_t)key * LN::PER_KEY + t] = v;
   }
 }
 
-// scaled FP32 operator of each key and the key's inverse scale powers sig^-n,
-// n = 0..P+1 (FP64)
-template <int P>
+// scaled FP32 operator of each key in the fragment order of MODE, and the
+// key's scale exponent
+template <int P, int MODE>
 __global__ void k_mx_operators32(const double* __restrict__ gtab, const double* __restrict__ dxy, uint32_t nk,
                                  double eps2, float* __restrict__ that, int* __restrict__ sig_exp) {
-  constexpr int S = num_moments(P), K = 16 * row_blocks(P), ST = 2 * P + 1, GS = ST * ST;
+  typedef Lanes<P, MODE> LN;
+  constexpr int S = num_moments(P), ST = 2 * P + 1, GS = ST * ST;
   uint32_t key = blockIdx.x;
   if (key >= nk) return;
   const double dx = dxy[2 * key], dy = dxy[2 * key + 1];
@@ -374,8 +399,9 @@ __global__ void k_mx_operators32(const double* __restrict__ gtab, const double* 
   double fact[P + 1];
   fact[0] = 1.0;
   for (int q = 1; q <= P; ++q) fact[q] = fact[q - 1] * q;
-  for (int t = threadIdx.x; t < K * K; t += blockDim.x) {
-    int r = t / K, c = t % K;
+  for (int t = threadIdx.x; t < LN::PER_KEY; t += blockDim.x) {
+    int r, c;
+    frag_pos<P, MODE>(t, r, c);
     float v = 0.f;
     if (r < S && c < S) {
       int I, J, i, j;
@@ -386,7 +412,7 @@ __global__ void k_mx_operators32(const double* __restrict__ gtab, const double* 
       v = (float)(w * g * pw[i + j] * pw[I + J + 1]);
       if (!isfinite(v)) v = 0.f;
     }
-    that[(size_t)key * K * K + t] = v;
+    that[(size_t)key * LN::PER_KEY + t] = v;
   }
 }
 
@@ -480,34 +506,31 @@ __global__ void __launch_bounds__(256) k_m2l_tiles(TileArgs a) {
     // (their address needs only the key), then the source rows (a missing id
     // reads row 0 and is masked; padding coefficients k >= S read k = S - 1)
     constexpr int KC = LN::KC;
-    constexpr int TW = MODE == MX_BF16X3 ? KC * 8 : KS;
     typedef typename std::conditional<MODE == MX_F64, double, float>::type TT;
-    TT tv[RB][TW];
+    // the operator in fragment order: slot e of row block rb for this lane
+    TT tv[RB][KL];
     if (MODE == MX_F64) {
-      const double* T = a.t64 + (size_t)key * K * K;
+      const double2* T = reinterpret_cast<const double2*>(a.t64 + (size_t)key * LN::PER_KEY);
 #pragma unroll
       for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-        for (int e = 0; e < KS; ++e)
-          tv[rb][e] = (TT)(TR ? T[(size_t)(kq * KS + e) * K + rb * 16 + col] : T[(size_t)(rb * 16 + col) * K + kq * KS + e]);
-    } else {
-      const float* T = a.that + (size_t)key * K * K;
-#pragma unroll
-      for (int rb = 0; rb < RB; ++rb) {
-        const float* Tr = T + (size_t)(rb * 16 + col) * K;
-        if (MODE == MX_F32) {
-#pragma unroll
-          for (int e = 0; e < KS; ++e) tv[rb][e] = (TT)Tr[kq * KS + e];
-        } else {
-#pragma unroll
-          for (int kc = 0; kc < KC; ++kc)
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-              const int k = kc * 32 + kq * 8 + e;
-              tv[rb][kc * 8 + e] = (TT)(k < K ? Tr[k] : 0.f);
-            }
+        for (int eh = 0; eh < KL / 2; ++eh) {
+          const double2 w = T[(rb * (KL / 2) + eh) * 64 + lane];
+          tv[rb][2 * eh] = (TT)w.x;
+          tv[rb][2 * eh + 1] = (TT)w.y;
         }
-      }
+    } else {
+      const float4* T = reinterpret_cast<const float4*>(a.that + (size_t)key * LN::PER_KEY);
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int eq = 0; eq < KL / 4; ++eq) {
+          const float4 w = T[(rb * (KL / 4) + eq) * 64 + lane];
+          tv[rb][4 * eq] = (TT)w.x;
+          tv[rb][4 * eq + 1] = (TT)w.y;
+          tv[rb][4 * eq + 2] = (TT)w.z;
+          tv[rb][4 * eq + 3] = (TT)w.w;
+        }
     }
     const int sx = MODE == MX_F64 ? 0 : a.sig_exp[key];
     // missing ids and padding coefficients k >= S load nothing (inactive
@@ -873,24 +896,39 @@ static int build_tiles(galax_fmm* h, const uint32_t* row_of_pair, const uint32_t
   return 0;
 }
 
-template <int P>
+template <int P, bool TR>
 static int operators64(galax_fmm* h) {
-  constexpr int K = 16 * row_blocks(P);
+  constexpr int PK = Lanes<P, MX_F64>::PER_KEY;
   const uint32_t nk = (uint32_t)h->n_keys;
+  DBuf& buf = TR ? h->mx_t64T : h->mx_t64;
   if (nk) {
-    GX_CHECK(dbuf_reserve(h->mx_t64, (size_t)nk * K * K * 8, h->stream));
-    k_mx_operators64<P><<<nk, 256, 0, h->stream>>>(h->gtab.as<double>(), nk, h->mx_t64.as<double>());
+    GX_CHECK(dbuf_reserve(buf, (size_t)nk * PK * 8, h->stream));
+    k_mx_operators64<P, TR><<<nk, 256, 0, h->stream>>>(h->gtab.as<double>(), nk, buf.as<double>());
     GX_HIP(hipGetLastError());
   }
-  h->mx_t64_valid = true;
+  return 0;
+}
+
+template <int P, int MODE>
+static int operators32(galax_fmm* h) {
+  constexpr int PK = Lanes<P, MODE>::PER_KEY;
+  const uint32_t nk = (uint32_t)h->n_keys;
+  hipStream_t s = h->stream;
+  if (nk) {
+    GX_CHECK(dbuf_reserve(h->mx_that, (size_t)nk * PK * 4, s));
+    GX_CHECK(dbuf_reserve(h->mx_sig, (size_t)nk * 4, s));
+    k_mx_operators32<P, MODE><<<nk, 256, 0, s>>>(h->gtab.as<double>(), h->key_dxy.as<double>(), nk,
+                                                 h->cfg.softening * h->cfg.softening, h->mx_that.as<float>(),
+                                                 h->mx_sig.as<int>());
+    GX_HIP(hipGetLastError());
+  }
   return 0;
 }
 
 template <int P>
 static int setup_impl(galax_fmm* h) {
-  constexpr int K = 16 * row_blocks(P);
   hipStream_t s = h->stream;
-  const uint32_t nk = (uint32_t)h->n_keys, ne = (uint32_t)h->n_live_pairs, nt = (uint32_t)h->n_live_targets;
+  const uint32_t ne = (uint32_t)h->n_live_pairs, nt = (uint32_t)h->n_live_targets;
   GX_CHECK(dbuf_reserve(h->ws.f, (size_t)std::max<uint32_t>(ne, 1) * 4, s));
   uint32_t* tgt = h->ws.f.as<uint32_t>();
   if (nt) {
@@ -899,14 +937,11 @@ static int setup_impl(galax_fmm* h) {
   }
   GX_CHECK(build_tiles(h, tgt, h->tree.m2l_dat.as<uint32_t>(), nt, h->mx_fwd));
   if (h->cfg.precision == GALAX_PREC_FP64) {
-    GX_CHECK(operators64<P>(h));
-  } else if (nk) {
-    GX_CHECK(dbuf_reserve(h->mx_that, (size_t)nk * K * K * 4, s));
-    GX_CHECK(dbuf_reserve(h->mx_sig, (size_t)nk * 4, s));
-    k_mx_operators32<P><<<nk, 256, 0, s>>>(h->gtab.as<double>(), h->key_dxy.as<double>(), nk,
-                                           h->cfg.softening * h->cfg.softening, h->mx_that.as<float>(),
-                                           h->mx_sig.as<int>());
-    GX_HIP(hipGetLastError());
+    GX_CHECK((operators64<P, false>(h)));
+  } else if (h->cfg.precision == GALAX_PREC_MIXED_BF16X3) {
+    GX_CHECK((operators32<P, MX_BF16X3>(h)));
+  } else {
+    GX_CHECK((operators32<P, MX_F32>(h)));
   }
   GX_HIP(hipStreamSynchronize(s));
   h->mx_valid = true;
@@ -925,7 +960,7 @@ static int adjoint_setup_impl(galax_fmm* h) {
   }
   // rows are sources; every source of a live pair is a live node (same level)
   GX_CHECK(build_tiles(h, h->tree.m2l_dat.as<uint32_t>(), tgt, nt, h->mx_adj));
-  if (!h->mx_t64_valid) GX_CHECK(operators64<P>(h));
+  GX_CHECK((operators64<P, true>(h)));
   GX_HIP(hipStreamSynchronize(s));
   h->mx_adj_valid = true;
   return 0;
@@ -971,7 +1006,7 @@ static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
     GX_HIP(hipMemsetAsync(h->mx_zero.p, 0, 16 * row_blocks(MAX_P) * 8, s));
   }
   a.zero = h->mx_zero.as<double>();
-  a.t64 = h->mx_t64.as<double>();
+  a.t64 = (TR ? h->mx_t64T : h->mx_t64).as<double>();
   a.that = h->mx_that.as<float>();
   a.sig_exp = h->mx_sig.as<int>();
   a.in = in;

@@ -780,8 +780,14 @@ int build_tree_constrained(galax_fmm* h, size_t n) {
 struct SegLevel {
   uint32_t base, p2, lo, n;  // heap entries [base, base + 2 p2); node ids [lo, lo + n)
 };
+// One 64-byte record per segment (a query reads it with two 32-byte loads
+// from one cache line): the box union x0, x1, y0, y1, then the children's
+// extremes ax = max(q.c - q.hw), bx = min(q.c + q.hw) in x, ay, by in y.
+struct SegRec {
+  double x0, x1, y0, y1, ax, bx, ay, by;
+};
 struct SegTree {
-  double *x0, *x1, *y0, *y1, *ax, *bx, *ay, *by;
+  SegRec* r;
 };
 
 __global__ void k_seg_leaves(NodeArrays a, const SegLevel* __restrict__ lvls, uint32_t nlv, SegTree st) {
@@ -795,18 +801,11 @@ __global__ void k_seg_leaves(NodeArrays a, const SegLevel* __restrict__ lvls, ui
   const size_t e = (size_t)L.base + L.p2 + k;
   const double inf = __builtin_huge_val();
   if (k >= L.n) {  // padding: meets nothing, no child
-    st.x0[e] = st.y0[e] = inf;
-    st.x1[e] = st.y1[e] = -inf;
-    st.ax[e] = st.ay[e] = -inf;
-    st.bx[e] = st.by[e] = inf;
+    st.r[e] = SegRec{inf, -inf, inf, -inf, -inf, inf, -inf, inf};
     return;
   }
   const uint32_t nd = L.lo + k;
   const double cx = a.cx[nd], cy = a.cy[nd], hw = a.hw[nd];
-  st.x0[e] = cx - hw;
-  st.x1[e] = cx + hw;
-  st.y0[e] = cy - hw;
-  st.y1[e] = cy + hw;
   double ax = -inf, bx = inf, ay = -inf, by = inf;
   const uint32_t cf = a.cf[nd];
   if (cf != NONE)
@@ -817,10 +816,7 @@ __global__ void k_seg_leaves(NodeArrays a, const SegLevel* __restrict__ lvls, ui
       ay = fmax(ay, qy - qh);
       by = fmin(by, qy + qh);
     }
-  st.ax[e] = ax;
-  st.bx[e] = bx;
-  st.ay[e] = ay;
-  st.by[e] = by;
+  st.r[e] = SegRec{cx - hw, cx + hw, cy - hw, cy + hw, ax, bx, ay, by};
 }
 
 // internal entries at height h (1 = parents of the leaf slots) of every level
@@ -831,15 +827,9 @@ __global__ void k_seg_up(const SegLevel* __restrict__ lvls, uint32_t nlv, uint32
     const uint32_t cnt = lvls[l].p2 >> h;
     if (i < cnt) {
       const size_t e = (size_t)lvls[l].base + cnt + i;
-      const size_t a = (size_t)lvls[l].base + 2 * (cnt + i), b = a + 1;
-      st.x0[e] = fmin(st.x0[a], st.x0[b]);
-      st.x1[e] = fmax(st.x1[a], st.x1[b]);
-      st.y0[e] = fmin(st.y0[a], st.y0[b]);
-      st.y1[e] = fmax(st.y1[a], st.y1[b]);
-      st.ax[e] = fmax(st.ax[a], st.ax[b]);
-      st.bx[e] = fmin(st.bx[a], st.bx[b]);
-      st.ay[e] = fmax(st.ay[a], st.ay[b]);
-      st.by[e] = fmin(st.by[a], st.by[b]);
+      const SegRec A = st.r[(size_t)lvls[l].base + 2 * (cnt + i)], B = st.r[(size_t)lvls[l].base + 2 * (cnt + i) + 1];
+      st.r[e] = SegRec{fmin(A.x0, B.x0), fmax(A.x1, B.x1), fmin(A.y0, B.y0), fmax(A.y1, B.y1),
+                       fmax(A.ax, B.ax), fmin(A.bx, B.bx), fmax(A.ay, B.ay), fmin(A.by, B.by)};
       return;
     }
     i -= cnt;
@@ -851,7 +841,8 @@ constexpr double SEG_SLACK = 1e-13;
 // Can the segment hold a node whose box meets [x0, x1] x [y0, y1]?
 __device__ __forceinline__ bool seg_meets(const SegTree& st, size_t e, double x0, double x1, double y0, double y1,
                                           double mag) {
-  const double sx0 = st.x0[e], sx1 = st.x1[e], sy0 = st.y0[e], sy1 = st.y1[e];
+  const double4 b = *reinterpret_cast<const double4*>(&st.r[e].x0);
+  const double sx0 = b.x, sx1 = b.y, sy0 = b.z, sy1 = b.w;
   if (!(sx0 <= sx1)) return false;  // empty
   const double sl = SEG_SLACK * (mag + fmax(fmax(fabs(sx0), fabs(sx1)), fmax(fabs(sy0), fabs(sy1))));
   return sx1 >= x0 - sl && sx0 <= x1 + sl && sy1 >= y0 - sl && sy0 <= y1 + sl;
@@ -861,14 +852,21 @@ __device__ __forceinline__ bool seg_meets(const SegTree& st, size_t e, double x0
 // width ch)?
 __device__ __forceinline__ bool seg_far_child(const SegTree& st, size_t e, double cx, double cy, double ch,
                                               double mag) {
-  const double ax = st.ax[e], bx = st.bx[e], ay = st.ay[e], by = st.by[e];
+  const double4 c = *reinterpret_cast<const double4*>(&st.r[e].ax);
+  const double ax = c.x, bx = c.y, ay = c.z, by = c.w;
   if (ax == -__builtin_huge_val()) return false;  // no child in the segment
   const double sl = SEG_SLACK * (mag + fmax(fmax(fabs(ax), fabs(bx)), fmax(fabs(ay), fabs(by))));
   const double r = ch + 1e-15;
   return ax > cx + r - sl || bx < cx - r + sl || ay > cy + r - sl || by < cy - r + sl;
 }
 
-constexpr int SEG_STACK = 40;
+// The walk needs no stack: in a heap (children of s are 2s and 2s + 1) the
+// subtree after s in depth-first, ascending order starts at the right
+// sibling of s or of its nearest ancestor that is a left child; 0 ends it.
+__device__ __forceinline__ uint32_t seg_next(uint32_t s) {
+  while (s & 1u) s >>= 1;
+  return s ? s + 1 : 0u;
+}
 
 // M2L rows: one thread per node c below the root.
 __global__ void __launch_bounds__(256) k_m2l_rows_seg(NodeArrays a, uint32_t nn, const SegLevel* __restrict__ lvls,
@@ -890,16 +888,13 @@ __global__ void __launch_bounds__(256) k_m2l_rows_seg(NodeArrays a, uint32_t nn,
   const double px0 = pcx - phw - 1e-15, px1 = pcx + phw + 1e-15, py0 = pcy - phw - 1e-15, py1 = pcy + phw + 1e-15;
   uint32_t* row = rows + (size_t)c * LIST_CAP;
   uint32_t len = 0;
-  uint32_t stack[SEG_STACK];
-  int sp = 0;
-  stack[sp++] = 1;
-  while (sp > 0 && len < LIST_CAP) {
-    const uint32_t s = stack[--sp];
+  uint32_t s = 1;
+  while (s && len < LIST_CAP) {
     if (s >= L.p2) {  // a single node
       const uint32_t k = s - L.p2;
-      if (k >= L.n) continue;
       const uint32_t pn = L.lo + k;
-      if (pn == P) continue;
+      s = seg_next(s);
+      if (k >= L.n || pn == P) continue;
       // galax-core/src/lib.rs:991-997
       const double dx = fabs(a.cx[pn] - pcx), dy = fabs(a.cy[pn] - pcy);
       const double h_sum = phw + a.hw[pn];
@@ -915,11 +910,8 @@ __global__ void __launch_bounds__(256) k_m2l_rows_seg(NodeArrays a, uint32_t nn,
       continue;
     }
     const size_t e = (size_t)L.base + s;
-    if (!seg_meets(st, e, px0, px1, py0, py1, magp) || !seg_far_child(st, e, ccx, ccy, chw, magc)) continue;
-    if (sp + 2 <= SEG_STACK) {
-      stack[sp++] = 2 * s + 1;  // right half after the left one
-      stack[sp++] = 2 * s;
-    }
+    s = (seg_meets(st, e, px0, px1, py0, py1, magp) && seg_far_child(st, e, ccx, ccy, chw, magc)) ? 2 * s
+                                                                                                   : seg_next(s);
   }
   lens[c] = len;
 }
@@ -938,24 +930,18 @@ __global__ void __launch_bounds__(256) k_p2p_rows_seg(NodeArrays a, uint32_t n_l
   uint32_t* row = rows + (size_t)c * LIST_CAP;
   row[0] = c;
   uint32_t len = 1;
-  uint32_t stack[SEG_STACK];
-  int sp = 0;
-  stack[sp++] = 1;
-  while (sp > 0 && len < LIST_CAP) {
-    const uint32_t s = stack[--sp];
+  uint32_t s = 1;
+  while (s && len < LIST_CAP) {
     if (s >= L.p2) {
-      const uint32_t b = L.lo + (s - L.p2);
-      if (s - L.p2 >= L.n || b == c) continue;
+      const uint32_t k = s - L.p2, b = L.lo + k;
+      s = seg_next(s);
+      if (k >= L.n || b == c) continue;
       const double dx = fabs(cx - a.cx[b]), dy = fabs(cy - a.cy[b]);
       const double hs = hw + a.hw[b];
       if (dx <= hs + 1e-15 && dy <= hs + 1e-15) row[len++] = b;
       continue;
     }
-    if (!seg_meets(st, (size_t)L.base + s, x0, x1, y0, y1, mag)) continue;
-    if (sp + 2 <= SEG_STACK) {
-      stack[sp++] = 2 * s + 1;
-      stack[sp++] = 2 * s;
-    }
+    s = seg_meets(st, (size_t)L.base + s, x0, x1, y0, y1, mag) ? 2 * s : seg_next(s);
   }
   lens[c] = len;
 }
@@ -1235,38 +1221,58 @@ __device__ __forceinline__ bool sym_owns(const uint32_t* __restrict__ goff, uint
   return gt > gb || (gt == gb && t < b);
 }
 
-// Per leaf: owned partners and the items they need.
-__global__ void k_sym_count(const uint32_t* __restrict__ goff, const uint32_t* __restrict__ gdat, uint32_t nl,
-                            uint32_t* __restrict__ nown, uint32_t* __restrict__ nit) {
+// Row of CSR entry g: the last r with off[r] <= g (off ascending, off[0] = 0)
+__device__ __forceinline__ uint32_t csr_row(const uint32_t* __restrict__ off, uint32_t nrows, uint32_t g) {
+  uint32_t lo = 0, hi = nrows;  // off[lo] <= g < off[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= g) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// One thread per gather entry (lists of hub leaves run to thousands of
+// entries): does the entry's leaf own the pair? Entry ng (the last thread)
+// writes 0, so the exclusive scan ends with the number of owned pairs.
+__global__ void k_sym_flag(const uint32_t* __restrict__ goff, const uint32_t* __restrict__ gdat, uint32_t nl,
+                           uint32_t ng, uint32_t* __restrict__ flag) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g > ng) return;
+  uint32_t f = 0;
+  if (g < ng) {
+    const uint32_t t = csr_row(goff, nl, g), b = gdat[g];
+    f = (b != t && sym_owns(goff, t, b)) ? 1u : 0u;
+  }
+  flag[g] = f;
+}
+
+// Per leaf: first owned pair (the scan at its first gather entry) and the
+// number of items (at least one: the self block).
+__global__ void k_sym_offsets(const uint32_t* __restrict__ goff, const uint32_t* __restrict__ pos, uint32_t nl,
+                              uint32_t* __restrict__ own_off, uint32_t* __restrict__ nit) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t > nl) return;
-  uint32_t c = 0;
-  if (t < nl)
-    for (uint32_t g = goff[t]; g < goff[t + 1]; ++g) {
-      const uint32_t b = gdat[g];
-      c += (b != t && sym_owns(goff, t, b)) ? 1u : 0u;
-    }
-  nown[t] = c;
-  nit[t] = t < nl ? max(1u, (c + SYM_ITEM_PARTNERS - 1) / SYM_ITEM_PARTNERS) : 0u;
+  own_off[t] = pos[goff[t]];
+  if (t < nl) {
+    const uint32_t c = pos[goff[t + 1]] - pos[goff[t]];
+    nit[t] = max(1u, (c + SYM_ITEM_PARTNERS - 1) / SYM_ITEM_PARTNERS);
+  } else {
+    nit[t] = 0;
+  }
 }
 
 // Owned partners in gather-list order; each pair's key (partner, owner)
 // sorts the non-owned pairs of every leaf by owner.
 __global__ void k_sym_own_fill(const uint32_t* __restrict__ goff, const uint32_t* __restrict__ gdat, uint32_t nl,
-                               const uint32_t* __restrict__ own_off, uint32_t* __restrict__ own_dat, int w,
-                               uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nl) return;
-  uint32_t u = own_off[t];
-  for (uint32_t g = goff[t]; g < goff[t + 1]; ++g) {
-    const uint32_t b = gdat[g];
-    if (b != t && sym_owns(goff, t, b)) {
-      own_dat[u] = b;
-      key[u] = ((uint64_t)b << w) | t;
-      val[u] = u;
-      ++u;
-    }
-  }
+                               uint32_t ng, const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                               uint32_t* __restrict__ own_dat, int w, uint64_t* __restrict__ key,
+                               uint32_t* __restrict__ val) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng || !flag[g]) return;
+  const uint32_t t = csr_row(goff, nl, g), b = gdat[g], u = pos[g];
+  own_dat[u] = b;
+  key[u] = ((uint64_t)b << w) | t;
+  val[u] = u;
 }
 
 __global__ void k_sym_low_split(const uint64_t* __restrict__ key, uint32_t np, int w, uint32_t* __restrict__ cnt) {
@@ -1367,15 +1373,26 @@ int build_sym_p2p(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, uint
   uint32_t* ch_off = ref_off + L1;
   uint32_t* part_off = ch_off + L1;
   uint32_t* heavy_off = part_off + L1;
-  k_sym_count<<<nblk(L1), 256, 0, s>>>(goff, gdat, nl, own_off, item_off);
+  // gather entries: owned flags, their exclusive scan, per-leaf offsets
+  uint32_t ng = 0;
+  GX_HIP(hipMemcpyAsync(&ng, goff + nl, 4, hipMemcpyDeviceToHost, s));
+  GX_HIP(hipStreamSynchronize(s));
+  ScopedDBuf gflag;
+  GX_CHECK(dbuf_reserve(gflag, ((size_t)ng + 1) * 8, s));
+  uint32_t* flag = gflag.as<uint32_t>();
+  uint32_t* pos = flag + ng + 1;
+  k_sym_flag<<<nblk((size_t)ng + 1), 256, 0, s>>>(goff, gdat, nl, ng, flag);
+  GX_HIP(hipGetLastError());
+  GX_CHECK(scan_u32(h, flag, pos, (size_t)ng + 1, false, false, nullptr));
+  k_sym_offsets<<<nblk(L1), 256, 0, s>>>(goff, pos, nl, own_off, item_off);
   GX_HIP(hipGetLastError());
   uint64_t np64 = 0, ni64 = 0;
-  GX_CHECK(sum_u32(h, own_off, L1, &np64));
+  GX_CHECK(sum_u32(h, flag, (size_t)ng + 1, &np64));
   GX_CHECK(sum_u32(h, item_off, L1, &ni64));
   if ((np64 + 1) * SYM_LEAF >= 0xFFFFFFF0ull || (ni64 + 1) * SYM_LEAF >= 0xFFFFFFF0ull)
     return set_error(GALAX_ERR_UNSUPPORTED, "symmetric near field: too many leaf pairs");
-  uint32_t np = 0, ni = 0;
-  GX_CHECK(scan_u32(h, own_off, own_off, L1, false, false, &np));
+  const uint32_t np = (uint32_t)np64;
+  uint32_t ni = 0;
   GX_CHECK(scan_u32(h, item_off, item_off, L1, false, false, &ni));
   GX_CHECK(dbuf_reserve(sp.own_off, L1 * 4, s));
   GX_HIP(hipMemcpyAsync(sp.own_off.p, own_off, L1 * 4, hipMemcpyDeviceToDevice, s));
@@ -1388,7 +1405,7 @@ int build_sym_p2p(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, uint
   GX_CHECK(dbuf_reserve(sp.own_dat, (size_t)np * 4 + 4, s));
   GX_HIP(hipMemsetAsync(low_off, 0, L1 * 4, s));
   const int w = bits_for(nl);
-  k_sym_own_fill<<<nblk(nl), 256, 0, s>>>(goff, gdat, nl, own_off, sp.own_dat.as<uint32_t>(), w,
+  k_sym_own_fill<<<nblk(ng), 256, 0, s>>>(goff, gdat, nl, ng, flag, pos, sp.own_dat.as<uint32_t>(), w,
                                           ws.sort_ka.as<uint64_t>(), ws.sort_va.as<uint32_t>());
   GX_HIP(hipGetLastError());
   DBuf low_slot;
@@ -1529,10 +1546,9 @@ int build_lists_reference(galax_fmm* h) {
     slots += p2;
     maxh = std::max(maxh, hgt);
   }
-  GX_CHECK(dbuf_reserve(h->ws.g, tot * 8 * sizeof(double) + t.n_levels * sizeof(SegLevel), s));
-  double* sb = h->ws.g.as<double>();
-  SegTree st{sb, sb + tot, sb + 2 * tot, sb + 3 * tot, sb + 4 * tot, sb + 5 * tot, sb + 6 * tot, sb + 7 * tot};
-  SegLevel* dlv = reinterpret_cast<SegLevel*>(sb + 8 * tot);
+  GX_CHECK(dbuf_reserve(h->ws.g, tot * sizeof(SegRec) + t.n_levels * sizeof(SegLevel), s));
+  SegTree st{h->ws.g.as<SegRec>()};
+  SegLevel* dlv = reinterpret_cast<SegLevel*>(st.r + tot);
   GX_HIP(hipMemcpyAsync(dlv, lv.data(), t.n_levels * sizeof(SegLevel), hipMemcpyHostToDevice, s));
   k_seg_leaves<<<nblk(slots), 256, 0, s>>>(a, dlv, t.n_levels, st);
   GX_HIP(hipGetLastError());
