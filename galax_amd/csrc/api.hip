@@ -422,7 +422,7 @@ int reserve_bodies(galax_fmm* h, size_t n) {
 int after_tree_installed(galax_fmm* h) {
   h->has_tree = true;
   h->keys_valid = false;
-  h->mx_valid = h->mx_adj_valid = false;
+  h->mx_valid = h->mx_adj_valid = h->mx_rows32 = false;
   h->corrected_ready = false;
   return 0;
 }
@@ -488,15 +488,15 @@ void galax_fmm_destroy(galax_fmm_t* h) {
   DBuf* bufs[] = {&h->x, &h->y, &h->m, &h->vx, &h->vy, &h->ax, &h->ay, &h->phi, &h->mp, &h->loc, &h->mp32,
                   &h->loc32, &h->gtab, &h->key_of_pair, &h->perm, &h->hash, &h->tmp_u32, &h->flag,
                   &h->c_m2l_off, &h->c_m2l_dat, &h->c_gat_off, &h->c_gat_dat, &h->c_dtt_pairs, &h->c_m2l_order,
-                  &h->key_dxy, &h->mx_t64, &h->mx_t64T, &h->mx_that, &h->mx_sig, &h->mx_zero,
+                  &h->key_dxy, &h->mx_t64, &h->mx_t64T, &h->mx_that, &h->mx_sig, &h->mx_zero, &h->mx_nexp,
                   &h->vj_lbar, &h->vj_mbar, &h->vj_leaf_of, &h->vj_fwd, &h->vj_out,
                   &h->mx_fwd.src, &h->mx_fwd.tile_off, &h->mx_fwd.step_key, &h->mx_fwd.step_first,
                   &h->mx_fwd.step_cnt, &h->mx_fwd.step_ids, &h->mx_fwd.item_tile, &h->mx_fwd.item_s0, &h->mx_fwd.item_s1,
                   &h->mx_fwd.item_slot, &h->mx_fwd.multi_tile, &h->mx_fwd.multi_slot, &h->mx_fwd.multi_cnt,
-                  &h->mx_fwd.part, &h->mx_fwd.lr_first, &h->mx_fwd.lr_len, &h->mx_fwd.lr_order, &h->mx_fwd.hub_first, &h->mx_fwd.hub_len, &h->mx_fwd.hub_coff, &h->mx_fwd.hub_part, &h->mx_fwd.virt, &h->mx_fwd.hubs, &h->mx_adj.src, &h->mx_adj.tile_off, &h->mx_adj.step_key,
+                  &h->mx_fwd.part, &h->mx_fwd.lr_first, &h->mx_fwd.lr_len, &h->mx_fwd.lr_order, &h->mx_fwd.hub_first, &h->mx_fwd.hub_len, &h->mx_fwd.hub_coff, &h->mx_fwd.hub_part, &h->mx_fwd.virt, &h->mx_fwd.hubs, &h->mx_fwd.lr_key, &h->mx_fwd.virt32, &h->mx_fwd.step_sh, &h->mx_adj.src, &h->mx_adj.tile_off, &h->mx_adj.step_key,
                   &h->mx_adj.step_first, &h->mx_adj.step_cnt, &h->mx_adj.step_ids, &h->mx_adj.item_tile, &h->mx_adj.item_s0,
                   &h->mx_adj.item_s1, &h->mx_adj.item_slot, &h->mx_adj.multi_tile, &h->mx_adj.multi_slot,
-                  &h->mx_adj.multi_cnt, &h->mx_adj.part, &h->mx_adj.lr_first, &h->mx_adj.lr_len, &h->mx_adj.lr_order, &h->mx_adj.hub_first, &h->mx_adj.hub_len, &h->mx_adj.hub_coff, &h->mx_adj.hub_part, &h->mx_adj.virt, &h->mx_adj.hubs};
+                  &h->mx_adj.multi_cnt, &h->mx_adj.part, &h->mx_adj.lr_first, &h->mx_adj.lr_len, &h->mx_adj.lr_order, &h->mx_adj.hub_first, &h->mx_adj.hub_len, &h->mx_adj.hub_coff, &h->mx_adj.hub_part, &h->mx_adj.virt, &h->mx_adj.hubs, &h->mx_adj.lr_key, &h->mx_adj.virt32, &h->mx_adj.step_sh};
   for (DBuf* b : bufs) dbuf_free(*b);
   DevTree& t = h->tree;
   DBuf* tb[] = {&t.center_x, &t.center_y, &t.half_width, &t.mass, &t.com_x, &t.com_y, &t.body_start, &t.body_end,
@@ -636,7 +636,7 @@ int galax_fmm_set_softening(galax_fmm_t* h, double softening) {
     return set_error(GALAX_ERR_INVALID, "softening must be finite and >= 0");
   h->cfg.softening = softening;
   h->keys_valid = false;  // derivative tables depend on eps
-  h->mx_valid = h->mx_adj_valid = false;
+  h->mx_valid = h->mx_adj_valid = h->mx_rows32 = false;
   return 0;
 }
 

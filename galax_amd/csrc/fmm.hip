@@ -91,7 +91,8 @@ __global__ void __launch_bounds__(256) k_p2m(double* __restrict__ x, double* __r
                                              const double* __restrict__ comx, const double* __restrict__ comy,
                                              uint32_t n_leaves, double* __restrict__ mp,
                                              uint32_t* __restrict__ changed, uint32_t* __restrict__ next,
-                                             float4* __restrict__ rel) {
+                                             float4* __restrict__ rel, float* __restrict__ mp32,
+                                             const int8_t* __restrict__ nexp) {
 #pragma clang fp contract(off)
   constexpr int S = num_moments(P);
   constexpr int LPW = S <= 32 ? 2 : 1;  // leaves per wave
@@ -163,6 +164,13 @@ __global__ void __launch_bounds__(256) k_p2m(double* __restrict__ x, double* __r
       }
     }
     if (live && k < S) mp[(size_t)leaf * S + k] = isfinite(acc) ? acc : 0.0;
+    if (mp32 && live) {  // the mixed forms' M2L rows: M_ij 2^(-e (i+j)), zero padded
+      constexpr int K32 = 16 * ((S + 15) / 16);
+      if (k < S)
+        mp32[(size_t)leaf * K32 + k] = (float)ldexp(isfinite(acc) ? acc : 0.0, -(int)nexp[leaf] * (i + j));
+      else if (k < K32)
+        mp32[(size_t)leaf * K32 + k] = 0.f;
+    }
   }
 }
 
@@ -177,7 +185,8 @@ __global__ void __launch_bounds__(256) k_p2m(double* __restrict__ x, double* __r
 template <int P>
 __global__ void __launch_bounds__(256) k_m2m(const uint32_t* __restrict__ cf, const uint8_t* __restrict__ cc,
                                              const double* __restrict__ comx, const double* __restrict__ comy,
-                                             uint32_t lo, uint32_t hi, double* __restrict__ mp) {
+                                             uint32_t lo, uint32_t hi, double* __restrict__ mp,
+                                             float* __restrict__ mp32, const int8_t* __restrict__ nexp) {
 #pragma clang fp contract(off)
   constexpr int S = num_moments(P);
   constexpr int PPW = S <= 32 ? 2 : 1;  // parents per wave
@@ -255,6 +264,16 @@ __global__ void __launch_bounds__(256) k_m2m(const uint32_t* __restrict__ cf, co
     for (int r = 0; r < (S + LW - 1) / LW; ++r) {
       const int k = sl + r * LW;
       if (k < S) mp[(size_t)p * S + k] = isfinite(acc[r]) ? acc[r] : 0.0;  // non-finite -> 0 (galax-core/src/lib.rs:934-936)
+      if (mp32) {  // the mixed forms' M2L rows: M_ij 2^(-e (i+j)), zero padded
+        constexpr int K32 = 16 * ((S + 15) / 16);
+        if (k < S) {
+          int oi, oj;
+          coef_ij<P>(k, oi, oj);
+          mp32[(size_t)p * K32 + k] = (float)ldexp(isfinite(acc[r]) ? acc[r] : 0.0, -(int)nexp[p] * (oi + oj));
+        } else if (k < K32) {
+          mp32[(size_t)p * K32 + k] = 0.f;
+        }
+      }
     }
   }
 }
@@ -1575,13 +1594,17 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
     }
     sym = &sp;
   }
+  // the mixed forms' M2L tiles read FP32 multipole rows, written by P2M and M2M
+  const bool rows32 = !corrected && h->mx_rows32 && h->cfg.precision != GALAX_PREC_FP64 &&
+                      h->cfg.precision != GALAX_PREC_FP64_EXACT;
   if (h->timing) GX_HIP(hipEventRecord(ev[1], s));
   {
     uint32_t* changed = h->flag.as<uint32_t>() + 4 + h->san_cur;
     uint32_t* next = h->flag.as<uint32_t>() + 4 + (h->san_cur ^ 1u);
     k_p2m<P><<<nblk(nl, num_moments(P) <= 32 ? 8 : 4), 256, 0, s>>>(
         d_x, d_y, d_m, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), comx, comy, (uint32_t)nl, mp, changed,
-        next, sym ? sym->rel.as<float4>() : nullptr);
+        next, sym ? sym->rel.as<float4>() : nullptr, rows32 ? h->mp32.as<float>() : nullptr,
+        rows32 ? h->mx_nexp.as<int8_t>() : nullptr);
     GX_HIP(hipGetLastError());
   }
   if (h->timing) GX_HIP(hipEventRecord(ev[2], s));
@@ -1597,7 +1620,8 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
         }
       }
       k_m2m<P><<<nblk(hi - lo, num_moments(P) <= 32 ? 8 : 4), 256, 0, s>>>(
-          t.child_first.as<uint32_t>(), t.child_count.as<uint8_t>(), comx, comy, lo, hi, mp);
+          t.child_first.as<uint32_t>(), t.child_count.as<uint8_t>(), comx, comy, lo, hi, mp,
+          rows32 && l == 1 ? h->mp32.as<float>() : nullptr, rows32 ? h->mx_nexp.as<int8_t>() : nullptr);
       GX_HIP(hipGetLastError());
     }
   }
@@ -1729,7 +1753,7 @@ int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, 
   if (!corrected && !h->keys_valid) {
     uint32_t top = std::min<uint32_t>(t.n_levels, 2);
     GX_CHECK(m2l_keys_setup(h, t.level_offset[top]));
-    h->mx_valid = h->mx_adj_valid = false;
+    h->mx_valid = h->mx_adj_valid = h->mx_rows32 = false;
   }
   if (!corrected && h->cfg.precision != GALAX_PREC_FP64_EXACT && !h->mx_valid) GX_CHECK(m2l_mfma_setup(h));
   GX_CHECK(dbuf_reserve(h->flag, 64, s));

@@ -76,18 +76,20 @@ __device__ __forceinline__ void coef_ij2(int k, int& i, int& j) {
 }
 
 // Contraction index carried by slot e (0 <= e < KL) of lane quarter kq:
-//     k = 8 (e / 2) + 2 kq + (e % 2).
+//     FP64 source rows (FP64 form):   k = 8 (e / 2) + 2 kq + (e % 2),
+//     FP32 source rows (mixed forms): k = 16 (e / 4) + 4 kq + (e % 4).
 // The 16x16x4 forms contract the KL = K/4 slots one per instruction; the
 // 16x16x32 bf16 form takes slots 8c .. 8c + 7 of each lane in instruction c.
-// Slots 2m and 2m + 1 of the four lane quarters cover k = 8m .. 8m + 7, so a
-// 16-byte load of a source row's pair (k, k + 1) per lane reads 64
-// contiguous bytes of each row per instruction.
+// A 16-byte load per lane of a source row's consecutive slots (2 doubles or 4
+// floats) then reads 64 contiguous bytes of each row per instruction.
 template <int P, int MODE>
 struct Lanes {
   static constexpr int S = num_moments(P), RB = row_blocks(P), K = 16 * RB;
   static constexpr int KC = (K + 31) / 32;
   static constexpr int KL = MODE == MX_BF16X3 ? KC * 8 : K / 4;
-  static __host__ __device__ __forceinline__ int k_of(int e, int kq) { return 8 * (e >> 1) + 2 * kq + (e & 1); }
+  static __host__ __device__ __forceinline__ int k_of(int e, int kq) {
+    return MODE == MX_F64 ? 8 * (e >> 1) + 2 * kq + (e & 1) : 16 * (e >> 2) + 4 * kq + (e & 3);
+  }
   // operator values per key, in fragment order (below)
   static constexpr int PER_KEY = RB * KL * 64;
 };
@@ -207,7 +209,8 @@ __global__ void k_mx_hub_list(const uint32_t* __restrict__ lr_len, const uint32_
 // long run's only entry is its pre-summed row, id n_in + its index
 __global__ void k_mx_step_ids(const uint32_t* __restrict__ step_grp, const uint32_t* __restrict__ gfirst,
                               const uint32_t* __restrict__ lrid, const uint32_t* __restrict__ src, size_t ns,
-                              uint32_t n_in, uint32_t* __restrict__ ids) {
+                              uint32_t n_in, const uint32_t* __restrict__ step_key, uint32_t* __restrict__ ids,
+                              uint32_t* __restrict__ lr_key) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ns * 16) return;
   const size_t st = i / 16, col = i % 16;
@@ -217,6 +220,7 @@ __global__ void k_mx_step_ids(const uint32_t* __restrict__ step_grp, const uint3
     f = gfirst[g];
     c = gfirst[g + 1] - f;
   }
+  if (c > MX_GROUP) lr_key[lrid[g]] = step_key[st];  // each long run is one (step, column)
   for (uint32_t q = 0; q < MX_GROUP; ++q) {
     uint32_t v = NONE;
     if (c > MX_GROUP)
@@ -225,6 +229,42 @@ __global__ void k_mx_step_ids(const uint32_t* __restrict__ step_grp, const uint3
       v = src[f + q];
     ids[(st * MX_GROUP + q) * 16 + col] = v;
   }
+}
+
+// Mixed forms. Exponent of each node's FP32 multipole row: the binary
+// exponent of its half width (rows hold M_ij 2^(-e (i+j)), i.e. the moments
+// in units of the node's size, far from FP32's range limits at any depth).
+__global__ void k_mx_node_exp(const double* __restrict__ hw, uint32_t nn, int8_t* __restrict__ nexp) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const double w = hw[i];
+  int e = (w > 0.0 && isfinite(w)) ? ilogb(w) : 0;
+  nexp[i] = (int8_t)min(max(e, -100), 100);
+}
+
+// Per (step, column): the shift e_s - a of each of its four source rows from
+// the row's own scale to the key's (a the key's exponent); 0 for a
+// pre-summed row, which is written at the key's scale. Packed as int8; a
+// shift outside +-127 (a node 2^127 times smaller or larger than its
+// distance) sets *err.
+__global__ void k_mx_step_shifts(const uint32_t* __restrict__ ids, const uint32_t* __restrict__ step_key,
+                                 const int* __restrict__ sig_exp, const int8_t* __restrict__ nexp, size_t ns,
+                                 uint32_t n_in, uint32_t* __restrict__ sh, uint32_t* __restrict__ err) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns * 16) return;
+  const size_t st = i / 16, col = i % 16;
+  const int a = sig_exp[step_key[st]];
+  uint32_t packed = 0;
+  for (uint32_t q = 0; q < MX_GROUP; ++q) {
+    const uint32_t id = ids[(st * MX_GROUP + q) * 16 + col];
+    int t = (id != NONE && id < n_in) ? (int)nexp[id] - a : 0;
+    if (t < -127 || t > 127) {
+      atomicOr(err, 1u);
+      t = 0;
+    }
+    packed |= (uint32_t)(uint8_t)(int8_t)t << (8 * q);
+  }
+  sh[i] = packed;
 }
 
 // Lanes per run of the pre-sums: a lane per coefficient pair when S is even
@@ -276,6 +316,51 @@ __global__ void __launch_bounds__(256) k_m2l_run_sums_w(const uint32_t* __restri
     }
     virt[(size_t)r * S + k] = acc;
   }
+}
+
+// Mixed forms: the same pre-sums from the FP32 node rows (K floats, zero past
+// S, each at its node's scale 2^(-e ord)): K/4 lanes per run, a lane per four
+// coefficients (one 16-byte load per row), each row rescaled to the key's
+// scale 2^(-a ord) and summed in FP64 in list order, rounded to FP32 once.
+// Runs of any length.
+template <int P>
+__host__ __device__ constexpr int sums32_lanes() {
+  return 4 * row_blocks(P);
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) k_m2l_run_sums32(const uint32_t* __restrict__ lr_order,
+                                                        const uint32_t* __restrict__ lr_first,
+                                                        const uint32_t* __restrict__ lr_len, uint32_t nlr,
+                                                        const uint32_t* __restrict__ src,
+                                                        const float* __restrict__ in32,
+                                                        const int8_t* __restrict__ nexp,
+                                                        const uint32_t* __restrict__ lr_key,
+                                                        const int* __restrict__ sig_exp, float* __restrict__ virt32) {
+  constexpr int K = 16 * row_blocks(P), LW = sums32_lanes<P>(), RPW = 64 / LW;
+  static_assert(LW <= 64, "a run's row must fit one wave");
+  const int lane = threadIdx.x & 63;
+  const uint32_t q = (xcd_runs(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6)) * RPW + lane / LW;
+  const int kl = lane % LW;
+  if (lane >= RPW * LW || q >= nlr) return;
+  const uint32_t r = lr_order[q], f = lr_first[r], len = lr_len[r];
+  const int a = sig_exp[lr_key[r]];
+  int o[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) o[j] = order_of<P>(4 * kl + j);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (uint32_t i = 0; i < len; ++i) {
+    const uint32_t sn = src[f + i];
+    const int t = (int)nexp[sn] - a;
+    const float4 v = *reinterpret_cast<const float4*>(in32 + (size_t)sn * K + 4 * kl);
+    acc[0] += ldexp((double)v.x, t * o[0]);
+    acc[1] += ldexp((double)v.y, t * o[1]);
+    acc[2] += ldexp((double)v.z, t * o[2]);
+    acc[3] += ldexp((double)v.w, t * o[3]);
+  }
+  *reinterpret_cast<float4*>(virt32 + (size_t)r * K + 4 * kl) =
+      make_float4((float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]);
 }
 
 // Hub runs in chunks of MX_HUB_CHUNK rows: sums_lanes(S) lanes per chunk sum
@@ -437,6 +522,12 @@ struct TileArgs {
   const double* in;  // multipoles (forward) or lbar (adjoint), S per node
   const double* virt;  // pre-summed long runs, S per run; ids n_in + r
   const double* zero;  // a row of zeros, read for missing ids
+  // mixed forms: FP32 source rows of K floats (node rows scaled by 2^(-e ord),
+  // pre-summed rows by the key's 2^(-a ord)) and per (step, column) the four
+  // sources' shifts e - a (int8, packed; 0 for a pre-summed row)
+  const float* in32;
+  const float* virt32;
+  const uint32_t* step_sh;
   uint32_t n_in;
   double* out;       // locals (forward) or mbar (adjoint), S per row
   double* part;      // partials of tiles with several items: [slot][16][S]
@@ -484,16 +575,17 @@ __global__ void __launch_bounds__(256) k_m2l_tiles(TileArgs a) {
   uint32_t st = s0 + w;
   // the step record (key, and the source ids of this lane's column) is
   // loaded one step ahead, so each step waits only on its multipole rows
-  uint32_t key_n = 0, id_n[MX_GROUP];
+  uint32_t key_n = 0, id_n[MX_GROUP], sh_n = 0;
 #pragma unroll
   for (uint32_t q = 0; q < MX_GROUP; ++q) id_n[q] = NONE;
   if (st < s1) {
     key_n = a.step_key[st];
 #pragma unroll
     for (uint32_t q = 0; q < MX_GROUP; ++q) id_n[q] = a.step_ids[((size_t)st * MX_GROUP + q) * 16 + col];
+    if (MODE != MX_F64) sh_n = a.step_sh[(size_t)st * 16 + col];
   }
   for (; st < s1; st += 4) {
-    const uint32_t key = key_n;
+    const uint32_t key = key_n, sh = sh_n;
     uint32_t id[MX_GROUP];
 #pragma unroll
     for (uint32_t q = 0; q < MX_GROUP; ++q) id[q] = id_n[q];
@@ -501,6 +593,7 @@ __global__ void __launch_bounds__(256) k_m2l_tiles(TileArgs a) {
       key_n = a.step_key[st + 4];
 #pragma unroll
       for (uint32_t q = 0; q < MX_GROUP; ++q) id_n[q] = a.step_ids[((size_t)(st + 4) * MX_GROUP + q) * 16 + col];
+      if (MODE != MX_F64) sh_n = a.step_sh[(size_t)(st + 4) * 16 + col];
     }
     // every load of the step is issued before any is used: the operator rows
     // (their address needs only the key), then the source rows (a missing id
@@ -536,58 +629,94 @@ __global__ void __launch_bounds__(256) k_m2l_tiles(TileArgs a) {
     // missing ids and padding coefficients k >= S load nothing (inactive
     // lanes make no cache accesses); the rows are finite (P2M and M2M map
     // non-finite sums to 0)
-    double v[MX_GROUP][KL];
+    // FP64 form: the group's sources summed in FP64 (list order)
+    double bs[MODE == MX_F64 ? KL : 1];
+    // mixed forms: B's column in FP32, each source row rescaled from its own
+    // scale to the key's (an exact power of two), summed in list order
+    float b32[MODE == MX_F64 ? 1 : KL];
+    if constexpr (MODE == MX_F64) {
+      double v[MX_GROUP][KL];
 #pragma unroll
-    for (uint32_t q = 0; q < MX_GROUP; ++q) {
-      const uint32_t iq = id[q];
+      for (uint32_t q = 0; q < MX_GROUP; ++q) {
+        const uint32_t iq = id[q];
 #pragma unroll
-      for (int e = 0; e < KL; ++e) v[q][e] = 0.0;
-      if (iq != NONE) {
-        const double* row = iq < a.n_in ? a.in + (size_t)iq * S : a.virt + (size_t)(iq - a.n_in) * S;
-        if constexpr (S % 2 == 0) {
-          // even rows on 16-byte aligned bases (checked by the host): each
-          // even k starts an aligned pair, and k < S holds for both or neither
+        for (int e = 0; e < KL; ++e) v[q][e] = 0.0;
+        if (iq != NONE) {
+          const double* row = iq < a.n_in ? a.in + (size_t)iq * S : a.virt + (size_t)(iq - a.n_in) * S;
+          if constexpr (S % 2 == 0) {
+            // even rows on 16-byte aligned bases (checked by the host): each
+            // even k starts an aligned pair, and k < S holds for both or neither
 #pragma unroll
-          for (int e = 0; e < KL; e += 2) {
-            const int k = LN::k_of(e, kq);
-            if (k < S) {
-              const double2 w = *reinterpret_cast<const double2*>(row + k);
-              v[q][e] = w.x;
-              v[q][e + 1] = w.y;
+            for (int e = 0; e < KL; e += 2) {
+              const int k = LN::k_of(e, kq);
+              if (k < S) {
+                const double2 w = *reinterpret_cast<const double2*>(row + k);
+                v[q][e] = w.x;
+                v[q][e + 1] = w.y;
+              }
             }
-          }
-        } else {
+          } else {
 #pragma unroll
-          for (int e = 0; e < KL; ++e) {
-            const int k = LN::k_of(e, kq);
-            if (k < S) v[q][e] = row[k];
+            for (int e = 0; e < KL; ++e) {
+              const int k = LN::k_of(e, kq);
+              if (k < S) v[q][e] = row[k];
+            }
           }
         }
       }
-    }
-    double bs[KL];
 #pragma unroll
-    for (int e = 0; e < KL; ++e) {  // a group's sources in list order
-      bs[e] = v[0][e];
+      for (int e = 0; e < KL; ++e) {  // a group's sources in list order
+        bs[e] = v[0][e];
 #pragma unroll
-      for (uint32_t q = 1; q < MX_GROUP; ++q) bs[e] += v[q][e];
+        for (uint32_t q = 1; q < MX_GROUP; ++q) bs[e] += v[q][e];
+      }
+    } else {
+      // rows of K floats, zero past S: slots 4m .. 4m + 3 are one aligned
+      // 16-byte load (k < K holds for all four or none)
+      float v[MX_GROUP][KL];
+#pragma unroll
+      for (uint32_t q = 0; q < MX_GROUP; ++q) {
+        const uint32_t iq = id[q];
+#pragma unroll
+        for (int e = 0; e < KL; ++e) v[q][e] = 0.f;
+        if (iq != NONE) {
+          const float* row = iq < a.n_in ? a.in32 + (size_t)iq * K : a.virt32 + (size_t)(iq - a.n_in) * K;
+#pragma unroll
+          for (int e = 0; e < KL; e += 4) {
+            const int k = LN::k_of(e, kq);
+            if (k < K) {
+              const float4 w = *reinterpret_cast<const float4*>(row + k);
+              v[q][e] = w.x;
+              v[q][e + 1] = w.y;
+              v[q][e + 2] = w.z;
+              v[q][e + 3] = w.w;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < KL; ++e) b32[e] = 0.f;
+#pragma unroll
+      for (uint32_t q = 0; q < MX_GROUP; ++q) {
+        const int t = (int)(int8_t)(uint8_t)(sh >> (8 * q));
+#pragma unroll
+        for (int e = 0; e < KL; ++e)
+          b32[e] += ldexpf(v[q][e], t * (int)((ord_in[e / 6] >> (5 * (e % 6))) & 31u));
+      }
     }
-    if (MODE == MX_F64) {
+    if constexpr (MODE == MX_F64) {
 #pragma unroll
       for (int rb = 0; rb < RB; ++rb) {
 #pragma unroll
         for (int e = 0; e < KS; ++e)
           L[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)tv[rb][e], bs[e], L[rb], 0, 0, 0);
       }
-    } else if (MODE == MX_F32) {
-      float b[KS];
-#pragma unroll
-      for (int e = 0; e < KS; ++e) b[e] = (float)ldexp(bs[e], -sx * (int)((ord_in[e / 6] >> (5 * (e % 6))) & 31u));
+    } else if constexpr (MODE == MX_F32) {
 #pragma unroll
       for (int rb = 0; rb < RB; ++rb) {
         f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int e = 0; e < KS; ++e) acc = __builtin_amdgcn_mfma_f32_16x16x4f32((float)tv[rb][e], b[e], acc, 0, 0, 0);
+        for (int e = 0; e < KS; ++e) acc = __builtin_amdgcn_mfma_f32_16x16x4f32((float)tv[rb][e], b32[e], acc, 0, 0, 0);
 #pragma unroll
         for (int j = 0; j < 4; ++j)
           L[rb][j] += ldexp((double)acc[j], -sx * (int)((ord_out[(rb * 4 + j) / 6] >> (5 * ((rb * 4 + j) % 6))) & 31u));
@@ -596,9 +725,8 @@ __global__ void __launch_bounds__(256) k_m2l_tiles(TileArgs a) {
       bf8 bh[KC], bm[KC], bl[KC];
 #pragma unroll
       for (int e = 0; e < KL; ++e) {
-        const float vv = (float)ldexp(bs[e], -sx * (int)((ord_in[e / 6] >> (5 * (e % 6))) & 31u));
         __bf16 hh, mm, ll;
-        split3(vv, hh, mm, ll);
+        split3(b32[e], hh, mm, ll);
         bh[e >> 3][e & 7] = hh;
         bm[e >> 3][e & 7] = mm;
         bl[e >> 3][e & 7] = ll;
@@ -829,9 +957,11 @@ static int build_tiles(galax_fmm* h, const uint32_t* row_of_pair, const uint32_t
       GX_HIP(hipStreamSynchronize(s));
     }
     GX_CHECK(dbuf_reserve(T.step_ids, (size_t)ns * MX_GROUP * 16 * 4, s));
+    GX_CHECK(dbuf_reserve(T.lr_key, (size_t)std::max<uint32_t>(nlr, 1) * 4, s));
     k_mx_step_ids<<<nblk((size_t)ns * 16), 256, 0, s>>>(T.step_first.as<uint32_t>(), gfirst, lrid,
                                                         T.src.as<uint32_t>(), ns, (uint32_t)h->tree.n_nodes,
-                                                        T.step_ids.as<uint32_t>());
+                                                        T.step_key.as<uint32_t>(), T.step_ids.as<uint32_t>(),
+                                                        T.lr_key.as<uint32_t>());
     GX_HIP(hipGetLastError());
   }
   uint32_t tot = 0;
@@ -925,6 +1055,42 @@ static int operators32(galax_fmm* h) {
   return 0;
 }
 
+// Mixed forms: per-node row exponents, per-step shifts, and the FP32 rows
+// (multipoles of the live levels 0-1, which P2M and M2M write each
+// evaluation, and the pre-summed long runs, whose padding stays zero).
+template <int P>
+static int rows32_setup(galax_fmm* h, M2LTiles& T) {
+  constexpr int K = 16 * row_blocks(P);
+  hipStream_t s = h->stream;
+  const DevTree& t = h->tree;
+  const size_t nn = t.n_nodes;
+  const size_t nlive = t.level_offset[std::min<uint32_t>(2, t.n_levels)];
+  GX_CHECK(dbuf_reserve(h->mx_nexp, std::max<size_t>(nn, 1), s));
+  if (nn) {
+    k_mx_node_exp<<<nblk(nn), 256, 0, s>>>(t.half_width.as<double>(), (uint32_t)nn, h->mx_nexp.as<int8_t>());
+    GX_HIP(hipGetLastError());
+  }
+  GX_CHECK(dbuf_reserve(h->mp32, std::max<size_t>(nlive, 1) * K * 4, s));
+  GX_CHECK(dbuf_reserve(T.virt32, std::max<size_t>(T.nlong, 1) * K * 4, s));
+  GX_HIP(hipMemsetAsync(T.virt32.p, 0, std::max<size_t>(T.nlong, 1) * K * 4, s));
+  GX_CHECK(dbuf_reserve(T.step_sh, std::max<size_t>(T.nsteps, 1) * 16 * 4, s));
+  if (T.nsteps) {
+    GX_CHECK(dbuf_reserve(h->flag, 64, s));
+    uint32_t* err = h->flag.as<uint32_t>() + 13;
+    GX_HIP(hipMemsetAsync(err, 0, 4, s));
+    k_mx_step_shifts<<<nblk(T.nsteps * 16), 256, 0, s>>>(T.step_ids.as<uint32_t>(), T.step_key.as<uint32_t>(),
+                                                         h->mx_sig.as<int>(), h->mx_nexp.as<int8_t>(), T.nsteps,
+                                                         (uint32_t)nn, T.step_sh.as<uint32_t>(), err);
+    GX_HIP(hipGetLastError());
+    uint32_t herr = 0;
+    GX_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s));
+    GX_HIP(hipStreamSynchronize(s));
+    if (herr) return set_error(GALAX_ERR_UNSUPPORTED, "M2L (mixed): a node's size and distance differ by over 2^127");
+  }
+  h->mx_rows32 = true;
+  return 0;
+}
+
 template <int P>
 static int setup_impl(galax_fmm* h) {
   hipStream_t s = h->stream;
@@ -938,10 +1104,12 @@ static int setup_impl(galax_fmm* h) {
   GX_CHECK(build_tiles(h, tgt, h->tree.m2l_dat.as<uint32_t>(), nt, h->mx_fwd));
   if (h->cfg.precision == GALAX_PREC_FP64) {
     GX_CHECK((operators64<P, false>(h)));
-  } else if (h->cfg.precision == GALAX_PREC_MIXED_BF16X3) {
-    GX_CHECK((operators32<P, MX_BF16X3>(h)));
   } else {
-    GX_CHECK((operators32<P, MX_F32>(h)));
+    if (h->cfg.precision == GALAX_PREC_MIXED_BF16X3)
+      GX_CHECK((operators32<P, MX_BF16X3>(h)));
+    else
+      GX_CHECK((operators32<P, MX_F32>(h)));
+    GX_CHECK(rows32_setup<P>(h, h->mx_fwd));
   }
   GX_HIP(hipStreamSynchronize(s));
   h->mx_valid = true;
@@ -982,7 +1150,15 @@ static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
   a.step_key = T.step_key.as<uint32_t>();
   a.step_ids = T.step_ids.as<uint32_t>();
   a.n_in = (uint32_t)h->tree.n_nodes;
-  if (T.nlong) {
+  constexpr bool rows32 = MODE != MX_F64;
+  if (rows32 && !h->mx_rows32) return set_error(GALAX_ERR_INVALID, "M2L (mixed): FP32 rows not set up");
+  if (rows32 && T.nlong) {
+    k_m2l_run_sums32<P><<<nblk(T.nlong, 4 * (64 / sums32_lanes<P>())), 256, 0, s>>>(
+        T.lr_order.as<uint32_t>(), T.lr_first.as<uint32_t>(), T.lr_len.as<uint32_t>(), (uint32_t)T.nlong,
+        T.src.as<uint32_t>(), h->mp32.as<float>(), h->mx_nexp.as<int8_t>(), T.lr_key.as<uint32_t>(),
+        h->mx_sig.as<int>(), T.virt32.as<float>());
+    GX_HIP(hipGetLastError());
+  } else if (T.nlong) {
     GX_CHECK(dbuf_reserve(T.virt, T.nlong * S * 8, s));
     if ((uintptr_t)in & 15) return set_error(GALAX_ERR_INVALID, "M2L source rows need 16-byte aligned bases");
     k_m2l_run_sums_w<P><<<nblk(T.nlong, 4 * (64 / sums_lanes(S))), 256, 0, s>>>(T.lr_order.as<uint32_t>(),
@@ -1010,6 +1186,9 @@ static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
   a.that = h->mx_that.as<float>();
   a.sig_exp = h->mx_sig.as<int>();
   a.in = in;
+  a.in32 = h->mp32.as<float>();
+  a.virt32 = T.virt32.as<float>();
+  a.step_sh = T.step_sh.as<uint32_t>();
   if (((uintptr_t)a.in | (uintptr_t)a.virt) & 15)
     return set_error(GALAX_ERR_INVALID, "M2L source rows need 16-byte aligned bases");
   a.out = out;
