@@ -540,8 +540,12 @@ struct TileArgs {
 //  D (16x16): v_mfma_f32_*: lane l holds D[4 kq + j][c], j = 0..3;
 //             v_mfma_f64_16x16x4_f64: lane l holds D[kq + 4 j][c] (rows interleaved).
 // TR (adjoint, FP64 only): A = T_K^T.
+// The mixed forms with two row blocks (p = 5, 6) are held to five waves per
+// SIMD (96 VGPRs, 16 bytes of spills: 0.114 -> 0.110 ms at 2^20 Plummer
+// bodies, p = 6); larger rows would spill heavily.
 template <int P, int MODE, bool TR>
-__global__ void __launch_bounds__(256) k_m2l_tiles(TileArgs a) {
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(MODE != MX_F64 && row_blocks(P) == 2 ? 5 : 1))) k_m2l_tiles(TileArgs a) {
   typedef Lanes<P, MODE> LN;
   constexpr int S = LN::S, RB = LN::RB, K = LN::K, KL = LN::KL, KS = K / 4;
   static_assert(!TR || MODE == MX_F64, "the transposed (adjoint) product is FP64");
