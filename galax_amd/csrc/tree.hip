@@ -63,8 +63,11 @@ constexpr int RS_BITS = 7, RS_RADIX = 1 << RS_BITS, RS_THREADS = 256, RS_ITEMS =
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_WAVES = RS_THREADS / 64;
 
+// Digit counts of each tile: hist[digit][tile] (scanned into global offsets
+// next) and a copy by tile, tcnt[tile][digit], for the scatter's local order.
 __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint64_t* __restrict__ keys, size_t n, int shift,
-                                                        uint32_t* __restrict__ hist, uint32_t nb) {
+                                                        uint32_t* __restrict__ hist, uint32_t nb,
+                                                        uint32_t* __restrict__ tcnt) {
   __shared__ uint32_t hcount[RS_RADIX];
   for (int t = threadIdx.x; t < RS_RADIX; t += RS_THREADS) hcount[t] = 0;
   __syncthreads();
@@ -75,58 +78,96 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint64_t* __restri
     if (i < n) atomicAdd(&hcount[(keys[i] >> shift) & (RS_RADIX - 1)], 1u);
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < RS_RADIX; t += RS_THREADS) hist[(size_t)t * nb + blockIdx.x] = hcount[t];
+  for (int t = threadIdx.x; t < RS_RADIX; t += RS_THREADS) {
+    hist[(size_t)t * nb + blockIdx.x] = hcount[t];
+    tcnt[(size_t)blockIdx.x * RS_RADIX + t] = hcount[t];
+  }
 }
 
+// One tile of RS_TILE keys per block, stably: the tile's digit histogram and
+// its exclusive scan (bucket starts within the tile); each key's rank among
+// the tile's keys of its digit (ballot ranking per wave, 16 rounds in key
+// order); the keys and values placed in LDS in (digit, rank) order; then
+// consecutive threads write consecutive LDS entries, so each bucket's keys
+// leave the tile as one contiguous run (~RS_TILE / RS_RADIX keys) instead of
+// one scattered 8-byte write per key.
 __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t* __restrict__ kin,
                                                            const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
                                                            uint32_t* __restrict__ vout, size_t n, int shift,
-                                                           const uint32_t* __restrict__ offs, uint32_t nb) {
+                                                           const uint32_t* __restrict__ offs, uint32_t nb,
+                                                           const uint32_t* __restrict__ tcnt) {
+  static_assert(RS_RADIX == 128, "the tile scan takes two digits per lane of one wave");
+  __shared__ uint64_t skey[RS_TILE];
+  __shared__ uint32_t sval[RS_TILE];
   __shared__ uint32_t run[RS_RADIX];
   __shared__ uint32_t boff[RS_RADIX];
+  __shared__ uint32_t tstart[RS_RADIX];
   __shared__ uint32_t cnt[RS_WAVES][RS_RADIX];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const size_t base = (size_t)blockIdx.x * RS_TILE;
+  const uint32_t tn = (uint32_t)min((size_t)RS_TILE, n - base);
   for (int t = tid; t < RS_RADIX; t += RS_THREADS) {
     run[t] = 0;
     boff[t] = offs[(size_t)t * nb + blockIdx.x];
   }
+  // bucket starts within the tile: exclusive scan of its digit counts, two
+  // digits per lane of wave 0
+  if (w == 0) {
+    const uint32_t c0 = tcnt[(size_t)blockIdx.x * RS_RADIX + 2 * lane];
+    const uint32_t c1 = tcnt[(size_t)blockIdx.x * RS_RADIX + 2 * lane + 1];
+    uint32_t incl = c0 + c1;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t v = (uint32_t)__shfl_up((int)incl, o, 64);
+      if (lane >= o) incl += v;
+    }
+    tstart[2 * lane] = incl - (c0 + c1);
+    tstart[2 * lane + 1] = incl - c1;
+  }
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  size_t base = (size_t)blockIdx.x * RS_TILE;
   for (int k = 0; k < RS_ITEMS; ++k) {
     for (int t = tid; t < RS_WAVES * RS_RADIX; t += RS_THREADS) (&cnt[0][0])[t] = 0;
     __syncthreads();
-    size_t i = base + (size_t)k * RS_THREADS + tid;
-    bool valid = i < n;
-    uint64_t key = valid ? kin[i] : 0ull;
-    uint32_t d = (uint32_t)(key >> shift) & (RS_RADIX - 1);
+    const uint32_t j = (uint32_t)k * RS_THREADS + tid;
+    const bool valid = j < tn;
+    const uint64_t key = valid ? kin[base + j] : 0ull;
+    const uint32_t val = valid ? vin[base + j] : 0u;
+    const uint32_t d = (uint32_t)(key >> shift) & (RS_RADIX - 1);
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < RS_BITS; ++b) {
-      bool bit = (d >> b) & 1u;
-      uint64_t bb = __ballot(bit);
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
       peers &= bit ? bb : ~bb;
     }
-    uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
-    bool leader = valid && ((peers & lt_mask) == 0);
+    const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+    const bool leader = valid && ((peers & lt_mask) == 0);
     if (leader) cnt[w][d] = (uint32_t)__popcll(peers);
     __syncthreads();
     for (int t = tid; t < RS_RADIX; t += RS_THREADS) {
-      uint32_t s = run[t];
+      uint32_t sacc = run[t];
 #pragma unroll
       for (int ww = 0; ww < RS_WAVES; ++ww) {
-        uint32_t c = cnt[ww][t];
-        cnt[ww][t] = s;
-        s += c;
+        const uint32_t c = cnt[ww][t];
+        cnt[ww][t] = sacc;
+        sacc += c;
       }
-      run[t] = s;
+      run[t] = sacc;
     }
     __syncthreads();
     if (valid) {
-      uint32_t pos = boff[d] + cnt[w][d] + rank;
-      kout[pos] = key;
-      vout[pos] = vin[i];
+      const uint32_t lpos = tstart[d] + cnt[w][d] + rank;
+      skey[lpos] = key;
+      sval[lpos] = val;
     }
-    __syncthreads();
+    __syncthreads();  // cnt is cleared for the next round
+  }
+  for (uint32_t j = tid; j < tn; j += RS_THREADS) {
+    const uint64_t key = skey[j];
+    const uint32_t d = (uint32_t)(key >> shift) & (RS_RADIX - 1);
+    const uint32_t pos = boff[d] + (j - tstart[d]);
+    kout[pos] = key;
+    vout[pos] = sval[j];
   }
 }
 
@@ -136,8 +177,9 @@ int radix_sort_bits(galax_fmm* h, uint64_t* ka, uint32_t* va, uint64_t* kb, uint
   if (n <= 1) return 0;
   hipStream_t s = h->stream;
   uint32_t nb = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
-  GX_CHECK(dbuf_reserve(h->ws.sort_hist, (size_t)RS_RADIX * nb * sizeof(uint32_t), s));
+  GX_CHECK(dbuf_reserve(h->ws.sort_hist, 2 * (size_t)RS_RADIX * nb * sizeof(uint32_t), s));
   uint32_t* hist = h->ws.sort_hist.as<uint32_t>();
+  uint32_t* tcnt = hist + (size_t)RS_RADIX * nb;
   int passes = (bits + RS_BITS - 1) / RS_BITS;
   if (passes & 1) ++passes;  // an even number of passes leaves the result in (ka, va)
   uint64_t *ki = ka, *ko = kb;
@@ -145,10 +187,10 @@ int radix_sort_bits(galax_fmm* h, uint64_t* ka, uint32_t* va, uint64_t* kb, uint
   for (int pass = 0; pass < passes; ++pass) {
     int shift = pass * RS_BITS;
     if (shift >= 64) shift = 63;  // padding pass: every digit equal, order kept
-    k_rs_hist<<<nb, RS_THREADS, 0, s>>>(ki, n, shift, hist, nb);
+    k_rs_hist<<<nb, RS_THREADS, 0, s>>>(ki, n, shift, hist, nb, tcnt);
     GX_HIP(hipGetLastError());
     GX_CHECK(scan_u32(h, hist, hist, (size_t)RS_RADIX * nb, false, false, nullptr));
-    k_rs_scatter<<<nb, RS_THREADS, 0, s>>>(ki, vi, ko, vo, n, shift, hist, nb);
+    k_rs_scatter<<<nb, RS_THREADS, 0, s>>>(ki, vi, ko, vo, n, shift, hist, nb, tcnt);
     GX_HIP(hipGetLastError());
     std::swap(ki, ko);
     std::swap(vi, vo);
