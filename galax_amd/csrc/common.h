@@ -119,10 +119,10 @@ struct M2LTiles {
   DBuf item_tile, item_s0, item_s1, item_slot;  // per work item
   DBuf multi_tile, multi_slot, multi_cnt;       // per tile with several items
   DBuf part;                                    // partial results, 16 x S doubles per slot
-  // mixed forms: FP32 source rows. lr_key: key of each long run; virt32: its
-  // pre-summed row in FP32, scaled by the key; step_sh: per (step, column) the
-  // four sources' row-scale shifts as int8 (packed)
-  DBuf lr_key, virt32, step_sh;
+  // mixed forms: lr_key: key of each long run (its pre-summed FP32 row, at the
+  // key's scale, follows the node rows in the handle's mp32); step_sh: per
+  // (step, column) the four sources' row-scale shifts as int8 (packed)
+  DBuf lr_key, step_sh;
   size_t n_rows = 0, ntiles = 0, ngroups = 0, nsteps = 0, nitems = 0, nmulti = 0, nslots = 0, nlong = 0, nhub = 0;
 };
 
@@ -174,8 +174,9 @@ struct galax_fmm {
   // adjoint tiles over sources, and the per-key operators
   gx::M2LTiles mx_fwd, mx_adj;
   gx::DBuf mx_t64, mx_t64T, mx_that, mx_sig, mx_zero;  // mx_zero: one zero source row (missing ids read it)
-  // mixed forms: per node the exponent e of its FP32 multipole row (mp32 holds
-  // M_ij 2^(-e (i+j)) in rows of 16 ceil(S/16) floats, zero padded)
+  // mixed forms: per node the exponent e of its FP32 multipole row. mp32
+  // holds rows of 16 ceil(S/16) floats, zero padded: the nodes' M_ij
+  // 2^(-e (i+j)), then the forward M2L's pre-summed long runs
   gx::DBuf mx_nexp;
   bool mx_rows32 = false;
   bool mx_valid = false, mx_adj_valid = false;

@@ -525,8 +525,7 @@ struct TileArgs {
   // mixed forms: FP32 source rows of K floats (node rows scaled by 2^(-e ord),
   // pre-summed rows by the key's 2^(-a ord)) and per (step, column) the four
   // sources' shifts e - a (int8, packed; 0 for a pre-summed row)
-  const float* in32;
-  const float* virt32;
+  const float* in32;  // node rows, then pre-summed rows (ids n_in + r)
   const uint32_t* step_sh;
   uint32_t n_in;
   double* out;       // locals (forward) or mbar (adjoint), S per row
@@ -677,6 +676,9 @@ __global__ void __launch_bounds__(256)
     } else {
       // rows of K floats, zero past S: slots 4m .. 4m + 3 are one aligned
       // 16-byte load (k < K holds for all four or none)
+      // a missing source loads nothing (inactive lanes make no cache
+      // accesses, and the kernel is bound by the L1's accesses as much as by
+      // its VALU work: loading a zero row instead was slower)
       float v[MX_GROUP][KL];
 #pragma unroll
       for (uint32_t q = 0; q < MX_GROUP; ++q) {
@@ -684,11 +686,11 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int e = 0; e < KL; ++e) v[q][e] = 0.f;
         if (iq != NONE) {
-          const float* row = iq < a.n_in ? a.in32 + (size_t)iq * K : a.virt32 + (size_t)(iq - a.n_in) * K;
+          const float* row = a.in32 + (size_t)iq * K;
 #pragma unroll
           for (int e = 0; e < KL; e += 4) {
             const int k = LN::k_of(e, kq);
-            if (k < K) {
+            if (MODE == MX_F32 || k < K) {  // F32: k < K always (KL = K / 4)
               const float4 w = *reinterpret_cast<const float4*>(row + k);
               v[q][e] = w.x;
               v[q][e + 1] = w.y;
@@ -705,7 +707,7 @@ __global__ void __launch_bounds__(256)
         const int t = (int)(int8_t)(uint8_t)(sh >> (8 * q));
 #pragma unroll
         for (int e = 0; e < KL; ++e)
-          b32[e] += ldexpf(v[q][e], t * (int)((ord_in[e / 6] >> (5 * (e % 6))) & 31u));
+          b32[e] += ldexpf(v[q][e], __mul24(t, (int)((ord_in[e / 6] >> (5 * (e % 6))) & 31u)));
       }
     }
     if constexpr (MODE == MX_F64) {
@@ -723,7 +725,7 @@ __global__ void __launch_bounds__(256)
         for (int e = 0; e < KS; ++e) acc = __builtin_amdgcn_mfma_f32_16x16x4f32((float)tv[rb][e], b32[e], acc, 0, 0, 0);
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          L[rb][j] += ldexp((double)acc[j], -sx * (int)((ord_out[(rb * 4 + j) / 6] >> (5 * ((rb * 4 + j) % 6))) & 31u));
+          L[rb][j] += ldexp((double)acc[j], -__mul24(sx, (int)((ord_out[(rb * 4 + j) / 6] >> (5 * ((rb * 4 + j) % 6))) & 31u)));
       }
     } else {
       bf8 bh[KC], bm[KC], bl[KC];
@@ -759,7 +761,7 @@ __global__ void __launch_bounds__(256)
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          L[rb][j] += ldexp((double)acc[j], -sx * (int)((ord_out[(rb * 4 + j) / 6] >> (5 * ((rb * 4 + j) % 6))) & 31u));
+          L[rb][j] += ldexp((double)acc[j], -__mul24(sx, (int)((ord_out[(rb * 4 + j) / 6] >> (5 * ((rb * 4 + j) % 6))) & 31u)));
       }
     }
   }
@@ -1068,15 +1070,16 @@ static int rows32_setup(galax_fmm* h, M2LTiles& T) {
   hipStream_t s = h->stream;
   const DevTree& t = h->tree;
   const size_t nn = t.n_nodes;
-  const size_t nlive = t.level_offset[std::min<uint32_t>(2, t.n_levels)];
   GX_CHECK(dbuf_reserve(h->mx_nexp, std::max<size_t>(nn, 1), s));
   if (nn) {
     k_mx_node_exp<<<nblk(nn), 256, 0, s>>>(t.half_width.as<double>(), (uint32_t)nn, h->mx_nexp.as<int8_t>());
     GX_HIP(hipGetLastError());
   }
-  GX_CHECK(dbuf_reserve(h->mp32, std::max<size_t>(nlive, 1) * K * 4, s));
-  GX_CHECK(dbuf_reserve(T.virt32, std::max<size_t>(T.nlong, 1) * K * 4, s));
-  GX_HIP(hipMemsetAsync(T.virt32.p, 0, std::max<size_t>(T.nlong, 1) * K * 4, s));
+  // node rows (written by P2M and M2M for the live levels), then pre-summed
+  // rows; all zero to begin with (the padding stays so)
+  const size_t rows = nn + T.nlong;
+  GX_CHECK(dbuf_reserve(h->mp32, rows * K * 4, s));
+  GX_HIP(hipMemsetAsync(h->mp32.p, 0, rows * K * 4, s));
   GX_CHECK(dbuf_reserve(T.step_sh, std::max<size_t>(T.nsteps, 1) * 16 * 4, s));
   if (T.nsteps) {
     GX_CHECK(dbuf_reserve(h->flag, 64, s));
@@ -1160,7 +1163,7 @@ static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
     k_m2l_run_sums32<P><<<nblk(T.nlong, 4 * (64 / sums32_lanes<P>())), 256, 0, s>>>(
         T.lr_order.as<uint32_t>(), T.lr_first.as<uint32_t>(), T.lr_len.as<uint32_t>(), (uint32_t)T.nlong,
         T.src.as<uint32_t>(), h->mp32.as<float>(), h->mx_nexp.as<int8_t>(), T.lr_key.as<uint32_t>(),
-        h->mx_sig.as<int>(), T.virt32.as<float>());
+        h->mx_sig.as<int>(), h->mp32.as<float>() + (size_t)h->tree.n_nodes * (16 * row_blocks(P)));
     GX_HIP(hipGetLastError());
   } else if (T.nlong) {
     GX_CHECK(dbuf_reserve(T.virt, T.nlong * S * 8, s));
@@ -1191,7 +1194,6 @@ static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
   a.sig_exp = h->mx_sig.as<int>();
   a.in = in;
   a.in32 = h->mp32.as<float>();
-  a.virt32 = T.virt32.as<float>();
   a.step_sh = T.step_sh.as<uint32_t>();
   if (((uintptr_t)a.in | (uintptr_t)a.virt) & 15)
     return set_error(GALAX_ERR_INVALID, "M2L source rows need 16-byte aligned bases");
