@@ -821,6 +821,83 @@ __global__ void __launch_bounds__(256) k_m2m_nodes(const uint32_t* __restrict__ 
   for (int k = 0; k < S; ++k) out[k] = isfinite(acc[k]) ? acc[k] : 0.0;  // non-finite -> 0 (galax-core/src/lib.rs:934-936)
 }
 
+// Corrected variant, levels whose parents each have one child (Morton-chunk
+// trees have runs of them: at 2^20 Plummer bodies levels 1-7 hold one parent
+// per leaf). A thread per (node, coefficient). A child at its parent's COM,
+// the usual case (the same bodies), shifts by zero: the parent's multipole is
+// the child's and the child's local gains the parent's, which is what the
+// full shifts of k_m2m_nodes / k_l2l_nodes give bit for bit (every other term
+// is +-0). An off-COM child takes its coefficient's full shift.
+template <int P>
+__global__ void __launch_bounds__(256) k_m2m_single(const uint32_t* __restrict__ cf, const double* __restrict__ comx,
+                                                    const double* __restrict__ comy, uint32_t lo, uint32_t hi,
+                                                    double* __restrict__ mp) {
+  constexpr int S = num_moments(P);
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t p = lo + (uint32_t)(t / S);
+  const int k = (int)(t % S);
+  if (p >= hi) return;
+  const uint32_t c = cf[p];
+  const double dx = comx[c] - comx[p], dy = comy[c] - comy[p];
+  const double* mc = mp + (size_t)c * S;
+  double v;
+  if (dx == 0.0 && dy == 0.0) {
+    v = mc[k];
+  } else {
+    // M_p(oi, oj) = sum_{a <= oi, b <= oj} M_c(oi - a, oj - b) dx^a / a! dy^b / b!
+    int oi, oj;
+    coef_ij<P>(k, oi, oj);
+    v = 0.0;
+    double ta = 1.0;
+    for (int a = 0; a <= oi; ++a) {
+      if (a > 0) ta *= dx / a;
+      double tb = 1.0;
+      for (int b = 0; b <= oj; ++b) {
+        if (b > 0) tb *= dy / b;
+        v = fma(mc[moment_index(oi - a, oj - b)], ta * tb, v);
+      }
+    }
+  }
+  mp[(size_t)p * S + k] = isfinite(v) ? v : 0.0;  // non-finite -> 0 (galax-core/src/lib.rs:934-936)
+}
+
+// children in [lo, hi), each its parent's only child
+template <int P>
+__global__ void __launch_bounds__(256) k_l2l_single(const uint32_t* __restrict__ par, const double* __restrict__ comx,
+                                                    const double* __restrict__ comy, uint32_t lo, uint32_t hi,
+                                                    double* __restrict__ loc) {
+  constexpr int S = num_moments(P);
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t c = lo + (uint32_t)(t / S);
+  const int k = (int)(t % S);
+  if (c >= hi) return;
+  const uint32_t p = par[c];
+  if (p == NONE) return;
+  const double dx = comx[c] - comx[p], dy = comy[c] - comy[p];
+  const double* lp = loc + (size_t)p * S;
+  double v;
+  if (dx == 0.0 && dy == 0.0) {
+    v = lp[k];
+  } else {
+    // L_c(oi, oj) += sum over i >= oi, j >= oj of L_p(i, j) C(i, oi) C(j, oj) dx^(i-oi) dy^(j-oj)
+    int oi, oj;
+    coef_ij<P>(k, oi, oj);
+    v = 0.0;
+    double ea = 1.0;
+    for (int i = oi; i <= P - oj; ++i) {
+      double eb = 1.0;
+      for (int j = oj; i + j <= P; ++j) {
+        const double bc = c_tabs.fact[i] / (c_tabs.fact[oi] * c_tabs.fact[i - oi]) * c_tabs.fact[j] /
+                          (c_tabs.fact[oj] * c_tabs.fact[j - oj]);
+        v = fma(lp[moment_index(i, j)] * bc, ea * eb, v);
+        eb *= dy;
+      }
+      ea *= dx;
+    }
+  }
+  loc[(size_t)c * S + k] += v;
+}
+
 // children in [lo, hi): L_c(oi, oj) += sum over i >= oi, j >= oj, i + j <= p
 // of L_p(i, j) C(i, oi) C(j, oj) dx^(i-oi) dy^(j-oj), (dx, dy) = com_c - com_p
 template <int P>
@@ -1611,6 +1688,12 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   {
     for (uint32_t l = 1; l < top; ++l) {
       uint32_t lo = t.level_offset[l], hi = t.level_offset[l + 1];
+      if (corrected && hi - lo == lo - t.level_offset[l - 1]) {  // every parent has one child
+        k_m2m_single<P><<<nblk((size_t)(hi - lo) * num_moments(P)), 256, 0, s>>>(
+            t.child_first.as<uint32_t>(), comx, comy, lo, hi, mp);
+        GX_HIP(hipGetLastError());
+        continue;
+      }
       if constexpr (P <= 8) {
         if (corrected) {
           k_m2m_nodes<P><<<nblk(hi - lo), 256, 0, s>>>(t.child_first.as<uint32_t>(), t.child_count.as<uint8_t>(),
@@ -1661,7 +1744,10 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   if (corrected) {
     for (uint32_t l = t.n_levels - 1; l >= l2l_stop; --l) {
       uint32_t lo = t.level_offset[l - 1], hi = t.level_offset[l];
-      if constexpr (P <= 8)
+      if (t.level_offset[l + 1] - hi == hi - lo)  // every parent has one child
+        k_l2l_single<P><<<nblk((size_t)(hi - lo) * S), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, lo, hi,
+                                                                     loc);
+      else if constexpr (P <= 8)
         k_l2l_nodes<P><<<nblk(hi - lo), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, lo, hi, loc);
       else
         k_l2l<P><<<nblk((size_t)(hi - lo) * S), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, lo, hi, loc);

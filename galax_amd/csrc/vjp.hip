@@ -382,6 +382,91 @@ __global__ void k_vjp_m2m(const uint32_t* __restrict__ par, const double* __rest
   mbar[(size_t)c * S + k] += acc;
 }
 
+// CORRECTED variant, levels whose parents each have one child: the two
+// adjoints with a thread per (node, coefficient). A child at its parent's COM
+// (the usual case) shifts by zero, so each adjoint adds the other node's
+// cotangent coefficient for coefficient (what the full shifts give bit for
+// bit); an off-COM child takes its coefficient's full adjoint shift.
+__device__ __forceinline__ void vj_coef_ij(int k, int& i, int& j) {
+  int n = 0;
+  while ((n + 1) * (n + 2) / 2 <= k) ++n;
+  j = k - n * (n + 1) / 2;
+  i = n - j;
+}
+
+__device__ __forceinline__ double vj_binom(int n, int k) {
+  double r = 1.0;
+  for (int q = 1; q <= k; ++q) r = r * (n - k + q) / q;
+  return r;
+}
+
+// parents in [lo, hi): Lbar_p(i, j) += sum_{oi <= i, oj <= j} Lbar_c(oi, oj) C(i, oi) C(j, oj) dx^(i-oi) dy^(j-oj)
+template <int P>
+__global__ void __launch_bounds__(256) k_vjp_l2l_single(const uint32_t* __restrict__ cf,
+                                                        const double* __restrict__ comx,
+                                                        const double* __restrict__ comy, uint32_t lo, uint32_t hi,
+                                                        double* __restrict__ lbar) {
+  constexpr int S = num_moments(P);
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t p = lo + (uint32_t)(t / S);
+  const int k = (int)(t % S);
+  if (p >= hi) return;
+  const uint32_t c = cf[p];
+  const double dx = comx[c] - comx[p], dy = comy[c] - comy[p];
+  const double* lc = lbar + (size_t)c * S;
+  double v;
+  if (dx == 0.0 && dy == 0.0) {
+    v = lc[k];
+  } else {
+    int i, j;
+    vj_coef_ij(k, i, j);
+    v = 0.0;
+    for (int oi = 0; oi <= i; ++oi)
+      for (int oj = 0; oj <= j; ++oj) {
+        double e = vj_binom(i, oi) * vj_binom(j, oj);
+        for (int q = 0; q < i - oi; ++q) e *= dx;
+        for (int q = 0; q < j - oj; ++q) e *= dy;
+        v = fma(lc[moment_index(oi, oj)], e, v);
+      }
+  }
+  lbar[(size_t)p * S + k] += v;
+}
+
+// children in [lo, hi): Mbar_c(i, j) += sum_{I >= i, J >= j} Mbar_p(I, J) dx^(I-i) / (I-i)! dy^(J-j) / (J-j)!
+template <int P>
+__global__ void __launch_bounds__(256) k_vjp_m2m_single(const uint32_t* __restrict__ par,
+                                                        const double* __restrict__ comx,
+                                                        const double* __restrict__ comy, uint32_t lo, uint32_t hi,
+                                                        double* __restrict__ mbar) {
+  constexpr int S = num_moments(P);
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t c = lo + (uint32_t)(t / S);
+  const int k = (int)(t % S);
+  if (c >= hi) return;
+  const uint32_t p = par[c];
+  if (p == NONE) return;
+  const double dx = comx[c] - comx[p], dy = comy[c] - comy[p];
+  const double* mp_ = mbar + (size_t)p * S;
+  double v;
+  if (dx == 0.0 && dy == 0.0) {
+    v = mp_[k];
+  } else {
+    int i, j;
+    vj_coef_ij(k, i, j);
+    v = 0.0;
+    double ta = 1.0;
+    for (int I = i; I <= P - j; ++I) {
+      double tb = 1.0;
+      for (int J = j; I + J <= P; ++J) {
+        v = fma(mp_[moment_index(I, J)], ta * tb, v);
+        tb *= dy / (J - j + 1);
+      }
+      ta *= dx / (I - i + 1);
+    }
+  }
+  mbar[(size_t)c * S + k] += v;
+}
+
 // CORRECTED variant, p <= 8: the same two adjoints with one thread per node,
 // the cotangents and the offset's powers in registers and the shift
 // constants known at compile time (as k_m2m_nodes / k_l2l_nodes).
@@ -643,6 +728,12 @@ static int vjp_impl(galax_fmm* h, const double* x, const double* y, const double
   const uint32_t top = corrected ? t.n_levels : std::min<uint32_t>(t.n_levels, 2);
   for (uint32_t l = 1; l < top; ++l) {
     uint32_t lo = t.level_offset[l], hi = t.level_offset[l + 1];
+    if (corrected && hi - lo == lo - t.level_offset[l - 1]) {  // every parent has one child
+      k_vjp_l2l_single<P><<<nblk((size_t)(hi - lo) * S), 256, 0, s>>>(t.child_first.as<uint32_t>(), comx, comy, lo,
+                                                                       hi, lbar);
+      GX_HIP(hipGetLastError());
+      continue;
+    }
     if constexpr (P <= 8) {
       if (corrected) {
         k_vjp_l2l_nodes<P><<<nblk(hi - lo), 256, 0, s>>>(t.child_first.as<uint32_t>(), t.child_count.as<uint8_t>(),
@@ -673,6 +764,12 @@ static int vjp_impl(galax_fmm* h, const double* x, const double* y, const double
   // M2M adjoint, top-down
   for (uint32_t l = top; l-- > 1;) {
     uint32_t lo = t.level_offset[l - 1], hi = t.level_offset[l];
+    if (corrected && t.level_offset[l + 1] - hi == hi - lo) {  // every parent has one child
+      k_vjp_m2m_single<P><<<nblk((size_t)(hi - lo) * S), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, lo, hi,
+                                                                       mbar);
+      GX_HIP(hipGetLastError());
+      continue;
+    }
     if constexpr (P <= 8) {
       if (corrected) {
         k_vjp_m2m_nodes<P><<<nblk(hi - lo), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, lo, hi, mbar);
