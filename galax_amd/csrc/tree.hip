@@ -388,43 +388,57 @@ __device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* wsum, 
 }
 
 // Every remaining parent round of at most FUSED_ROUND nodes in one
-// 1024-thread workgroup: per round the steps of k_round_heads, the inclusive
-// max scan, k_round_chunks, the exclusive sum scan, k_round_mark and
-// round_parent (the same arithmetic), with barriers between them, so the
-// ~20 rounds cost one launch instead of ~6 kernels and a host read each.
-// A node's group key is the Morton key of its first body (a parent shares
-// its first child's, galax-core/src/lib.rs:278-282), kept per node in nkey so
-// the head test reads one array in order; run starts and chunk heads are
-// bytes in LDS. Stops early (state left in st) when a round is larger than
-// FUSED_ROUND or its parents would not fit in `cap` nodes; the host goes on
-// from there. st: [cs, ce, shift, level, rounds done]; lvl_end[r] = ce after
-// round r.
+// 1024-thread workgroup, topology only: which nodes start a parent (runs of
+// equal key >> shift, a new parent every 4 nodes of a run; k_round_heads,
+// the inclusive max scan, k_round_chunks, the exclusive sum scan and
+// k_round_mark of the multi-kernel path) and each parent's first child. The
+// parents' geometry (round_parent) runs afterwards per level with a thread
+// per parent over the whole GPU (k_round_parents): one workgroup computing it
+// round by round waited on memory for ~1.4 ms at 2^20 bodies.
+// A node's group key is the Morton key of its first body (a parent shares its
+// first child's, galax-core/src/lib.rs:278-282), and nodes are in key order,
+// so the run test needs only hb[j], the bit length of key_j ^ key_(j-1):
+// (key_j >> sh) != (key_(j-1) >> sh) iff hb[j] > sh. A parent's hb is the
+// largest hb over its first child and the previous parent's other children
+// (the bit length of an XOR of sorted keys is the largest over the steps
+// between them). hb and the chunk-head flags are bytes in LDS (96 KB). Stops
+// early (state left in st) when a round is larger than FUSED_ROUND or its
+// parents would not fit in `cap` nodes; the host goes on from there.
+// st: [cs, ce, shift, level, rounds done]; lvl_end[r] = ce after round r.
 constexpr uint32_t FUSED_ROUND = 32768;
-__global__ void __launch_bounds__(1024) k_rounds_fused(NodeArrays a, const uint64_t* __restrict__ keys,
-                                                       int constrained_style, uint32_t cap, uint64_t* __restrict__ nkey,
-                                                       uint32_t* __restrict__ st, uint32_t* __restrict__ lvl_end,
-                                                       uint32_t max_rounds) {
+
+__global__ void __launch_bounds__(1024) k_rounds_topo(NodeArrays a, const uint64_t* __restrict__ keys, uint32_t cap,
+                                                      uint32_t* __restrict__ st, uint32_t* __restrict__ lvl_end,
+                                                      uint32_t max_rounds) {
   __shared__ uint32_t wsum[17];
+  __shared__ uint8_t hb[2][FUSED_ROUND];
   __shared__ uint8_t flg[FUSED_ROUND];
   uint32_t cs = st[0], ce = st[1], shift = st[2], lvl = st[3], r = 0;
   const uint32_t tid = threadIdx.x;
-  for (uint32_t j = cs + tid; j < ce; j += 1024) nkey[j] = keys[a.bs[j]];
+  int cur = 0;
+  if (ce - cs <= FUSED_ROUND)
+    for (uint32_t j = tid; j < ce - cs; j += 1024) {
+      uint32_t v = 255;  // the first node always starts a run
+      if (j > 0) {
+        const uint64_t x = keys[a.bs[cs + j]] ^ keys[a.bs[cs + j - 1]];
+        v = x ? 64u - (uint32_t)__clzll((long long)x) : 0u;
+      }
+      hb[0][j] = (uint8_t)v;
+    }
   __syncthreads();
   while (ce - cs > 1 && ce - cs <= FUSED_ROUND && r < max_rounds && lvl < 250) {
     const uint32_t c = ce - cs;
     const uint32_t sh = shift & 63;  // release-mode Rust masks the shift amount
-    for (uint32_t j = tid; j < c; j += 1024)
-      flg[j] = (j == 0 || (nkey[cs + j] >> sh) != (nkey[cs + j - 1] >> sh)) ? 1 : 0;
-    __syncthreads();
+    const uint8_t* h = hb[cur];
     const uint32_t K = (c + 1023) / 1024, lo = min(tid * K, c), hi = min(lo + K, c);
     uint32_t m = 0;  // last run start in this thread's chunk (0 = none: j = 0 always starts one)
     for (uint32_t j = lo; j < hi; ++j)
-      if (flg[j]) m = j;
+      if (h[j] > sh) m = j;
     uint32_t tot;
     uint32_t run = block_scan_excl<true>(m, wsum, &tot);
     uint32_t nf = 0;
     for (uint32_t j = lo; j < hi; ++j) {
-      if (flg[j]) run = j;
+      if (h[j] > sh) run = j;
       const uint8_t f = ((j - run) % 4u == 0u) ? 1 : 0;  // a new parent every 4 nodes of a run
       flg[j] = f;
       nf += f;
@@ -432,15 +446,19 @@ __global__ void __launch_bounds__(1024) k_rounds_fused(NodeArrays a, const uint6
     uint32_t np = 0;
     uint32_t pidx = block_scan_excl<false>(nf, wsum, &np);
     if (ce + np > cap) break;  // uniform: the host grows the arrays and goes on
+    uint8_t* hn = hb[cur ^ 1];
     for (uint32_t j = lo; j < hi; ++j)
-      if (flg[j]) a.cf[ce + pidx++] = cs + j;
+      if (flg[j]) {
+        const uint32_t q = pidx++;
+        a.cf[ce + q] = cs + j;
+        uint32_t mx = h[j];
+        if (j > 0)
+          for (uint32_t t = j - 1; !flg[t]; --t) mx = max(mx, (uint32_t)h[t]);  // flg[0] is set
+        hn[q] = (uint8_t)(q == 0 ? 255u : mx);
+      }
     __syncthreads();
+    cur ^= 1;
     ++lvl;
-    for (uint32_t q = tid; q < np; q += 1024) {
-      round_parent(a, ce, np, q, (uint8_t)lvl, constrained_style);
-      nkey[ce + q] = nkey[a.cf[ce + q]];
-    }
-    __syncthreads();
     cs = ce;
     ce += np;
     shift += 2;
@@ -518,17 +536,22 @@ static int build_from_leaf_starts(galax_fmm* h, size_t n, size_t n_leaves, bool 
       const uint32_t max_rounds = 64;
       GX_CHECK(ensure_node_capacity(h, ce + (size_t)c * 32 + 64));
       a = node_arrays(h);
-      GX_CHECK(dbuf_reserve(h->ws.h, (8 + max_rounds) * sizeof(uint32_t) + t.capacity * sizeof(uint64_t), s));
+      GX_CHECK(dbuf_reserve(h->ws.h, (8 + max_rounds) * sizeof(uint32_t), s));
       uint32_t* dst = h->ws.h.as<uint32_t>();
-      uint64_t* nkey = reinterpret_cast<uint64_t*>(dst + 8 + max_rounds);
       uint32_t hst[8 + 64] = {(uint32_t)cs, (uint32_t)ce, shift, lvl, 0, 0, 0, 0};
       GX_HIP(hipMemcpyAsync(dst, hst, 8 * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-      k_rounds_fused<<<1, 1024, 0, s>>>(a, t.keys.as<uint64_t>(), constrained_style ? 1 : 0, (uint32_t)t.capacity,
-                                         nkey, dst, dst + 8, max_rounds);
+      k_rounds_topo<<<1, 1024, 0, s>>>(a, t.keys.as<uint64_t>(), (uint32_t)t.capacity, dst, dst + 8, max_rounds);
       GX_HIP(hipGetLastError());
       GX_HIP(hipMemcpyAsync(hst, dst, (8 + max_rounds) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
       GX_HIP(hipStreamSynchronize(s));
-      for (uint32_t k = 0; k < hst[4]; ++k) t.level_offset.push_back(hst[8 + k]);
+      // the rounds' geometry, level by level (each reads its children's)
+      for (uint32_t k = 0, pe = (uint32_t)ce; k < hst[4]; ++k) {
+        const uint32_t ne = hst[8 + k], np = ne - pe;
+        k_round_parents<<<nblk(np), 256, 0, s>>>(a, pe, np, (uint8_t)(lvl + 1 + k), constrained_style ? 1 : 0);
+        GX_HIP(hipGetLastError());
+        t.level_offset.push_back(ne);
+        pe = ne;
+      }
       cs = hst[0];
       ce = hst[1];
       shift = hst[2];
