@@ -845,12 +845,19 @@ int build_tree_constrained(galax_fmm* h, size_t n) {
 struct SegLevel {
   uint32_t base, p2, lo, n;  // heap entries [base, base + 2 p2); node ids [lo, lo + n)
 };
-// One 64-byte record per segment (a query reads it with two 32-byte loads
-// from one cache line): the box union x0, x1, y0, y1, then the children's
-// extremes ax = max(q.c - q.hw), bx = min(q.c + q.hw) in x, ay, by in y.
+// One 32-byte record per segment (a query reads it with two 16-byte loads):
+// the box union x0, x1, y0, y1, then the children's extremes
+// ax = max(q.c - q.hw), bx = min(q.c + q.hw) in x, ay, by in y. In FP32,
+// each bound rounded outwards (minima down, maxima up), so a record holds
+// at least what the FP64 one would and the pruning stays conservative.
 struct SegRec {
-  double x0, x1, y0, y1, ax, bx, ay, by;
+  float x0, x1, y0, y1, ax, bx, ay, by;
 };
+__device__ __forceinline__ SegRec seg_rec(double x0, double x1, double y0, double y1, double ax, double bx, double ay,
+                                          double by) {
+  return SegRec{__double2float_rd(x0), __double2float_ru(x1), __double2float_rd(y0), __double2float_ru(y1),
+                __double2float_ru(ax), __double2float_rd(bx), __double2float_ru(ay), __double2float_rd(by)};
+}
 struct SegTree {
   SegRec* r;
 };
@@ -866,7 +873,7 @@ __global__ void k_seg_leaves(NodeArrays a, const SegLevel* __restrict__ lvls, ui
   const size_t e = (size_t)L.base + L.p2 + k;
   const double inf = __builtin_huge_val();
   if (k >= L.n) {  // padding: meets nothing, no child
-    st.r[e] = SegRec{inf, -inf, inf, -inf, -inf, inf, -inf, inf};
+    st.r[e] = seg_rec(inf, -inf, inf, -inf, -inf, inf, -inf, inf);
     return;
   }
   const uint32_t nd = L.lo + k;
@@ -881,7 +888,7 @@ __global__ void k_seg_leaves(NodeArrays a, const SegLevel* __restrict__ lvls, ui
       ay = fmax(ay, qy - qh);
       by = fmin(by, qy + qh);
     }
-  st.r[e] = SegRec{cx - hw, cx + hw, cy - hw, cy + hw, ax, bx, ay, by};
+  st.r[e] = seg_rec(cx - hw, cx + hw, cy - hw, cy + hw, ax, bx, ay, by);
 }
 
 // internal entries at height h (1 = parents of the leaf slots) of every level
@@ -893,8 +900,8 @@ __global__ void k_seg_up(const SegLevel* __restrict__ lvls, uint32_t nlv, uint32
     if (i < cnt) {
       const size_t e = (size_t)lvls[l].base + cnt + i;
       const SegRec A = st.r[(size_t)lvls[l].base + 2 * (cnt + i)], B = st.r[(size_t)lvls[l].base + 2 * (cnt + i) + 1];
-      st.r[e] = SegRec{fmin(A.x0, B.x0), fmax(A.x1, B.x1), fmin(A.y0, B.y0), fmax(A.y1, B.y1),
-                       fmax(A.ax, B.ax), fmin(A.bx, B.bx), fmax(A.ay, B.ay), fmin(A.by, B.by)};
+      st.r[e] = SegRec{fminf(A.x0, B.x0), fmaxf(A.x1, B.x1), fminf(A.y0, B.y0), fmaxf(A.y1, B.y1),
+                       fmaxf(A.ax, B.ax), fminf(A.bx, B.bx), fmaxf(A.ay, B.ay), fminf(A.by, B.by)};
       return;
     }
     i -= cnt;
@@ -906,7 +913,7 @@ constexpr double SEG_SLACK = 1e-13;
 // Can the segment hold a node whose box meets [x0, x1] x [y0, y1]?
 __device__ __forceinline__ bool seg_meets(const SegTree& st, size_t e, double x0, double x1, double y0, double y1,
                                           double mag) {
-  const double4 b = *reinterpret_cast<const double4*>(&st.r[e].x0);
+  const float4 b = *reinterpret_cast<const float4*>(&st.r[e].x0);
   const double sx0 = b.x, sx1 = b.y, sy0 = b.z, sy1 = b.w;
   if (!(sx0 <= sx1)) return false;  // empty
   const double sl = SEG_SLACK * (mag + fmax(fmax(fabs(sx0), fabs(sx1)), fmax(fabs(sy0), fabs(sy1))));
@@ -917,7 +924,7 @@ __device__ __forceinline__ bool seg_meets(const SegTree& st, size_t e, double x0
 // width ch)?
 __device__ __forceinline__ bool seg_far_child(const SegTree& st, size_t e, double cx, double cy, double ch,
                                               double mag) {
-  const double4 c = *reinterpret_cast<const double4*>(&st.r[e].ax);
+  const float4 c = *reinterpret_cast<const float4*>(&st.r[e].ax);
   const double ax = c.x, bx = c.y, ay = c.z, by = c.w;
   if (ax == -__builtin_huge_val()) return false;  // no child in the segment
   const double sl = SEG_SLACK * (mag + fmax(fmax(fabs(ax), fabs(bx)), fmax(fabs(ay), fabs(by))));
