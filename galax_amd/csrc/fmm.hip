@@ -1344,15 +1344,7 @@ __global__ void __launch_bounds__(64) k_p2p_sym(
     v = make_float4(FAR, FAR, 0.f, 0.f);
     if (jb < nb) v = rel[b0 + jb];
   };
-  uint32_t nb_n = 0;
-  float4 v_n = make_float4(FAR, FAR, 0.f, 0.f);
-  double cx_n = 0.0, cy_n = 0.0;
-  if (u0 < u1) fetch(u0, nb_n, v_n, cx_n, cy_n);
-  for (uint32_t u = u0; u < u1; ++u) {
-    const uint32_t nb = nb_n;
-    const float4 v = v_n;
-    const double cxb = cx_n, cyb = cy_n;
-    if (u + 1 < u1) fetch(u + 1, nb_n, v_n, cx_n, cy_n);
+  auto partner = [&](uint32_t u, uint32_t nb, const float4& v, double cxb, double cyb) {
     float xj = FAR, yj = FAR, mj = 0.f;
     if (jb < nb) {
       xj = v.x + (float)(cxb - ox);
@@ -1385,6 +1377,19 @@ __global__ void __launch_bounds__(64) k_p2p_sym(
       pair2[o] = make_float2(ajx, ajy);
       if (PHI) pairp[o] = ajp;
     }
+  };
+  // two partners per iteration with their data in two register sets (A, B),
+  // each fetched while the other is computed: no copies between iterations
+  uint32_t nbA = 0, nbB = 0;
+  float4 vA = make_float4(FAR, FAR, 0.f, 0.f), vB = vA;
+  double cxA = 0.0, cyA = 0.0, cxB = 0.0, cyB = 0.0;
+  if (u0 < u1) fetch(u0, nbA, vA, cxA, cyA);
+  for (uint32_t u = u0; u < u1; u += 2) {
+    if (u + 1 < u1) fetch(u + 1, nbB, vB, cxB, cyB);
+    partner(u, nbA, vA, cxA, cyA);
+    if (u + 1 >= u1) break;
+    if (u + 2 < u1) fetch(u + 2, nbA, vA, cxA, cyA);
+    partner(u + 1, nbB, vB, cxB, cyB);
   }
   // the i side: acc_i[K] of lane q belongs to the i of its rotated lane
   float tix = 0.f, tiy = 0.f, tip = 0.f;
