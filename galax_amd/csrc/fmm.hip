@@ -1520,6 +1520,7 @@ __global__ void __launch_bounds__(256) k_sym_heavy_sum(uint32_t nheavy, const ui
   if (e >= be[t] - b0) return;
   double sx = 0.0, sy = 0.0, sp = 0.0;
   const uint32_t p0 = heavy_p0[k], n = heavy_n[k];
+#pragma unroll 8
   for (uint32_t c = 0; c < n; ++c) {
     const double2 v = cpart2[(size_t)(p0 + c) * SYM_LEAF + e];
     sx += v.x;
