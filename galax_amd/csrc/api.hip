@@ -493,10 +493,10 @@ void galax_fmm_destroy(galax_fmm_t* h) {
                   &h->mx_fwd.src, &h->mx_fwd.tile_off, &h->mx_fwd.step_key, &h->mx_fwd.step_first,
                   &h->mx_fwd.step_cnt, &h->mx_fwd.step_ids, &h->mx_fwd.item_tile, &h->mx_fwd.item_s0, &h->mx_fwd.item_s1,
                   &h->mx_fwd.item_slot, &h->mx_fwd.multi_tile, &h->mx_fwd.multi_slot, &h->mx_fwd.multi_cnt,
-                  &h->mx_fwd.part, &h->mx_fwd.lr_first, &h->mx_fwd.lr_len, &h->mx_fwd.lr_order, &h->mx_fwd.hub_first, &h->mx_fwd.hub_len, &h->mx_fwd.hub_coff, &h->mx_fwd.hub_part, &h->mx_fwd.virt, &h->mx_fwd.hubs, &h->mx_fwd.lr_key, &h->mx_fwd.step_sh, &h->mx_adj.src, &h->mx_adj.tile_off, &h->mx_adj.step_key,
+                  &h->mx_fwd.part, &h->mx_fwd.lr_first, &h->mx_fwd.lr_len, &h->mx_fwd.lr_order, &h->mx_fwd.hub_first, &h->mx_fwd.hub_len, &h->mx_fwd.hub_coff, &h->mx_fwd.hub_part, &h->mx_fwd.virt, &h->mx_fwd.hubs, &h->mx_fwd.lr_key, &h->mx_fwd.step_sh, &h->mx_fwd.lr_tab, &h->mx_fwd.lr_psh, &h->mx_adj.src, &h->mx_adj.tile_off, &h->mx_adj.step_key,
                   &h->mx_adj.step_first, &h->mx_adj.step_cnt, &h->mx_adj.step_ids, &h->mx_adj.item_tile, &h->mx_adj.item_s0,
                   &h->mx_adj.item_s1, &h->mx_adj.item_slot, &h->mx_adj.multi_tile, &h->mx_adj.multi_slot,
-                  &h->mx_adj.multi_cnt, &h->mx_adj.part, &h->mx_adj.lr_first, &h->mx_adj.lr_len, &h->mx_adj.lr_order, &h->mx_adj.hub_first, &h->mx_adj.hub_len, &h->mx_adj.hub_coff, &h->mx_adj.hub_part, &h->mx_adj.virt, &h->mx_adj.hubs, &h->mx_adj.lr_key, &h->mx_adj.step_sh};
+                  &h->mx_adj.multi_cnt, &h->mx_adj.part, &h->mx_adj.lr_first, &h->mx_adj.lr_len, &h->mx_adj.lr_order, &h->mx_adj.hub_first, &h->mx_adj.hub_len, &h->mx_adj.hub_coff, &h->mx_adj.hub_part, &h->mx_adj.virt, &h->mx_adj.hubs, &h->mx_adj.lr_key, &h->mx_adj.step_sh, &h->mx_adj.lr_tab, &h->mx_adj.lr_psh};
   for (DBuf* b : bufs) dbuf_free(*b);
   DevTree& t = h->tree;
   DBuf* tb[] = {&t.center_x, &t.center_y, &t.half_width, &t.mass, &t.com_x, &t.com_y, &t.body_start, &t.body_end,

@@ -121,8 +121,12 @@ struct M2LTiles {
   DBuf part;                                    // partial results, 16 x S doubles per slot
   // mixed forms: lr_key: key of each long run (its pre-summed FP32 row, at the
   // key's scale, follows the node rows in the handle's mp32); step_sh: per
-  // (step, column) the four sources' row-scale shifts as int8 (packed)
+  // (step, column) the sources' row-scale shifts as int8 (packed)
   DBuf lr_key, step_sh;
+  // mixed forms: per long run in the pre-sum kernel's order (first pair, length,
+  // pre-summed row), and per pair of a long run its row's shift to the key's scale
+  DBuf lr_tab, lr_psh;
+  uint32_t group = 4;  // longest run summed inside the product kernel (1, 2 or 4); longer runs are pre-summed
   size_t n_rows = 0, ntiles = 0, ngroups = 0, nsteps = 0, nitems = 0, nmulti = 0, nslots = 0, nlong = 0, nhub = 0;
 };
 

@@ -97,7 +97,6 @@ struct Lanes {
 // ------------------------------------------------------------- setup ----
 
 constexpr uint32_t MX_ITEM_STEPS = 32;  // steps per work item
-constexpr uint32_t MX_GROUP = 4;        // longest run summed inside the product kernel; longer runs are pre-summed
 constexpr uint32_t MX_WAVE_RUN = 256;   // longest run pre-summed by one wave; longer runs (hubs) go in chunks
 constexpr uint32_t MX_HUB_CHUNK = 128;  // rows per chunk of a hub run
 
@@ -177,18 +176,18 @@ __global__ void k_mx_steps(const unsigned long long* __restrict__ g2k, const uin
 
 __global__ void k_mx_set_u32(uint32_t* __restrict__ p, uint32_t v) { *p = v; }
 
-// groups of more than MX_GROUP pairs (long runs): index and source range
-__global__ void k_mx_long_flags(const uint32_t* __restrict__ gfirst, size_t ng, uint32_t* __restrict__ f) {
+// groups of more than G pairs (long runs): index and source range
+__global__ void k_mx_long_flags(const uint32_t* __restrict__ gfirst, size_t ng, uint32_t G, uint32_t* __restrict__ f) {
   size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g < ng) f[g] = gfirst[g + 1] - gfirst[g] > MX_GROUP ? 1u : 0u;
+  if (g < ng) f[g] = gfirst[g + 1] - gfirst[g] > G ? 1u : 0u;
 }
 
 __global__ void k_mx_long_table(const uint32_t* __restrict__ gfirst, const uint32_t* __restrict__ lrid, size_t ng,
-                                uint32_t* __restrict__ lr_first, uint32_t* __restrict__ lr_len) {
+                                uint32_t G, uint32_t* __restrict__ lr_first, uint32_t* __restrict__ lr_len) {
   size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ng) return;
   const uint32_t c = gfirst[g + 1] - gfirst[g];
-  if (c > MX_GROUP) {
+  if (c > G) {
     lr_first[lrid[g]] = gfirst[g];
     lr_len[lrid[g]] = c;
   }
@@ -205,12 +204,12 @@ __global__ void k_mx_hub_list(const uint32_t* __restrict__ lr_len, const uint32_
   if (r < n && lr_len[r] > MX_WAVE_RUN) hubs[pos[r]] = (uint32_t)r;
 }
 
-// source ids of every step, [step][q][column], NONE past a group's end; a
-// long run's only entry is its pre-summed row, id n_in + its index
+// source ids of every step, [step][q][column] (q < G), NONE past a group's
+// end; a long run's only entry is its pre-summed row, id n_in + its index
 __global__ void k_mx_step_ids(const uint32_t* __restrict__ step_grp, const uint32_t* __restrict__ gfirst,
                               const uint32_t* __restrict__ lrid, const uint32_t* __restrict__ src, size_t ns,
-                              uint32_t n_in, const uint32_t* __restrict__ step_key, uint32_t* __restrict__ ids,
-                              uint32_t* __restrict__ lr_key) {
+                              uint32_t n_in, uint32_t G, const uint32_t* __restrict__ step_key,
+                              uint32_t* __restrict__ ids, uint32_t* __restrict__ lr_key) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ns * 16) return;
   const size_t st = i / 16, col = i % 16;
@@ -220,14 +219,14 @@ __global__ void k_mx_step_ids(const uint32_t* __restrict__ step_grp, const uint3
     f = gfirst[g];
     c = gfirst[g + 1] - f;
   }
-  if (c > MX_GROUP) lr_key[lrid[g]] = step_key[st];  // each long run is one (step, column)
-  for (uint32_t q = 0; q < MX_GROUP; ++q) {
+  if (c > G) lr_key[lrid[g]] = step_key[st];  // each long run is one (step, column)
+  for (uint32_t q = 0; q < G; ++q) {
     uint32_t v = NONE;
-    if (c > MX_GROUP)
+    if (c > G)
       v = q == 0 ? n_in + lrid[g] : NONE;
     else if (q < c)
       v = src[f + q];
-    ids[(st * MX_GROUP + q) * 16 + col] = v;
+    ids[(st * G + q) * 16 + col] = v;
   }
 }
 
@@ -242,21 +241,21 @@ __global__ void k_mx_node_exp(const double* __restrict__ hw, uint32_t nn, int8_t
   nexp[i] = (int8_t)min(max(e, -100), 100);
 }
 
-// Per (step, column): the shift e_s - a of each of its four source rows from
+// Per (step, column): the shift e_s - a of each of its G source rows from
 // the row's own scale to the key's (a the key's exponent); 0 for a
 // pre-summed row, which is written at the key's scale. Packed as int8; a
 // shift outside +-127 (a node 2^127 times smaller or larger than its
 // distance) sets *err.
 __global__ void k_mx_step_shifts(const uint32_t* __restrict__ ids, const uint32_t* __restrict__ step_key,
                                  const int* __restrict__ sig_exp, const int8_t* __restrict__ nexp, size_t ns,
-                                 uint32_t n_in, uint32_t* __restrict__ sh, uint32_t* __restrict__ err) {
+                                 uint32_t n_in, uint32_t G, uint32_t* __restrict__ sh, uint32_t* __restrict__ err) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ns * 16) return;
   const size_t st = i / 16, col = i % 16;
   const int a = sig_exp[step_key[st]];
   uint32_t packed = 0;
-  for (uint32_t q = 0; q < MX_GROUP; ++q) {
-    const uint32_t id = ids[(st * MX_GROUP + q) * 16 + col];
+  for (uint32_t q = 0; q < G; ++q) {
+    const uint32_t id = ids[(st * G + q) * 16 + col];
     int t = (id != NONE && id < n_in) ? (int)nexp[id] - a : 0;
     if (t < -127 || t > 127) {
       atomicOr(err, 1u);
@@ -322,29 +321,26 @@ __global__ void __launch_bounds__(256) k_m2l_run_sums_w(const uint32_t* __restri
 // S, each at its node's scale 2^(-e ord)): K/4 lanes per run, a lane per four
 // coefficients (one 16-byte load per row), each row rescaled to the key's
 // scale 2^(-a ord) and summed in FP64 in list order, rounded to FP32 once.
-// Runs of any length.
+// Runs of any length, in tab order: (first pair, length, pre-summed row);
+// psh holds each pair's shift e - a, so a row's load waits only on its id.
 template <int P>
 __host__ __device__ constexpr int sums32_lanes() {
   return 4 * row_blocks(P);
 }
 
 template <int P>
-__global__ void __launch_bounds__(256) k_m2l_run_sums32(const uint32_t* __restrict__ lr_order,
-                                                        const uint32_t* __restrict__ lr_first,
-                                                        const uint32_t* __restrict__ lr_len, uint32_t nlr,
+__global__ void __launch_bounds__(256) k_m2l_run_sums32(const uint4* __restrict__ tab, uint32_t nlr,
                                                         const uint32_t* __restrict__ src,
-                                                        const float* __restrict__ in32,
-                                                        const int8_t* __restrict__ nexp,
-                                                        const uint32_t* __restrict__ lr_key,
-                                                        const int* __restrict__ sig_exp, float* __restrict__ virt32) {
+                                                        const int8_t* __restrict__ psh,
+                                                        const float* __restrict__ in32, float* __restrict__ virt32) {
   constexpr int K = 16 * row_blocks(P), LW = sums32_lanes<P>(), RPW = 64 / LW;
   static_assert(LW <= 64, "a run's row must fit one wave");
   const int lane = threadIdx.x & 63;
   const uint32_t q = (xcd_runs(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6)) * RPW + lane / LW;
   const int kl = lane % LW;
   if (lane >= RPW * LW || q >= nlr) return;
-  const uint32_t r = lr_order[q], f = lr_first[r], len = lr_len[r];
-  const int a = sig_exp[lr_key[r]];
+  const uint4 rt = tab[q];
+  const uint32_t f = rt.x, len = rt.y, r = rt.z;
   int o[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) o[j] = order_of<P>(4 * kl + j);
@@ -352,7 +348,7 @@ __global__ void __launch_bounds__(256) k_m2l_run_sums32(const uint32_t* __restri
 #pragma unroll 4
   for (uint32_t i = 0; i < len; ++i) {
     const uint32_t sn = src[f + i];
-    const int t = (int)nexp[sn] - a;
+    const int t = psh[f + i];
     const float4 v = *reinterpret_cast<const float4*>(in32 + (size_t)sn * K + 4 * kl);
     acc[0] += ldexp((double)v.x, t * o[0]);
     acc[1] += ldexp((double)v.y, t * o[1]);
@@ -361,6 +357,29 @@ __global__ void __launch_bounds__(256) k_m2l_run_sums32(const uint32_t* __restri
   }
   *reinterpret_cast<float4*>(virt32 + (size_t)r * K + 4 * kl) =
       make_float4((float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]);
+}
+
+// The pre-sum table of the mixed forms: per run q of the pre-sum order its
+// (first pair, length, row), and per pair of the run its shift e_s - a (the
+// node's row exponent less the key's); a shift outside +-127 sets *err.
+__global__ void k_mx_run_tab32(const uint32_t* __restrict__ lr_order, const uint32_t* __restrict__ lr_first,
+                               const uint32_t* __restrict__ lr_len, const uint32_t* __restrict__ lr_key, uint32_t nlr,
+                               const uint32_t* __restrict__ src, const int* __restrict__ sig_exp,
+                               const int8_t* __restrict__ nexp, uint4* __restrict__ tab, int8_t* __restrict__ psh,
+                               uint32_t* __restrict__ err) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nlr) return;
+  const uint32_t r = lr_order[q], f = lr_first[r], len = lr_len[r];
+  tab[q] = make_uint4(f, len, r, 0u);
+  const int a = sig_exp[lr_key[r]];
+  for (uint32_t i = 0; i < len; ++i) {
+    int t = (int)nexp[src[f + i]] - a;
+    if (t < -127 || t > 127) {
+      atomicOr(err, 1u);
+      t = 0;
+    }
+    psh[f + i] = (int8_t)t;
+  }
 }
 
 // Hub runs in chunks of MX_HUB_CHUNK rows: sums_lanes(S) lanes per chunk sum
@@ -532,7 +551,16 @@ struct TileArgs {
   double* part;      // partials of tiles with several items: [slot][16][S]
 };
 
+// Longest (row, key) run a product kernel sums itself; longer runs are
+// pre-summed into one row. At 2^20 Plummer bodies, p = 6, 82% of the runs
+// have one pair: the FP32 kernel takes one source per column (G = 1: 0.112 ->
+// 0.084 ms, the M2L pass 0.156 -> 0.138 ms against G = 4), the FP64 kernel,
+// bound by its matrix cores, two.
+constexpr uint32_t mx_group_of(int mode) { return mode == MX_F64 ? 2u : 1u; }
+
 // One block (4 waves) per work item; wave w takes steps s0 + w, s0 + w + 4, ...
+// Each column of a step sums up to G source rows (a (row, key) run of at most
+// G pairs; longer runs arrive as one pre-summed row).
 // Operand layouts (lane l, column/row c = l % 16, quarter kq = l / 16):
 //  v_mfma_*_16x16x4: A lane l holds A[c][slot kq]; B lane l holds B[slot kq][c];
 //  v_mfma_f32_16x16x32_bf16: 8 consecutive slots 8 kq + 0..7 per lane;
@@ -542,7 +570,7 @@ struct TileArgs {
 // The mixed forms with two row blocks (p = 5, 6) are held to five waves per
 // SIMD (96 VGPRs, 16 bytes of spills: 0.114 -> 0.110 ms at 2^20 Plummer
 // bodies, p = 6); larger rows would spill heavily.
-template <int P, int MODE, bool TR>
+template <int P, int MODE, bool TR, uint32_t G>
 __global__ void __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu(MODE != MX_F64 && row_blocks(P) == 2 ? 5 : 1))) k_m2l_tiles(TileArgs a) {
   typedef Lanes<P, MODE> LN;
@@ -578,24 +606,24 @@ __global__ void __launch_bounds__(256)
   uint32_t st = s0 + w;
   // the step record (key, and the source ids of this lane's column) is
   // loaded one step ahead, so each step waits only on its multipole rows
-  uint32_t key_n = 0, id_n[MX_GROUP], sh_n = 0;
+  uint32_t key_n = 0, id_n[G], sh_n = 0;
 #pragma unroll
-  for (uint32_t q = 0; q < MX_GROUP; ++q) id_n[q] = NONE;
+  for (uint32_t q = 0; q < G; ++q) id_n[q] = NONE;
   if (st < s1) {
     key_n = a.step_key[st];
 #pragma unroll
-    for (uint32_t q = 0; q < MX_GROUP; ++q) id_n[q] = a.step_ids[((size_t)st * MX_GROUP + q) * 16 + col];
+    for (uint32_t q = 0; q < G; ++q) id_n[q] = a.step_ids[((size_t)st * G + q) * 16 + col];
     if (MODE != MX_F64) sh_n = a.step_sh[(size_t)st * 16 + col];
   }
   for (; st < s1; st += 4) {
     const uint32_t key = key_n, sh = sh_n;
-    uint32_t id[MX_GROUP];
+    uint32_t id[G];
 #pragma unroll
-    for (uint32_t q = 0; q < MX_GROUP; ++q) id[q] = id_n[q];
+    for (uint32_t q = 0; q < G; ++q) id[q] = id_n[q];
     if (st + 4 < s1) {
       key_n = a.step_key[st + 4];
 #pragma unroll
-      for (uint32_t q = 0; q < MX_GROUP; ++q) id_n[q] = a.step_ids[((size_t)(st + 4) * MX_GROUP + q) * 16 + col];
+      for (uint32_t q = 0; q < G; ++q) id_n[q] = a.step_ids[((size_t)(st + 4) * G + q) * 16 + col];
       if (MODE != MX_F64) sh_n = a.step_sh[(size_t)(st + 4) * 16 + col];
     }
     // every load of the step is issued before any is used: the operator rows
@@ -628,7 +656,7 @@ __global__ void __launch_bounds__(256)
           tv[rb][4 * eq + 3] = (TT)w.w;
         }
     }
-    const int sx = MODE == MX_F64 ? 0 : a.sig_exp[key];
+    const int nsx = MODE == MX_F64 ? 0 : -a.sig_exp[key];  // output scale 2^(-a (ord + 1)) per row
     // missing ids and padding coefficients k >= S load nothing (inactive
     // lanes make no cache accesses); the rows are finite (P2M and M2M map
     // non-finite sums to 0)
@@ -638,9 +666,9 @@ __global__ void __launch_bounds__(256)
     // scale to the key's (an exact power of two), summed in list order
     float b32[MODE == MX_F64 ? 1 : KL];
     if constexpr (MODE == MX_F64) {
-      double v[MX_GROUP][KL];
+      double v[G][KL];
 #pragma unroll
-      for (uint32_t q = 0; q < MX_GROUP; ++q) {
+      for (uint32_t q = 0; q < G; ++q) {
         const uint32_t iq = id[q];
 #pragma unroll
         for (int e = 0; e < KL; ++e) v[q][e] = 0.0;
@@ -671,7 +699,7 @@ __global__ void __launch_bounds__(256)
       for (int e = 0; e < KL; ++e) {  // a group's sources in list order
         bs[e] = v[0][e];
 #pragma unroll
-        for (uint32_t q = 1; q < MX_GROUP; ++q) bs[e] += v[q][e];
+        for (uint32_t q = 1; q < G; ++q) bs[e] += v[q][e];
       }
     } else {
       // rows of K floats, zero past S: slots 4m .. 4m + 3 are one aligned
@@ -679,9 +707,9 @@ __global__ void __launch_bounds__(256)
       // a missing source loads nothing (inactive lanes make no cache
       // accesses, and the kernel is bound by the L1's accesses as much as by
       // its VALU work: loading a zero row instead was slower)
-      float v[MX_GROUP][KL];
+      float v[G][KL];
 #pragma unroll
-      for (uint32_t q = 0; q < MX_GROUP; ++q) {
+      for (uint32_t q = 0; q < G; ++q) {
         const uint32_t iq = id[q];
 #pragma unroll
         for (int e = 0; e < KL; ++e) v[q][e] = 0.f;
@@ -701,13 +729,13 @@ __global__ void __launch_bounds__(256)
         }
       }
 #pragma unroll
-      for (int e = 0; e < KL; ++e) b32[e] = 0.f;
-#pragma unroll
-      for (uint32_t q = 0; q < MX_GROUP; ++q) {
+      for (uint32_t q = 0; q < G; ++q) {
         const int t = (int)(int8_t)(uint8_t)(sh >> (8 * q));
 #pragma unroll
-        for (int e = 0; e < KL; ++e)
-          b32[e] += ldexpf(v[q][e], __mul24(t, (int)((ord_in[e / 6] >> (5 * (e % 6))) & 31u)));
+        for (int e = 0; e < KL; ++e) {
+          const float w = ldexpf(v[q][e], __mul24(t, (int)((ord_in[e / 6] >> (5 * (e % 6))) & 31u)));
+          b32[e] = q == 0 ? w : b32[e] + w;
+        }
       }
     }
     if constexpr (MODE == MX_F64) {
@@ -725,7 +753,7 @@ __global__ void __launch_bounds__(256)
         for (int e = 0; e < KS; ++e) acc = __builtin_amdgcn_mfma_f32_16x16x4f32((float)tv[rb][e], b32[e], acc, 0, 0, 0);
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          L[rb][j] += ldexp((double)acc[j], -__mul24(sx, (int)((ord_out[(rb * 4 + j) / 6] >> (5 * ((rb * 4 + j) % 6))) & 31u)));
+          L[rb][j] += ldexp((double)acc[j], __mul24(nsx, (int)((ord_out[(rb * 4 + j) / 6] >> (5 * ((rb * 4 + j) % 6))) & 31u)));
       }
     } else {
       bf8 bh[KC], bm[KC], bl[KC];
@@ -761,7 +789,7 @@ __global__ void __launch_bounds__(256)
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          L[rb][j] += ldexp((double)acc[j], -__mul24(sx, (int)((ord_out[(rb * 4 + j) / 6] >> (5 * ((rb * 4 + j) % 6))) & 31u)));
+          L[rb][j] += ldexp((double)acc[j], __mul24(nsx, (int)((ord_out[(rb * 4 + j) / 6] >> (5 * ((rb * 4 + j) % 6))) & 31u)));
       }
     }
   }
@@ -904,16 +932,17 @@ static int build_tiles(galax_fmm* h, const uint32_t* row_of_pair, const uint32_t
     k_mx_steps<<<nblk(ng), 256, 0, s>>>(g2k, g2v, sid, ng, kb, rkb, T.step_key.as<uint32_t>(),
                                         T.step_first.as<uint32_t>(), tile_off);
     GX_HIP(hipGetLastError());
-    // long runs (more than MX_GROUP pairs) become one pre-summed row each
+    // long runs (more than T.group pairs) become one pre-summed row each
     uint32_t* lrid = sid;  // the step ids are no longer needed
-    k_mx_long_flags<<<nblk(ng), 256, 0, s>>>(gfirst, ng, lrid);
+    k_mx_long_flags<<<nblk(ng), 256, 0, s>>>(gfirst, ng, T.group, lrid);
     GX_HIP(hipGetLastError());
     uint32_t nlr = 0;
     GX_CHECK(scan_u32(h, lrid, lrid, ng, false, false, &nlr));
     T.nlong = nlr;
     GX_CHECK(dbuf_reserve(T.lr_first, (size_t)std::max<uint32_t>(nlr, 1) * 4, s));
     GX_CHECK(dbuf_reserve(T.lr_len, (size_t)std::max<uint32_t>(nlr, 1) * 4, s));
-    k_mx_long_table<<<nblk(ng), 256, 0, s>>>(gfirst, lrid, ng, T.lr_first.as<uint32_t>(), T.lr_len.as<uint32_t>());
+    k_mx_long_table<<<nblk(ng), 256, 0, s>>>(gfirst, lrid, ng, T.group, T.lr_first.as<uint32_t>(),
+                                             T.lr_len.as<uint32_t>());
     GX_HIP(hipGetLastError());
     // runs too long for one wave (hubs) get a block each
     T.nhub = 0;
@@ -962,11 +991,11 @@ static int build_tiles(galax_fmm* h, const uint32_t* row_of_pair, const uint32_t
       }
       GX_HIP(hipStreamSynchronize(s));
     }
-    GX_CHECK(dbuf_reserve(T.step_ids, (size_t)ns * MX_GROUP * 16 * 4, s));
+    GX_CHECK(dbuf_reserve(T.step_ids, (size_t)ns * T.group * 16 * 4, s));
     GX_CHECK(dbuf_reserve(T.lr_key, (size_t)std::max<uint32_t>(nlr, 1) * 4, s));
     k_mx_step_ids<<<nblk((size_t)ns * 16), 256, 0, s>>>(T.step_first.as<uint32_t>(), gfirst, lrid,
                                                         T.src.as<uint32_t>(), ns, (uint32_t)h->tree.n_nodes,
-                                                        T.step_key.as<uint32_t>(), T.step_ids.as<uint32_t>(),
+                                                        T.group, T.step_key.as<uint32_t>(), T.step_ids.as<uint32_t>(),
                                                         T.lr_key.as<uint32_t>());
     GX_HIP(hipGetLastError());
   }
@@ -1081,13 +1110,23 @@ static int rows32_setup(galax_fmm* h, M2LTiles& T) {
   GX_CHECK(dbuf_reserve(h->mp32, rows * K * 4, s));
   GX_HIP(hipMemsetAsync(h->mp32.p, 0, rows * K * 4, s));
   GX_CHECK(dbuf_reserve(T.step_sh, std::max<size_t>(T.nsteps, 1) * 16 * 4, s));
+  GX_CHECK(dbuf_reserve(T.lr_tab, std::max<size_t>(T.nlong, 1) * 16, s));
+  GX_CHECK(dbuf_reserve(T.lr_psh, std::max<size_t>(h->n_live_pairs, 1), s));
   if (T.nsteps) {
     GX_CHECK(dbuf_reserve(h->flag, 64, s));
     uint32_t* err = h->flag.as<uint32_t>() + 13;
     GX_HIP(hipMemsetAsync(err, 0, 4, s));
+    if (T.nlong) {
+      k_mx_run_tab32<<<nblk(T.nlong), 256, 0, s>>>(T.lr_order.as<uint32_t>(), T.lr_first.as<uint32_t>(),
+                                                   T.lr_len.as<uint32_t>(), T.lr_key.as<uint32_t>(),
+                                                   (uint32_t)T.nlong, T.src.as<uint32_t>(), h->mx_sig.as<int>(),
+                                                   h->mx_nexp.as<int8_t>(), T.lr_tab.as<uint4>(),
+                                                   T.lr_psh.as<int8_t>(), err);
+      GX_HIP(hipGetLastError());
+    }
     k_mx_step_shifts<<<nblk(T.nsteps * 16), 256, 0, s>>>(T.step_ids.as<uint32_t>(), T.step_key.as<uint32_t>(),
                                                          h->mx_sig.as<int>(), h->mx_nexp.as<int8_t>(), T.nsteps,
-                                                         (uint32_t)nn, T.step_sh.as<uint32_t>(), err);
+                                                         (uint32_t)nn, T.group, T.step_sh.as<uint32_t>(), err);
     GX_HIP(hipGetLastError());
     uint32_t herr = 0;
     GX_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s));
@@ -1108,6 +1147,7 @@ static int setup_impl(galax_fmm* h) {
     k_mx_pair_rows<<<nblk(nt), 256, 0, s>>>(h->tree.m2l_off.as<uint32_t>(), nt, tgt);
     GX_HIP(hipGetLastError());
   }
+  h->mx_fwd.group = h->cfg.precision == GALAX_PREC_FP64 ? mx_group_of(MX_F64) : mx_group_of(MX_F32);
   GX_CHECK(build_tiles(h, tgt, h->tree.m2l_dat.as<uint32_t>(), nt, h->mx_fwd));
   if (h->cfg.precision == GALAX_PREC_FP64) {
     GX_CHECK((operators64<P, false>(h)));
@@ -1134,6 +1174,7 @@ static int adjoint_setup_impl(galax_fmm* h) {
     GX_HIP(hipGetLastError());
   }
   // rows are sources; every source of a live pair is a live node (same level)
+  h->mx_adj.group = mx_group_of(MX_F64);
   GX_CHECK(build_tiles(h, h->tree.m2l_dat.as<uint32_t>(), tgt, nt, h->mx_adj));
   GX_CHECK((operators64<P, true>(h)));
   GX_HIP(hipStreamSynchronize(s));
@@ -1161,9 +1202,8 @@ static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
   if (rows32 && !h->mx_rows32) return set_error(GALAX_ERR_INVALID, "M2L (mixed): FP32 rows not set up");
   if (rows32 && T.nlong) {
     k_m2l_run_sums32<P><<<nblk(T.nlong, 4 * (64 / sums32_lanes<P>())), 256, 0, s>>>(
-        T.lr_order.as<uint32_t>(), T.lr_first.as<uint32_t>(), T.lr_len.as<uint32_t>(), (uint32_t)T.nlong,
-        T.src.as<uint32_t>(), h->mp32.as<float>(), h->mx_nexp.as<int8_t>(), T.lr_key.as<uint32_t>(),
-        h->mx_sig.as<int>(), h->mp32.as<float>() + (size_t)h->tree.n_nodes * (16 * row_blocks(P)));
+        T.lr_tab.as<uint4>(), (uint32_t)T.nlong, T.src.as<uint32_t>(), T.lr_psh.as<int8_t>(), h->mp32.as<float>(),
+        h->mp32.as<float>() + (size_t)h->tree.n_nodes * (16 * row_blocks(P)));
     GX_HIP(hipGetLastError());
   } else if (T.nlong) {
     GX_CHECK(dbuf_reserve(T.virt, T.nlong * S * 8, s));
@@ -1200,7 +1240,8 @@ static int run_tiles(galax_fmm* h, M2LTiles& T, const double* in, double* out) {
   a.out = out;
   a.part = T.part.as<double>();
   if (!TR && h->timing) GX_HIP(hipEventRecord(h->ev[8], s));
-  k_m2l_tiles<P, MODE, TR><<<a.nitems, 256, 0, s>>>(a);
+  if (T.group != mx_group_of(MODE)) return set_error(GALAX_ERR_INVALID, "M2L tiles: built for another form");
+  k_m2l_tiles<P, MODE, TR, mx_group_of(MODE)><<<a.nitems, 256, 0, s>>>(a);
   GX_HIP(hipGetLastError());
   if (!TR && h->timing) GX_HIP(hipEventRecord(h->ev[9], s));
   if (T.nmulti) {
