@@ -98,8 +98,17 @@ struct SymP2P {
   DBuf heavy_leaf, heavy_p0, heavy_n;      // leaves with several chunks: first chunk sum, count
   DBuf rel;                                // float4 per body: x - com, y - com, m, 0 (leaf COM)
   DBuf own2, ownp, pair2, pairp, cpart2, cpartp;  // partial sums
+  // Leaves of up to 2 SYM_LEAF bodies (halves): the structure above is built
+  // over halves of at most SYM_LEAF bodies (the first SYM_LEAF bodies, then
+  // the rest), each with its body range, its leaf's COM, and its leaf (whose
+  // local expansion and parent give its far field).
+  bool halves = false;
+  DBuf vbs, vbe, vcx, vcy, vleaf;
 };
 int build_sym_p2p(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, uint32_t n_leaves, SymP2P& sp);
+// The symmetric near field of the tree's leaves (n_max <= 2 SYM_LEAF): over the
+// leaves themselves, or over their halves when n_max > SYM_LEAF.
+int build_sym_near(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, uint32_t n_leaves, SymP2P& sp);
 void sym_p2p_free(SymP2P& sp);
 
 // M2L work in tiles of 16 rows (targets, or sources for the adjoint). The live

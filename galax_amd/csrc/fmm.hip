@@ -1281,10 +1281,14 @@ __device__ __forceinline__ float row_rot(float v) {
 // block (j-side sums only, every ordered pair of t once, the diagonal masked)
 // and its far field in FP64 (as in k_l2p_p2p), go to the item's own slot.
 // Padding bodies sit at 3e19 with zero mass: r^2 overflows to inf, 1/r = 0.
+// Leaves here are the tree's leaves, or halves of them (vleaf: each half's
+// leaf, whose local expansion and parent give the far field; NULL: the
+// leaves themselves); bs, be, cx, cy: body range and COM of each.
 template <int P, bool PHI>
 __global__ void __launch_bounds__(64) k_p2p_sym(
     const float4* __restrict__ rel, const double* __restrict__ x, const double* __restrict__ y,
-    const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be, const double* __restrict__ comx,
+    const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be, const double* __restrict__ cx,
+    const double* __restrict__ cy, const uint32_t* __restrict__ vleaf, const double* __restrict__ comx,
     const double* __restrict__ comy, const double* __restrict__ loc, const uint32_t* __restrict__ par, int fuse_l2l,
     const uint32_t* __restrict__ own_off, const uint32_t* __restrict__ own_dat, const uint32_t* __restrict__ itgt,
     const uint32_t* __restrict__ iu0, const uint32_t* __restrict__ iu1, const uint32_t* __restrict__ iown,
@@ -1298,7 +1302,7 @@ __global__ void __launch_bounds__(64) k_p2p_sym(
   const bool first = u0 == own_off[t];
   const int lane = threadIdx.x, r = lane >> 4, q = lane & 15;
   const uint32_t ts = bs[t], na = be[t] - ts;
-  const double ox = comx[t], oy = comy[t];
+  const double ox = cx[t], oy = cy[t];
   const uint32_t ia = 16u * (uint32_t)(r & 1) + (uint32_t)q;   // i-layout
   const uint32_t jb = 16u * (uint32_t)(r >> 1) + (uint32_t)q;  // j-layout
   float xi = FAR, yi = FAR, mi = 0.f;
@@ -1335,12 +1339,12 @@ __global__ void __launch_bounds__(64) k_p2p_sym(
   }
   // partner data one step ahead: its loads are in flight during the
   // previous partner's arithmetic
-  auto fetch = [&](uint32_t u, uint32_t& nb, float4& v, double& cx, double& cy) {
+  auto fetch = [&](uint32_t u, uint32_t& nb, float4& v, double& bx, double& by) {
     const uint32_t b = own_dat[u];
     const uint32_t b0 = bs[b];
     nb = be[b] - b0;
-    cx = comx[b];
-    cy = comy[b];
+    bx = cx[b];
+    by = cy[b];
     v = make_float4(FAR, FAR, 0.f, 0.f);
     if (jb < nb) v = rel[b0 + jb];
   };
@@ -1412,9 +1416,10 @@ __global__ void __launch_bounds__(64) k_p2p_sym(
     sx = __shfl(sx, src, 64);
     sy = __shfl(sy, src, 64);
     if (PHI) sp = __shfl(sp, src, 64);
-    const uint32_t pa = fuse_l2l ? par[t] : NONE;
+    const uint32_t lt = vleaf ? vleaf[t] : t;
+    const uint32_t pa = fuse_l2l ? par[lt] : NONE;
     for (int k = lane; k < S; k += 64) {
-      sloc[k] = loc[(size_t)t * S + k];
+      sloc[k] = loc[(size_t)lt * S + k];
       if (pa != NONE) spar[k] = loc[(size_t)pa * S + k];
     }
     __syncthreads();
@@ -1699,9 +1704,9 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   // unordered leaf pair once (k_p2p_sym); P2M then also writes the bodies'
   // leaf-relative FP32 coordinates
   SymP2P* sym = nullptr;
-  if (near && fp32_near && h->cfg.n_max <= SYM_LEAF && !h->near_gather && nl > 0) {
+  if (near && fp32_near && h->cfg.n_max <= 2 * SYM_LEAF && !h->near_gather && nl > 0) {
     SymP2P& sp = corrected ? h->c_sym : t.sym;
-    if (!sp.valid) GX_CHECK(build_sym_p2p(h, goff, gdat, (uint32_t)nl, sp));
+    if (!sp.valid) GX_CHECK(build_sym_near(h, goff, gdat, (uint32_t)nl, sp));
     if (d_phi && !sp.pairp.p) {
       GX_CHECK(dbuf_reserve(sp.ownp, std::max<size_t>(sp.n_items, 1) * SYM_LEAF * 8, s));
       GX_CHECK(dbuf_reserve(sp.pairp, std::max<size_t>(sp.n_pairs, 1) * SYM_LEAF * 4, s));
@@ -1811,9 +1816,15 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
     return 0;
   }
   if (sym) {
+    // the leaves, or their halves
+    const uint32_t* sbs = sym->halves ? sym->vbs.as<uint32_t>() : t.body_start.as<uint32_t>();
+    const uint32_t* sbe = sym->halves ? sym->vbe.as<uint32_t>() : t.body_end.as<uint32_t>();
+    const double* scx = sym->halves ? sym->vcx.as<double>() : comx;
+    const double* scy = sym->halves ? sym->vcy.as<double>() : comy;
+    const uint32_t* svl = sym->halves ? sym->vleaf.as<uint32_t>() : nullptr;
 #define GX_LAUNCH_SYM(PH)                                                                                       \
   k_p2p_sym<P, PH><<<(unsigned)sym->n_items, 64, 0, s>>>(                                                     \
-      sym->rel.as<float4>(), d_x, d_y, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), comx, comy, loc, \
+      sym->rel.as<float4>(), d_x, d_y, sbs, sbe, scx, scy, svl, comx, comy, loc,                                \
       t.parent.as<uint32_t>(), fuse_l2l ? 1 : 0, sym->own_off.as<uint32_t>(), sym->own_dat.as<uint32_t>(),      \
       sym->itgt.as<uint32_t>(), sym->iu0.as<uint32_t>(), sym->iu1.as<uint32_t>(), sym->iown.as<uint32_t>(),      \
       (float)(eps * eps), sym->own2.as<double2>(), sym->ownp.as<double>(), sym->pair2.as<float2>(),            \
@@ -1822,14 +1833,14 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   if (h->timing) GX_HIP(hipEventRecord(ev[11], s));                                                             \
   k_sym_chunk_sum<PH><<<(unsigned)((sym->n_chunks + 7) / 8), 256, 0, s>>>(                                     \
       (uint32_t)sym->n_chunks, sym->ch_leaf.as<uint32_t>(), sym->ch_r0.as<uint32_t>(), sym->ch_r1.as<uint32_t>(), \
-      sym->ch_dst.as<uint32_t>(), sym->refs.as<uint32_t>(), t.body_start.as<uint32_t>(),                        \
-      t.body_end.as<uint32_t>(), sym->own2.as<double2>(), sym->ownp.as<double>(), sym->pair2.as<float2>(),     \
+      sym->ch_dst.as<uint32_t>(), sym->refs.as<uint32_t>(), sbs, sbe, sym->own2.as<double2>(),                  \
+      sym->ownp.as<double>(), sym->pair2.as<float2>(),                                                          \
       sym->pairp.as<float>(), sym->cpart2.as<double2>(), sym->cpartp.as<double>(), d_ax, d_ay, d_phi);         \
   GX_HIP(hipGetLastError());                                                                                    \
   if (sym->n_heavy)                                                                                             \
     k_sym_heavy_sum<PH><<<(unsigned)((sym->n_heavy + 7) / 8), 256, 0, s>>>(                                    \
         (uint32_t)sym->n_heavy, sym->heavy_leaf.as<uint32_t>(), sym->heavy_p0.as<uint32_t>(),                   \
-        sym->heavy_n.as<uint32_t>(), t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(),                    \
+        sym->heavy_n.as<uint32_t>(), sbs, sbe,                                                                  \
         sym->cpart2.as<double2>(), sym->cpartp.as<double>(), d_ax, d_ay, d_phi)
     if (d_phi) {
       GX_LAUNCH_SYM(true);

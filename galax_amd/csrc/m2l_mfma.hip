@@ -746,15 +746,21 @@ __global__ void __launch_bounds__(256)
           L[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)tv[rb][e], bs[e], L[rb], 0, 0, 0);
       }
     } else if constexpr (MODE == MX_F32) {
+      // the row blocks' product chains interleaved: each MFMA's successor in
+      // issue order is independent of it
+      f4 acc[RB];
 #pragma unroll
-      for (int rb = 0; rb < RB; ++rb) {
-        f4 acc = {0.f, 0.f, 0.f, 0.f};
+      for (int rb = 0; rb < RB; ++rb) acc[rb] = (f4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int e = 0; e < KS; ++e) acc = __builtin_amdgcn_mfma_f32_16x16x4f32((float)tv[rb][e], b32[e], acc, 0, 0, 0);
+      for (int e = 0; e < KS; ++e)
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+          acc[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)tv[rb][e], b32[e], acc[rb], 0, 0, 0);
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          L[rb][j] += ldexp((double)acc[j], __mul24(nsx, (int)((ord_out[(rb * 4 + j) / 6] >> (5 * ((rb * 4 + j) % 6))) & 31u)));
-      }
+          L[rb][j] += ldexp((double)acc[rb][j], __mul24(nsx, (int)((ord_out[(rb * 4 + j) / 6] >> (5 * ((rb * 4 + j) % 6))) & 31u)));
     } else {
       bf8 bh[KC], bm[KC], bl[KC];
 #pragma unroll

@@ -1424,9 +1424,10 @@ __global__ void k_sym_ref_fill(const uint32_t* __restrict__ item_off, const uint
 void sym_p2p_free(SymP2P& sp) {
   DBuf* b[] = {&sp.own_off, &sp.own_dat, &sp.itgt,       &sp.iu0,      &sp.iu1,     &sp.iown,  &sp.refs,
                &sp.ch_leaf, &sp.ch_r0,   &sp.ch_r1,      &sp.ch_dst,   &sp.heavy_leaf, &sp.heavy_p0, &sp.heavy_n,
-               &sp.rel,     &sp.own2,    &sp.ownp,       &sp.pair2,    &sp.pairp,   &sp.cpart2, &sp.cpartp};
+               &sp.rel,     &sp.own2,    &sp.ownp,       &sp.pair2,    &sp.pairp,   &sp.cpart2, &sp.cpartp,
+               &sp.vbs,     &sp.vbe,     &sp.vcx,        &sp.vcy,      &sp.vleaf};
   for (DBuf* d : b) dbuf_free(*d);
-  sp.valid = false;
+  sp.valid = sp.halves = false;
   sp.n_items = sp.n_pairs = sp.n_leaves = sp.n_chunks = sp.n_heavy = sp.n_cpart = 0;
 }
 
@@ -1550,6 +1551,106 @@ int build_sym_p2p(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, uint
   sp.n_heavy = nheavy;
   sp.n_cpart = ncp;
   sp.valid = true;
+  return 0;
+}
+
+// ---- halves of leaves with more than SYM_LEAF bodies ----
+
+// halves per leaf (1 or 2); entry nl is 0, so the exclusive scan ends with the count
+__global__ void k_vh_count(const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be, uint32_t nl,
+                           uint32_t* __restrict__ nh) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > nl) return;
+  nh[t] = t < nl ? (be[t] - bs[t] > SYM_LEAF ? 2u : 1u) : 0u;
+}
+
+// each leaf's halves: the first SYM_LEAF bodies, then the rest
+__global__ void k_vh_fill(const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be,
+                          const double* __restrict__ comx, const double* __restrict__ comy, uint32_t nl,
+                          const uint32_t* __restrict__ voff, uint32_t* __restrict__ vbs, uint32_t* __restrict__ vbe,
+                          double* __restrict__ vcx, double* __restrict__ vcy, uint32_t* __restrict__ vleaf) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nl) return;
+  const uint32_t v = voff[t], b0 = bs[t], b1 = be[t], mid = min(b1, b0 + SYM_LEAF);
+  vbs[v] = b0;
+  vbe[v] = mid;
+  vcx[v] = comx[t];
+  vcy[v] = comy[t];
+  vleaf[v] = t;
+  if (b1 > mid) {
+    vbs[v + 1] = mid;
+    vbe[v + 1] = b1;
+    vcx[v + 1] = comx[t];
+    vcy[v + 1] = comy[t];
+    vleaf[v + 1] = t;
+  }
+}
+
+// gather-list length of each half: its leaf's partners' halves; entry nv is 0
+__global__ void k_vg_count(const uint32_t* __restrict__ goff, const uint32_t* __restrict__ gdat, uint32_t nl,
+                           const uint32_t* __restrict__ voff, uint32_t* __restrict__ vcnt) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > nl) return;
+  if (t == nl) {
+    vcnt[voff[nl]] = 0u;
+    return;
+  }
+  uint32_t c = 0;
+  for (uint32_t g = goff[t]; g < goff[t + 1]; ++g) c += voff[gdat[g] + 1] - voff[gdat[g]];
+  for (uint32_t v = voff[t]; v < voff[t + 1]; ++v) vcnt[v] = c;
+}
+
+// gather list of each half: its leaf's list, every partner replaced by its halves in order
+__global__ void k_vg_fill(const uint32_t* __restrict__ goff, const uint32_t* __restrict__ gdat, uint32_t nl,
+                          const uint32_t* __restrict__ voff, const uint32_t* __restrict__ vgoff,
+                          uint32_t* __restrict__ vgdat) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nl) return;
+  for (uint32_t v = voff[t]; v < voff[t + 1]; ++v) {
+    uint32_t q = vgoff[v];
+    for (uint32_t g = goff[t]; g < goff[t + 1]; ++g)
+      for (uint32_t w = voff[gdat[g]]; w < voff[gdat[g] + 1]; ++w) vgdat[q++] = w;
+  }
+}
+
+int build_sym_near(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, uint32_t nl, SymP2P& sp) {
+  if (h->cfg.n_max > 2 * SYM_LEAF) return set_error(GALAX_ERR_INVALID, "symmetric near field: leaves too large");
+  sp.halves = h->cfg.n_max > SYM_LEAF;
+  if (!sp.halves) return build_sym_p2p(h, goff, gdat, nl, sp);
+  hipStream_t s = h->stream;
+  const DevTree& t = h->tree;
+  const size_t L1 = (size_t)nl + 1;
+  ScopedDBuf voffb, vgoffb, vgdatb;
+  GX_CHECK(dbuf_reserve(voffb, L1 * 4, s));
+  uint32_t* voff = voffb.as<uint32_t>();
+  k_vh_count<<<nblk(L1), 256, 0, s>>>(t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), nl, voff);
+  GX_HIP(hipGetLastError());
+  uint32_t nv = 0;
+  GX_CHECK(scan_u32(h, voff, voff, L1, false, false, &nv));
+  const size_t V1 = (size_t)nv + 1;
+  GX_CHECK(dbuf_reserve(sp.vbs, V1 * 4, s));
+  GX_CHECK(dbuf_reserve(sp.vbe, V1 * 4, s));
+  GX_CHECK(dbuf_reserve(sp.vcx, V1 * 8, s));
+  GX_CHECK(dbuf_reserve(sp.vcy, V1 * 8, s));
+  GX_CHECK(dbuf_reserve(sp.vleaf, V1 * 4, s));
+  k_vh_fill<<<nblk(nl), 256, 0, s>>>(t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), t.com_x.as<double>(),
+                                     t.com_y.as<double>(), nl, voff, sp.vbs.as<uint32_t>(), sp.vbe.as<uint32_t>(),
+                                     sp.vcx.as<double>(), sp.vcy.as<double>(), sp.vleaf.as<uint32_t>());
+  GX_HIP(hipGetLastError());
+  GX_CHECK(dbuf_reserve(vgoffb, V1 * 4, s));
+  uint32_t* vgoff = vgoffb.as<uint32_t>();
+  k_vg_count<<<nblk(L1), 256, 0, s>>>(goff, gdat, nl, voff, vgoff);
+  GX_HIP(hipGetLastError());
+  uint64_t nvg64 = 0;
+  GX_CHECK(sum_u32(h, vgoff, V1, &nvg64));
+  if (nvg64 >= 0xFFFFFFF0ull) return set_error(GALAX_ERR_UNSUPPORTED, "symmetric near field: too many half pairs");
+  uint32_t nvg = 0;
+  GX_CHECK(scan_u32(h, vgoff, vgoff, V1, false, false, &nvg));
+  GX_CHECK(dbuf_reserve(vgdatb, (size_t)nvg * 4 + 4, s));
+  k_vg_fill<<<nblk(nl), 256, 0, s>>>(goff, gdat, nl, voff, vgoff, vgdatb.as<uint32_t>());
+  GX_HIP(hipGetLastError());
+  GX_CHECK(build_sym_p2p(h, vgoff, vgdatb.as<uint32_t>(), nv, sp));
+  GX_HIP(hipStreamSynchronize(s));
   return 0;
 }
 

@@ -676,14 +676,20 @@ SYM_CASES = [
     ("uniform_nmax13_ref", lambda: ic.uniform(9_001, seed=313), 13, fmm.ALGO_REFERENCE),
     ("plummer_nmax32_corrected", lambda: ic.plummer(30_000, seed=314), 32, fmm.ALGO_CORRECTED),
     ("galaxy_nmax16_corrected", lambda: ic.galaxy(8_192, seed=315), 16, fmm.ALGO_CORRECTED),
+    # leaves of more than 32 bodies: the pairs of their halves
+    ("plummer_nmax64_ref", lambda: ic.plummer(40_000, seed=316), 64, fmm.ALGO_REFERENCE),
+    ("uniform_nmax48_ref", lambda: ic.uniform(9_001, seed=317), 48, fmm.ALGO_REFERENCE),
+    ("plummer_nmax64_corrected", lambda: ic.plummer(30_000, seed=318), 64, fmm.ALGO_CORRECTED),
+    ("galaxy_nmax33_corrected", lambda: ic.galaxy(8_192, seed=319), 33, fmm.ALGO_CORRECTED),
 ]
 
 
 @pytest.mark.parametrize("name,gen,n_max,algo", SYM_CASES, ids=[c[0] for c in SYM_CASES])
 def test_symmetric_near_field_matches_gather_kernel(name, gen, n_max, algo, monkeypatch):
     """The FP32 forms' near field evaluates each unordered leaf pair once
-    (k_p2p_sym, the reference's pair rule a < b, galax-core/src/lib.rs:1415-1437);
-    the gather kernel (GALAX_NEAR_GATHER=1) evaluates every directed pair. Both
+    (k_p2p_sym, the reference's pair rule a < b, galax-core/src/lib.rs:1415-1437;
+    leaves of more than 32 bodies as the pairs of their halves); the gather
+    kernel (GALAX_NEAR_GATHER=1) evaluates every directed pair. Both
     are FP32 on leaf-relative coordinates: forces and potentials agree to FP32
     rounding, on full and partly filled leaves, in both variants. Both stay
     within 1e-5 RMS of FP64."""
