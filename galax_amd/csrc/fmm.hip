@@ -769,12 +769,17 @@ __global__ void k_l2l(const uint32_t* __restrict__ par, const double* __restrict
 
 // parents in [lo, hi): M_p(oi, oj) = sum over children c and i <= oi, j <= oj
 // of M_c(i, j) dx^(oi-i) dy^(oj-j) / ((oi-i)! (oj-j)!), (dx, dy) = com_c - com_p
+// Blocks of 64 threads spread a level's waves over every CU; the register
+// budget (up to 256 per lane) holds the child's expansion, the sums and the
+// powers without spills (at the default budget of 128, p = 8 spilled 1 KB per
+// lane: 240 -> 166 us per evaluation at 2^20 Plummer bodies).
 template <int P>
-__global__ void __launch_bounds__(256) k_m2m_nodes(const uint32_t* __restrict__ cf, const uint8_t* __restrict__ cc,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
+k_m2m_nodes(const uint32_t* __restrict__ cf, const uint8_t* __restrict__ cc,
                                                    const double* __restrict__ comx, const double* __restrict__ comy,
                                                    uint32_t lo, uint32_t hi, double* __restrict__ mp) {
   constexpr int S = num_moments(P);
-  const uint32_t p = lo + blockIdx.x * 256u + threadIdx.x;
+  const uint32_t p = lo + blockIdx.x * 64u + threadIdx.x;
   if (p >= hi) return;
   const uint32_t c0 = cf[p], nc = cc[p];
   const double px = comx[p], py = comy[p];
@@ -901,11 +906,12 @@ __global__ void __launch_bounds__(256) k_l2l_single(const uint32_t* __restrict__
 // children in [lo, hi): L_c(oi, oj) += sum over i >= oi, j >= oj, i + j <= p
 // of L_p(i, j) C(i, oi) C(j, oj) dx^(i-oi) dy^(j-oj), (dx, dy) = com_c - com_p
 template <int P>
-__global__ void __launch_bounds__(256) k_l2l_nodes(const uint32_t* __restrict__ par, const double* __restrict__ comx,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
+k_l2l_nodes(const uint32_t* __restrict__ par, const double* __restrict__ comx,
                                                    const double* __restrict__ comy, uint32_t lo, uint32_t hi,
                                                    double* __restrict__ loc) {
   constexpr int S = num_moments(P);
-  const uint32_t c = lo + blockIdx.x * 256u + threadIdx.x;
+  const uint32_t c = lo + blockIdx.x * 64u + threadIdx.x;
   if (c >= hi) return;
   const uint32_t p = par[c];
   if (p == NONE) return;
@@ -1739,7 +1745,7 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
       }
       if constexpr (P <= 8) {
         if (corrected) {
-          k_m2m_nodes<P><<<nblk(hi - lo), 256, 0, s>>>(t.child_first.as<uint32_t>(), t.child_count.as<uint8_t>(),
+          k_m2m_nodes<P><<<nblk(hi - lo, 64), 64, 0, s>>>(t.child_first.as<uint32_t>(), t.child_count.as<uint8_t>(),
                                                         comx, comy, lo, hi, mp);
           GX_HIP(hipGetLastError());
           continue;
@@ -1791,7 +1797,7 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
         k_l2l_single<P><<<nblk((size_t)(hi - lo) * S), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, lo, hi,
                                                                      loc);
       else if constexpr (P <= 8)
-        k_l2l_nodes<P><<<nblk(hi - lo), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, lo, hi, loc);
+        k_l2l_nodes<P><<<nblk(hi - lo, 64), 64, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, lo, hi, loc);
       else
         k_l2l<P><<<nblk((size_t)(hi - lo) * S), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, lo, hi, loc);
       GX_HIP(hipGetLastError());

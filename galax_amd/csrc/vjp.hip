@@ -471,13 +471,14 @@ __global__ void __launch_bounds__(256) k_vjp_m2m_single(const uint32_t* __restri
 // the cotangents and the offset's powers in registers and the shift
 // constants known at compile time (as k_m2m_nodes / k_l2l_nodes).
 template <int P>
-__global__ void __launch_bounds__(256) k_vjp_l2l_nodes(const uint32_t* __restrict__ cf,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
+k_vjp_l2l_nodes(const uint32_t* __restrict__ cf,
                                                        const uint8_t* __restrict__ cc,
                                                        const double* __restrict__ comx,
                                                        const double* __restrict__ comy, uint32_t lo, uint32_t hi,
                                                        double* __restrict__ lbar) {
   constexpr int S = num_moments(P);
-  const uint32_t p = lo + blockIdx.x * 256u + threadIdx.x;
+  const uint32_t p = lo + blockIdx.x * 64u + threadIdx.x;
   if (p >= hi) return;
   const uint32_t c0 = cf[p], nc = cc[p];
   const double px = comx[p], py = comy[p];
@@ -521,12 +522,13 @@ __global__ void __launch_bounds__(256) k_vjp_l2l_nodes(const uint32_t* __restric
 }
 
 template <int P>
-__global__ void __launch_bounds__(256) k_vjp_m2m_nodes(const uint32_t* __restrict__ par,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
+k_vjp_m2m_nodes(const uint32_t* __restrict__ par,
                                                        const double* __restrict__ comx,
                                                        const double* __restrict__ comy, uint32_t lo, uint32_t hi,
                                                        double* __restrict__ mbar) {
   constexpr int S = num_moments(P);
-  const uint32_t c = lo + blockIdx.x * 256u + threadIdx.x;
+  const uint32_t c = lo + blockIdx.x * 64u + threadIdx.x;
   if (c >= hi) return;
   const uint32_t p = par[c];
   if (p == NONE) return;
@@ -736,7 +738,7 @@ static int vjp_impl(galax_fmm* h, const double* x, const double* y, const double
     }
     if constexpr (P <= 8) {
       if (corrected) {
-        k_vjp_l2l_nodes<P><<<nblk(hi - lo), 256, 0, s>>>(t.child_first.as<uint32_t>(), t.child_count.as<uint8_t>(),
+        k_vjp_l2l_nodes<P><<<nblk(hi - lo, 64), 64, 0, s>>>(t.child_first.as<uint32_t>(), t.child_count.as<uint8_t>(),
                                                           comx, comy, lo, hi, lbar);
         GX_HIP(hipGetLastError());
         continue;
@@ -772,7 +774,7 @@ static int vjp_impl(galax_fmm* h, const double* x, const double* y, const double
     }
     if constexpr (P <= 8) {
       if (corrected) {
-        k_vjp_m2m_nodes<P><<<nblk(hi - lo), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, lo, hi, mbar);
+        k_vjp_m2m_nodes<P><<<nblk(hi - lo, 64), 64, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, lo, hi, mbar);
         GX_HIP(hipGetLastError());
         continue;
       }
