@@ -160,6 +160,10 @@ struct DevTree {
   uint64_t p2p_interactions = 0;  // directed body-body pairs of the gather lists
   P2PItems items;
   SymP2P sym;  // symmetric near field over the gather lists (built on first use)
+  // per level l >= 1: every parent of level l has one child, the parent
+  // level_offset[l] + i that of child level_offset[l - 1] + i (chains of such
+  // levels are shifted in one launch); empty until checked for this tree
+  std::vector<uint8_t> chain_ok;
   size_t capacity = 0;  // node capacity of the node arrays
 };
 

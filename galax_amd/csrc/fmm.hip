@@ -903,6 +903,110 @@ __global__ void __launch_bounds__(256) k_l2l_single(const uint32_t* __restrict__
   loc[(size_t)c * S + k] += v;
 }
 
+// Chains of one-child levels in one launch (corrected variant): levels
+// where the parent lo + i of each level has the child lo - n + i (n nodes per
+// level; checked once per tree by k_chain_check). A block holds 256 / S
+// chains, a thread per (chain, coefficient); the expansion of the chain's
+// current node stays in LDS between levels, and each level's terms are those
+// of k_m2m_single / k_l2l_single (a copy for a child at its parent's COM).
+__global__ void k_chain_check(const uint32_t* __restrict__ cf, uint32_t lo, uint32_t n, uint32_t* __restrict__ bad) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && cf[lo + i] != lo - n + i) atomicOr(bad, 1u);
+}
+
+// M2M up nlev levels: parents lo0 + l n + i, l = 0 .. nlev - 1
+template <int P>
+__global__ void __launch_bounds__(256) k_m2m_chain(const double* __restrict__ comx, const double* __restrict__ comy,
+                                                   uint32_t lo0, uint32_t n, uint32_t nlev, double* __restrict__ mp) {
+  constexpr int S = num_moments(P), NB = 256 / S;
+  __shared__ double sm[NB][S];
+  const int ns = (int)threadIdx.x / S, k = (int)threadIdx.x % S;
+  const uint32_t i = blockIdx.x * NB + (uint32_t)ns;
+  const bool live = ns < NB && i < n;  // every thread reaches the barriers
+  int oi = 0, oj = 0;
+  if (live) {
+    coef_ij<P>(k, oi, oj);
+    sm[ns][k] = mp[(size_t)(lo0 - n + i) * S + k];
+  }
+  __syncthreads();
+  for (uint32_t l = 0; l < nlev; ++l) {
+    const uint32_t p = lo0 + l * n + i, c = p - n;
+    double v = 0.0;
+    if (live) {
+      const double dx = comx[c] - comx[p], dy = comy[c] - comy[p];
+      if (dx == 0.0 && dy == 0.0) {
+        v = sm[ns][k];
+      } else {  // M_p(oi, oj) = sum_{a <= oi, b <= oj} M_c(oi - a, oj - b) dx^a / a! dy^b / b!
+        double ta = 1.0;
+        for (int a = 0; a <= oi; ++a) {
+          if (a > 0) ta *= dx / a;
+          double tb = 1.0;
+          for (int b = 0; b <= oj; ++b) {
+            if (b > 0) tb *= dy / b;
+            v = fma(sm[ns][moment_index(oi - a, oj - b)], ta * tb, v);
+          }
+        }
+      }
+      v = isfinite(v) ? v : 0.0;  // non-finite -> 0 (galax-core/src/lib.rs:934-936)
+    }
+    __syncthreads();
+    if (live) {
+      sm[ns][k] = v;
+      mp[(size_t)p * S + k] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// L2L down nlev levels: children hi0 - l n + i - n of parents hi0 - l n + i,
+// l = 0 .. nlev - 1 (hi0: the first level of the top parents)
+template <int P>
+__global__ void __launch_bounds__(256) k_l2l_chain(const double* __restrict__ comx, const double* __restrict__ comy,
+                                                   uint32_t hi0, uint32_t n, uint32_t nlev, double* __restrict__ loc) {
+  constexpr int S = num_moments(P), NB = 256 / S;
+  __shared__ double sl[NB][S];
+  const int ns = (int)threadIdx.x / S, k = (int)threadIdx.x % S;
+  const uint32_t i = blockIdx.x * NB + (uint32_t)ns;
+  const bool live = ns < NB && i < n;  // every thread reaches the barriers
+  int oi = 0, oj = 0;
+  if (live) {
+    coef_ij<P>(k, oi, oj);
+    sl[ns][k] = loc[(size_t)(hi0 + i) * S + k];
+  }
+  __syncthreads();
+  for (uint32_t l = 0; l < nlev; ++l) {
+    const uint32_t p = hi0 - l * n + i, c = p - n;
+    double v = 0.0;
+    if (live) {
+      const double dx = comx[c] - comx[p], dy = comy[c] - comy[p];
+      double w;
+      if (dx == 0.0 && dy == 0.0) {
+        w = sl[ns][k];
+      } else {  // sum over i >= oi, j >= oj of L_p(i, j) C(i, oi) C(j, oj) dx^(i-oi) dy^(j-oj)
+        w = 0.0;
+        double ea = 1.0;
+        for (int a = oi; a <= P - oj; ++a) {
+          double eb = 1.0;
+          for (int b = oj; a + b <= P; ++b) {
+            const double bc = c_tabs.fact[a] / (c_tabs.fact[oi] * c_tabs.fact[a - oi]) * c_tabs.fact[b] /
+                              (c_tabs.fact[oj] * c_tabs.fact[b - oj]);
+            w = fma(sl[ns][moment_index(a, b)] * bc, ea * eb, w);
+            eb *= dy;
+          }
+          ea *= dx;
+        }
+      }
+      v = loc[(size_t)c * S + k] + w;
+    }
+    __syncthreads();
+    if (live) {
+      sl[ns][k] = v;
+      loc[(size_t)c * S + k] = v;
+    }
+    __syncthreads();
+  }
+}
+
 // children in [lo, hi): L_c(oi, oj) += sum over i >= oi, j >= oj, i + j <= p
 // of L_p(i, j) C(i, oi) C(j, oj) dx^(i-oi) dy^(j-oj), (dx, dy) = com_c - com_p
 template <int P>
@@ -1679,6 +1783,31 @@ static int m2l_keys_setup(galax_fmm* h, uint32_t n_targets) {
   return 0;
 }
 
+// per level l >= 1 of the tree: do its parents form one-child chains
+// aligned with the level below (k_m2m_chain / k_l2l_chain)? Once per tree.
+static int chain_check(galax_fmm* h) {
+  DevTree& t = h->tree;
+  if (t.chain_ok.size() == (size_t)t.n_levels + 1) return 0;
+  hipStream_t s = h->stream;
+  std::vector<uint8_t> ok(t.n_levels + 1, 0);
+  std::vector<uint32_t> bad(t.n_levels + 1, 0);
+  ScopedDBuf flags;
+  GX_CHECK(dbuf_reserve(flags, (t.n_levels + 1) * 4, s));
+  GX_HIP(hipMemsetAsync(flags.p, 0, (t.n_levels + 1) * 4, s));
+  for (uint32_t l = 1; l < t.n_levels; ++l) {
+    const uint32_t lo = t.level_offset[l], n = t.level_offset[l + 1] - lo;
+    if (n == 0 || n != lo - t.level_offset[l - 1]) continue;
+    ok[l] = 1;
+    k_chain_check<<<nblk(n), 256, 0, s>>>(t.child_first.as<uint32_t>(), lo, n, flags.as<uint32_t>() + l);
+    GX_HIP(hipGetLastError());
+  }
+  GX_HIP(hipMemcpyAsync(bad.data(), flags.p, (t.n_levels + 1) * 4, hipMemcpyDeviceToHost, s));
+  GX_HIP(hipStreamSynchronize(s));
+  for (uint32_t l = 0; l <= t.n_levels; ++l) ok[l] = ok[l] && !bad[l];
+  t.chain_ok.swap(ok);
+  return 0;
+}
+
 template <int P>
 static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, double* d_ay,
                       double* d_phi, bool near) {
@@ -1735,8 +1864,18 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   }
   if (h->timing) GX_HIP(hipEventRecord(ev[2], s));
   {
+    if (corrected) GX_CHECK(chain_check(h));
     for (uint32_t l = 1; l < top; ++l) {
       uint32_t lo = t.level_offset[l], hi = t.level_offset[l + 1];
+      if (corrected && t.chain_ok[l]) {  // a chain of one-child levels l .. l2 in one launch
+        uint32_t l2 = l;
+        while (l2 + 1 < top && t.chain_ok[l2 + 1]) ++l2;
+        constexpr int NB = 256 / num_moments(P);
+        k_m2m_chain<P><<<nblk(hi - lo, NB), 256, 0, s>>>(comx, comy, lo, hi - lo, l2 - l + 1, mp);
+        GX_HIP(hipGetLastError());
+        l = l2;
+        continue;
+      }
       if (corrected && hi - lo == lo - t.level_offset[l - 1]) {  // every parent has one child
         k_m2m_single<P><<<nblk((size_t)(hi - lo) * num_moments(P)), 256, 0, s>>>(
             t.child_first.as<uint32_t>(), comx, comy, lo, hi, mp);
@@ -1793,6 +1932,15 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   if (corrected) {
     for (uint32_t l = t.n_levels - 1; l >= l2l_stop; --l) {
       uint32_t lo = t.level_offset[l - 1], hi = t.level_offset[l];
+      if (t.chain_ok[l]) {  // a chain of one-child levels l .. l2 (descending) in one launch
+        uint32_t l2 = l;
+        while (l2 - 1 >= l2l_stop && t.chain_ok[l2 - 1]) --l2;
+        constexpr int NB = 256 / num_moments(P);
+        k_l2l_chain<P><<<nblk(hi - lo, NB), 256, 0, s>>>(comx, comy, hi, hi - lo, l - l2 + 1, loc);
+        GX_HIP(hipGetLastError());
+        l = l2;
+        continue;
+      }
       if (t.level_offset[l + 1] - hi == hi - lo)  // every parent has one child
         k_l2l_single<P><<<nblk((size_t)(hi - lo) * S), 256, 0, s>>>(t.parent.as<uint32_t>(), comx, comy, lo, hi,
                                                                      loc);

@@ -520,6 +520,7 @@ static int build_from_leaf_starts(galax_fmm* h, size_t n, size_t n_leaves, bool 
   k_leaves<<<nblk(n_leaves), 256, 0, s>>>(h->x.as<double>(), h->y.as<double>(), h->m.as<double>(),
                                           h->ws.c.as<uint32_t>(), (uint32_t)n_leaves, a);
   GX_HIP(hipGetLastError());
+  t.chain_ok.clear();
   t.level_offset.assign(1, 0);
   t.level_offset.push_back((uint32_t)n_leaves);
   size_t cs = 0, ce = n_leaves;
@@ -603,6 +604,7 @@ int build_tree_plain(galax_fmm* h, size_t n) {
   t.n_nodes = t.n_leaves = 0;
   t.n_levels = 0;
   t.n_bodies = n;
+  t.chain_ok.clear();
   t.level_offset.assign(1, 0);
   if (n == 0) return 0;
   GX_CHECK(dbuf_reserve(t.keys, n * sizeof(uint64_t), s));
@@ -756,6 +758,7 @@ int build_tree_constrained(galax_fmm* h, size_t n) {
   t.n_nodes = t.n_leaves = 0;
   t.n_levels = 0;
   t.n_bodies = n;
+  t.chain_ok.clear();
   t.level_offset.assign(1, 0);
   if (n == 0) return 0;
   GX_CHECK(dbuf_reserve(t.keys, n * sizeof(uint64_t), s));
@@ -1864,6 +1867,7 @@ int install_tree_host(galax_fmm* h, const galax_tree_view_t* tv, const galax_lis
   t.n_leaves = nl;
   t.n_bodies = nb;
   t.n_levels = nn ? level[nn - 1] + 1 : 0;
+  t.chain_ok.clear();
   t.level_offset.assign(t.n_levels + 1, (uint32_t)nn);
   t.level_offset[0] = 0;
   for (uint32_t l = 1; l <= t.n_levels; ++l) {
