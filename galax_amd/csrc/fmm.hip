@@ -1842,7 +1842,7 @@ static int run_passes(galax_fmm* h, double* d_x, double* d_y, double* d_m, doubl
   if (near && fp32_near && h->cfg.n_max <= 2 * SYM_LEAF && !h->near_gather && nl > 0) {
     SymP2P& sp = corrected ? h->c_sym : t.sym;
     if (!sp.valid) GX_CHECK(build_sym_near(h, goff, gdat, (uint32_t)nl, sp));
-    if (d_phi && !sp.pairp.p) {
+    if (d_phi) {  // the potentials' slots, sized for this tree (a rebuilt tree may have more pairs)
       GX_CHECK(dbuf_reserve(sp.ownp, std::max<size_t>(sp.n_items, 1) * SYM_LEAF * 8, s));
       GX_CHECK(dbuf_reserve(sp.pairp, std::max<size_t>(sp.n_pairs, 1) * SYM_LEAF * 4, s));
       GX_CHECK(dbuf_reserve(sp.cpartp, std::max<size_t>(sp.n_cpart, 1) * SYM_LEAF * 8, s));

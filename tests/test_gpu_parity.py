@@ -712,6 +712,34 @@ def test_symmetric_near_field_matches_gather_kernel(name, gen, n_max, algo, monk
         assert np.sqrt(np.mean(r ** 2)) <= 1e-5, (label, np.sqrt(np.mean(r ** 2)))
 
 
+@pytest.mark.parametrize("algo,n_max", [(fmm.ALGO_REFERENCE, 32), (fmm.ALGO_CORRECTED, 32), (fmm.ALGO_REFERENCE, 64)],
+                         ids=["reference", "corrected", "reference_nmax64"])
+def test_symmetric_near_field_after_rebuild_with_potential(algo, n_max):
+    """A handle that evaluated forces and potentials on one tree, then builds a
+    larger one (more leaf pairs, more work items) and evaluates again: its
+    results equal a fresh handle's bit for bit. (The potentials' near-field
+    slots were once sized at their first use only, so the second tree's
+    evaluation wrote past them.)"""
+    bounds = (-50.0, 50.0, -50.0, 50.0)
+    small, big = ic.plummer(12_000, seed=321), ic.plummer(40_000, seed=322)
+    kw = dict(p=6, eps=0.05, n_max=n_max, bounds=bounds, precision=fmm.PREC_MIXED_FP32, algo=algo, theta=0.4)
+    ctx = _ctx(len(big["x"]), **kw)
+    b = _bodies(small)
+    ctx.build_tree(b)
+    b.phi = np.zeros(len(b))
+    ctx.evaluate(b)
+    b2 = _bodies(big)
+    ctx.build_tree(b2)
+    b2.phi = np.zeros(len(b2))
+    ctx.evaluate(b2)
+    fresh, b3, _ = _gpu_build(big, **kw)
+    b3.phi = np.zeros(len(b3))
+    fresh.evaluate(b3)
+    for u, v in ((b2.ax, b3.ax), (b2.ay, b3.ay), (b2.phi, b3.phi)):
+        assert np.isfinite(u).all()
+        assert np.array_equal(u, v)
+
+
 def test_mixed_forms_match_fp64_plummer1m_p6():
     """The full 2^20-body Plummer workload at p = 6 (size-independent check:
     mixed vs FP64 of the same algorithm)."""
