@@ -193,6 +193,16 @@ __global__ void k_mx_long_table(const uint32_t* __restrict__ gfirst, const uint3
   }
 }
 
+// sort key of each long run: 32 - its length's bit length (longest class first), 6 bits
+__global__ void k_mx_len_class(const uint32_t* __restrict__ lr_len, uint32_t n, unsigned long long* __restrict__ key,
+                               uint32_t* __restrict__ val) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const uint32_t len = lr_len[r];
+  key[r] = (unsigned long long)(len ? __clz(len) : 32);
+  val[r] = r;
+}
+
 __global__ void k_mx_hub_flags(const uint32_t* __restrict__ lr_len, size_t n, uint32_t* __restrict__ f) {
   size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r < n) f[r] = lr_len[r] > MX_WAVE_RUN ? 1u : 0u;
@@ -964,19 +974,18 @@ static int build_tiles(galax_fmm* h, const uint32_t* row_of_pair, const uint32_t
       k_mx_hub_list<<<nblk(nlr), 256, 0, s>>>(T.lr_len.as<uint32_t>(), hf, nlr, T.hubs.as<uint32_t>());
       GX_HIP(hipGetLastError());
       // the pre-sum kernel's order: decreasing power-of-two length class
-      // (stable), so that the two runs of a wave are alike, the longest start
-      // first and neighbouring runs of one class stay together
-      std::vector<uint32_t> len(nlr), ord(nlr);
-      GX_HIP(hipMemcpyAsync(len.data(), T.lr_len.p, (size_t)nlr * 4, hipMemcpyDeviceToHost, s));
-      GX_HIP(hipStreamSynchronize(s));
-      for (uint32_t q = 0; q < nlr; ++q) ord[q] = q;
-      auto cls = [](uint32_t n) { return n ? 32 - __builtin_clz(n) : 0; };
-      std::stable_sort(ord.begin(), ord.end(), [&](uint32_t u, uint32_t v) { return cls(len[u]) > cls(len[v]); });
-      GX_CHECK(upload_u32(T.lr_order, ord, s));
+      // (a stable sort on the device), so that the runs of a wave are alike,
+      // the longest start first and neighbouring runs of one class stay together
+      k_mx_len_class<<<nblk(nlr), 256, 0, s>>>(T.lr_len.as<uint32_t>(), nlr, ka, va);
+      GX_HIP(hipGetLastError());
+      GX_CHECK(radix_sort_bits(h, reinterpret_cast<uint64_t*>(ka), va, reinterpret_cast<uint64_t*>(kb2), vb, nlr, 6));
+      GX_CHECK(dbuf_reserve(T.lr_order, (size_t)nlr * 4, s));
+      GX_HIP(hipMemcpyAsync(T.lr_order.p, va, (size_t)nlr * 4, hipMemcpyDeviceToDevice, s));
       // hub runs in chunks of MX_HUB_CHUNK rows
       T.nhub_chunks = 0;
       if (nh) {
-        std::vector<uint32_t> hubs(nh), first(nlr), cf, cl, coff(nh + 1, 0);
+        std::vector<uint32_t> len(nlr), hubs(nh), first(nlr), cf, cl, coff(nh + 1, 0);
+        GX_HIP(hipMemcpyAsync(len.data(), T.lr_len.p, (size_t)nlr * 4, hipMemcpyDeviceToHost, s));
         GX_HIP(hipMemcpyAsync(hubs.data(), T.hubs.p, (size_t)nh * 4, hipMemcpyDeviceToHost, s));
         GX_HIP(hipMemcpyAsync(first.data(), T.lr_first.p, (size_t)nlr * 4, hipMemcpyDeviceToHost, s));
         GX_HIP(hipStreamSynchronize(s));
