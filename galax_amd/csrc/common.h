@@ -104,8 +104,13 @@ struct SymP2P {
   // local expansion and parent give its far field).
   bool halves = false;
   DBuf vbs, vbe, vcx, vcy, vleaf;
+  DBuf ab2;  // the VJP's cotangents per body in FP32 (its near-field adjoint reuses this structure)
 };
 int build_sym_p2p(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, uint32_t n_leaves, SymP2P& sp);
+// k_sym_chunk_sum / k_sym_heavy_sum over the structure's leaves (or halves)
+// bs, be: each leaf's slots summed into (o0, o1) and, if o2, the third value.
+int sym_combine(const SymP2P& sp, const uint32_t* bs, const uint32_t* be, double* o0, double* o1, double* o2,
+                hipStream_t s);
 // The symmetric near field of the tree's leaves (n_max <= 2 SYM_LEAF): over the
 // leaves themselves, or over their halves when n_max > SYM_LEAF.
 int build_sym_near(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, uint32_t n_leaves, SymP2P& sp);
@@ -297,6 +302,17 @@ struct Near<float> {
   typedef float4 V4;
   static __device__ __forceinline__ float rsq(float v) { return __builtin_amdgcn_rsqf(v); }
 };
+
+// v from lane (q + K) mod 16 or (q - K) mod 16 of the caller's 16-lane row
+// (a DPP row rotation; the compiler folds it into the instruction that reads
+// v). Which of the two does not matter: rotation 16 - K undoes rotation K.
+template <int K>
+__device__ __forceinline__ float row_rot(float v) {
+  if constexpr ((K & 15) == 0)
+    return v;
+  else
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x120 + (K & 15), 0xF, 0xF, true));
+}
 
 // ---------------------------------------------------------- evaluation ----
 // near = false computes the expansions only (P2M .. L2L), for the VJP

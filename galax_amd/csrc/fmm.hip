@@ -1366,17 +1366,6 @@ hipError_t p2p_combine(const P2PItems& it, const uint32_t* goff, const uint32_t*
 
 // ------------------------------------------------ symmetric near field ----
 
-// v from lane (q + K) mod 16 or (q - K) mod 16 of the caller's 16-lane row
-// (a DPP row rotation; the compiler folds it into the instruction that reads
-// v). Which of the two does not matter: rotation 16 - K undoes rotation K.
-template <int K>
-__device__ __forceinline__ float row_rot(float v) {
-  if constexpr ((K & 15) == 0)
-    return v;
-  else
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x120 + (K & 15), 0xF, 0xF, true));
-}
-
 // One wave per work item: leaf t (target side, "i") with its owned partners
 // own_dat[u0 .. u1) ("j"), in FP32 on coordinates relative to t's COM (each
 // partner's leaf-relative coordinates plus the FP64 COM difference, rounded
@@ -1645,6 +1634,29 @@ __global__ void __launch_bounds__(256) k_sym_heavy_sum(uint32_t nheavy, const ui
   ax[b0 + e] = sx;
   ay[b0 + e] = sy;
   if (PHI) phi[b0 + e] = sp;
+}
+
+// The symmetric near field's combine: each leaf's slots into (o0, o1[, o2]),
+// the two force components and, with o2, the third value (the potential, or
+// in the adjoint m-bar). bs, be: the structure's leaves (or halves).
+int sym_combine(const SymP2P& sp, const uint32_t* bs, const uint32_t* be, double* o0, double* o1, double* o2,
+                hipStream_t s) {
+  auto go = [&](auto phc) -> int {
+    constexpr bool PH = decltype(phc)::value;
+    k_sym_chunk_sum<PH><<<(unsigned)((sp.n_chunks + 7) / 8), 256, 0, s>>>(
+        (uint32_t)sp.n_chunks, sp.ch_leaf.as<uint32_t>(), sp.ch_r0.as<uint32_t>(), sp.ch_r1.as<uint32_t>(),
+        sp.ch_dst.as<uint32_t>(), sp.refs.as<uint32_t>(), bs, be, sp.own2.as<double2>(), sp.ownp.as<double>(),
+        sp.pair2.as<float2>(), sp.pairp.as<float>(), sp.cpart2.as<double2>(), sp.cpartp.as<double>(), o0, o1, o2);
+    GX_HIP(hipGetLastError());
+    if (sp.n_heavy) {
+      k_sym_heavy_sum<PH><<<(unsigned)((sp.n_heavy + 7) / 8), 256, 0, s>>>(
+          (uint32_t)sp.n_heavy, sp.heavy_leaf.as<uint32_t>(), sp.heavy_p0.as<uint32_t>(), sp.heavy_n.as<uint32_t>(),
+          bs, be, sp.cpart2.as<double2>(), sp.cpartp.as<double>(), o0, o1, o2);
+      GX_HIP(hipGetLastError());
+    }
+    return 0;
+  };
+  return o2 ? go(std::true_type{}) : go(std::false_type{});
 }
 
 // ------------------------------------------------------ direct sum ----

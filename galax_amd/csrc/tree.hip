@@ -1428,7 +1428,7 @@ void sym_p2p_free(SymP2P& sp) {
   DBuf* b[] = {&sp.own_off, &sp.own_dat, &sp.itgt,       &sp.iu0,      &sp.iu1,     &sp.iown,  &sp.refs,
                &sp.ch_leaf, &sp.ch_r0,   &sp.ch_r1,      &sp.ch_dst,   &sp.heavy_leaf, &sp.heavy_p0, &sp.heavy_n,
                &sp.rel,     &sp.own2,    &sp.ownp,       &sp.pair2,    &sp.pairp,   &sp.cpart2, &sp.cpartp,
-               &sp.vbs,     &sp.vbe,     &sp.vcx,        &sp.vcy,      &sp.vleaf};
+               &sp.vbs,     &sp.vbe,     &sp.vcx,        &sp.vcy,      &sp.vleaf,   &sp.ab2};
   for (DBuf* d : b) dbuf_free(*d);
   sp.valid = sp.halves = false;
   sp.n_items = sp.n_pairs = sp.n_leaves = sp.n_chunks = sp.n_heavy = sp.n_cpart = 0;

@@ -16,6 +16,7 @@
 // Every stage is a gather (one writer per output), so results are deterministic.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -183,6 +184,157 @@ __global__ void __launch_bounds__(64) k_vjp_near(const double* __restrict__ x, c
         pp[2] = (double)gm_;
       }
     }
+  }
+}
+
+// Adjoint of the FP32 forms' symmetric near field (k_p2p_sym's work items,
+// leaves or halves, and pair layout): each unordered pair once, in FP32 on
+// leaf-relative coordinates. With d = x_i - x_j, w = m_j abar_i - m_i abar_j
+// and v = w / r^3 - 3 d (d . w) / r^5 (the gather form's terms for both bodies):
+//   xbar_j += v, xbar_i -= v, mbar_j -= abar_i . d / r^3, mbar_i += abar_j . d / r^3.
+// The i side (the item's leaf, sums per rotation K rotated back at the end)
+// and, on its first item, the leaf's self block (every ordered pair, j side)
+// go to the item's own slot; the j side to the pair's slot. sym_combine then
+// adds each body's slots (xbar, ybar as the force pair, mbar as the third
+// value). Padding bodies sit at 3e19 with zero mass and cotangent: r^2
+// overflows to inf, 1/r = 0, and every term of a real body with them is 0.
+__global__ void k_vjp_sym_prep(uint32_t nlv, const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be,
+                               const double* __restrict__ cx, const double* __restrict__ cy,
+                               const double* __restrict__ x, const double* __restrict__ y,
+                               const double* __restrict__ m, const double* __restrict__ axb,
+                               const double* __restrict__ ayb, float4* __restrict__ rel, float2* __restrict__ ab) {
+  const uint32_t v = blockIdx.x * 8u + (threadIdx.x >> 5), e = threadIdx.x & 31u;
+  if (v >= nlv) return;
+  const uint32_t b = bs[v] + e;
+  if (b >= be[v]) return;
+  rel[b] = make_float4((float)(x[b] - cx[v]), (float)(y[b] - cy[v]), (float)m[b], 0.f);
+  ab[b] = make_float2((float)axb[b], (float)ayb[b]);
+}
+
+__global__ void __launch_bounds__(64) k_vjp_sym(const float4* __restrict__ rel, const float2* __restrict__ ab,
+                                                const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be,
+                                                const double* __restrict__ cx, const double* __restrict__ cy,
+                                                const uint32_t* __restrict__ own_off,
+                                                const uint32_t* __restrict__ own_dat,
+                                                const uint32_t* __restrict__ itgt, const uint32_t* __restrict__ iu0,
+                                                const uint32_t* __restrict__ iu1, const uint32_t* __restrict__ iown,
+                                                float eps2, double2* __restrict__ own2, double* __restrict__ ownp,
+                                                float2* __restrict__ pair2, float* __restrict__ pairp) {
+  constexpr float FAR = 3e19f;
+  const uint32_t item = xcd_runs(blockIdx.x, gridDim.x);
+  const uint32_t t = itgt[item], u0 = iu0[item], u1 = iu1[item], own = iown[item];
+  const bool first = u0 == own_off[t];
+  const int lane = threadIdx.x, r = lane >> 4, q = lane & 15;
+  const uint32_t ts = bs[t], na = be[t] - ts;
+  const double ox = cx[t], oy = cy[t];
+  const uint32_t ia = 16u * (uint32_t)(r & 1) + (uint32_t)q;   // i-layout
+  const uint32_t jb = 16u * (uint32_t)(r >> 1) + (uint32_t)q;  // j-layout
+  float xi = FAR, yi = FAR, mi = 0.f, axi = 0.f, ayi = 0.f;
+  if (ia < na) {
+    const float4 v = rel[ts + ia];
+    const float2 a = ab[ts + ia];
+    xi = v.x;
+    yi = v.y;
+    mi = v.z;
+    axi = a.x;
+    ayi = a.y;
+  }
+  float gix[16], giy[16], gim[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) gix[k] = giy[k] = gim[k] = 0.f;
+  // one pair (i rotated by K, j in the lane): the j side's terms; the i side's
+  // into gix[K] etc. when BOTH
+  auto pair = [&](auto kc, float xj, float yj, float mj, float axj, float ayj, bool zero, float& sx, float& sy,
+                  float& sm, auto both) {
+    constexpr int K = decltype(kc)::value;
+    const float dx = row_rot<K>(xi) - xj, dy = row_rot<K>(yi) - yj;  // x_i - x_j
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, eps2));
+    float rs = Near<float>::rsq(r2);
+    if (zero) rs = 0.f;
+    const float rs2 = rs * rs, r3 = rs2 * rs, r5x3 = r3 * (3.f * rs2);
+    const float mri = row_rot<K>(mi);
+    const float wx = fmaf(mj, row_rot<K>(axi), -mri * axj), wy = fmaf(mj, row_rot<K>(ayi), -mri * ayj);
+    const float tt = fmaf(dx, wx, dy * wy) * r5x3;
+    const float vx = fmaf(wx, r3, -dx * tt), vy = fmaf(wy, r3, -dy * tt);
+    sx += vx;
+    sy += vy;
+    sm -= fmaf(row_rot<K>(axi), dx, row_rot<K>(ayi) * dy) * r3;
+    if constexpr (decltype(both)::value) {
+      gix[K] -= vx;
+      giy[K] -= vy;
+      gim[K] += fmaf(axj, dx, ayj * dy) * r3;
+    }
+  };
+  // self block: every ordered pair (i, j) of t, sums on the j side
+  float sx = 0.f, sy = 0.f, sm = 0.f;
+  if (first) {
+    float xj = FAR, yj = FAR, mj = 0.f, axj = 0.f, ayj = 0.f;
+    if (jb < na) {
+      const float4 v = rel[ts + jb];
+      const float2 a = ab[ts + jb];
+      xj = v.x;
+      yj = v.y;
+      mj = v.z;
+      axj = a.x;
+      ayj = a.y;
+    }
+    const bool diag = (r & 1) == (r >> 1);  // rows 0 and 3: i == j at K = 0
+    static_for<0, 16>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      pair(kc, xj, yj, mj, axj, ayj, K == 0 && diag, sx, sy, sm, std::false_type{});
+    });
+  }
+  for (uint32_t u = u0; u < u1; ++u) {
+    const uint32_t b = own_dat[u];
+    const uint32_t b0 = bs[b], nb = be[b] - b0;
+    float xj = FAR, yj = FAR, mj = 0.f, axj = 0.f, ayj = 0.f;
+    if (jb < nb) {
+      const float4 v = rel[b0 + jb];
+      const float2 a = ab[b0 + jb];
+      xj = v.x + (float)(cx[b] - ox);
+      yj = v.y + (float)(cy[b] - oy);
+      mj = v.z;
+      axj = a.x;
+      ayj = a.y;
+    }
+    float ajx = 0.f, ajy = 0.f, ajm = 0.f;
+    static_for<0, 16>([&](auto kc) { pair(kc, xj, yj, mj, axj, ayj, false, ajx, ajy, ajm, std::true_type{}); });
+    // rows (0, 1) and (2, 3) hold the same j-bodies
+    ajx += __shfl_xor(ajx, 16, 64);
+    ajy += __shfl_xor(ajy, 16, 64);
+    ajm += __shfl_xor(ajm, 16, 64);
+    if ((r & 1) == 0 && jb < nb) {
+      const size_t o = (size_t)u * SYM_LEAF + jb;
+      pair2[o] = make_float2(ajx, ajy);
+      pairp[o] = ajm;
+    }
+  }
+  // the i side: acc_i[K] of lane q belongs to the i of its rotated lane
+  float tix = 0.f, tiy = 0.f, tim = 0.f;
+  static_for<0, 16>([&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    tix += row_rot<16 - K>(gix[K]);
+    tiy += row_rot<16 - K>(giy[K]);
+    tim += row_rot<16 - K>(gim[K]);
+  });
+  tix += __shfl_xor(tix, 32, 64);  // rows r and r ^ 2 hold the same i-bodies
+  tiy += __shfl_xor(tiy, 32, 64);
+  tim += __shfl_xor(tim, 32, 64);
+  if (first) {  // uniform over the block: the self block's j-side sums to the i layout
+    sx += __shfl_xor(sx, 16, 64);
+    sy += __shfl_xor(sy, 16, 64);
+    sm += __shfl_xor(sm, 16, 64);
+    const int src = lane < 16 ? lane : lane + 16;  // body 16 + q sits in row 2
+    sx = __shfl(sx, src, 64);
+    sy = __shfl(sy, src, 64);
+    sm = __shfl(sm, src, 64);
+  } else {
+    sx = sy = sm = 0.f;
+  }
+  if (lane < 32 && (uint32_t)lane < na) {
+    const size_t o = (size_t)own * SYM_LEAF + (size_t)lane;
+    own2[o] = make_double2((double)(tix + sx), (double)(tiy + sy));
+    ownp[o] = (double)(tim + sm);
   }
 }
 
@@ -709,6 +861,30 @@ static int vjp_impl(galax_fmm* h, const double* x, const double* y, const double
   GX_CHECK(dbuf_reserve(h->p2p_part, std::max<size_t>(it.n_part, 1) * nmax * 3 * 8, s));
   const bool fp32_near =
       h->cfg.precision == GALAX_PREC_MIXED_FP32 || h->cfg.precision == GALAX_PREC_MIXED_BF16X3;
+  if (fp32_near && h->cfg.n_max <= 2 * SYM_LEAF && !h->near_gather && nl > 0) {
+    // the FP32 forms: each unordered leaf pair once (the forward's structure)
+    SymP2P& sp = corrected ? h->c_sym : t.sym;
+    if (!sp.valid) GX_CHECK(build_sym_near(h, goff, gdat, (uint32_t)nl, sp));
+    GX_CHECK(dbuf_reserve(sp.ownp, std::max<size_t>(sp.n_items, 1) * SYM_LEAF * 8, s));
+    GX_CHECK(dbuf_reserve(sp.pairp, std::max<size_t>(sp.n_pairs, 1) * SYM_LEAF * 4, s));
+    GX_CHECK(dbuf_reserve(sp.cpartp, std::max<size_t>(sp.n_cpart, 1) * SYM_LEAF * 8, s));
+    GX_CHECK(dbuf_reserve(sp.ab2, std::max<size_t>(n, 1) * 8, s));
+    const uint32_t* sbs = sp.halves ? sp.vbs.as<uint32_t>() : t.body_start.as<uint32_t>();
+    const uint32_t* sbe = sp.halves ? sp.vbe.as<uint32_t>() : t.body_end.as<uint32_t>();
+    const double* scx = sp.halves ? sp.vcx.as<double>() : comx;
+    const double* scy = sp.halves ? sp.vcy.as<double>() : comy;
+    const uint32_t nlv = (uint32_t)sp.n_leaves;
+    k_vjp_sym_prep<<<(nlv + 7) / 8, 256, 0, s>>>(nlv, sbs, sbe, scx, scy, x, y, m, axb, ayb, sp.rel.as<float4>(),
+                                                 sp.ab2.as<float2>());
+    GX_HIP(hipGetLastError());
+    k_vjp_sym<<<(unsigned)sp.n_items, 64, 0, s>>>(
+        sp.rel.as<float4>(), sp.ab2.as<float2>(), sbs, sbe, scx, scy, sp.own_off.as<uint32_t>(),
+        sp.own_dat.as<uint32_t>(), sp.itgt.as<uint32_t>(), sp.iu0.as<uint32_t>(), sp.iu1.as<uint32_t>(),
+        sp.iown.as<uint32_t>(), (float)eps2, sp.own2.as<double2>(), sp.ownp.as<double>(), sp.pair2.as<float2>(),
+        sp.pairp.as<float>());
+    GX_HIP(hipGetLastError());
+    GX_CHECK(sym_combine(sp, sbs, sbe, xb, yb, mb, s));
+  } else {
 #define GX_LAUNCH_VJP_NEAR(T)                                                                                  \
   k_vjp_near<T><<<(unsigned)it.n_items, 64, 0, s>>>(                                                          \
       x, y, m, axb, ayb, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), t.com_x.as<double>(),          \
@@ -719,9 +895,10 @@ static int vjp_impl(galax_fmm* h, const double* x, const double* y, const double
   else
     GX_LAUNCH_VJP_NEAR(double);
 #undef GX_LAUNCH_VJP_NEAR
-  GX_HIP(hipGetLastError());
-  GX_HIP(p2p_combine(it, goff, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), nmax,
-                     h->p2p_part.as<double>(), xb, yb, mb, s));
+    GX_HIP(hipGetLastError());
+    GX_HIP(p2p_combine(it, goff, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), nmax,
+                       h->p2p_part.as<double>(), xb, yb, mb, s));
+  }
   // L2P adjoint -> Lbar of leaves
   k_vjp_l2p_coef<P><<<nblk(nl, 4), 256, 0, s>>>(x, y, axb, ayb, t.body_start.as<uint32_t>(),
                                                 t.body_end.as<uint32_t>(), comx, comy, (uint32_t)nl, lbar);

@@ -51,6 +51,53 @@ def test_vjp_matches_central_differences(algo, p):
     assert worst < 1e-5, worst
 
 
+SYM_VJP_CASES = [
+    ("plummer_nmax32_ref", lambda: ic.plummer(12_000, seed=411), 32, fmm.ALGO_REFERENCE),
+    ("plummer_nmax64_ref", lambda: ic.plummer(20_000, seed=412), 64, fmm.ALGO_REFERENCE),
+    ("uniform_nmax13_ref", lambda: ic.uniform(9_001, seed=413), 13, fmm.ALGO_REFERENCE),
+    ("plummer_nmax32_corrected", lambda: ic.plummer(10_000, seed=414), 32, fmm.ALGO_CORRECTED),
+    ("galaxy_nmax48_corrected", lambda: ic.galaxy(8_192, seed=415), 48, fmm.ALGO_CORRECTED),
+]
+
+
+@pytest.mark.parametrize("name,gen,n_max,algo", SYM_VJP_CASES, ids=[c[0] for c in SYM_VJP_CASES])
+def test_symmetric_near_adjoint_matches_gather_adjoint(name, gen, n_max, algo, monkeypatch):
+    """The FP32 forms' near-field adjoint evaluates each unordered leaf pair
+    once (k_vjp_sym, the symmetric near field's items; leaves of more than 32
+    bodies as their halves); with GALAX_NEAR_GATHER=1 it is the gather kernel,
+    every directed pair. Both are FP32 on leaf-relative coordinates: the
+    cotangents agree to FP32 rounding (relative norm 2e-6 on these cases,
+    1e-5 where a 1000:1e-3 mass ratio makes mbar cancel), and both stay within
+    1e-5 of FP64."""
+    d = gen()
+    bounds = (0.0, 100.0, 0.0, 100.0) if name.startswith("uniform") else (-50.0, 50.0, -50.0, 50.0)
+    r = ic.SplitMix64(17)
+    axb, ayb = r.normal(len(d["x"])), r.normal(len(d["x"]))
+    out = {}
+    for label, env, prec in (("sym", "0", fmm.PREC_MIXED_FP32), ("gather", "1", fmm.PREC_MIXED_FP32),
+                             ("fp64", "0", fmm.PREC_FP64)):
+        monkeypatch.setenv("GALAX_NEAR_GATHER", env)
+        n = len(d["x"])
+        ctx = fmm.GpuContext(fmm.GpuConfig(max_n=n, p=6, eps=0.05, n_max=n_max, bounds=bounds, algo=algo,
+                                           theta=0.4, precision=prec))
+        b = fmm.BodiesSoA.from_dict(d)
+        ctx.build_tree(b)
+        _forward(ctx, b)
+        g = ctx.vjp(b, axb, ayb)
+        assert all(np.isfinite(v).all() for v in g), label
+        out[label] = g
+    tol = 1e-5 if name.startswith("galaxy") else 2e-6
+    for k in range(3):
+        ref = np.linalg.norm(out["gather"][k])
+        print(name, k, "sym-gather", np.linalg.norm(out["sym"][k] - out["gather"][k]) / ref,
+              "sym-fp64", np.linalg.norm(out["sym"][k] - out["fp64"][k]) / np.linalg.norm(out["fp64"][k]),
+              "gather-fp64", np.linalg.norm(out["gather"][k] - out["fp64"][k]) / np.linalg.norm(out["fp64"][k]),
+              flush=True)
+        assert np.linalg.norm(out["sym"][k] - out["gather"][k]) <= tol * ref, (k, name)
+        for label in ("sym", "gather"):
+            assert np.linalg.norm(out[label][k] - out["fp64"][k]) <= 1e-5 * np.linalg.norm(out["fp64"][k]), (label, k)
+
+
 @pytest.mark.parametrize("algo", [fmm.ALGO_REFERENCE, fmm.ALGO_CORRECTED])
 def test_mixed_form_vjp_matches_fp64_vjp(algo):
     """The mixed forms run the near-field adjoint in FP32 on leaf-relative
