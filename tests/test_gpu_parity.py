@@ -285,6 +285,52 @@ def test_upload_tree_matches_device_build():
     assert np.median(_rel_diff((b.ax, b.ay), (ax, ay))) <= 1e-6
 
 
+def _swap_nodes(tree, lists, p, q):
+    """The same tree with nodes p and q exchanged: every per-node array
+    reordered, every stored node id (children, parent, list entries) mapped."""
+    nn = len(tree["center_x"])
+    perm = np.arange(nn)
+    perm[p], perm[q] = q, p  # an involution: new node j is old node perm[j], old id i is new id perm[i]
+    new = {k: np.asarray(v)[perm].copy() for k, v in tree.items()}
+    for k in ("children", "parent"):
+        a = new[k]
+        a[a >= 0] = perm[a[a >= 0]]
+    out = []
+    for off, dat in lists:
+        off = np.asarray(off, np.int64)
+        rows = [np.asarray(dat[off[perm[j]]:off[perm[j] + 1]], np.int64) for j in range(nn)]
+        noff = np.zeros(nn + 1, np.uint64)
+        noff[1:] = np.cumsum([len(r) for r in rows])
+        ndat = perm[np.concatenate(rows)].astype(np.uint32) if noff[-1] else np.zeros(0, np.uint32)
+        out.append((noff, ndat))
+    return new, out
+
+
+def test_corrected_one_child_levels_not_aligned():
+    """An uploaded tree whose one-child levels are not aligned with the level
+    below (two parents of such a level exchanged): those levels take the
+    per-level shifts instead of the fused chains (k_chain_check), and the
+    corrected variant's forces equal those on the same tree in its built
+    order to rounding."""
+    d = ic.plummer(20_000, seed=203)
+    bounds = (-50.0, 50.0, -50.0, 50.0)
+    ot, od, _ = _oracle_build(d, 32, bounds)
+    tree, lv = ot.export(), ot.levels()
+    off = np.searchsorted(lv, np.arange(lv.max() + 2))
+    sizes = np.diff(off)
+    assert sizes[3] == sizes[2] == sizes[4] and sizes[3] > 1  # levels 1..: one parent per leaf here
+    lists = [O.m2l_lists(ot).csr(), O.p2p_lists(ot).csr()]
+    res = []
+    for t, (m2l, p2p) in ((tree, lists), _swap_nodes(tree, lists, int(off[3]), int(off[3]) + 1)):
+        ctx = _ctx(len(od["x"]), p=6, eps=0.05, bounds=bounds, algo=fmm.ALGO_CORRECTED, theta=0.4)
+        ctx.upload_tree_data(t, m2l, p2p)
+        b = _bodies(od)
+        ctx.evaluate(b)
+        res.append((b.ax.copy(), b.ay.copy()))
+    rd = _rel_diff(res[1], res[0])
+    assert np.isfinite(rd).all() and rd.max() <= 1e-10, rd.max()
+
+
 CORRECTED_LIST_CASES = [
     ("plummer_t03", lambda: ic.plummer(20000, seed=203), 32, (-50.0, 50.0, -50.0, 50.0), 0.3),
     ("uniform_t05", lambda: ic.uniform(6000, seed=101), 32, (0.0, 100.0, 0.0, 100.0), 0.5),
