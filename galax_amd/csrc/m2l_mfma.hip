@@ -96,7 +96,7 @@ struct Lanes {
 
 // ------------------------------------------------------------- setup ----
 
-constexpr uint32_t MX_ITEM_STEPS = 32;  // steps per work item
+constexpr uint32_t MX_ITEM_STEPS = 64;  // steps per work item (32: +2% on the FP32 pass, 128: +5%)
 constexpr uint32_t MX_WAVE_RUN = 256;   // longest run pre-summed by one wave; longer runs (hubs) go in chunks
 constexpr uint32_t MX_HUB_CHUNK = 128;  // rows per chunk of a hub run
 
