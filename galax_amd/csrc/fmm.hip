@@ -81,8 +81,9 @@ static inline unsigned nblk(size_t n, unsigned bs = 256) { return (unsigned)((n 
 // compute_fmm_force's sanitize to every body (each body is in one leaf),
 // writing fixed values back and counting them in *changed; it zeroes *next,
 // the counter of the following evaluation (two counters in turn). The wave stages a chunk of
-// the leaf's bodies in LDS as m and the power tables dx^0..dx^P, dy^0..dy^P
-// (dx = x - com, powers by repeated products: the values every lane would
+// the leaf's bodies in LDS as the tables m dx^0..m dx^P and dy^0..dy^P
+// (dx = x - com, powers by repeated products; m dx^i is the first product of
+// the reference's m dx^i dy^j, so every lane's term is the value it would
 // compute), then each lane sums its coefficient over the chunk in body order.
 // With S <= 32 the two half-waves take two leaves.
 template <int P>
@@ -97,7 +98,7 @@ __global__ void __launch_bounds__(256) k_p2m(double* __restrict__ x, double* __r
   constexpr int S = num_moments(P);
   constexpr int LPW = S <= 32 ? 2 : 1;  // leaves per wave
   constexpr int LW = 64 / LPW;          // lanes per leaf = staging chunk
-  __shared__ double spx[4][64][P + 1], spy[4][64][P + 1], smb[4][64];
+  __shared__ double smx[4][64][P + 1], spy[4][64][P + 1];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int sub = lane / LW, sl = lane % LW;
   if (blockIdx.x == 0 && threadIdx.x == 0) *next = 0u;
@@ -145,21 +146,21 @@ __global__ void __launch_bounds__(256) k_p2m(double* __restrict__ x, double* __r
         // the symmetric FP32 near field's sources: leaf-relative coordinates
         if (rel && k0 == 0) rel[b] = make_float4((float)dx, (float)dy, (float)mb, 0.f);
         double px = 1.0, py = 1.0;
-        spx[w][base + sl][0] = 1.0;
+        smx[w][base + sl][0] = mb * px;
         spy[w][base + sl][0] = 1.0;
 #pragma unroll
         for (int r = 1; r <= P; ++r) {
           px = px * dx;
           py = py * dy;
-          spx[w][base + sl][r] = px;
+          smx[w][base + sl][r] = mb * px;
           spy[w][base + sl][r] = py;
         }
-        smb[w][base + sl] = mb;
       }
       __syncthreads();
       const uint32_t n = e > s + c ? min((uint32_t)LW, e - s - c) : 0u;
+#pragma unroll 4
       for (uint32_t q = 0; q < n; ++q) {
-        const double t = smb[w][base + q] * spx[w][base + q][i] * spy[w][base + q][j];
+        const double t = smx[w][base + q][i] * spy[w][base + q][j];
         acc += div_exact(t, div, rdiv);
       }
     }
