@@ -943,11 +943,10 @@ __device__ __forceinline__ uint32_t seg_next(uint32_t s) {
   return s ? s + 1 : 0u;
 }
 
-// M2L rows: one thread per node c below the root.
-__global__ void __launch_bounds__(256) k_m2l_rows_seg(NodeArrays a, uint32_t nn, const SegLevel* __restrict__ lvls,
-                                                      uint32_t nlv, SegTree st, uint32_t* __restrict__ rows,
-                                                      uint32_t* __restrict__ lens) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+// M2L row of node c (below the root).
+__device__ __forceinline__ void m2l_row_seg(const NodeArrays& a, uint32_t nn, const SegLevel* __restrict__ lvls,
+                                            uint32_t nlv, const SegTree& st, uint32_t* __restrict__ rows,
+                                            uint32_t* __restrict__ lens, uint32_t c) {
   if (c >= nn) return;
   const uint32_t P = a.par[c];
   const uint32_t lc = a.lv[c];
@@ -963,6 +962,18 @@ __global__ void __launch_bounds__(256) k_m2l_rows_seg(NodeArrays a, uint32_t nn,
   const double px0 = pcx - phw - 1e-15, px1 = pcx + phw + 1e-15, py0 = pcy - phw - 1e-15, py1 = pcy + phw + 1e-15;
   uint32_t* row = rows + (size_t)c * LIST_CAP;
   uint32_t len = 0;
+  // entries go out four at a time (16-byte stores; rows are 256-byte aligned
+  // and LIST_CAP is a multiple of 4): a thread's stores are far apart in
+  // time, and single 4-byte ones each cost a partial-line write to HBM
+  uint32_t b0 = 0, b1 = 0, b2 = 0;
+  auto put = [&](uint32_t q) {
+    const uint32_t r = len & 3u;
+    if (r == 3u) *reinterpret_cast<uint4*>(row + (len - 3u)) = make_uint4(b0, b1, b2, q);
+    b0 = r == 0u ? q : b0;
+    b1 = r == 1u ? q : b1;
+    b2 = r == 2u ? q : b2;
+    ++len;
+  };
   uint32_t s = 1;
   while (s && len < LIST_CAP) {
     if (s >= L.p2) {  // a single node
@@ -980,7 +991,7 @@ __global__ void __launch_bounds__(256) k_m2l_rows_seg(NodeArrays a, uint32_t nn,
         const double cdx = fabs(a.cx[q] - ccx), cdy = fabs(a.cy[q] - ccy);
         const double hs = chw + a.hw[q];
         if (cdx <= hs + 1e-15 && cdy <= hs + 1e-15) continue;
-        row[len++] = q;
+        put(q);
       }
       continue;
     }
@@ -988,15 +999,18 @@ __global__ void __launch_bounds__(256) k_m2l_rows_seg(NodeArrays a, uint32_t nn,
     s = (seg_meets(st, e, px0, px1, py0, py1, magp) && seg_far_child(st, e, ccx, ccy, chw, magc)) ? 2 * s
                                                                                                    : seg_next(s);
   }
+  const uint32_t r = len & 3u, l0 = len - r;  // the last, partial group
+  if (r > 0u) row[l0] = b0;
+  if (r > 1u) row[l0 + 1] = b1;
+  if (r > 2u) row[l0 + 2] = b2;
   lens[c] = len;
 }
 
-// P2P rows: one thread per leaf; the leaf itself first, then every other leaf
-// whose box touches it, ascending (galax-core/src/lib.rs:1371-1389).
-__global__ void __launch_bounds__(256) k_p2p_rows_seg(NodeArrays a, uint32_t n_leaves, const SegLevel* __restrict__ lvls,
-                                                      SegTree st, uint32_t* __restrict__ rows,
-                                                      uint32_t* __restrict__ lens) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+// P2P row of leaf c: the leaf itself first, then every other leaf whose box
+// touches it, ascending (galax-core/src/lib.rs:1371-1389).
+__device__ __forceinline__ void p2p_row_seg(const NodeArrays& a, uint32_t n_leaves, const SegLevel* __restrict__ lvls,
+                                            const SegTree& st, uint32_t* __restrict__ rows,
+                                            uint32_t* __restrict__ lens, uint32_t c) {
   if (c >= n_leaves) return;
   const SegLevel L = lvls[0];
   const double cx = a.cx[c], cy = a.cy[c], hw = a.hw[c];
@@ -1019,6 +1033,19 @@ __global__ void __launch_bounds__(256) k_p2p_rows_seg(NodeArrays a, uint32_t n_l
     s = seg_meets(st, (size_t)L.base + s, x0, x1, y0, y1, mag) ? 2 * s : seg_next(s);
   }
   lens[c] = len;
+}
+
+// Both list builds in one launch, a thread per row: the first nbp blocks walk
+// the leaves' P2P rows (few rows, long walks: they start first), the rest the
+// nodes' M2L rows. Each build has its own rows and lengths.
+__global__ void __launch_bounds__(256) k_rows_seg(NodeArrays a, uint32_t nn, uint32_t n_leaves, uint32_t nbp,
+                                                  const SegLevel* __restrict__ lvls, uint32_t nlv, SegTree st,
+                                                  uint32_t* __restrict__ mrows, uint32_t* __restrict__ mlens,
+                                                  uint32_t* __restrict__ prows, uint32_t* __restrict__ plens) {
+  if (blockIdx.x < nbp)
+    p2p_row_seg(a, n_leaves, lvls, st, prows, plens, blockIdx.x * blockDim.x + threadIdx.x);
+  else
+    m2l_row_seg(a, nn, lvls, nlv, st, mrows, mlens, (blockIdx.x - nbp) * blockDim.x + threadIdx.x);
 }
 
 __global__ void k_rows_compact(const uint32_t* __restrict__ rows, const uint32_t* __restrict__ off, uint32_t nn,
@@ -1734,9 +1761,18 @@ int build_lists_reference(galax_fmm* h) {
     k_seg_up<<<nblk(cnt), 256, 0, s>>>(dlv, t.n_levels, hh, st);
     GX_HIP(hipGetLastError());
   }
-  // M2L lists
-  k_m2l_rows_seg<<<nblk(nn), 256, 0, s>>>(a, (uint32_t)nn, dlv, t.n_levels, st, rows, lens);
-  GX_HIP(hipGetLastError());
+  // M2L and P2P rows in one launch (P2P: leaves only; other rows empty)
+  GX_CHECK(dbuf_reserve(h->ws.e, std::max<size_t>(t.n_leaves, 1) * LIST_CAP * sizeof(uint32_t), s));
+  GX_CHECK(dbuf_reserve(h->ws.f, (nn + 1) * sizeof(uint32_t), s));
+  uint32_t* prows = h->ws.e.as<uint32_t>();
+  uint32_t* plens = h->ws.f.as<uint32_t>();
+  GX_HIP(hipMemsetAsync(plens, 0, (nn + 1) * sizeof(uint32_t), s));
+  {
+    const uint32_t nbp = nblk(t.n_leaves);
+    k_rows_seg<<<nbp + nblk(nn), 256, 0, s>>>(a, (uint32_t)nn, (uint32_t)t.n_leaves, nbp, dlv, t.n_levels, st, rows,
+                                              lens, prows, plens);
+    GX_HIP(hipGetLastError());
+  }
   GX_HIP(hipMemsetAsync(lens + nn, 0, sizeof(uint32_t), s));
   GX_CHECK(dbuf_reserve(t.m2l_off, (nn + 1) * sizeof(uint32_t), s));
   uint32_t mt = 0;
@@ -1746,16 +1782,13 @@ int build_lists_reference(galax_fmm* h) {
   k_rows_compact<<<nblk(nn * LIST_CAP), 256, 0, s>>>(rows, t.m2l_off.as<uint32_t>(), (uint32_t)nn,
                                                      t.m2l_dat.as<uint32_t>());
   GX_HIP(hipGetLastError());
-  // P2P lists (leaves only; other rows empty)
-  GX_HIP(hipMemsetAsync(lens, 0, (nn + 1) * sizeof(uint32_t), s));
-  k_p2p_rows_seg<<<nblk(t.n_leaves), 256, 0, s>>>(a, (uint32_t)t.n_leaves, dlv, st, rows, lens);
-  GX_HIP(hipGetLastError());
+  // P2P lists
   GX_CHECK(dbuf_reserve(t.p2p_off, (nn + 1) * sizeof(uint32_t), s));
   uint32_t pt = 0;
-  GX_CHECK(scan_u32(h, lens, t.p2p_off.as<uint32_t>(), nn + 1, false, false, &pt));
+  GX_CHECK(scan_u32(h, plens, t.p2p_off.as<uint32_t>(), nn + 1, false, false, &pt));
   t.p2p_total = pt;
   GX_CHECK(dbuf_reserve(t.p2p_dat, (size_t)pt * sizeof(uint32_t) + 4, s));
-  k_rows_compact<<<nblk(t.n_leaves * LIST_CAP), 256, 0, s>>>(rows, t.p2p_off.as<uint32_t>(), (uint32_t)t.n_leaves,
+  k_rows_compact<<<nblk(t.n_leaves * LIST_CAP), 256, 0, s>>>(prows, t.p2p_off.as<uint32_t>(), (uint32_t)t.n_leaves,
                                                              t.p2p_dat.as<uint32_t>());
   GX_HIP(hipGetLastError());
   GX_CHECK(build_mark(h, 4));
