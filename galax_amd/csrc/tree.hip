@@ -1127,13 +1127,21 @@ __global__ void k_pair_emit(uint32_t n_leaves, const uint32_t* __restrict__ off,
   }
 }
 
-__global__ void k_pair_split(const uint64_t* __restrict__ keys, uint32_t n, int w, uint32_t* __restrict__ dat,
-                             uint32_t* __restrict__ cnt) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// The sorted keys as CSR: dat[i] = source of key i; off[u] = the first i
+// whose target is u (key i starts every leaf u after the previous key's
+// target, up to its own), off[n_leaves] = n. (Counting with atomics instead
+// serialises: neighbouring keys share their target.)
+__global__ void k_pair_split(const uint64_t* __restrict__ keys, uint32_t n, int w, uint32_t n_leaves,
+                             uint32_t* __restrict__ dat, uint32_t* __restrict__ off) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint64_t k = keys[i];
+  const uint64_t k = keys[i];
   dat[i] = (uint32_t)(k & ((1ull << w) - 1));
-  atomicAdd(&cnt[(uint32_t)(k >> w)], 1u);
+  const uint32_t t = (uint32_t)(k >> w);
+  const uint32_t u0 = i ? (uint32_t)(keys[i - 1] >> w) + 1u : 0u;
+  for (uint32_t u = u0; u <= t; ++u) off[u] = i;
+  if (i == n - 1)
+    for (uint32_t u = t + 1; u <= n_leaves; ++u) off[u] = n;
 }
 
 // Directed body-body interactions of the gather lists (self pairs excluded).
@@ -1206,22 +1214,30 @@ __global__ void k_items_count(const uint32_t* __restrict__ goff, uint32_t nl, ui
   ismulti[t] = n > 1 ? 1u : 0u;
 }
 
-__global__ void k_items_fill(const uint32_t* __restrict__ goff, uint32_t nl, const uint32_t* __restrict__ ioff,
-                             const uint32_t* __restrict__ poff, const uint32_t* __restrict__ moff,
-                             uint32_t* __restrict__ tgt, uint32_t* __restrict__ g0, uint32_t* __restrict__ g1,
-                             uint32_t* __restrict__ slot, uint32_t* __restrict__ multi) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nl) return;
-  uint32_t n = ioff[t + 1] - ioff[t];
-  for (uint32_t k = 0; k < n; ++k) {
-    uint32_t i = ioff[t] + k;
-    uint32_t a = goff[t] + k * P2P_ITEM_LEAVES;
-    tgt[i] = t;
-    g0[i] = a;
-    g1[i] = min(a + P2P_ITEM_LEAVES, goff[t + 1]);
-    slot[i] = n > 1 ? poff[t] + k : NONE;
+// One thread per item i: its leaf t (ioff[t] <= i < ioff[t + 1], every leaf
+// has an item) by binary search, so a leaf with very many items is not left
+// to one thread.
+__global__ void k_items_fill(const uint32_t* __restrict__ goff, uint32_t nl, uint32_t ni,
+                             const uint32_t* __restrict__ ioff, const uint32_t* __restrict__ poff,
+                             const uint32_t* __restrict__ moff, uint32_t* __restrict__ tgt, uint32_t* __restrict__ g0,
+                             uint32_t* __restrict__ g1, uint32_t* __restrict__ slot, uint32_t* __restrict__ multi) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ni) return;
+  uint32_t lo = 0, hi = nl;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (ioff[mid] <= i)
+      lo = mid;
+    else
+      hi = mid;
   }
-  if (n > 1) multi[moff[t]] = t;
+  const uint32_t t = lo, k = i - ioff[t], n = ioff[t + 1] - ioff[t];
+  const uint32_t a = goff[t] + k * P2P_ITEM_LEAVES;
+  tgt[i] = t;
+  g0[i] = a;
+  g1[i] = min(a + P2P_ITEM_LEAVES, goff[t + 1]);
+  slot[i] = n > 1 ? poff[t] + k : NONE;
+  if (k == 0 && n > 1) multi[moff[t]] = t;
 }
 
 // Work class of an item: target bodies in 32-body passes times source bodies,
@@ -1278,9 +1294,11 @@ int build_p2p_items(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, ui
   GX_CHECK(dbuf_reserve(it.g1, (size_t)ni * 4 + 4, s));
   GX_CHECK(dbuf_reserve(it.slot, (size_t)ni * 4 + 4, s));
   GX_CHECK(dbuf_reserve(it.multi, (size_t)nm * 4 + 4, s));
-  k_items_fill<<<nblk(nl), 256, 0, s>>>(goff, nl, ioff, poff, moff, it.tgt.as<uint32_t>(), it.g0.as<uint32_t>(),
-                                        it.g1.as<uint32_t>(), it.slot.as<uint32_t>(), it.multi.as<uint32_t>());
-  GX_HIP(hipGetLastError());
+  if (ni) {
+    k_items_fill<<<nblk(ni), 256, 0, s>>>(goff, nl, ni, ioff, poff, moff, it.tgt.as<uint32_t>(), it.g0.as<uint32_t>(),
+                                          it.g1.as<uint32_t>(), it.slot.as<uint32_t>(), it.multi.as<uint32_t>());
+    GX_HIP(hipGetLastError());
+  }
   // items by decreasing power-of-two work class (stable): the costly items of
   // dense regions start first instead of forming the last round of blocks.
   // Slots, and so the combine's summation order, do not depend on the order
@@ -1297,9 +1315,8 @@ int build_p2p_items(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, ui
     GX_HIP(hipGetLastError());
     GX_CHECK(radix_sort_bits(h, ws.sort_ka.as<uint64_t>(), ws.sort_va.as<uint32_t>(), ws.sort_kb.as<uint64_t>(),
                              ws.sort_vb.as<uint32_t>(), ni, 7));
-    DBuf tmp;
-    GX_CHECK(dbuf_reserve(tmp, (size_t)ni * 16, s));
-    uint32_t* b = tmp.as<uint32_t>();
+    GX_CHECK(dbuf_reserve(ws.e, (size_t)ni * 16, s));  // free here in both list builds
+    uint32_t* b = ws.e.as<uint32_t>();
     k_items_permute<<<nblk(ni), 256, 0, s>>>(ws.sort_va.as<uint32_t>(), ni, it.tgt.as<uint32_t>(),
                                              it.g0.as<uint32_t>(), it.g1.as<uint32_t>(), it.slot.as<uint32_t>(), b);
     GX_HIP(hipGetLastError());
@@ -1307,8 +1324,6 @@ int build_p2p_items(galax_fmm* h, const uint32_t* goff, const uint32_t* gdat, ui
     GX_HIP(hipMemcpyAsync(it.g0.p, b + ni, (size_t)ni * 4, hipMemcpyDeviceToDevice, s));
     GX_HIP(hipMemcpyAsync(it.g1.p, b + 2 * (size_t)ni, (size_t)ni * 4, hipMemcpyDeviceToDevice, s));
     GX_HIP(hipMemcpyAsync(it.slot.p, b + 3 * (size_t)ni, (size_t)ni * 4, hipMemcpyDeviceToDevice, s));
-    GX_HIP(hipStreamSynchronize(s));
-    dbuf_free(tmp);
   }
   GX_HIP(hipStreamSynchronize(s));
   return 0;
@@ -1712,9 +1727,10 @@ static int build_gather(galax_fmm* h) {
   GX_CHECK(dbuf_reserve(t.gat_dat, (size_t)total * sizeof(uint32_t) + 4, s));
   uint32_t* goff = t.gat_off.as<uint32_t>();
   GX_HIP(hipMemsetAsync(goff, 0, (nl + 1) * sizeof(uint32_t), s));
-  k_pair_split<<<nblk(total), 256, 0, s>>>(keys, total, w, t.gat_dat.as<uint32_t>(), goff);
-  GX_HIP(hipGetLastError());
-  GX_CHECK(scan_u32(h, goff, goff, nl + 1, false, false, nullptr));
+  if (total) {
+    k_pair_split<<<nblk(total), 256, 0, s>>>(keys, total, w, nl, t.gat_dat.as<uint32_t>(), goff);
+    GX_HIP(hipGetLastError());
+  }
   GX_CHECK(count_interactions(h, goff, t.gat_dat.as<uint32_t>(), &t.p2p_interactions));
   GX_CHECK(build_p2p_items(h, goff, t.gat_dat.as<uint32_t>(), nl, t.items));
   return 0;
