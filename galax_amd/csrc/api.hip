@@ -646,7 +646,10 @@ int galax_fmm_tree_info(galax_fmm_t* h, size_t* n_nodes, size_t* n_leaves, uint3
   if (n_nodes) *n_nodes = h->tree.n_nodes;
   if (n_leaves) *n_leaves = h->tree.n_leaves;
   if (n_levels) *n_levels = h->tree.n_levels;
-  if (m2l_total) *m2l_total = h->tree.m2l_total;
+  if (m2l_total) {
+    GX_CHECK(ensure_full_m2l_lists(h));
+    *m2l_total = h->tree.m2l_total;
+  }
   if (p2p_total) *p2p_total = h->tree.p2p_total;
   return 0;
 }
@@ -729,6 +732,7 @@ int galax_fmm_export_tree(galax_fmm_t* h, galax_tree_view_t* tv, galax_lists_vie
     return 0;
   };
   if (nn) {
+    if (m2l) GX_CHECK(ensure_full_m2l_lists(h));
     GX_CHECK(ex(m2l, t.m2l_off, t.m2l_dat, t.m2l_total));
     GX_CHECK(ex(p2p, t.p2p_off, t.p2p_dat, t.p2p_total));
   }

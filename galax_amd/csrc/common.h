@@ -162,6 +162,10 @@ struct DevTree {
   // node, and per leaf the symmetric P2P gather list (self first).
   DBuf m2l_off, m2l_dat, p2p_off, p2p_dat, gat_off, gat_dat;
   size_t m2l_total = 0, p2p_total = 0, gat_total = 0;
+  // false: the M2L rows of levels >= 2, which no evaluation reads (a leaf
+  // receives only its own and its parent's M2L), are not built yet
+  // (ensure_full_m2l_lists builds them; m2l_total then counts every row)
+  bool m2l_full = true;
   uint64_t p2p_interactions = 0;  // directed body-body pairs of the gather lists
   P2PItems items;
   SymP2P sym;  // symmetric near field over the gather lists (built on first use)
@@ -240,6 +244,7 @@ int scan_u32(galax_fmm* h, const uint32_t* d_in, uint32_t* d_out, size_t n, bool
 int build_tree_plain(galax_fmm* h, size_t n);             // bodies already in h->x/y/... unsorted
 int build_tree_constrained(galax_fmm* h, size_t n);       // bodies in h->x/y/m, current order
 int build_lists_reference(galax_fmm* h);                  // M2L + P2P lists + gather CSR
+int ensure_full_m2l_lists(galax_fmm* h);                 // the M2L rows of levels >= 2, once per tree
 // Records build-stage event i (0..5) on the handle's stream when the handle
 // has its build events (created by the API entry points).
 int build_mark(galax_fmm* h, int i);

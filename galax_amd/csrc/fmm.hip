@@ -2095,7 +2095,7 @@ int evaluate(galax_fmm* h, double* d_x, double* d_y, double* d_m, double* d_ax, 
   if (h->timing) GX_HIP(hipEventElapsedTime(&tot, h->ev[0], h->ev[7]));
   tm.total = tot;
   tm.m2l_pairs_executed = corrected ? h->c_m2l_total : h->n_live_pairs;
-  tm.m2l_pairs_reference = corrected ? h->c_m2l_total : t.m2l_total;
+  tm.m2l_pairs_reference = corrected ? h->c_m2l_total : (t.m2l_full ? t.m2l_total : 0);  // 0: not built yet
   tm.m2l_keys = corrected ? 0 : h->n_keys;
   tm.p2p_interactions = corrected ? h->c_p2p_interactions : t.p2p_interactions;
   const bool tiled = !corrected && h->cfg.precision != GALAX_PREC_FP64_EXACT;

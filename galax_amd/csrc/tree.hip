@@ -943,11 +943,15 @@ __device__ __forceinline__ uint32_t seg_next(uint32_t s) {
   return s ? s + 1 : 0u;
 }
 
-// M2L row of node c (below the root).
-__device__ __forceinline__ void m2l_row_seg(const NodeArrays& a, uint32_t nn, const SegLevel* __restrict__ lvls,
-                                            uint32_t nlv, const SegTree& st, uint32_t* __restrict__ rows,
-                                            uint32_t* __restrict__ lens, uint32_t c) {
+// M2L row of node c (below the root); rows of nodes c >= nwalk are left empty.
+__device__ __forceinline__ void m2l_row_seg(const NodeArrays& a, uint32_t nn, uint32_t nwalk,
+                                            const SegLevel* __restrict__ lvls, uint32_t nlv, const SegTree& st,
+                                            uint32_t* __restrict__ rows, uint32_t* __restrict__ lens, uint32_t c) {
   if (c >= nn) return;
+  if (c >= nwalk) {
+    lens[c] = 0;
+    return;
+  }
   const uint32_t P = a.par[c];
   const uint32_t lc = a.lv[c];
   if (P == NONE || lc + 1 >= nlv) {
@@ -1037,15 +1041,16 @@ __device__ __forceinline__ void p2p_row_seg(const NodeArrays& a, uint32_t n_leav
 
 // Both list builds in one launch, a thread per row: the first nbp blocks walk
 // the leaves' P2P rows (few rows, long walks: they start first), the rest the
-// nodes' M2L rows. Each build has its own rows and lengths.
-__global__ void __launch_bounds__(256) k_rows_seg(NodeArrays a, uint32_t nn, uint32_t n_leaves, uint32_t nbp,
-                                                  const SegLevel* __restrict__ lvls, uint32_t nlv, SegTree st,
-                                                  uint32_t* __restrict__ mrows, uint32_t* __restrict__ mlens,
+// nodes' M2L rows (walked for nodes below nwalk). Each build has its own rows
+// and lengths.
+__global__ void __launch_bounds__(256) k_rows_seg(NodeArrays a, uint32_t nn, uint32_t nwalk, uint32_t n_leaves,
+                                                  uint32_t nbp, const SegLevel* __restrict__ lvls, uint32_t nlv,
+                                                  SegTree st, uint32_t* __restrict__ mrows, uint32_t* __restrict__ mlens,
                                                   uint32_t* __restrict__ prows, uint32_t* __restrict__ plens) {
   if (blockIdx.x < nbp)
     p2p_row_seg(a, n_leaves, lvls, st, prows, plens, blockIdx.x * blockDim.x + threadIdx.x);
   else
-    m2l_row_seg(a, nn, lvls, nlv, st, mrows, mlens, (blockIdx.x - nbp) * blockDim.x + threadIdx.x);
+    m2l_row_seg(a, nn, nwalk, lvls, nlv, st, mrows, mlens, (blockIdx.x - nbp) * blockDim.x + threadIdx.x);
 }
 
 __global__ void k_rows_compact(const uint32_t* __restrict__ rows, const uint32_t* __restrict__ off, uint32_t nn,
@@ -1736,20 +1741,11 @@ static int build_gather(galax_fmm* h) {
   return 0;
 }
 
-int build_lists_reference(galax_fmm* h) {
+// Segment trees over every level's nodes (the candidates of both list
+// builds), in h->ws.g.
+static int build_segtrees(galax_fmm* h, const NodeArrays& a, SegTree& st, SegLevel*& dlv) {
   DevTree& t = h->tree;
   hipStream_t s = h->stream;
-  size_t nn = t.n_nodes;
-  if (nn == 0) {
-    t.m2l_total = t.p2p_total = t.gat_total = 0;
-    return 0;
-  }
-  NodeArrays a = node_arrays(h);
-  GX_CHECK(dbuf_reserve(h->ws.d, nn * LIST_CAP * sizeof(uint32_t), s));
-  GX_CHECK(dbuf_reserve(h->ws.c, (nn + 1) * sizeof(uint32_t), s));
-  uint32_t* rows = h->ws.d.as<uint32_t>();
-  uint32_t* lens = h->ws.c.as<uint32_t>();
-  // segment trees over every level's nodes (candidates of both list builds)
   std::vector<SegLevel> lv(t.n_levels);
   size_t tot = 0, slots = 0;
   uint32_t maxh = 0;
@@ -1766,8 +1762,8 @@ int build_lists_reference(galax_fmm* h) {
     maxh = std::max(maxh, hgt);
   }
   GX_CHECK(dbuf_reserve(h->ws.g, tot * sizeof(SegRec) + t.n_levels * sizeof(SegLevel), s));
-  SegTree st{h->ws.g.as<SegRec>()};
-  SegLevel* dlv = reinterpret_cast<SegLevel*>(st.r + tot);
+  st = SegTree{h->ws.g.as<SegRec>()};
+  dlv = reinterpret_cast<SegLevel*>(st.r + tot);
   GX_HIP(hipMemcpyAsync(dlv, lv.data(), t.n_levels * sizeof(SegLevel), hipMemcpyHostToDevice, s));
   k_seg_leaves<<<nblk(slots), 256, 0, s>>>(a, dlv, t.n_levels, st);
   GX_HIP(hipGetLastError());
@@ -1777,18 +1773,17 @@ int build_lists_reference(galax_fmm* h) {
     k_seg_up<<<nblk(cnt), 256, 0, s>>>(dlv, t.n_levels, hh, st);
     GX_HIP(hipGetLastError());
   }
-  // M2L and P2P rows in one launch (P2P: leaves only; other rows empty)
-  GX_CHECK(dbuf_reserve(h->ws.e, std::max<size_t>(t.n_leaves, 1) * LIST_CAP * sizeof(uint32_t), s));
-  GX_CHECK(dbuf_reserve(h->ws.f, (nn + 1) * sizeof(uint32_t), s));
-  uint32_t* prows = h->ws.e.as<uint32_t>();
-  uint32_t* plens = h->ws.f.as<uint32_t>();
-  GX_HIP(hipMemsetAsync(plens, 0, (nn + 1) * sizeof(uint32_t), s));
-  {
-    const uint32_t nbp = nblk(t.n_leaves);
-    k_rows_seg<<<nbp + nblk(nn), 256, 0, s>>>(a, (uint32_t)nn, (uint32_t)t.n_leaves, nbp, dlv, t.n_levels, st, rows,
-                                              lens, prows, plens);
-    GX_HIP(hipGetLastError());
-  }
+  // the host vector lv must outlive the copy
+  GX_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// The M2L CSR (m2l_off, m2l_dat, m2l_total) from per-node rows and lengths
+// (lens: nn + 1 entries; the last is cleared here).
+static int compact_m2l(galax_fmm* h, const uint32_t* rows, uint32_t* lens) {
+  DevTree& t = h->tree;
+  hipStream_t s = h->stream;
+  const size_t nn = t.n_nodes;
   GX_HIP(hipMemsetAsync(lens + nn, 0, sizeof(uint32_t), s));
   GX_CHECK(dbuf_reserve(t.m2l_off, (nn + 1) * sizeof(uint32_t), s));
   uint32_t mt = 0;
@@ -1798,6 +1793,47 @@ int build_lists_reference(galax_fmm* h) {
   k_rows_compact<<<nblk(nn * LIST_CAP), 256, 0, s>>>(rows, t.m2l_off.as<uint32_t>(), (uint32_t)nn,
                                                      t.m2l_dat.as<uint32_t>());
   GX_HIP(hipGetLastError());
+  return 0;
+}
+
+// The reference's lists. The M2L rows are walked for the nodes of levels 0
+// and 1 only, the rows any evaluation reads (a leaf receives its own M2L and
+// its parent's: the ascending-id L2L, galax-core/src/lib.rs:1266); the rows of
+// levels >= 2 stay empty until ensure_full_m2l_lists builds them (for an
+// export of the lists or their total). Node ids run level by level, so the
+// live rows, their offsets and their entries are the same either way.
+int build_lists_reference(galax_fmm* h) {
+  DevTree& t = h->tree;
+  hipStream_t s = h->stream;
+  size_t nn = t.n_nodes;
+  t.m2l_full = true;
+  if (nn == 0) {
+    t.m2l_total = t.p2p_total = t.gat_total = 0;
+    return 0;
+  }
+  NodeArrays a = node_arrays(h);
+  GX_CHECK(dbuf_reserve(h->ws.d, nn * LIST_CAP * sizeof(uint32_t), s));
+  GX_CHECK(dbuf_reserve(h->ws.c, (nn + 1) * sizeof(uint32_t), s));
+  uint32_t* rows = h->ws.d.as<uint32_t>();
+  uint32_t* lens = h->ws.c.as<uint32_t>();
+  SegTree st{};
+  SegLevel* dlv = nullptr;
+  GX_CHECK(build_segtrees(h, a, st, dlv));
+  // M2L and P2P rows in one launch (P2P: leaves only; other rows empty)
+  GX_CHECK(dbuf_reserve(h->ws.e, std::max<size_t>(t.n_leaves, 1) * LIST_CAP * sizeof(uint32_t), s));
+  GX_CHECK(dbuf_reserve(h->ws.f, (nn + 1) * sizeof(uint32_t), s));
+  uint32_t* prows = h->ws.e.as<uint32_t>();
+  uint32_t* plens = h->ws.f.as<uint32_t>();
+  GX_HIP(hipMemsetAsync(plens, 0, (nn + 1) * sizeof(uint32_t), s));
+  const uint32_t nwalk = t.level_offset[std::min<uint32_t>(t.n_levels, 2)];
+  {
+    const uint32_t nbp = nblk(t.n_leaves);
+    k_rows_seg<<<nbp + nblk(nn), 256, 0, s>>>(a, (uint32_t)nn, nwalk, (uint32_t)t.n_leaves, nbp, dlv, t.n_levels, st,
+                                              rows, lens, prows, plens);
+    GX_HIP(hipGetLastError());
+  }
+  GX_CHECK(compact_m2l(h, rows, lens));
+  t.m2l_full = nwalk >= nn;
   // P2P lists
   GX_CHECK(dbuf_reserve(t.p2p_off, (nn + 1) * sizeof(uint32_t), s));
   uint32_t pt = 0;
@@ -1810,6 +1846,28 @@ int build_lists_reference(galax_fmm* h) {
   GX_CHECK(build_mark(h, 4));
   GX_CHECK(build_gather(h));
   h->keys_valid = false;
+  return 0;
+}
+
+int ensure_full_m2l_lists(galax_fmm* h) {
+  DevTree& t = h->tree;
+  if (t.m2l_full || t.n_nodes == 0) return 0;
+  hipStream_t s = h->stream;
+  const size_t nn = t.n_nodes;
+  NodeArrays a = node_arrays(h);
+  GX_CHECK(dbuf_reserve(h->ws.d, nn * LIST_CAP * sizeof(uint32_t), s));
+  GX_CHECK(dbuf_reserve(h->ws.c, (nn + 1) * sizeof(uint32_t), s));
+  uint32_t* rows = h->ws.d.as<uint32_t>();
+  uint32_t* lens = h->ws.c.as<uint32_t>();
+  SegTree st{};
+  SegLevel* dlv = nullptr;
+  GX_CHECK(build_segtrees(h, a, st, dlv));
+  k_rows_seg<<<nblk(nn), 256, 0, s>>>(a, (uint32_t)nn, (uint32_t)nn, (uint32_t)t.n_leaves, 0u, dlv, t.n_levels, st,
+                                      rows, lens, nullptr, nullptr);
+  GX_HIP(hipGetLastError());
+  GX_CHECK(compact_m2l(h, rows, lens));
+  GX_HIP(hipStreamSynchronize(s));
+  t.m2l_full = true;
   return 0;
 }
 
@@ -1935,6 +1993,7 @@ int install_tree_host(galax_fmm* h, const galax_tree_view_t* tv, const galax_lis
     GX_HIP(hipStreamSynchronize(s));
     return 0;
   };
+  t.m2l_full = true;  // the caller's lists, every row
   GX_CHECK(up_lists(m2l, t.m2l_off, t.m2l_dat, t.m2l_total));
   GX_CHECK(up_lists(p2p, t.p2p_off, t.p2p_dat, t.p2p_total));
   GX_CHECK(build_gather(h));

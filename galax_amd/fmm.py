@@ -287,13 +287,18 @@ class GpuContext:
         _check(lib().galax_fmm_set_softening(self.h, eps))
         self.cfg.eps = eps
 
-    def tree_info(self) -> dict:
+    def tree_info(self, m2l_total: bool = True) -> dict:
+        """Sizes of the installed tree. With m2l_total, also the M2L list
+        length: the first such call on a tree builds the M2L rows of levels
+        >= 2 (no evaluation reads them; build_tree leaves them for later)."""
         nn, nl, m2l, p2p = C.c_size_t(), C.c_size_t(), C.c_size_t(), C.c_size_t()
         nlev = C.c_uint32()
-        _check(lib().galax_fmm_tree_info(self.h, C.byref(nn), C.byref(nl), C.byref(nlev), C.byref(m2l),
-                                         C.byref(p2p)))
-        return {"n_nodes": nn.value, "n_leaves": nl.value, "n_levels": nlev.value, "m2l_total": m2l.value,
-                "p2p_total": p2p.value}
+        _check(lib().galax_fmm_tree_info(self.h, C.byref(nn), C.byref(nl), C.byref(nlev),
+                                         C.byref(m2l) if m2l_total else None, C.byref(p2p)))
+        out = {"n_nodes": nn.value, "n_leaves": nl.value, "n_levels": nlev.value, "p2p_total": p2p.value}
+        if m2l_total:
+            out["m2l_total"] = m2l.value
+        return out
 
     def export_tree(self):
         """Returns (tree dict, m2l (offsets, data), p2p (offsets, data))."""
@@ -315,7 +320,7 @@ class GpuContext:
 
     def export_expansions(self):
         """(multipoles, locals) of the last evaluation, each (n_nodes, S)."""
-        nn = self.tree_info()["n_nodes"]
+        nn = self.tree_info(m2l_total=False)["n_nodes"]
         S = (self.cfg.p + 1) * (self.cfg.p + 2) // 2
         mp, loc = np.zeros((nn, S)), np.zeros((nn, S))
         _check(lib().galax_fmm_export_expansions(self.h, _ptr(mp), _ptr(loc), nn))
@@ -325,7 +330,7 @@ class GpuContext:
         """The corrected variant's lists, built on the device at the first
         evaluation: (m2l (offsets, data), p2p (offsets, data)), one row per
         node (P2P rows: the leaf itself first, internal nodes empty)."""
-        nn = self.tree_info()["n_nodes"]
+        nn = self.tree_info(m2l_total=False)["n_nodes"]
         out = []
         for which in (0, 1):
             off = np.zeros(nn + 1, np.uint64)

@@ -129,6 +129,8 @@ typedef struct galax_fmm galax_fmm_t;
  * kernel: its time is part of p2m and `sanitize` is the gap before it. */
 typedef struct {
   double sanitize, p2m, m2m, m2l_setup, m2l, l2l, l2p_p2p, total;
+  /* m2l_pairs_reference: every M2L pair of the reference's lists, 0 while
+   * the rows of levels >= 2 are not built (see galax_fmm_tree_info) */
   uint64_t m2l_pairs_executed, m2l_pairs_reference, m2l_keys, p2p_interactions;
   /* matrix-core M2L work units: (target, key) groups (runs of more than 4
    * pairs are pre-summed into one row each evaluation),
@@ -174,7 +176,10 @@ GALAX_API int galax_fmm_upload_tree(galax_fmm_t* h, const galax_tree_view_t* tre
  * from the Morton-sorted bodies after Tree::build, galax-cli/src/main.rs:182-190). */
 GALAX_API int galax_fmm_set_softening(galax_fmm_t* h, double softening);
 
-/* Sizes of the installed tree: nodes, leaves, levels, M2L and P2P list lengths. */
+/* Sizes of the installed tree: nodes, leaves, levels, M2L and P2P list lengths.
+ * galax_fmm_build_tree walks the M2L rows of levels 0 and 1 only, the rows
+ * an evaluation reads; asking for m2l_total (or exporting the M2L lists)
+ * first builds the rows of the other levels, once per tree. */
 GALAX_API int galax_fmm_tree_info(galax_fmm_t* h, size_t* n_nodes, size_t* n_leaves, uint32_t* n_levels, size_t* m2l_total,
                         size_t* p2p_total);
 

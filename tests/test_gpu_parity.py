@@ -108,6 +108,31 @@ def test_tree_and_lists_bit_exact(name, gen, n_max, bounds):
     _assert_lists_equal(gp2p, O.p2p_lists(ot))
 
 
+@pytest.mark.parametrize("prec", [fmm.PREC_FP64, fmm.PREC_FP64_EXACT, fmm.PREC_MIXED_FP32])
+def test_upper_level_m2l_rows_built_on_demand(prec):
+    """build_tree walks the M2L rows of levels 0 and 1 only, the rows an
+    evaluation reads; the others are built when the lists or their total are
+    asked for. The forces before and after are equal bit for bit, the
+    reported reference pair count is 0 until then, and the completed lists
+    equal the oracle's."""
+    d = ic.plummer(12_000, seed=91)
+    bounds = (-50.0, 50.0, -50.0, 50.0)
+    ctx, b, _ = _gpu_build(d, p=4, eps=0.05, bounds=bounds, precision=prec)
+    ctx.evaluate(b)
+    first = (b.ax.copy(), b.ay.copy())
+    assert ctx.timings().m2l_pairs_reference == 0
+    info = ctx.tree_info()
+    assert info["n_levels"] > 2
+    gt, gm2l, gp2p = ctx.export_tree()
+    assert gm2l[1].size == info["m2l_total"]
+    ctx.evaluate(b)
+    assert np.array_equal(b.ax, first[0]) and np.array_equal(b.ay, first[1])
+    assert ctx.timings().m2l_pairs_reference == info["m2l_total"]
+    ot, _, _ = _oracle_build(d, 32, bounds)
+    _assert_lists_equal(gm2l, O.m2l_lists(ot))
+    _assert_lists_equal(gp2p, O.p2p_lists(ot))
+
+
 def test_tree_bit_exact_full_size_plummer1m():
     """Tree and permutation at the 2^20-body Plummer workload (lists are
     checked at the smaller sizes above: the oracle's list builder is O(n^2))."""
