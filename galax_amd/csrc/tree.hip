@@ -474,36 +474,37 @@ __global__ void __launch_bounds__(1024) k_rounds_topo(NodeArrays a, const uint64
   }
 }
 
-template <class T>
-static int grow_keep(DBuf& b, size_t old_n, size_t new_n, hipStream_t s) {
-  if (new_n * sizeof(T) <= b.bytes) return 0;
-  DBuf nb;
-  GX_CHECK(dbuf_reserve(nb, new_n * sizeof(T), s));
-  if (b.p && old_n) GX_HIP(hipMemcpyAsync(nb.p, b.p, old_n * sizeof(T), hipMemcpyDeviceToDevice, s));
-  GX_HIP(hipStreamSynchronize(s));
-  dbuf_free(b);
-  b = nb;
-  return 0;
-}
-
+// Grows the node arrays to at least `nodes` entries, keeping their contents:
+// every new array is allocated and its copy queued first, then one wait, then
+// the old arrays are freed (one round trip instead of one per array).
 int ensure_node_capacity(galax_fmm* h, size_t nodes) {
   DevTree& t = h->tree;
   if (nodes <= t.capacity) return 0;
-  size_t nc = std::max(nodes, t.capacity * 3 / 2 + 64);
-  size_t old = t.capacity;
+  const size_t nc = std::max(nodes, t.capacity * 3 / 2 + 64);
+  const size_t old = t.capacity;
   hipStream_t s = h->stream;
-  GX_CHECK(grow_keep<double>(t.center_x, old, nc, s));
-  GX_CHECK(grow_keep<double>(t.center_y, old, nc, s));
-  GX_CHECK(grow_keep<double>(t.half_width, old, nc, s));
-  GX_CHECK(grow_keep<double>(t.mass, old, nc, s));
-  GX_CHECK(grow_keep<double>(t.com_x, old, nc, s));
-  GX_CHECK(grow_keep<double>(t.com_y, old, nc, s));
-  GX_CHECK(grow_keep<uint32_t>(t.body_start, old, nc, s));
-  GX_CHECK(grow_keep<uint32_t>(t.body_end, old, nc, s));
-  GX_CHECK(grow_keep<uint32_t>(t.child_first, old, nc, s));
-  GX_CHECK(grow_keep<uint32_t>(t.parent, old, nc, s));
-  GX_CHECK(grow_keep<uint8_t>(t.child_count, old, nc, s));
-  GX_CHECK(grow_keep<uint8_t>(t.level, old, nc, s));
+  DBuf* arr[] = {&t.center_x, &t.center_y, &t.half_width, &t.mass,        &t.com_x,       &t.com_y,
+                 &t.body_start, &t.body_end, &t.child_first, &t.parent, &t.child_count, &t.level};
+  const size_t esz[] = {8, 8, 8, 8, 8, 8, 4, 4, 4, 4, 1, 1};
+  constexpr int NA = sizeof(esz) / sizeof(esz[0]);
+  static_assert(sizeof(arr) / sizeof(arr[0]) == NA, "one element size per array");
+  DBuf nb[NA];
+  bool copied = false;
+  for (int i = 0; i < NA; ++i) {
+    if (int rc = dbuf_reserve(nb[i], nc * esz[i], s)) {
+      for (int k = 0; k <= i; ++k) dbuf_free(nb[k]);
+      return rc;
+    }
+    if (arr[i]->p && old) {
+      GX_HIP(hipMemcpyAsync(nb[i].p, arr[i]->p, old * esz[i], hipMemcpyDeviceToDevice, s));
+      copied = true;
+    }
+  }
+  if (copied) GX_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < NA; ++i) {
+    dbuf_free(*arr[i]);
+    *arr[i] = nb[i];
+  }
   t.capacity = nc;
   return 0;
 }
@@ -515,7 +516,9 @@ static inline unsigned nblk(size_t n, unsigned bs = 256) { return (unsigned)((n 
 static int build_from_leaf_starts(galax_fmm* h, size_t n, size_t n_leaves, bool constrained_style) {
   DevTree& t = h->tree;
   hipStream_t s = h->stream;
-  GX_CHECK(ensure_node_capacity(h, n_leaves * 4 + 64));
+  // room for the rounds: a tree small enough for the fused rounds gets the
+  // capacity they ask for (32 more rounds of n_leaves nodes) at once
+  GX_CHECK(ensure_node_capacity(h, n_leaves <= FUSED_ROUND ? n_leaves * 33 + 64 : n_leaves * 4 + 64));
   NodeArrays a = node_arrays(h);
   k_leaves<<<nblk(n_leaves), 256, 0, s>>>(h->x.as<double>(), h->y.as<double>(), h->m.as<double>(),
                                           h->ws.c.as<uint32_t>(), (uint32_t)n_leaves, a);
