@@ -1745,11 +1745,13 @@ static int build_gather(galax_fmm* h) {
 }
 
 // Segment trees over every level's nodes (the candidates of both list
-// builds), in h->ws.g.
-static int build_segtrees(galax_fmm* h, const NodeArrays& a, SegTree& st, SegLevel*& dlv) {
+// builds), in h->ws.g. lv (host, the levels' layout) is copied from
+// asynchronously: the caller keeps it until its next wait on the stream.
+static int build_segtrees(galax_fmm* h, const NodeArrays& a, std::vector<SegLevel>& lv, SegTree& st,
+                          SegLevel*& dlv) {
   DevTree& t = h->tree;
   hipStream_t s = h->stream;
-  std::vector<SegLevel> lv(t.n_levels);
+  lv.assign(t.n_levels, SegLevel{});
   size_t tot = 0, slots = 0;
   uint32_t maxh = 0;
   for (uint32_t l = 0; l < t.n_levels; ++l) {
@@ -1776,14 +1778,13 @@ static int build_segtrees(galax_fmm* h, const NodeArrays& a, SegTree& st, SegLev
     k_seg_up<<<nblk(cnt), 256, 0, s>>>(dlv, t.n_levels, hh, st);
     GX_HIP(hipGetLastError());
   }
-  // the host vector lv must outlive the copy
-  GX_HIP(hipStreamSynchronize(s));
   return 0;
 }
 
-// The M2L CSR (m2l_off, m2l_dat, m2l_total) from per-node rows and lengths
-// (lens: nn + 1 entries; the last is cleared here).
-static int compact_m2l(galax_fmm* h, const uint32_t* rows, uint32_t* lens) {
+// The M2L CSR (m2l_off, m2l_dat, m2l_total) from per-node rows (of the first
+// nrows nodes) and lengths (lens: nn + 1 entries, 0 for rows from nrows on;
+// the last is cleared here).
+static int compact_m2l(galax_fmm* h, const uint32_t* rows, uint32_t* lens, size_t nrows) {
   DevTree& t = h->tree;
   hipStream_t s = h->stream;
   const size_t nn = t.n_nodes;
@@ -1793,9 +1794,11 @@ static int compact_m2l(galax_fmm* h, const uint32_t* rows, uint32_t* lens) {
   GX_CHECK(scan_u32(h, lens, t.m2l_off.as<uint32_t>(), nn + 1, false, false, &mt));
   t.m2l_total = mt;
   GX_CHECK(dbuf_reserve(t.m2l_dat, (size_t)mt * sizeof(uint32_t) + 4, s));
-  k_rows_compact<<<nblk(nn * LIST_CAP), 256, 0, s>>>(rows, t.m2l_off.as<uint32_t>(), (uint32_t)nn,
-                                                     t.m2l_dat.as<uint32_t>());
-  GX_HIP(hipGetLastError());
+  if (nrows) {
+    k_rows_compact<<<nblk(nrows * LIST_CAP), 256, 0, s>>>(rows, t.m2l_off.as<uint32_t>(), (uint32_t)nrows,
+                                                          t.m2l_dat.as<uint32_t>());
+    GX_HIP(hipGetLastError());
+  }
   return 0;
 }
 
@@ -1815,27 +1818,28 @@ int build_lists_reference(galax_fmm* h) {
     return 0;
   }
   NodeArrays a = node_arrays(h);
-  GX_CHECK(dbuf_reserve(h->ws.d, nn * LIST_CAP * sizeof(uint32_t), s));
+  const uint32_t nwalk = t.level_offset[std::min<uint32_t>(t.n_levels, 2)];
+  GX_CHECK(dbuf_reserve(h->ws.d, (size_t)nwalk * LIST_CAP * sizeof(uint32_t), s));  // rows of the walked nodes
   GX_CHECK(dbuf_reserve(h->ws.c, (nn + 1) * sizeof(uint32_t), s));
   uint32_t* rows = h->ws.d.as<uint32_t>();
   uint32_t* lens = h->ws.c.as<uint32_t>();
   SegTree st{};
   SegLevel* dlv = nullptr;
-  GX_CHECK(build_segtrees(h, a, st, dlv));
+  std::vector<SegLevel> lv;  // kept until the scans below wait on the stream
+  GX_CHECK(build_segtrees(h, a, lv, st, dlv));
   // M2L and P2P rows in one launch (P2P: leaves only; other rows empty)
   GX_CHECK(dbuf_reserve(h->ws.e, std::max<size_t>(t.n_leaves, 1) * LIST_CAP * sizeof(uint32_t), s));
   GX_CHECK(dbuf_reserve(h->ws.f, (nn + 1) * sizeof(uint32_t), s));
   uint32_t* prows = h->ws.e.as<uint32_t>();
   uint32_t* plens = h->ws.f.as<uint32_t>();
   GX_HIP(hipMemsetAsync(plens, 0, (nn + 1) * sizeof(uint32_t), s));
-  const uint32_t nwalk = t.level_offset[std::min<uint32_t>(t.n_levels, 2)];
   {
     const uint32_t nbp = nblk(t.n_leaves);
     k_rows_seg<<<nbp + nblk(nn), 256, 0, s>>>(a, (uint32_t)nn, nwalk, (uint32_t)t.n_leaves, nbp, dlv, t.n_levels, st,
                                               rows, lens, prows, plens);
     GX_HIP(hipGetLastError());
   }
-  GX_CHECK(compact_m2l(h, rows, lens));
+  GX_CHECK(compact_m2l(h, rows, lens, nwalk));
   t.m2l_full = nwalk >= nn;
   // P2P lists
   GX_CHECK(dbuf_reserve(t.p2p_off, (nn + 1) * sizeof(uint32_t), s));
@@ -1864,11 +1868,12 @@ int ensure_full_m2l_lists(galax_fmm* h) {
   uint32_t* lens = h->ws.c.as<uint32_t>();
   SegTree st{};
   SegLevel* dlv = nullptr;
-  GX_CHECK(build_segtrees(h, a, st, dlv));
+  std::vector<SegLevel> lv;  // kept until the scans below wait on the stream
+  GX_CHECK(build_segtrees(h, a, lv, st, dlv));
   k_rows_seg<<<nblk(nn), 256, 0, s>>>(a, (uint32_t)nn, (uint32_t)nn, (uint32_t)t.n_leaves, 0u, dlv, t.n_levels, st,
                                       rows, lens, nullptr, nullptr);
   GX_HIP(hipGetLastError());
-  GX_CHECK(compact_m2l(h, rows, lens));
+  GX_CHECK(compact_m2l(h, rows, lens, nn));
   GX_HIP(hipStreamSynchronize(s));
   t.m2l_full = true;
   return 0;
