@@ -184,7 +184,8 @@ GALAX_API int galax_fmm_tree_info(galax_fmm_t* h, size_t* n_nodes, size_t* n_lea
                         size_t* p2p_total);
 
 /* Copies the installed tree and lists out (for bit-exactness checks).
- * Views must be sized from galax_fmm_tree_info; any view may be NULL. */
+ * Views must be sized from galax_fmm_tree_info; any view may be NULL. An M2L
+ * view gets every row (the rows of levels >= 2 are built first if needed). */
 GALAX_API int galax_fmm_export_tree(galax_fmm_t* h, galax_tree_view_t* tree, galax_lists_view_t* m2l,
                           galax_lists_view_t* p2p);
 
