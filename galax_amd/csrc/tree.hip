@@ -1856,9 +1856,18 @@ int build_lists_reference(galax_fmm* h) {
   return 0;
 }
 
+static int build_full_m2l_lists(galax_fmm* h);
+
 int ensure_full_m2l_lists(galax_fmm* h) {
+  if (h->tree.m2l_full || h->tree.n_nodes == 0) return 0;
+  const int rc = build_full_m2l_lists(h);
+  // a failure can leave the live rows overwritten: the tree is then unusable
+  if (rc) h->has_tree = false;
+  return rc;
+}
+
+static int build_full_m2l_lists(galax_fmm* h) {
   DevTree& t = h->tree;
-  if (t.m2l_full || t.n_nodes == 0) return 0;
   hipStream_t s = h->stream;
   const size_t nn = t.n_nodes;
   NodeArrays a = node_arrays(h);
