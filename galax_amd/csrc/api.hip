@@ -17,22 +17,53 @@ static inline unsigned nb256(size_t n) { return (unsigned)((n + 255) / 256); }
 
 // --------------------------------------------- corrected-variant lists ----
 
-// R_n = max |body - com_n| over the node's bodies; one wave per node.
+// R_n = max |body - com_n| over the node's bodies. The bodies of a node are
+// cut into chunks of RAD_CHUNK, one wave per (node, chunk) item (a node near
+// the root has most of the bodies: one wave per node left them to one wave);
+// each item's max of the squared distances goes into the node's slot by an
+// integer atomic max on its bit pattern (non-negative doubles order as their
+// bits; fmax drops NaN, so none arrives), then one pass takes the roots.
+// Max is exact in any order: the radii are those of a serial loop.
+constexpr uint32_t RAD_CHUNK = 4096;
+
+__global__ void k_radii_items(const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be, uint32_t nn,
+                              uint32_t* __restrict__ nit) {
+  const uint32_t nd = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= nn) return;
+  nit[nd] = max(1u, (be[nd] - bs[nd] + RAD_CHUNK - 1) / RAD_CHUNK);
+}
+
 __global__ void k_radii(const double* __restrict__ x, const double* __restrict__ y, const uint32_t* __restrict__ bs,
                         const uint32_t* __restrict__ be, const double* __restrict__ comx,
-                        const double* __restrict__ comy, uint32_t nn, double* __restrict__ R) {
+                        const double* __restrict__ comy, uint32_t nn, const uint32_t* __restrict__ ioff,
+                        uint32_t nitems, unsigned long long* __restrict__ r2bits) {
 #pragma clang fp contract(off)
-  uint32_t nd = blockIdx.x * 4 + (threadIdx.x >> 6);
-  int lane = threadIdx.x & 63;
-  if (nd >= nn) return;
-  double cx = comx[nd], cy = comy[nd];
+  const uint32_t item = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (item >= nitems) return;
+  uint32_t lo = 0, hi = nn;  // the node nd with ioff[nd] <= item < ioff[nd + 1]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (ioff[mid] <= item)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const uint32_t nd = lo;
+  const uint32_t b0 = bs[nd] + (item - ioff[nd]) * RAD_CHUNK, b1 = min(be[nd], b0 + RAD_CHUNK);
+  const double cx = comx[nd], cy = comy[nd];
   double r2 = 0.0;
-  for (uint32_t b = bs[nd] + lane; b < be[nd]; b += 64) {
+  for (uint32_t b = b0 + lane; b < b1; b += 64) {
     double dx = x[b] - cx, dy = y[b] - cy;
     r2 = fmax(r2, dx * dx + dy * dy);
   }
   for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o, 64));
-  if (lane == 0) R[nd] = sqrt(r2);
+  if (lane == 0) atomicMax(r2bits + nd, (unsigned long long)__double_as_longlong(r2));
+}
+
+__global__ void k_radii_sqrt(uint32_t nn, double* __restrict__ R) {
+  const uint32_t nd = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd < nn) R[nd] = sqrt(R[nd]);  // R holds the squared radius's bits, read as a double
 }
 
 // The dual tree traversal, breadth-first on the device. Every visited pair
@@ -133,15 +164,21 @@ __global__ void k_dtt_self_keys(uint32_t nl, int wp, uint64_t* __restrict__ pkey
   if (a < nl) pkeys[a] = (uint64_t)a << wp;
 }
 
-// sorted keys -> CSR entries and per-row counts (cnt zeroed, rows + 1 entries)
-__global__ void k_dtt_split(const uint64_t* __restrict__ keys, uint32_t n, int w, bool self_first,
-                            uint32_t* __restrict__ dat, uint32_t* __restrict__ cnt) {
+// sorted keys -> CSR entries and row offsets (rows + 1 entries): off[u] is
+// the first i whose row is u (key i starts every row after the previous
+// key's, up to its own), off[rows] = n. (Counting with atomics instead
+// serialises: neighbouring keys share their row.)
+__global__ void k_dtt_split(const uint64_t* __restrict__ keys, uint32_t n, int w, bool self_first, uint32_t rows,
+                            uint32_t* __restrict__ dat, uint32_t* __restrict__ off) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint64_t k = keys[i];
   const uint32_t row = (uint32_t)(k >> w), col = (uint32_t)(k & ((1ull << w) - 1));
   dat[i] = self_first ? (col == 0 ? row : col - 1u) : col;
-  atomicAdd(&cnt[row], 1u);
+  const uint32_t u0 = i ? (uint32_t)(keys[i - 1] >> w) + 1u : 0u;
+  for (uint32_t u = u0; u <= row; ++u) off[u] = i;
+  if (i == n - 1)
+    for (uint32_t u = row + 1; u <= rows; ++u) off[u] = n;
 }
 
 // Targets in decreasing M2L list length (stable), for the lane-per-pair M2L
@@ -186,11 +223,10 @@ int keys_to_csr(galax_fmm* h, size_t n, int w, int bits, bool self_first, uint32
   GX_CHECK(dbuf_reserve(dat, n * 4 + 4, s));
   GX_HIP(hipMemsetAsync(off.p, 0, ((size_t)rows + 1) * 4, s));
   if (n) {
-    k_dtt_split<<<nb256(n), 256, 0, s>>>(ws.sort_ka.as<uint64_t>(), (uint32_t)n, w, self_first, dat.as<uint32_t>(),
-                                         off.as<uint32_t>());
+    k_dtt_split<<<nb256(n), 256, 0, s>>>(ws.sort_ka.as<uint64_t>(), (uint32_t)n, w, self_first, rows,
+                                         dat.as<uint32_t>(), off.as<uint32_t>());
     GX_HIP(hipGetLastError());
   }
-  GX_CHECK(scan_u32(h, off.as<uint32_t>(), off.as<uint32_t>(), (size_t)rows + 1, false, false, nullptr));
   return 0;
 }
 
@@ -210,10 +246,23 @@ int build_lists_corrected(galax_fmm* h, const double* d_x, const double* d_y) {
   }
   Workspace& ws = h->ws;
   GX_CHECK(dbuf_reserve(ws.a, nn * 8, s));
-  k_radii<<<(unsigned)((nn + 3) / 4), 256, 0, s>>>(d_x, d_y, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(),
-                                                   t.com_x.as<double>(), t.com_y.as<double>(), (uint32_t)nn,
-                                                   ws.a.as<double>());
-  GX_HIP(hipGetLastError());
+  {
+    GX_CHECK(dbuf_reserve(ws.b, (nn + 1) * 4, s));
+    uint32_t* ioff = ws.b.as<uint32_t>();
+    k_radii_items<<<nb256(nn), 256, 0, s>>>(t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(), (uint32_t)nn,
+                                            ioff);
+    GX_HIP(hipGetLastError());
+    GX_HIP(hipMemsetAsync(ioff + nn, 0, 4, s));
+    uint32_t nitems = 0;
+    GX_CHECK(scan_u32(h, ioff, ioff, nn + 1, false, false, &nitems));
+    GX_HIP(hipMemsetAsync(ws.a.p, 0, nn * 8, s));
+    k_radii<<<(nitems + 3) / 4, 256, 0, s>>>(d_x, d_y, t.body_start.as<uint32_t>(), t.body_end.as<uint32_t>(),
+                                             t.com_x.as<double>(), t.com_y.as<double>(), (uint32_t)nn, ioff, nitems,
+                                             ws.a.as<unsigned long long>());
+    GX_HIP(hipGetLastError());
+    k_radii_sqrt<<<nb256(nn), 256, 0, s>>>((uint32_t)nn, ws.a.as<double>());
+    GX_HIP(hipGetLastError());
+  }
   const DttTree dt{t.com_x.as<double>(), t.com_y.as<double>(), ws.a.as<double>(), t.child_first.as<uint32_t>(),
                    t.child_count.as<uint8_t>(), h->cfg.theta};
   // rounds: pairs [lo, hi) expand into [hi, hi + total)
