@@ -278,10 +278,14 @@ int build_lists_corrected(galax_fmm* h, const double* d_x, const double* d_y) {
     uint32_t* cnt = ws.b.as<uint32_t>();
     k_dtt_count<<<nb256(n), 256, 0, s>>>(dt, P.as<uint64_t>(), (uint32_t)lo, (uint32_t)hi, cnt);
     GX_HIP(hipGetLastError());
-    // pair ids and the scan are 32-bit: check the exact 64-bit total first
-    uint64_t total64 = 0;
-    GX_CHECK(sum_u32(h, cnt, n, &total64));
-    if (hi + total64 >= (1ull << 32)) return set_error(GALAX_ERR_UNSUPPORTED, "dual tree traversal above 2^32 pairs");
+    // pair ids and the scan are 32-bit: a pair has at most 10 child pairs
+    // (a self pair of a node with 4 children), so the exact 64-bit total is
+    // needed only when hi + 10 n could reach 2^32 (it costs a wait per round)
+    if (hi + 10ull * n >= (1ull << 32)) {
+      uint64_t total64 = 0;
+      GX_CHECK(sum_u32(h, cnt, n, &total64));
+      if (hi + total64 >= (1ull << 32)) return set_error(GALAX_ERR_UNSUPPORTED, "dual tree traversal above 2^32 pairs");
+    }
     uint32_t total = 0;
     GX_CHECK(scan_u32(h, cnt, cnt, n, false, false, &total));
     GX_CHECK(grow_pairs(P, hi, hi + total, s));
