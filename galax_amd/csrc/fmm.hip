@@ -1384,8 +1384,11 @@ hipError_t p2p_combine(const P2PItems& it, const uint32_t* goff, const uint32_t*
 // Leaves here are the tree's leaves, or halves of them (vleaf: each half's
 // leaf, whose local expansion and parent give the far field; NULL: the
 // leaves themselves); bs, be, cx, cy: body range and COM of each.
+// Five waves per SIMD for the forces alone (96 VGPRs, a few spilled: the
+// kernel is VALU-bound and the extra wave hides its loads; 4% faster than four
+// waves); the potential's extra sums would spill heavily at five.
 template <int P, bool PHI>
-__global__ void __launch_bounds__(64) k_p2p_sym(
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PHI ? 1 : 5))) k_p2p_sym(
     const float4* __restrict__ rel, const double* __restrict__ x, const double* __restrict__ y,
     const uint32_t* __restrict__ bs, const uint32_t* __restrict__ be, const double* __restrict__ cx,
     const double* __restrict__ cy, const uint32_t* __restrict__ vleaf, const double* __restrict__ comx,
@@ -1482,18 +1485,28 @@ __global__ void __launch_bounds__(64) k_p2p_sym(
       if (PHI) pairp[o] = ajp;
     }
   };
-  // two partners per iteration with their data in two register sets (A, B),
-  // each fetched while the other is computed: no copies between iterations
-  uint32_t nbA = 0, nbB = 0;
-  float4 vA = make_float4(FAR, FAR, 0.f, 0.f), vB = vA;
-  double cxA = 0.0, cyA = 0.0, cxB = 0.0, cyB = 0.0;
-  if (u0 < u1) fetch(u0, nbA, vA, cxA, cyA);
-  for (uint32_t u = u0; u < u1; u += 2) {
-    if (u + 1 < u1) fetch(u + 1, nbB, vB, cxB, cyB);
-    partner(u, nbA, vA, cxA, cyA);
-    if (u + 1 >= u1) break;
-    if (u + 2 < u1) fetch(u + 2, nbA, vA, cxA, cyA);
-    partner(u + 1, nbB, vB, cxB, cyB);
+  if constexpr (!PHI) {  // one partner at a time: five waves per SIMD hide its loads
+    for (uint32_t u = u0; u < u1; ++u) {
+      uint32_t nb = 0;
+      float4 v;
+      double bx, by;
+      fetch(u, nb, v, bx, by);
+      partner(u, nb, v, bx, by);
+    }
+  } else {
+    // two partners per iteration with their data in two register sets (A, B),
+    // each fetched while the other is computed: no copies between iterations
+    uint32_t nbA = 0, nbB = 0;
+    float4 vA = make_float4(FAR, FAR, 0.f, 0.f), vB = vA;
+    double cxA = 0.0, cyA = 0.0, cxB = 0.0, cyB = 0.0;
+    if (u0 < u1) fetch(u0, nbA, vA, cxA, cyA);
+    for (uint32_t u = u0; u < u1; u += 2) {
+      if (u + 1 < u1) fetch(u + 1, nbB, vB, cxB, cyB);
+      partner(u, nbA, vA, cxA, cyA);
+      if (u + 1 >= u1) break;
+      if (u + 2 < u1) fetch(u + 2, nbA, vA, cxA, cyA);
+      partner(u + 1, nbB, vB, cxB, cyB);
+    }
   }
   // the i side: acc_i[K] of lane q belongs to the i of its rotated lane
   float tix = 0.f, tiy = 0.f, tip = 0.f;
